@@ -1,0 +1,1196 @@
+// gain_engine context and C-ABI entry points (the device-manager role of
+// GAIN-MPI's GpuManager, BaseClasses/GpuManager.cpp).
+//
+// One context = one GPU + one HIP stream.  Uploaded data (dipole, bases,
+// inflation, eigenvector blocks) stay resident in HBM; per-call work buffers
+// grow on demand and are reused.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "gemm.hpp"
+#include "kernels.hpp"
+
+namespace gain {
+
+static thread_local std::string t_last_error;
+void set_last_error(const char* msg) { t_last_error = msg; }
+
+// ---------------------------------------------------------------------------
+// Wigner 3j symbol (j1 j2 j3; m1 m2 m3) by the Racah formula in long double,
+// the quantity the reference obtains from TROVE's three_j
+// (fort_func.f90:7-104) for its table (GpuManager.cpp:164-184).
+static long double lfact(int n) {
+  long double f = 1.0L;
+  for (int i = 2; i <= n; ++i) f *= (long double)i;
+  return f;
+}
+
+static double three_j(int j1, int j2, int j3, int m1, int m2, int m3) {
+  if (j3 > j1 + j2 || j3 < std::abs(j1 - j2)) return 0.0;
+  if (j1 < 0 || j2 < 0 || j3 < 0) return 0.0;
+  if (j1 < std::abs(m1) || j2 < std::abs(m2) || j3 < std::abs(m3)) return 0.0;
+  if (m1 + m2 + m3 != 0) return 0.0;
+  const long double tri = lfact(j1 + j2 - j3) * lfact(j1 - j2 + j3) *
+                          lfact(-j1 + j2 + j3) / lfact(j1 + j2 + j3 + 1);
+  const long double pre =
+      std::sqrt(tri * lfact(j1 + m1) * lfact(j1 - m1) * lfact(j2 + m2) *
+                lfact(j2 - m2) * lfact(j3 + m3) * lfact(j3 - m3));
+  const int kmin = std::max({0, j2 - j3 - m1, j1 - j3 + m2});
+  const int kmax = std::min({j1 + j2 - j3, j1 - m1, j2 + m2});
+  long double sum = 0.0L;
+  for (int k = kmin; k <= kmax; ++k) {
+    const long double den = lfact(k) * lfact(j3 - j2 + k + m1) *
+                            lfact(j3 - j1 + k - m2) * lfact(j1 + j2 - j3 - k) *
+                            lfact(j1 - k - m1) * lfact(j2 - k + m2);
+    sum += (k & 1) ? -1.0L / den : 1.0L / den;
+  }
+  const int ph = ((j1 - j2 - m3) % 2 + 2) % 2;
+  return (double)((ph ? -1.0L : 1.0L) * pre * sum);
+}
+
+// ---------------------------------------------------------------------------
+struct DeviceArena {
+  std::vector<void*> ptrs;
+  template <class T>
+  T* alloc(size_t n) {
+    if (n == 0) n = 1;
+    void* p = nullptr;
+    if (hipMalloc(&p, n * sizeof(T)) != hipSuccess)
+      fail(GAIN_ERR_NOMEM, "hipMalloc of %zu bytes failed", n * sizeof(T));
+    ptrs.push_back(p);
+    return (T*)p;
+  }
+  template <class T>
+  T* upload(const T* h, size_t n) {
+    T* d = alloc<T>(n);
+    if (n) GAIN_HIP(hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+  }
+  void release(void* p) {
+    auto it = std::find(ptrs.begin(), ptrs.end(), p);
+    if (it != ptrs.end()) {
+      (void)hipFree(*it);
+      ptrs.erase(it);
+    }
+  }
+  ~DeviceArena() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+};
+
+// Grow-only device work buffer.
+struct Work {
+  void* p = nullptr;
+  size_t bytes = 0;
+  template <class T>
+  T* get(size_t n) {
+    const size_t need = std::max<size_t>(n, 1) * sizeof(T);
+    if (need > bytes) {
+      if (p) {
+        // earlier launches may still read the old buffer
+        (void)hipDeviceSynchronize();
+        (void)hipFree(p);
+      }
+      p = nullptr;
+      size_t nb = std::max(need, bytes * 3 / 2);
+      if (hipMalloc(&p, nb) != hipSuccess) {
+        bytes = 0;
+        fail(GAIN_ERR_NOMEM, "work buffer allocation of %zu bytes failed", nb);
+      }
+      bytes = nb;
+    }
+    return (T*)p;
+  }
+  ~Work() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+// Pinned host staging ring for per-call parameter arrays: a slot is reused
+// only after the copy that read it has executed.
+struct PinnedRing {
+  static constexpr int kSlots = 4;
+  void* h[kSlots] = {};
+  size_t bytes[kSlots] = {};
+  hipEvent_t ev[kSlots] = {};
+  bool used[kSlots] = {};
+  int next = 0;
+  void* acquire(size_t n, int* slot) {
+    const int s = next;
+    next = (next + 1) % kSlots;
+    if (used[s]) GAIN_HIP(hipEventSynchronize(ev[s]));
+    if (n > bytes[s]) {
+      if (h[s]) (void)hipHostFree(h[s]);
+      h[s] = nullptr;
+      size_t nb = std::max<size_t>(n, 1 << 16);
+      GAIN_HIP(hipHostMalloc(&h[s], nb, hipHostMallocDefault));
+      bytes[s] = nb;
+    }
+    if (!ev[s]) GAIN_HIP(hipEventCreateWithFlags(&ev[s], hipEventDisableTiming));
+    *slot = s;
+    return h[s];
+  }
+  void release(int s, hipStream_t st) {
+    GAIN_HIP(hipEventRecord(ev[s], st));
+    used[s] = true;
+  }
+  ~PinnedRing() {
+    for (int s = 0; s < kSlots; ++s) {
+      if (ev[s]) {
+        (void)hipEventSynchronize(ev[s]);
+        (void)hipEventDestroy(ev[s]);
+      }
+      if (h[s]) (void)hipHostFree(h[s]);
+    }
+  }
+};
+
+struct Irrep {
+  int ntot = 0, sdeg = 0;
+  int* N = nullptr;
+  double* repres = nullptr;
+  int* tcase = nullptr;
+};
+
+struct Basis {
+  bool present = false, has_infl = false;
+  int J = 0, nprim = 0, ncol = 0, nvib = 0;
+  std::vector<int> kstart, kblock, hk, htau, hvib;
+  int *kval = nullptr, *tau = nullptr, *vib = nullptr;
+  // scatter CSR over (col, v) slots, built for a given nc
+  int slot_nc = -1;
+  int *slot_ptr = nullptr, *slot_prim = nullptr;
+  // inflation
+  int maxsymcoeffs = 0, matsize = 0;
+  int *ijterms = nullptr, *icontr = nullptr;
+  int *case_ptr = nullptr, *case_prim = nullptr;
+  Irrep irrep[GAIN_MAX_IRREPS];
+};
+
+struct Block {
+  int n = 0, rec_len = 0;
+  int64_t ld = 0;
+  const double* vec = nullptr;  // device, may be null (energies only)
+  bool owned = false;
+  std::vector<double> energy;
+  double* d_energy = nullptr;
+  bool sorted = true;
+};
+
+struct TimerSlot {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending, pool;
+  double total_ms = 0.0;
+  int64_t launches = 0;
+};
+
+}  // namespace gain
+
+using namespace gain;
+
+struct gain_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int jmax = 0, nrepres = 0, maxdeg = 0;
+  std::vector<double> threej;
+  double* d_threej = nullptr;
+  gain_params params{};
+  int pair[GAIN_MAX_IRREPS] = {};
+  bool params_set = false;
+  double* d_mu = nullptr;
+  int nc = 0;
+  std::vector<Basis> bases;
+  std::map<std::pair<int, int>, Block> blocks;
+  DeviceArena arena;
+  Work w_c, w_X, w_Y, w_H, w_G, w_part, w_small, w_out;
+  PinnedRing ring;
+  // tuning
+  int gemv_rows = 4, gemv_ksplit = 8, mu_ksplit = 4;
+  bool timing = false;
+  std::map<std::string, TimerSlot> timers;
+};
+
+namespace gain {
+namespace {
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    GAIN_HIP(hipGetDevice(&prev));
+    if (prev != dev) GAIN_HIP(hipSetDevice(dev));
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+// RAII event pair around the launches of one kernel family.
+struct Timed {
+  gain_ctx* c;
+  TimerSlot* slot = nullptr;
+  std::pair<hipEvent_t, hipEvent_t> ev{};
+  Timed(gain_ctx* ctx, const char* name) : c(ctx) {
+    if (!c->timing) return;
+    slot = &c->timers[name];
+    if (!slot->pool.empty()) {
+      ev = slot->pool.back();
+      slot->pool.pop_back();
+    } else {
+      GAIN_HIP(hipEventCreate(&ev.first));
+      GAIN_HIP(hipEventCreate(&ev.second));
+    }
+    GAIN_HIP(hipEventRecord(ev.first, c->stream));
+  }
+  ~Timed() {
+    if (!slot) return;
+    (void)hipEventRecord(ev.second, c->stream);
+    slot->pending.push_back(ev);
+  }
+};
+
+void resolve_timers(gain_ctx* c, TimerSlot& t) {
+  for (auto& e : t.pending) {
+    GAIN_HIP(hipEventSynchronize(e.second));
+    float ms = 0.f;
+    GAIN_HIP(hipEventElapsedTime(&ms, e.first, e.second));
+    t.total_ms += ms;
+    t.launches += 1;
+    t.pool.push_back(e);
+  }
+  t.pending.clear();
+  (void)c;
+}
+
+gain_ctx* need(gain_ctx* c) {
+  if (!c) fail(GAIN_ERR_ARG, "null context");
+  return c;
+}
+
+Basis& basis_of(gain_ctx* c, int jind, bool need_infl) {
+  if (jind < 0 || jind >= (int)c->bases.size() || !c->bases[jind].present)
+    fail(GAIN_ERR_STATE, "basis for J index %d not uploaded", jind);
+  Basis& b = c->bases[jind];
+  if (need_infl && !b.has_infl)
+    fail(GAIN_ERR_STATE, "inflation data for J index %d not uploaded", jind);
+  return b;
+}
+
+Block& block_of(gain_ctx* c, int jind, int gamma) {
+  auto it = c->blocks.find({jind, gamma});
+  if (it == c->blocks.end())
+    fail(GAIN_ERR_STATE, "state block (J index %d, irrep %d) not uploaded",
+         jind, gamma);
+  return it->second;
+}
+
+// Scatter CSR for a basis: slot = (2K+tau)*nc + v -> primitives.
+void ensure_slots(gain_ctx* c, Basis& b) {
+  if (b.slot_nc == c->nc) return;
+  if (b.nvib > c->nc)
+    fail(GAIN_ERR_ARG,
+         "vibrational index %d of J=%d exceeds the dipole dimension %d",
+         b.nvib - 1, b.J, c->nc);
+  const int nslot = b.ncol * c->nc;
+  std::vector<int> cnt(nslot + 1, 0);
+  std::vector<int> slot(b.nprim);
+  for (int r = 0; r < b.nprim; ++r) {
+    slot[r] = (2 * b.hk[r] + (b.htau[r] & 1)) * c->nc + b.hvib[r];
+    cnt[slot[r] + 1]++;
+  }
+  for (int s = 0; s < nslot; ++s) cnt[s + 1] += cnt[s];
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1), prim(b.nprim);
+  for (int r = 0; r < b.nprim; ++r) prim[fill[slot[r]]++] = r;
+  if (b.slot_ptr) c->arena.release(b.slot_ptr);
+  if (b.slot_prim) c->arena.release(b.slot_prim);
+  b.slot_ptr = c->arena.upload(cnt.data(), cnt.size());
+  b.slot_prim = c->arena.upload(prim.data(), prim.size());
+  b.slot_nc = c->nc;
+}
+
+bool degfilter(const gain_ctx* c, int gI, int gF, int dI, int dF) {
+  // States::DegeneracyFilter, States.cpp:95-116
+  const gain_params& p = c->params;
+  if (p.sym_degen[gI] == 1 || p.sym_degen[gF] == 1) return true;
+  if (!p.reduced) return true;
+  if (p.symmetry_type == 1) return true;
+  if (p.symmetry_type == 2) return dI == 1 && dF == 0;
+  if (p.symmetry_type == 3) {
+    if (gI == 3 && gF == 4 && dI == 1 && dF == 0) return true;
+    if (gI == 4 && gF == 3 && dI == 2 && dF == 0) return true;
+    if (gI == 2 && gF == 2 && dI == 1 && dF == 0) return true;
+  }
+  return false;
+}
+
+bool filter_lower(const gain_ctx* c, double e, int g) {
+  const gain_params& p = c->params;
+  return (e - p.zpe) <= p.erange_lower[1] && (e - p.zpe) >= p.erange_lower[0] &&
+         p.isym_do[g];
+}
+
+// First index in [0,n) where pred holds, for a monotone predicate.
+template <class P>
+int lower_bound_pred(int n, P pred) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = lo + (hi - lo) / 2;
+    if (pred(mid))
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  return lo;
+}
+
+// Rows of the final block that can pass the energy part of the filter for
+// any of the initial energies (superset; the exact test runs per pair).
+void final_range(const gain_ctx* c, const Block& bF,
+                 const std::vector<double>& eI, const std::vector<uint8_t>& ok,
+                 int* lo, int* hi) {
+  const gain_params& p = c->params;
+  if (!bF.sorted) {
+    *lo = 0;
+    *hi = bF.n;
+    return;
+  }
+  int L = bF.n, H = 0;
+  const std::vector<double>& e = bF.energy;
+  for (size_t i = 0; i < eI.size(); ++i) {
+    if (!ok[i]) continue;
+    const double ei = eI[i];
+    // nu >= fw0 and eF - zpe >= eu0 are monotone in eF
+    const int l = lower_bound_pred(bF.n, [&](int f) {
+      return (e[f] - ei) >= p.freq_window[0] && (e[f] - p.zpe) >= p.erange_upper[0];
+    });
+    // first index where nu > fw1 or eF - zpe > eu1
+    const int h = lower_bound_pred(bF.n, [&](int f) {
+      return !((e[f] - ei) <= p.freq_window[1]) ||
+             !((e[f] - p.zpe) <= p.erange_upper[1]);
+    });
+    if (l < h) {
+      L = std::min(L, l);
+      H = std::max(H, h);
+    }
+  }
+  if (L >= H) L = H = 0;
+  *lo = L;
+  *hi = H;
+}
+
+struct Plan {
+  int jindI, gI, jindF, gF;
+  Basis* bI;
+  Basis* bF;
+  Block* blkI;
+  Block* blkF;
+  int nI;
+  std::vector<int> levels;
+  std::vector<double> eI;
+  std::vector<uint8_t> okI;
+  std::vector<int> DI;                   // initial components computed
+  std::vector<std::pair<int, int>> vec;  // (dI, dF) in Output.cpp order
+  bool block_ok = false;
+  int Flo = 0, Fhi = 0;
+};
+
+Plan make_plan(gain_ctx* c, int jindI, int gI, const int* levels, int nI,
+               int jindF, int gF) {
+  if (!c->params_set) fail(GAIN_ERR_STATE, "gain_set_params not called");
+  if (!c->d_mu) fail(GAIN_ERR_STATE, "dipole not uploaded");
+  GAIN_CHECK_ARG(gI >= 0 && gI < c->nrepres && gF >= 0 && gF < c->nrepres,
+                 "irrep index out of range");
+  Plan P;
+  P.jindI = jindI;
+  P.gI = gI;
+  P.jindF = jindF;
+  P.gF = gF;
+  P.bI = &basis_of(c, jindI, true);
+  P.bF = &basis_of(c, jindF, true);
+  P.blkI = &block_of(c, jindI, gI);
+  P.blkF = &block_of(c, jindF, gF);
+  if (!P.blkF->vec)
+    fail(GAIN_ERR_STATE, "final block (%d,%d) has no eigenvectors", jindF, gF);
+  GAIN_CHECK_ARG(P.blkI->rec_len == P.bI->irrep[gI].ntot,
+                 "initial block record length %d != Ntot %d",
+                 P.blkI->rec_len, P.bI->irrep[gI].ntot);
+  GAIN_CHECK_ARG(P.blkF->rec_len == P.bF->irrep[gF].ntot,
+                 "final block record length %d != Ntot %d", P.blkF->rec_len,
+                 P.bF->irrep[gF].ntot);
+  GAIN_CHECK_ARG(P.bI->J <= c->jmax && P.bF->J <= c->jmax,
+                 "J above the context's jmax %d", c->jmax);
+  GAIN_CHECK_ARG(c->params.sym_degen[gI] == P.bI->irrep[gI].sdeg &&
+                     c->params.sym_degen[gF] == P.bF->irrep[gF].sdeg,
+                 "irrep degeneracy of the parameters differs from the "
+                 "inflation data");
+  GAIN_CHECK_ARG(nI >= 1, "no initial states");
+  P.nI = nI;
+  const gain_params& p = c->params;
+  for (int i = 0; i < nI; ++i) {
+    GAIN_CHECK_ARG(levels[i] >= 0 && levels[i] < P.blkI->n,
+                   "initial level %d out of range [0,%d)", levels[i],
+                   P.blkI->n);
+    P.levels.push_back(levels[i]);
+    const double e = P.blkI->energy[levels[i]];
+    P.eI.push_back(e);
+    P.okI.push_back(filter_lower(c, e, gI) ? 1 : 0);
+  }
+  const int jI = P.bI->J, jF = P.bF->J;
+  P.block_ok = p.gns[gI] > 0.0 && p.gns[gF] > 0.0 &&
+               (jF != p.maxJ || jI != p.maxJ) &&
+               p.isym_pairs[gI] == p.isym_pairs[gF] && c->pair[gI] == gF &&
+               std::abs(jF - jI) <= 1 && jI + jF >= 1;
+  const int ndegI = p.sym_degen[gI], ndegF = p.sym_degen[gF];
+  for (int dI = 0; dI < ndegI; ++dI)
+    if (degfilter(c, gI, c->pair[gI], dI, 0)) P.DI.push_back(dI);
+  for (int dF = 0; dF < ndegF; ++dF)
+    for (int dI = 0; dI < ndegI; ++dI)
+      if (degfilter(c, gI, gF, dI, dF)) P.vec.push_back({dI, dF});
+  if (P.block_ok) final_range(c, *P.blkF, P.eI, P.okI, &P.Flo, &P.Fhi);
+  return P;
+}
+
+// Device scratch laid out from one buffer of small per-call arrays.
+struct Small {
+  int64_t* c_off;
+  int* ideg;
+  int* h_of_vec;
+  int* ideg_of_vec;
+  int* slot_of_vec;
+  double* eI;
+  int* levelI;
+  uint8_t* okI;
+};
+
+// Half line strengths of every (initial, dI) vector: scatter, mu GEMM and
+// gather.  H rows: i * |DI| + k.  Returns the H stride.
+int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
+                    const Small& sm, int nvecX, double* H) {
+  Basis& bI = *P.bI;
+  Basis& bF = *P.bF;
+  ensure_slots(c, bI);
+  if (bF.nvib > c->nc)
+    fail(GAIN_ERR_ARG,
+         "vibrational index %d of J=%d exceeds the dipole dimension %d",
+         bF.nvib - 1, bF.J, c->nc);
+  const Irrep& ir = bI.irrep[P.gI];
+  const int nc = c->nc;
+  const int64_t xstride = (int64_t)bI.ncol * nc;
+  double* X = c->w_X.get<double>((size_t)nvecX * xstride);
+  const int ncolTot = nvecX * bI.ncol;
+  const int ks = c->mu_ksplit;
+  double* Y = c->w_Y.get<double>((size_t)ks * 3 * ncolTot * nc);
+  Timed t(c, "half_ls");
+  ScatterArgs sa{};
+  sa.nslot = bI.ncol * nc;
+  sa.nprim = bI.nprim;
+  sa.slot_ptr = bI.slot_ptr;
+  sa.slot_prim = bI.slot_prim;
+  sa.kval = bI.kval;
+  sa.tau = bI.tau;
+  sa.icontr = bI.icontr;
+  sa.ijt = bI.ijterms + (int64_t)P.gI * bI.maxsymcoeffs;
+  sa.N = ir.N;
+  sa.repres = ir.repres;
+  sa.sdeg = ir.sdeg;
+  sa.ntot = ir.ntot;
+  sa.cbase = cbase;
+  sa.c_off = sm.c_off;
+  sa.ideg = sm.ideg;
+  sa.nvec = nvecX;
+  sa.X = X;
+  sa.xstride = xstride;
+  launch_scatter_expand(sa, c->stream);
+  launch_mu_gemm_valu(c->d_mu, nc, X, ncolTot, Y, ks, c->stream);
+  GatherArgs ga{};
+  ga.nprimF = bF.nprim;
+  ga.nc = nc;
+  ga.jI = bI.J;
+  ga.jF = bF.J;
+  ga.jmax = c->jmax;
+  ga.ncolI = bI.ncol;
+  ga.nvec = nvecX;
+  ga.ksplit = ks;
+  ga.kvalF = bF.kval;
+  ga.tauF = bF.tau;
+  ga.vibF = bF.vib;
+  ga.threej = c->d_threej;
+  ga.Y = Y;
+  ga.ys_alpha = (int64_t)ncolTot * nc;
+  ga.ys_ks = 3 * (int64_t)ncolTot * nc;
+  ga.H = H;
+  ga.hstride = bF.nprim;
+  launch_gather_h(ga, c->stream);
+  return bF.nprim;
+}
+
+// Lines for nI initial states of one block against one final block.
+void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
+               int precision, bool plain, const gain_lines* out) {
+  const gain_params& p = c->params;
+  Basis& bF = *P.bF;
+  Block& blkI = *P.blkI;
+  Block& blkF = *P.blkF;
+  const int nI = P.nI, nF = blkF.n;
+  const int nDI = (int)P.DI.size();
+  const int nvpi = (int)P.vec.size();
+  const int md2 = c->maxdeg * c->maxdeg;
+  const bool compute = P.Fhi > P.Flo && nvpi > 0 && nDI > 0;
+  const int nvecX = nI * nDI;
+  const int nvecG = nI * nvpi;
+
+  // ---- small per-call arrays through the pinned ring
+  const size_t off_c = 0;
+  const size_t off_ideg = off_c + sizeof(int64_t) * std::max(nvecX, 1);
+  const size_t off_hv = off_ideg + sizeof(int) * std::max(nvecX, 1);
+  const size_t off_dv = off_hv + sizeof(int) * std::max(nvecG, 1);
+  const size_t off_sv = off_dv + sizeof(int) * std::max(nvecG, 1);
+  const size_t off_e = (off_sv + sizeof(int) * 16 + 7) / 8 * 8;
+  const size_t off_l = off_e + sizeof(double) * nI;
+  const size_t off_ok = off_l + sizeof(int) * nI;
+  const size_t total = off_ok + nI + 16;
+  int slot;
+  char* hs = (char*)c->ring.acquire(total, &slot);
+  int64_t* h_c = (int64_t*)(hs + off_c);
+  int* h_ideg = (int*)(hs + off_ideg);
+  int* h_hv = (int*)(hs + off_hv);
+  int* h_dv = (int*)(hs + off_dv);
+  int* h_sv = (int*)(hs + off_sv);
+  for (int i = 0; i < nI; ++i)
+    for (int k = 0; k < nDI; ++k) {
+      h_c[i * nDI + k] = cI_host ? 0 : (int64_t)P.levels[i] * blkI.ld;
+      h_ideg[i * nDI + k] = P.DI[k];
+    }
+  for (int i = 0; i < nI; ++i)
+    for (int v = 0; v < nvpi; ++v) {
+      const int dI = P.vec[v].first, dF = P.vec[v].second;
+      int pos = -1;
+      for (int k = 0; k < nDI; ++k)
+        if (P.DI[k] == dI) pos = k;
+      // a (dI,dF) pair whose dI has no half line strength reads a zero
+      // vector (the reference reads its memset-zero buffer)
+      h_hv[i * nvpi + v] = pos >= 0 ? i * nDI + pos : nvecX;
+      h_dv[i * nvpi + v] = dF;
+    }
+  for (int v = 0; v < nvpi; ++v)
+    h_sv[v] = P.vec[v].first + P.vec[v].second * c->maxdeg;
+  std::memcpy(hs + off_e, P.eI.data(), sizeof(double) * nI);
+  std::memcpy(hs + off_l, P.levels.data(), sizeof(int) * nI);
+  std::memcpy(hs + off_ok, P.okI.data(), nI);
+  char* ds = c->w_small.get<char>(total);
+  GAIN_HIP(hipMemcpyAsync(ds, hs, total, hipMemcpyHostToDevice, c->stream));
+  Small sm{(int64_t*)(ds + off_c), (int*)(ds + off_ideg),
+           (int*)(ds + off_hv),    (int*)(ds + off_dv),
+           (int*)(ds + off_sv),    (double*)(ds + off_e),
+           (int*)(ds + off_l),     (uint8_t*)(ds + off_ok)};
+  const double* cbase = blkI.vec;
+  int cslot = -1;
+  if (cI_host) {
+    const size_t nb = sizeof(double) * blkI.rec_len;
+    void* hc = c->ring.acquire(nb, &cslot);
+    std::memcpy(hc, cI_host, nb);
+    double* dc = c->w_c.get<double>(blkI.rec_len);
+    GAIN_HIP(hipMemcpyAsync(dc, hc, nb, hipMemcpyHostToDevice, c->stream));
+    cbase = dc;
+  }
+  c->ring.release(slot, c->stream);
+  if (cslot >= 0) c->ring.release(cslot, c->stream);
+  if (!cbase && compute)
+    fail(GAIN_ERR_STATE, "initial block has no eigenvectors and no cI given");
+
+  const int nR = P.Fhi - P.Flo;
+  const Irrep& irF = bF.irrep[P.gF];
+  const int ntotF = irF.ntot;
+  const int64_t gstride = (ntotF + 1) / 2 * 2;
+  const double* src = nullptr;
+  int ksplit = 1;
+  int64_t src_ks = 0, src_f = 0, src_v = 0;
+  if (compute) {
+    const int64_t hstride = bF.nprim;
+    double* H = c->w_H.get<double>((size_t)(nvecX + 1) * hstride);
+    launch_fill_zero(H + (size_t)nvecX * hstride, hstride, c->stream);
+    run_half_ls(c, P, cbase, sm, nvecX, H);
+    double* G = c->w_G.get<double>((size_t)nvecG * gstride);
+    {
+      Timed t(c, "expand_T");
+      ExpandTArgs ea{};
+      ea.ntot = ntotF;
+      ea.nprim = bF.nprim;
+      ea.sdeg = irF.sdeg;
+      ea.tcase = irF.tcase;
+      ea.case_ptr = bF.case_ptr;
+      ea.case_prim = bF.case_prim;
+      ea.icontr = bF.icontr;
+      ea.repres = irF.repres;
+      ea.H = H;
+      ea.hstride = hstride;
+      ea.h_of_vec = sm.h_of_vec;
+      ea.ideg_of_vec = sm.ideg_of_vec;
+      ea.nvec = nvecG;
+      ea.G = G;
+      ea.gstride = gstride;
+      launch_expand_T(ea, c->stream);
+    }
+    const double* Crows = blkF.vec;
+    if (plain && nI == 1) {
+      ksplit = c->gemv_ksplit;
+      double* part = c->w_part.get<double>((size_t)ksplit * nR * nvpi);
+      Timed t(c, "gemv");
+      GemvArgs a{};
+      a.C = Crows;
+      a.ld = blkF.ld;
+      a.ntot = ntotF;
+      a.F0 = P.Flo;
+      a.nF = nR;
+      a.G = G;
+      a.gstride = gstride;
+      a.nvec = nvpi;
+      a.partial = part;
+      a.ksplit = ksplit;
+      a.rows = c->gemv_rows;
+      launch_gemv(a, c->stream);
+      src = part;
+      src_ks = (int64_t)nR * nvpi;
+      src_f = nvpi;
+      src_v = 1;
+    } else {
+      double* LS = c->w_part.get<double>((size_t)nR * nvecG);
+      Timed t(c, "gemm_proj");
+      gemm_projection(precision, nR, nvecG, ntotF, Crows + (int64_t)P.Flo * blkF.ld,
+                      blkF.ld, G, gstride, LS, nR, c->stream);
+      src = LS;
+      src_f = 1;
+      src_v = nR;
+    }
+  }
+  // ---- epilogue
+  const bool host_out = !out->on_device;
+  const size_t nPairs = (size_t)nI * nF;
+  double *o_ls = out->ls, *o_S = out->S, *o_A = out->A, *o_I = out->I;
+  uint8_t* o_fl = out->flags;
+  char* stage = nullptr;
+  size_t s_ls = 0, s_S = 0, s_A = 0, s_I = 0, s_fl = 0, s_tot = 0;
+  if (host_out) {
+    s_ls = 0;
+    s_S = s_ls + (out->ls ? nPairs * md2 * sizeof(double) : 0);
+    s_A = s_S + (out->S ? nPairs * sizeof(double) : 0);
+    s_I = s_A + (out->A ? nPairs * sizeof(double) : 0);
+    s_fl = s_I + (out->I ? nPairs * sizeof(double) : 0);
+    s_tot = s_fl + (out->flags ? nPairs : 0);
+    stage = c->w_out.get<char>(s_tot);
+    o_ls = out->ls ? (double*)(stage + s_ls) : nullptr;
+    o_S = out->S ? (double*)(stage + s_S) : nullptr;
+    o_A = out->A ? (double*)(stage + s_A) : nullptr;
+    o_I = out->I ? (double*)(stage + s_I) : nullptr;
+    o_fl = out->flags ? (uint8_t*)(stage + s_fl) : nullptr;
+  }
+  {
+    Timed t(c, "epilogue");
+    LinesArgs la{};
+    la.nI = nI;
+    la.nF = nF;
+    la.Flo = compute ? P.Flo : 0;
+    la.Fhi = compute ? P.Fhi : 0;
+    la.maxdeg = c->maxdeg;
+    la.nvec_per_i = nvpi;
+    la.slot_of_vec = sm.slot_of_vec;
+    la.src = src;
+    la.ksplit = ksplit;
+    la.src_ks = src_ks;
+    la.src_f = src_f;
+    la.src_v = src_v;
+    la.eI = sm.eI;
+    la.levelI = sm.levelI;
+    la.okI = sm.okI;
+    la.eF = blkF.d_energy;
+    la.same_block = (P.jindI == P.jindF && P.gI == P.gF);
+    la.block_ok = P.block_ok;
+    la.jI = P.bI->J;
+    la.jF = bF.J;
+    la.ndegI = p.sym_degen[P.gI];
+    la.ndegF = p.sym_degen[P.gF];
+    la.reduced = p.reduced;
+    la.gnsF = p.gns[P.gF];
+    la.zpe = p.zpe;
+    la.threshold = p.threshold;
+    la.beta = kPlanck * kVellgt / (kBoltz * p.temperature);
+    la.partition = p.partition;
+    la.fw0 = p.freq_window[0];
+    la.fw1 = p.freq_window[1];
+    la.eu0 = p.erange_upper[0];
+    la.eu1 = p.erange_upper[1];
+    la.isym_do_F = p.isym_do[P.gF];
+    la.ls = o_ls;
+    la.S = o_S;
+    la.A = o_A;
+    la.I = o_I;
+    la.flags = o_fl;
+    launch_lines(la, c->stream);
+  }
+  if (host_out) {
+    if (out->ls)
+      GAIN_HIP(hipMemcpyAsync(out->ls, o_ls, nPairs * md2 * sizeof(double),
+                              hipMemcpyDeviceToHost, c->stream));
+    if (out->S)
+      GAIN_HIP(hipMemcpyAsync(out->S, o_S, nPairs * sizeof(double),
+                              hipMemcpyDeviceToHost, c->stream));
+    if (out->A)
+      GAIN_HIP(hipMemcpyAsync(out->A, o_A, nPairs * sizeof(double),
+                              hipMemcpyDeviceToHost, c->stream));
+    if (out->I)
+      GAIN_HIP(hipMemcpyAsync(out->I, o_I, nPairs * sizeof(double),
+                              hipMemcpyDeviceToHost, c->stream));
+    if (out->flags)
+      GAIN_HIP(hipMemcpyAsync(out->flags, o_fl, nPairs, hipMemcpyDeviceToHost,
+                              c->stream));
+    GAIN_HIP(hipStreamSynchronize(c->stream));
+  }
+}
+
+}  // namespace
+}  // namespace gain
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" const char* gain_last_error(void) { return t_last_error.c_str(); }
+
+extern "C" const char* gain_version(void) { return "gain_engine 0.1 gfx950"; }
+
+extern "C" int gain_create(gain_ctx** out, int device, int jmax, int nrepres,
+                           int maxdegen) {
+  return guarded([&] {
+    GAIN_CHECK_ARG(out, "null output pointer");
+    *out = nullptr;
+    GAIN_CHECK_ARG(jmax >= 0 && jmax <= 200, "jmax %d out of range", jmax);
+    GAIN_CHECK_ARG(nrepres >= 1 && nrepres <= GAIN_MAX_IRREPS,
+                   "nrepres %d out of range", nrepres);
+    GAIN_CHECK_ARG(maxdegen >= 1 && maxdegen <= 3, "maxdegen %d out of range",
+                   maxdegen);
+    int ndev = 0;
+    GAIN_HIP(hipGetDeviceCount(&ndev));
+    GAIN_CHECK_ARG(device >= 0 && device < ndev, "device %d not present (%d)",
+                   device, ndev);
+    gain_ctx* c = new gain_ctx();
+    try {
+      c->device = device;
+      DeviceGuard g(device);
+      GAIN_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+      c->jmax = jmax;
+      c->nrepres = nrepres;
+      c->maxdeg = maxdegen;
+      const int n1 = jmax + 1;
+      c->threej.assign((size_t)n1 * n1 * 9, 0.0);
+      for (int jI = 0; jI <= jmax; ++jI)
+        for (int jF = std::max(jI - 1, 0); jF <= std::min(jI + 1, jmax); ++jF)
+          for (int kI = 0; kI <= jI; ++kI)
+            for (int kF = std::max(kI - 1, 0); kF <= std::min(kI + 1, jF);
+                 ++kF)
+              c->threej[jI + kI * n1 + (jF - jI + 1) * n1 * n1 +
+                        (kF - kI + 1) * n1 * n1 * 3] =
+                  three_j(jI, 1, jF, kI, kF - kI, -kF);
+      c->d_threej = c->arena.upload(c->threej.data(), c->threej.size());
+    } catch (...) {
+      if (c->stream) (void)hipStreamDestroy(c->stream);
+      delete c;
+      throw;
+    }
+    *out = c;
+  });
+}
+
+extern "C" int gain_destroy(gain_ctx* c) {
+  return guarded([&] {
+    if (!c) return;
+    {
+      DeviceGuard g(c->device);
+      (void)hipStreamSynchronize(c->stream);
+      for (auto& kv : c->timers) {
+        for (auto& e : kv.second.pending) {
+          (void)hipEventDestroy(e.first);
+          (void)hipEventDestroy(e.second);
+        }
+        for (auto& e : kv.second.pool) {
+          (void)hipEventDestroy(e.first);
+          (void)hipEventDestroy(e.second);
+        }
+      }
+      (void)hipStreamDestroy(c->stream);
+    }
+    delete c;
+  });
+}
+
+extern "C" int gain_get_threej(gain_ctx* c, double* host_out) {
+  return guarded([&] {
+    need(c);
+    GAIN_CHECK_ARG(host_out, "null output");
+    std::memcpy(host_out, c->threej.data(), sizeof(double) * c->threej.size());
+  });
+}
+
+extern "C" int gain_get_stream(gain_ctx* c, void** s) {
+  return guarded([&] {
+    need(c);
+    GAIN_CHECK_ARG(s, "null output");
+    *s = (void*)c->stream;
+  });
+}
+
+extern "C" int gain_synchronize(gain_ctx* c) {
+  return guarded([&] {
+    need(c);
+    DeviceGuard g(c->device);
+    GAIN_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+extern "C" int gain_set_params(gain_ctx* c, const gain_params* p) {
+  return guarded([&] {
+    need(c);
+    GAIN_CHECK_ARG(p, "null params");
+    GAIN_CHECK_ARG(p->nrepres == c->nrepres, "nrepres %d != context's %d",
+                   p->nrepres, c->nrepres);
+    GAIN_CHECK_ARG(p->maxdeg == c->maxdeg, "maxdeg %d != context's %d",
+                   p->maxdeg, c->maxdeg);
+    GAIN_CHECK_ARG(p->temperature > 0.0 && p->partition != 0.0,
+                   "temperature and partition function must be set");
+    for (int g = 0; g < p->nrepres; ++g)
+      GAIN_CHECK_ARG(p->sym_degen[g] >= 1 && p->sym_degen[g] <= p->maxdeg,
+                     "degeneracy of irrep %d out of range", g);
+    // TroveInput::ComputeIGammaPair, TroveInput.cpp:202-236
+    int pair[GAIN_MAX_IRREPS];
+    for (int gI = 0; gI < p->nrepres; ++gI) {
+      int ng = 0;
+      pair[gI] = gI;
+      for (int gF = 0; gF < p->nrepres; ++gF)
+        if (gI != gF && p->isym_pairs[gI] == p->isym_pairs[gF]) {
+          pair[gI] = gF;
+          if (++ng > 1)
+            fail(GAIN_ERR_CONFIG,
+                 "selection rules do not come in pairs (irrep %d)", gI);
+        }
+      if (p->gns[gI] != p->gns[pair[gI]])
+        fail(GAIN_ERR_CONFIG, "selection rules do not agree with gns "
+                              "(irrep %d)", gI);
+    }
+    c->params = *p;
+    std::memcpy(c->pair, pair, sizeof(pair));
+    c->params_set = true;
+  });
+}
+
+extern "C" int gain_get_gamma_pair(gain_ctx* c, int gamma, int* pair) {
+  return guarded([&] {
+    need(c);
+    if (!c->params_set) fail(GAIN_ERR_STATE, "gain_set_params not called");
+    GAIN_CHECK_ARG(gamma >= 0 && gamma < c->nrepres && pair, "bad irrep");
+    *pair = c->pair[gamma];
+  });
+}
+
+extern "C" int gain_upload_dipole(gain_ctx* c, const double* mu, int nc,
+                                  int startF, int endF) {
+  return guarded([&] {
+    need(c);
+    GAIN_CHECK_ARG(mu && nc > 0, "bad dipole");
+    if (startF != 0 || endF != nc)
+      fail(GAIN_ERR_ARG,
+           "the engine keeps the whole dipole resident: need startF=0, "
+           "endF=nc (got %d,%d of %d)", startF, endF, nc);
+    DeviceGuard g(c->device);
+    if (c->d_mu) c->arena.release(c->d_mu);
+    c->d_mu = c->arena.upload(mu, (size_t)3 * nc * nc);
+    c->nc = nc;
+  });
+}
+
+extern "C" int gain_upload_basis(gain_ctx* c, int jind, int J, int nprim,
+                                 const int* kstart, const int* kblock,
+                                 const int* ktau, const int* vib) {
+  return guarded([&] {
+    need(c);
+    GAIN_CHECK_ARG(jind >= 0 && jind < 1024, "J index out of range");
+    GAIN_CHECK_ARG(J >= 0 && J <= c->jmax, "J=%d above jmax %d", J, c->jmax);
+    GAIN_CHECK_ARG(nprim > 0 && kstart && kblock && ktau && vib,
+                   "bad basis arrays");
+    // K blocks must tile [0, nprim) in order (TroveBasisSet.cpp:658-681)
+    int pos = 0;
+    for (int K = 0; K <= J; ++K) {
+      GAIN_CHECK_ARG(kstart[K] == pos && kblock[K] >= 0,
+                     "K block %d does not start at %d", K, pos);
+      pos += kblock[K];
+    }
+    GAIN_CHECK_ARG(pos == nprim, "K blocks cover %d of %d primitives", pos,
+                   nprim);
+    DeviceGuard g(c->device);
+    if ((int)c->bases.size() <= jind) c->bases.resize(jind + 1);
+    Basis& b = c->bases[jind];
+    if (b.present) fail(GAIN_ERR_STATE, "J index %d already uploaded", jind);
+    b.J = J;
+    b.nprim = nprim;
+    b.ncol = 2 * (J + 1);
+    b.kstart.assign(kstart, kstart + J + 1);
+    b.kblock.assign(kblock, kblock + J + 1);
+    b.hk.resize(nprim);
+    b.htau.assign(ktau, ktau + nprim);
+    b.hvib.assign(vib, vib + nprim);
+    int nv = 0;
+    for (int K = 0; K <= J; ++K)
+      for (int r = kstart[K]; r < kstart[K] + kblock[K]; ++r) b.hk[r] = K;
+    for (int r = 0; r < nprim; ++r) {
+      GAIN_CHECK_ARG(vib[r] >= 0, "negative vibrational index");
+      nv = std::max(nv, vib[r] + 1);
+      b.htau[r] &= 1;
+    }
+    b.nvib = nv;
+    b.kval = c->arena.upload(b.hk.data(), nprim);
+    b.tau = c->arena.upload(b.htau.data(), nprim);
+    b.vib = c->arena.upload(b.hvib.data(), nprim);
+    b.present = true;
+  });
+}
+
+extern "C" int gain_upload_inflation(gain_ctx* c, int jind, int maxsymcoeffs,
+                                     const int* ijterms, const int* icontr,
+                                     int matsize, const int* ntot,
+                                     const int* sdeg, const int** N,
+                                     const double** repres) {
+  return guarded([&] {
+    need(c);
+    Basis& b = basis_of(c, jind, false);
+    GAIN_CHECK_ARG(maxsymcoeffs > 0 && ijterms && icontr && matsize > 0 &&
+                       ntot && sdeg && N && repres,
+                   "bad inflation arrays");
+    if (b.has_infl) fail(GAIN_ERR_STATE, "inflation of J index %d already set",
+                         jind);
+    DeviceGuard g(c->device);
+    for (int r = 0; r < b.nprim; ++r) {
+      GAIN_CHECK_ARG(icontr[r] >= 0 && icontr[r] < maxsymcoeffs,
+                     "icontr case out of range at %d", r);
+      GAIN_CHECK_ARG(icontr[r + b.nprim] >= 0 && icontr[r + b.nprim] < matsize,
+                     "icontr lambda out of range at %d", r);
+    }
+    b.maxsymcoeffs = maxsymcoeffs;
+    b.matsize = matsize;
+    b.ijterms = c->arena.upload(ijterms, (size_t)maxsymcoeffs * c->nrepres);
+    b.icontr = c->arena.upload(icontr, (size_t)2 * b.nprim);
+    // case -> primitives CSR (for the transposed expansion)
+    std::vector<int> cp(maxsymcoeffs + 1, 0), prim(b.nprim);
+    for (int r = 0; r < b.nprim; ++r) cp[icontr[r] + 1]++;
+    for (int k = 0; k < maxsymcoeffs; ++k) cp[k + 1] += cp[k];
+    std::vector<int> fill(cp.begin(), cp.end() - 1);
+    for (int r = 0; r < b.nprim; ++r) prim[fill[icontr[r]]++] = r;
+    b.case_ptr = c->arena.upload(cp.data(), cp.size());
+    b.case_prim = c->arena.upload(prim.data(), prim.size());
+    for (int gm = 0; gm < c->nrepres; ++gm) {
+      Irrep& ir = b.irrep[gm];
+      ir.ntot = ntot[gm];
+      ir.sdeg = sdeg[gm];
+      GAIN_CHECK_ARG(ir.ntot >= 0 && ir.sdeg >= 1 && ir.sdeg <= c->maxdeg,
+                     "irrep %d: bad Ntot/sdeg", gm);
+      const int* ijt = ijterms + (size_t)gm * maxsymcoeffs;
+      std::vector<int> tcase(std::max(ir.ntot, 1), 0);
+      for (int k = 0; k < maxsymcoeffs; ++k) {
+        GAIN_CHECK_ARG(N[gm][k] >= 0 && ijt[k] >= 0 &&
+                           ijt[k] + N[gm][k] <= ir.ntot,
+                       "irrep %d case %d: ijterms/N exceed Ntot", gm, k);
+        for (int i = 0; i < N[gm][k]; ++i) tcase[ijt[k] + i] = k;
+      }
+      ir.N = c->arena.upload(N[gm], maxsymcoeffs);
+      const size_t nrep = (size_t)ir.ntot * ir.sdeg * matsize;
+      ir.repres = nrep ? c->arena.upload(repres[gm], nrep)
+                       : c->arena.alloc<double>(1);
+      ir.tcase = c->arena.upload(tcase.data(), tcase.size());
+    }
+    b.has_infl = true;
+  });
+}
+
+static void register_block(gain_ctx* c, int jind, int gamma, int nstates,
+                           int rec_len, int64_t ld, const double* energies,
+                           const double* vec, bool device) {
+  need(c);
+  GAIN_CHECK_ARG(gamma >= 0 && gamma < c->nrepres, "irrep out of range");
+  GAIN_CHECK_ARG(nstates >= 0 && rec_len >= 0 && ld >= rec_len,
+                 "bad block sizes");
+  GAIN_CHECK_ARG(energies || nstates == 0, "null energies");
+  DeviceGuard g(c->device);
+  auto it = c->blocks.find({jind, gamma});
+  if (it != c->blocks.end()) {
+    if (it->second.owned && it->second.vec)
+      c->arena.release((void*)it->second.vec);
+    if (it->second.d_energy) c->arena.release(it->second.d_energy);
+    c->blocks.erase(it);
+  }
+  Block b;
+  b.n = nstates;
+  b.rec_len = rec_len;
+  b.ld = ld;
+  b.energy.assign(energies, energies + nstates);
+  for (int i = 1; i < nstates; ++i)
+    if (!(b.energy[i - 1] <= b.energy[i])) b.sorted = false;
+  b.d_energy = c->arena.upload(b.energy.data(), nstates);
+  if (vec) {
+    if (device) {
+      b.vec = vec;
+      b.owned = false;
+    } else {
+      double* d = c->arena.alloc<double>((size_t)nstates * ld);
+      GAIN_HIP(hipMemcpy(d, vec, sizeof(double) * nstates * ld,
+                         hipMemcpyHostToDevice));
+      b.vec = d;
+      b.owned = true;
+    }
+  }
+  c->blocks[{jind, gamma}] = std::move(b);
+}
+
+extern "C" int gain_upload_block(gain_ctx* c, int jind, int gamma,
+                                 int nstates, int rec_len,
+                                 const double* energies,
+                                 const double* vectors) {
+  return guarded([&] {
+    register_block(c, jind, gamma, nstates, rec_len, rec_len, energies,
+                   vectors, false);
+  });
+}
+
+extern "C" int gain_bind_block_device(gain_ctx* c, int jind, int gamma,
+                                      int nstates, int rec_len, int64_t ld,
+                                      const double* energies,
+                                      const double* vectors_device) {
+  return guarded([&] {
+    GAIN_CHECK_ARG(vectors_device, "null device pointer");
+    register_block(c, jind, gamma, nstates, rec_len, ld, energies,
+                   vectors_device, true);
+  });
+}
+
+extern "C" int gain_half_ls(gain_ctx* c, int jindI, int gammaI, int idegI,
+                            const double* cI, int jindF, double* h_out) {
+  return guarded([&] {
+    need(c);
+    if (!c->d_mu) fail(GAIN_ERR_STATE, "dipole not uploaded");
+    GAIN_CHECK_ARG(cI && h_out, "null vector");
+    GAIN_CHECK_ARG(gammaI >= 0 && gammaI < c->nrepres, "irrep out of range");
+    DeviceGuard g(c->device);
+    Plan P;
+    P.jindI = jindI;
+    P.gI = gammaI;
+    P.jindF = jindF;
+    P.gF = gammaI;
+    P.bI = &basis_of(c, jindI, true);
+    P.bF = &basis_of(c, jindF, false);
+    GAIN_CHECK_ARG(std::abs(P.bI->J - P.bF->J) <= 1, "|J_F - J_I| > 1");
+    const Irrep& ir = P.bI->irrep[gammaI];
+    GAIN_CHECK_ARG(idegI >= 0 && idegI < ir.sdeg, "component out of range");
+    int slot, cslot;
+    char* hs = (char*)c->ring.acquire(sizeof(int64_t) + sizeof(int), &slot);
+    *(int64_t*)hs = 0;
+    *(int*)(hs + sizeof(int64_t)) = idegI;
+    char* ds = c->w_small.get<char>(sizeof(int64_t) + sizeof(int));
+    GAIN_HIP(hipMemcpyAsync(ds, hs, sizeof(int64_t) + sizeof(int),
+                            hipMemcpyHostToDevice, c->stream));
+    const size_t nb = sizeof(double) * std::max(ir.ntot, 1);
+    void* hc = c->ring.acquire(nb, &cslot);
+    std::memcpy(hc, cI, sizeof(double) * ir.ntot);
+    double* dc = c->w_c.get<double>(std::max(ir.ntot, 1));
+    GAIN_HIP(hipMemcpyAsync(dc, hc, nb, hipMemcpyHostToDevice, c->stream));
+    c->ring.release(slot, c->stream);
+    c->ring.release(cslot, c->stream);
+    Small sm{};
+    sm.c_off = (int64_t*)ds;
+    sm.ideg = (int*)(ds + sizeof(int64_t));
+    double* H = c->w_H.get<double>(P.bF->nprim);
+    run_half_ls(c, P, dc, sm, 1, H);
+    GAIN_HIP(hipMemcpyAsync(h_out, H, sizeof(double) * P.bF->nprim,
+                            hipMemcpyDeviceToHost, c->stream));
+    GAIN_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+extern "C" int gain_plain_lines(gain_ctx* c, int jindI, int gammaI,
+                                int levelI, const double* cI, int jindF,
+                                int gammaF, const gain_lines* out) {
+  return guarded([&] {
+    need(c);
+    GAIN_CHECK_ARG(out, "null output descriptor");
+    DeviceGuard g(c->device);
+    Plan P = make_plan(c, jindI, gammaI, &levelI, 1, jindF, gammaF);
+    run_lines(c, P, cI, GAIN_PREC_FP64, true, out);
+  });
+}
+
+extern "C" int gain_batched_lines(gain_ctx* c, int jindI, int gammaI,
+                                  const int* levelsI, int nI, int jindF,
+                                  int gammaF, int precision,
+                                  const gain_lines* out) {
+  return guarded([&] {
+    need(c);
+    GAIN_CHECK_ARG(out && levelsI && nI > 0, "bad batch");
+    GAIN_CHECK_ARG(precision >= GAIN_PREC_FP64 && precision <= GAIN_PREC_BF16X3,
+                   "unknown precision %d", precision);
+    DeviceGuard g(c->device);
+    Plan P = make_plan(c, jindI, gammaI, levelsI, nI, jindF, gammaF);
+    if (!P.blkI->vec)
+      fail(GAIN_ERR_STATE, "batched path needs the initial block's vectors "
+                           "resident (gain_upload_block with vectors)");
+    run_lines(c, P, nullptr, precision, false, out);
+  });
+}
+
+extern "C" int gain_set_timing(gain_ctx* c, int enable) {
+  return guarded([&] {
+    need(c);
+    c->timing = enable != 0;
+  });
+}
+
+extern "C" int gain_get_timing(gain_ctx* c, const char* name, double* total_ms,
+                               int64_t* launches, int reset) {
+  return guarded([&] {
+    need(c);
+    GAIN_CHECK_ARG(name && total_ms && launches, "null argument");
+    DeviceGuard g(c->device);
+    auto it = c->timers.find(name);
+    if (it == c->timers.end()) {
+      *total_ms = 0.0;
+      *launches = 0;
+      return;
+    }
+    resolve_timers(c, it->second);
+    *total_ms = it->second.total_ms;
+    *launches = it->second.launches;
+    if (reset) {
+      it->second.total_ms = 0.0;
+      it->second.launches = 0;
+    }
+  });
+}
+
+// Tuning knobs (not part of the public header; used by the autotuner).
+extern "C" int gain_set_tuning(gain_ctx* c, const char* key, int value) {
+  return guarded([&] {
+    need(c);
+    GAIN_CHECK_ARG(key, "null key");
+    const std::string k(key);
+    if (k == "gemv_rows") {
+      GAIN_CHECK_ARG(value == 2 || value == 4 || value == 8, "rows 2/4/8");
+      c->gemv_rows = value;
+    } else if (k == "gemv_ksplit") {
+      GAIN_CHECK_ARG(value >= 1 && value <= 256, "ksplit 1..256");
+      c->gemv_ksplit = value;
+    } else if (k == "mu_ksplit") {
+      GAIN_CHECK_ARG(value >= 1 && value <= 64, "ksplit 1..64");
+      c->mu_ksplit = value;
+    } else {
+      fail(GAIN_ERR_ARG, "unknown tuning key %s", key);
+    }
+  });
+}

@@ -1,0 +1,469 @@
+// Plain FP64 path kernels for gfx950 (wave64).  See kernels.hpp for the HBM
+// layout of every buffer.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace gain {
+
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
+// ---------------------------------------------------------------------------
+// Expand + scatter (dipole_GPU.cu:859-889 fused with the K/tau grouping of
+// the half line strength).  One thread per (column, v) slot of X:
+//   X[vec][2K+tau][v] = sum_{r in slot} (-1)^{(K mod 3) tau + K} x[r],
+//   x[r] = sum_{i<N[case]} c[t0+i] repres[(t0+i) + d*Ntot + lambda*sdeg*Ntot]
+// The sign is the initial-side factor of dipole_GPU.cu:437-438.
+__global__ __launch_bounds__(256) void k_scatter_expand(ScatterArgs a) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (s >= a.nslot) return;
+  const double* c = a.cbase + a.c_off[b];
+  const int ideg = a.ideg[b];
+  double acc = 0.0;
+  for (int q = a.slot_ptr[s]; q < a.slot_ptr[s + 1]; ++q) {
+    const int r = a.slot_prim[q];
+    const int icase = a.icontr[r];
+    const int lam = a.icontr[r + a.nprim];
+    const int t0 = a.ijt[icase];
+    const int ne = a.N[icase];
+    const double* rp = a.repres + (int64_t)ideg * a.ntot +
+                       (int64_t)lam * a.sdeg * a.ntot;
+    double x = 0.0;
+    for (int i = 0; i < ne; ++i) x = fma(c[t0 + i], rp[t0 + i], x);
+    const int K = a.kval[r], t = a.tau[r];
+    acc += ((((K % 3) * t + K) & 1) ? -x : x);
+  }
+  a.X[(int64_t)b * a.xstride + s] = acc;
+}
+
+void launch_scatter_expand(const ScatterArgs& a, hipStream_t s) {
+  if (a.nslot == 0 || a.nvec == 0) return;
+  dim3 grid((a.nslot + 255) / 256, a.nvec);
+  hipLaunchKernelGGL(k_scatter_expand, grid, dim3(256), 0, s, a);
+  check_launch("scatter_expand");
+}
+
+// ---------------------------------------------------------------------------
+// mu_alpha * X on the vector ALUs (FP64 FMA).  Workgroup: 64 rows (vF) x 64
+// columns, 4 waves each owning 16 columns; X staged through LDS in 32-row
+// slices.  Split-K over vI writes partials (summed by the gather kernel).
+__global__ __launch_bounds__(256) void k_mu_gemm_valu(
+    const double* __restrict__ mu, int nc, const double* __restrict__ X,
+    int ncolTot, double* __restrict__ Ypart, int ksplit, int kc) {
+  __shared__ double xs[32][65];
+  const int tx = threadIdx.x & 63;
+  const int cg = threadIdx.x >> 6;
+  const int row = blockIdx.x * 64 + tx;
+  const int j0 = blockIdx.y * 64;
+  const int alpha = blockIdx.z % 3;
+  const int ks = blockIdx.z / 3;
+  const int kb = ks * kc;
+  const int ke = min(nc, kb + kc);
+  const double* mua = mu + (int64_t)alpha * nc * nc;
+  const int rrow = row < nc ? row : nc - 1;
+  double acc[16];
+#pragma unroll
+  for (int c = 0; c < 16; ++c) acc[c] = 0.0;
+  for (int k0 = kb; k0 < ke; k0 += 32) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < 32 * 64; e += 256) {
+      const int k = e & 31, j = e >> 5;
+      const int gk = k0 + k, gj = j0 + j;
+      xs[k][j] = (gk < ke && gj < ncolTot) ? X[(int64_t)gj * nc + gk] : 0.0;
+    }
+    __syncthreads();
+    const int kn = min(32, ke - k0);
+    for (int k = 0; k < kn; ++k) {
+      const double m = mua[(int64_t)(k0 + k) * nc + rrow];
+#pragma unroll
+      for (int c = 0; c < 16; ++c) acc[c] = fma(m, xs[k][cg * 16 + c], acc[c]);
+    }
+  }
+  if (row >= nc) return;
+  double* yo = Ypart + ((int64_t)ks * 3 + alpha) * ncolTot * (int64_t)nc;
+#pragma unroll
+  for (int c = 0; c < 16; ++c) {
+    const int j = j0 + cg * 16 + c;
+    if (j < ncolTot) yo[(int64_t)j * nc + row] = acc[c];
+  }
+  (void)ksplit;
+}
+
+void launch_mu_gemm_valu(const double* mu, int nc, const double* X,
+                         int ncolTot, double* Ypart, int ksplit,
+                         hipStream_t s) {
+  int kc = (nc + ksplit - 1) / ksplit;
+  kc = (kc + 31) / 32 * 32;
+  dim3 grid((nc + 63) / 64, (ncolTot + 63) / 64, 3 * ksplit);
+  hipLaunchKernelGGL(k_mu_gemm_valu, grid, dim3(256), 0, s, mu, nc, X,
+                     ncolTot, Ypart, ksplit, kc);
+  check_launch("mu_gemm_valu");
+}
+
+// ---------------------------------------------------------------------------
+// Gather of the half line strength from mu_alpha X (dipole_GPU.cu:430-511):
+//   h[r'] = (-1)^{(K_F mod 3) tau_F} sum_{K_I = K_F-1..K_F+1, K_I<=J_I}
+//           f3j * s * sum_{tau_I} m * Y_alpha[v(r'), 2 K_I + tau_I]
+// with alpha, m and s as in the reference kernel (s = truncated SQRT2 when
+// K_I != K_F and K_I K_F != 0).
+__global__ __launch_bounds__(256) void k_gather_h(GatherArgs a) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (r >= a.nprimF) return;
+  const int K = a.kvalF[r];
+  const int tF = a.tauF[r] & 1;
+  const int v = a.vibF[r];
+  const int n1 = a.jmax + 1;
+  double acc = 0.0;
+  const int klo = max(K - 1, 0), khi = min(K + 1, a.jI);
+  for (int KI = klo; KI <= khi; ++KI) {
+    const int dk = K - KI;
+    const double f3j =
+        a.threej[a.jI + KI * n1 + (a.jF - a.jI + 1) * n1 * n1 +
+                 (dk + 1) * n1 * n1 * 3];
+    const double sq = (dk != 0 && KI * K != 0) ? kSqrt2Ref : 1.0;
+#pragma unroll
+    for (int tI = 0; tI < 2; ++tI) {
+      int alpha, m;
+      if (dk == 0) {
+        alpha = 2;
+        m = tF - tI;
+      } else if (tF != tI) {
+        alpha = 0;
+        m = (tF - tI) * dk;
+      } else {
+        alpha = 1;
+        m = -1;
+      }
+      if (m == 0) continue;
+      const int64_t col = (int64_t)b * a.ncolI + 2 * KI + tI;
+      const double* yp = a.Y + alpha * a.ys_alpha + col * a.nc + v;
+      double y = 0.0;
+      for (int ks = 0; ks < a.ksplit; ++ks) y += yp[ks * a.ys_ks];
+      acc = fma(f3j * sq * (double)m, y, acc);
+    }
+  }
+  const bool neg = ((K % 3) * tF) & 1;
+  a.H[(int64_t)b * a.hstride + r] = neg ? -acc : acc;
+}
+
+void launch_gather_h(const GatherArgs& a, hipStream_t s) {
+  if (a.nprimF == 0 || a.nvec == 0) return;
+  dim3 grid((a.nprimF + 255) / 256, a.nvec);
+  hipLaunchKernelGGL(k_gather_h, grid, dim3(256), 0, s, a);
+  check_launch("gather_h");
+}
+
+// ---------------------------------------------------------------------------
+// Transpose of the expansion: g[t] = sum_{r in case(t)}
+//   repres[t + d*Ntot + lambda(r)*sdeg*Ntot] h[r]
+// so that <P c_F, h> = <c_F, g> for every final record c_F
+// (GpuManager.cpp:733-741 expands each c_F and dots it with h instead).
+__global__ __launch_bounds__(256) void k_expand_T(ExpandTArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = blockIdx.y;
+  if (t >= a.ntot) return;
+  const int c = a.tcase[t];
+  const int d = a.ideg_of_vec[v];
+  const double* h = a.H + (int64_t)a.h_of_vec[v] * a.hstride;
+  const double* rp = a.repres + t + (int64_t)d * a.ntot;
+  double acc = 0.0;
+  for (int q = a.case_ptr[c]; q < a.case_ptr[c + 1]; ++q) {
+    const int r = a.case_prim[q];
+    const int lam = a.icontr[r + a.nprim];
+    acc = fma(rp[(int64_t)lam * a.sdeg * a.ntot], h[r], acc);
+  }
+  a.G[(int64_t)v * a.gstride + t] = acc;
+}
+
+void launch_expand_T(const ExpandTArgs& a, hipStream_t s) {
+  if (a.ntot == 0 || a.nvec == 0) return;
+  dim3 grid((a.ntot + 255) / 256, a.nvec);
+  hipLaunchKernelGGL(k_expand_T, grid, dim3(256), 0, s, a);
+  check_launch("expand_T");
+}
+
+// ---------------------------------------------------------------------------
+// FP64 GEMV over HBM-resident eigenvector records (the plain drop-in
+// projection).  One wave per (group of R rows, K chunk); each lane streams
+// 16-byte non-temporal loads of R rows while the NV projection vectors come
+// from L2.  Wave-level tree reduction; split-K partials reduced by the
+// epilogue in a fixed order.
+template <int R, int NV, bool VEC>
+__global__ __launch_bounds__(256) void k_gemv(GemvArgs a, int kchunk) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int ngroups = (a.nF + R - 1) / R;
+  if (wave >= (int64_t)ngroups * a.ksplit) return;
+  const int grp = (int)(wave / a.ksplit);
+  const int ks = (int)(wave % a.ksplit);
+  const int tbeg = ks * kchunk;
+  const int tend = min(a.ntot, tbeg + kchunk);
+  const double* rowp[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    int F = grp * R + i;
+    if (F >= a.nF) F = a.nF - 1;
+    rowp[i] = a.C + (int64_t)(a.F0 + F) * a.ld;
+  }
+  double acc[R][NV];
+#pragma unroll
+  for (int i = 0; i < R; ++i)
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[i][v] = 0.0;
+
+  if (VEC) {
+    int t = tbeg + 2 * lane;
+    // two 1 KiB steps per iteration keep 2R loads in flight per lane
+    for (; t + 129 < tend; t += 256) {
+      dbl2 c0[R], c1[R];
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+        c0[i] = __builtin_nontemporal_load((const dbl2*)(rowp[i] + t));
+        c1[i] = __builtin_nontemporal_load((const dbl2*)(rowp[i] + t + 128));
+      }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        if (v >= a.nvec) continue;
+        const dbl2 g0 = *(const dbl2*)(a.G + v * a.gstride + t);
+        const dbl2 g1 = *(const dbl2*)(a.G + v * a.gstride + t + 128);
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          double s = acc[i][v];
+          s = fma(c0[i].x, g0.x, s);
+          s = fma(c0[i].y, g0.y, s);
+          s = fma(c1[i].x, g1.x, s);
+          s = fma(c1[i].y, g1.y, s);
+          acc[i][v] = s;
+        }
+      }
+    }
+    for (; t + 1 < tend; t += 128) {
+      dbl2 c0[R];
+#pragma unroll
+      for (int i = 0; i < R; ++i)
+        c0[i] = __builtin_nontemporal_load((const dbl2*)(rowp[i] + t));
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        if (v >= a.nvec) continue;
+        const dbl2 g0 = *(const dbl2*)(a.G + v * a.gstride + t);
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          acc[i][v] = fma(c0[i].x, g0.x, acc[i][v]);
+          acc[i][v] = fma(c0[i].y, g0.y, acc[i][v]);
+        }
+      }
+    }
+    if (t < tend) {  // odd-length tail: one element left for one lane
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        if (v >= a.nvec) continue;
+        const double g = a.G[v * a.gstride + t];
+#pragma unroll
+        for (int i = 0; i < R; ++i) acc[i][v] = fma(rowp[i][t], g, acc[i][v]);
+      }
+    }
+  } else {
+    for (int t = tbeg + lane; t < tend; t += 64) {
+      double c[R];
+#pragma unroll
+      for (int i = 0; i < R; ++i) c[i] = __builtin_nontemporal_load(rowp[i] + t);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        if (v >= a.nvec) continue;
+        const double g = a.G[v * a.gstride + t];
+#pragma unroll
+        for (int i = 0; i < R; ++i) acc[i][v] = fma(c[i], g, acc[i][v]);
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const int F = grp * R + i;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      if (v >= a.nvec) continue;
+      double x = acc[i][v];
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off);
+      if (lane == 0 && F < a.nF)
+        a.partial[((int64_t)ks * a.nF + F) * a.nvec + v] = x;
+    }
+  }
+}
+
+int gemv_kchunk(int ntot, int ksplit) {
+  int kc = (ntot + ksplit - 1) / ksplit;
+  return (kc + 127) / 128 * 128;
+}
+
+template <int R, int NV>
+static void gemv_dispatch_vec(const GemvArgs& a, bool vec, hipStream_t s) {
+  const int kc = gemv_kchunk(a.ntot, a.ksplit);
+  const int64_t waves = (int64_t)((a.nF + R - 1) / R) * a.ksplit;
+  dim3 grid((unsigned)((waves + 3) / 4));
+  if (vec)
+    hipLaunchKernelGGL((k_gemv<R, NV, true>), grid, dim3(256), 0, s, a, kc);
+  else
+    hipLaunchKernelGGL((k_gemv<R, NV, false>), grid, dim3(256), 0, s, a, kc);
+}
+
+template <int R>
+static void gemv_dispatch_nv(const GemvArgs& a, bool vec, hipStream_t s) {
+  if (a.nvec <= 1)
+    gemv_dispatch_vec<R, 1>(a, vec, s);
+  else if (a.nvec <= 2)
+    gemv_dispatch_vec<R, 2>(a, vec, s);
+  else if (a.nvec <= 4)
+    gemv_dispatch_vec<R, 4>(a, vec, s);
+  else
+    gemv_dispatch_vec<R, 9>(a, vec, s);
+}
+
+void launch_gemv(const GemvArgs& a, hipStream_t s) {
+  if (a.nF <= 0) return;
+  GAIN_CHECK_ARG(a.nvec >= 1 && a.nvec <= 9, "gemv: nvec %d out of range",
+                 a.nvec);
+  const bool vec = (a.ld % 2 == 0) && (a.gstride % 2 == 0) &&
+                   ((uintptr_t)a.C % 16 == 0) && ((uintptr_t)a.G % 16 == 0);
+  switch (a.rows) {
+    case 2: gemv_dispatch_nv<2>(a, vec, s); break;
+    case 8: gemv_dispatch_nv<8>(a, vec, s); break;
+    default: gemv_dispatch_nv<4>(a, vec, s); break;
+  }
+  check_launch("gemv");
+}
+
+// ---------------------------------------------------------------------------
+// Epilogue: transition filters (States.cpp:46-76), S from the degenerate
+// components, Einstein A, threshold and intensity (Output.cpp:52-112).
+__global__ __launch_bounds__(256) void k_lines(LinesArgs a) {
+  const int F = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (F >= a.nF) return;
+  const int64_t o = (int64_t)i * a.nF + F;
+  const double eI = a.eI[i];
+  const double eF = a.eF[F];
+  const double nu = eF - eI;
+  const double eu = eF - a.zpe;
+  const bool pass = a.block_ok && a.okI[i] && nu >= a.fw0 && nu <= a.fw1 &&
+                    eu <= a.eu1 && eu >= a.eu0 && a.isym_do_F &&
+                    !(a.same_block && F == a.levelI[i]);
+  const bool inside = F >= a.Flo && F < a.Fhi;
+  const int md2 = a.maxdeg * a.maxdeg;
+  double lsv[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) lsv[q] = 0.0;
+  double ssum = 0.0;
+  if (pass && inside) {
+    for (int v = 0; v < a.nvec_per_i; ++v) {
+      const double* p = a.src + (int64_t)(F - a.Flo) * a.src_f +
+                        ((int64_t)i * a.nvec_per_i + v) * a.src_v;
+      double val = 0.0;
+      for (int ks = 0; ks < a.ksplit; ++ks) val += p[ks * a.src_ks];
+      lsv[a.slot_of_vec[v]] = val;
+      ssum += val * val;
+    }
+  }
+  double S = 0.0, A = 0.0, In = 0.0;
+  uint8_t fl = 0;
+  if (pass) {
+    fl = 1;
+    if (!(nu < 1e-40)) {
+      double s = ssum / (double)a.ndegI;
+      if (a.reduced && a.ndegF != 1 && a.ndegI != 1) s *= (double)a.ndegI;
+      A = kACoef * (double)(2 * a.jI + 1) * s * fabs(nu) * fabs(nu) *
+          fabs(nu);
+      S = s;
+      if (!(A < a.threshold)) {
+        fl |= 2;
+        In = kCMCoef * A * a.gnsF * (2.0 * (double)a.jF + 1.0) *
+             exp(-a.beta * (eI - a.zpe)) * (1.0 - exp(-a.beta * nu)) /
+             (nu * nu * a.partition);
+      }
+    }
+  }
+  if (a.ls)
+    for (int q = 0; q < md2; ++q) a.ls[o * md2 + q] = lsv[q];
+  if (a.S) a.S[o] = S;
+  if (a.A) a.A[o] = A;
+  if (a.I) a.I[o] = In;
+  if (a.flags) a.flags[o] = fl;
+}
+
+void launch_lines(const LinesArgs& a, hipStream_t s) {
+  if (a.nF == 0 || a.nI == 0) return;
+  dim3 grid((a.nF + 255) / 256, a.nI);
+  hipLaunchKernelGGL(k_lines, grid, dim3(256), 0, s, a);
+  check_launch("lines");
+}
+
+// ---------------------------------------------------------------------------
+// FP64 TN GEMM on the vector ALUs: 64x64 tile, K step 16, 4x4 outputs per
+// thread, both operands staged k-major in LDS.
+__global__ __launch_bounds__(256) void k_gemm_tn_valu(
+    int M, int N, int K, const double* __restrict__ A, int64_t lda,
+    const double* __restrict__ B, int64_t ldb, double* __restrict__ C,
+    int64_t ldc) {
+  __shared__ double as[16][65];
+  __shared__ double bs[16][65];
+  const int tid = threadIdx.x;
+  const int tm = tid & 15, tn = tid >> 4;
+  const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+  double acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + 256 * q;
+      const int k = e & 15, r = e >> 4;
+      const int gk = k0 + k;
+      const int gm = m0 + r, gn = n0 + r;
+      as[k][r] = (gk < K && gm < M) ? A[(int64_t)gm * lda + gk] : 0.0;
+      bs[k][r] = (gk < K && gn < N) ? B[(int64_t)gn * ldb + gk] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = as[k][tm + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = bs[k][tn + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int n = n0 + tn + 16 * j;
+    if (n >= N) continue;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int m = m0 + tm + 16 * i;
+      if (m < M) C[(int64_t)n * ldc + m] = acc[i][j];
+    }
+  }
+}
+
+void launch_gemm_tn_valu(int M, int N, int K, const double* A, int64_t lda,
+                         const double* B, int64_t ldb, double* C,
+                         int64_t ldc, hipStream_t s) {
+  if (M <= 0 || N <= 0) return;
+  dim3 grid((M + 63) / 64, (N + 63) / 64);
+  hipLaunchKernelGGL(k_gemm_tn_valu, grid, dim3(256), 0, s, M, N, K, A, lda,
+                     B, ldb, C, ldc);
+  check_launch("gemm_tn_valu");
+}
+
+}  // namespace gain

@@ -1,0 +1,129 @@
+// Device kernels of the plain FP64 path and their host launchers.
+//
+// Data layout in HBM (one context, one GPU):
+//   mu      [alpha][vI][vF]            3*nc*nc doubles (reference piece layout)
+//   X       [vec][col][vI]             scattered initial vectors, col = 2K+tau
+//   Ypart   [ks][alpha][vec*ncol][vF]  mu_alpha * X, split-K partials
+//   H       [vec][r']                  half line strengths (primitive basis F)
+//   G       [vec][t]                   P^T h in the symmetrised basis of F
+//   C       [F][ld]                    final eigenvector records (rows)
+//   partial [ks][F-F0][vec]            GEMV split-K partials
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gain {
+
+// Truncated constant of the reference (defines.h:23, dipole_GPU.cu:24).
+constexpr double kSqrt2Ref = 0.70710678118;
+constexpr double kACoef = 3.13618923e-7;   // defines.h:22
+constexpr double kCMCoef = 1.327209365e-12;  // defines.h:24
+constexpr double kPlanck = 6.62606896e-27;   // defines.h:19
+constexpr double kVellgt = 2.99792458e+10;   // defines.h:20
+constexpr double kBoltz = 1.380658e-16;      // defines.h:21
+
+struct ScatterArgs {
+  int nslot;  // ncol * nc
+  int nprim;
+  const int* slot_ptr;   // [nslot+1]
+  const int* slot_prim;  // primitives grouped by slot
+  const int* kval;
+  const int* tau;
+  const int* icontr;     // [2*nprim]
+  const int* ijt;        // ijterms + gamma*maxsymcoeffs
+  const int* N;          // N of this irrep
+  const double* repres;  // repres of this irrep
+  int sdeg, ntot;
+  const double* cbase;   // initial records
+  const int64_t* c_off;  // [nvec] offset of each vector's record in cbase
+  const int* ideg;       // [nvec] component of each vector
+  int nvec;
+  double* X;
+  int64_t xstride;       // ncol * nc
+};
+void launch_scatter_expand(const ScatterArgs& a, hipStream_t s);
+
+// Ypart[ks][alpha][j][vF] = sum_{vI in chunk ks} mu[alpha][vI][vF] X[j][vI]
+void launch_mu_gemm_valu(const double* mu, int nc, const double* X,
+                         int ncolTot, double* Ypart, int ksplit,
+                         hipStream_t s);
+
+struct GatherArgs {
+  int nprimF, nc, jI, jF, jmax, ncolI, nvec, ksplit;
+  const int* kvalF;
+  const int* tauF;
+  const int* vibF;
+  const double* threej;
+  const double* Y;
+  int64_t ys_alpha;  // ncolTot * nc
+  int64_t ys_ks;     // 3 * ncolTot * nc
+  double* H;
+  int64_t hstride;
+};
+void launch_gather_h(const GatherArgs& a, hipStream_t s);
+
+struct ExpandTArgs {
+  int ntot, nprim, sdeg;
+  const int* tcase;      // [ntot] case of each symmetrised function
+  const int* case_ptr;   // [ncase+1]
+  const int* case_prim;  // primitives of each case
+  const int* icontr;     // lambda = icontr[r + nprim]
+  const double* repres;
+  const double* H;
+  int64_t hstride;
+  const int* h_of_vec;   // [nvec] which H vector feeds output vector v
+  const int* ideg_of_vec;  // [nvec] final-side component
+  int nvec;
+  double* G;
+  int64_t gstride;
+};
+void launch_expand_T(const ExpandTArgs& a, hipStream_t s);
+
+struct GemvArgs {
+  const double* C;
+  int64_t ld;
+  int ntot;
+  int F0, nF;
+  const double* G;
+  int64_t gstride;
+  int nvec;
+  double* partial;  // [ksplit][nF][nvec]
+  int ksplit;
+  int rows;         // rows per wave: 2, 4 or 8
+};
+void launch_gemv(const GemvArgs& a, hipStream_t s);
+int gemv_kchunk(int ntot, int ksplit);
+
+struct LinesArgs {
+  int nI, nF, Flo, Fhi, maxdeg;
+  int nvec_per_i;             // vectors per initial state
+  const int* slot_of_vec;     // [nvec_per_i] -> dI + dF*maxdeg
+  const double* src;          // projected line strengths
+  int ksplit;
+  int64_t src_ks, src_f, src_v;  // strides (F index is F - Flo)
+  const double* eI;           // [nI]
+  const int* levelI;          // [nI]
+  const uint8_t* okI;         // [nI] lower-state filter
+  const double* eF;           // [nF]
+  int same_block;
+  int block_ok;
+  int jI, jF, ndegI, ndegF, reduced;
+  double gnsF, zpe, threshold, beta, partition;
+  double fw0, fw1, eu0, eu1;
+  int isym_do_F;
+  // outputs (device, may be null)
+  double* ls;
+  double* S;
+  double* A;
+  double* I;
+  uint8_t* flags;
+};
+void launch_lines(const LinesArgs& a, hipStream_t s);
+
+// C[n*ldc + m] = sum_k A[m*lda + k] * B[n*ldb + k]  (FP64, vector ALUs)
+void launch_gemm_tn_valu(int M, int N, int K, const double* A, int64_t lda,
+                         const double* B, int64_t ldb, double* C,
+                         int64_t ldc, hipStream_t s);
+
+}  // namespace gain

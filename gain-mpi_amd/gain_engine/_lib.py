@@ -1,0 +1,118 @@
+"""ctypes binding of libgain_engine.so (the C ABI in include/gain_engine.h).
+
+The library is built in-tree by `make -C gain-mpi_amd`; there is no CPU
+fallback: loading fails loudly when the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgain_engine.so")
+MAX_IRREPS = 16
+
+OK, ERR_ARG, ERR_HIP, ERR_STATE, ERR_NOMEM, ERR_CONFIG = range(6)
+PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
+
+# Every symbol include/gain_engine.h declares.
+EXPORTS = (
+    "gain_last_error", "gain_version", "gain_create", "gain_destroy",
+    "gain_get_threej", "gain_get_stream", "gain_synchronize",
+    "gain_set_params", "gain_get_gamma_pair", "gain_upload_dipole",
+    "gain_upload_basis", "gain_upload_inflation", "gain_upload_block",
+    "gain_bind_block_device", "gain_half_ls", "gain_plain_lines",
+    "gain_batched_lines", "gain_set_timing", "gain_get_timing",
+    "gain_copy_symmetry_constants", "gain_copy_jmax_constant",
+    "gain_copy_dipole_constant", "gain_transform_vector_primitive",
+    "gain_compute_half_linestrength", "gain_ddot",
+)
+
+
+class GainError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"gain_engine error {code}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [("nrepres", C.c_int), ("maxdeg", C.c_int), ("maxJ", C.c_int),
+                ("reduced", C.c_int), ("symmetry_type", C.c_int),
+                ("sym_degen", C.c_int * MAX_IRREPS),
+                ("isym_do", C.c_int * MAX_IRREPS),
+                ("isym_pairs", C.c_int * MAX_IRREPS),
+                ("gns", C.c_double * MAX_IRREPS),
+                ("erange_lower", C.c_double * 2),
+                ("erange_upper", C.c_double * 2),
+                ("freq_window", C.c_double * 2), ("zpe", C.c_double),
+                ("temperature", C.c_double), ("partition", C.c_double),
+                ("threshold", C.c_double)]
+
+
+class Lines(C.Structure):
+    _fields_ = [("ls", C.c_void_p), ("S", C.c_void_p), ("A", C.c_void_p),
+                ("I", C.c_void_p), ("flags", C.c_void_p),
+                ("on_device", C.c_int)]
+
+
+_lib = None
+
+_vp, _ip, _dp = C.c_void_p, C.c_int, C.c_double
+
+
+def load():
+    """Load the in-tree library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `make -C gain-mpi_amd` "
+            "(or __graft_entry__.build()); there is no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "gain_last_error": (C.c_char_p, []),
+        "gain_version": (C.c_char_p, []),
+        "gain_create": (_ip, [C.POINTER(_vp), _ip, _ip, _ip, _ip]),
+        "gain_destroy": (_ip, [_vp]),
+        "gain_get_threej": (_ip, [_vp, _vp]),
+        "gain_get_stream": (_ip, [_vp, C.POINTER(_vp)]),
+        "gain_synchronize": (_ip, [_vp]),
+        "gain_set_params": (_ip, [_vp, C.POINTER(Params)]),
+        "gain_get_gamma_pair": (_ip, [_vp, _ip, C.POINTER(_ip)]),
+        "gain_upload_dipole": (_ip, [_vp, _vp, _ip, _ip, _ip]),
+        "gain_upload_basis": (_ip, [_vp, _ip, _ip, _ip, _vp, _vp, _vp, _vp]),
+        "gain_upload_inflation": (_ip, [_vp, _ip, _ip, _vp, _vp, _ip, _vp,
+                                        _vp, _vp, _vp]),
+        "gain_upload_block": (_ip, [_vp, _ip, _ip, _ip, _ip, _vp, _vp]),
+        "gain_bind_block_device": (_ip, [_vp, _ip, _ip, _ip, _ip,
+                                         C.c_int64, _vp, _vp]),
+        "gain_half_ls": (_ip, [_vp, _ip, _ip, _ip, _vp, _ip, _vp]),
+        "gain_plain_lines": (_ip, [_vp, _ip, _ip, _ip, _vp, _ip, _ip,
+                                   C.POINTER(Lines)]),
+        "gain_batched_lines": (_ip, [_vp, _ip, _ip, _vp, _ip, _ip, _ip, _ip,
+                                     C.POINTER(Lines)]),
+        "gain_set_timing": (_ip, [_vp, _ip]),
+        "gain_get_timing": (_ip, [_vp, C.c_char_p, C.POINTER(_dp),
+                                  C.POINTER(C.c_int64), _ip]),
+        "gain_set_tuning": (_ip, [_vp, C.c_char_p, _ip]),
+        "gain_copy_symmetry_constants": (_ip, [_ip, _ip]),
+        "gain_copy_jmax_constant": (_ip, [_ip]),
+        "gain_copy_dipole_constant": (_ip, [_ip]),
+        "gain_transform_vector_primitive": (_ip, [_ip] * 6 + [_vp] * 7),
+        "gain_compute_half_linestrength": (
+            _ip, [_ip] * 5 + [_vp] * 4 + [_ip] * 6 + [_vp] * 5 + [_vp]),
+        "gain_ddot": (_ip, [_ip, _vp, _vp, _vp, _vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc: int):
+    if rc != OK:
+        msg = load().gain_last_error().decode(errors="replace")
+        raise GainError(rc, msg)

@@ -1,0 +1,238 @@
+"""Host-side front end of the line-strength engine (Python over the C ABI).
+
+`Engine` plays the role of the reference's GpuManager
+(BaseClasses/GpuManager.h:136-172): it owns one device context, receives
+the dipole, the per-J basis and inflation data and the state blocks, and
+runs the hot path per initial state (plain FP64 GEMV drop-in) or per batch
+of initial states (GEMM engine).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+_i32 = np.int32
+_f64 = np.float64
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def params_struct(p) -> _lib.Params:
+    s = _lib.Params()
+    s.nrepres = p.nrepres
+    s.maxdeg = p.maxdeg
+    s.maxJ = p.maxJ
+    s.reduced = int(bool(p.reduced))
+    s.symmetry_type = p.symmetry_type
+    for g in range(p.nrepres):
+        s.sym_degen[g] = p.sym_degen[g]
+        s.isym_do[g] = int(p.isym_do[g])
+        s.isym_pairs[g] = p.isym_pairs[g]
+        s.gns[g] = p.gns[g]
+    s.erange_lower[:] = list(p.erange_lower)
+    s.erange_upper[:] = list(p.erange_upper)
+    s.freq_window[:] = list(p.freq_window)
+    s.zpe, s.temperature = p.zpe, p.temperature
+    s.partition, s.threshold = p.partition, p.threshold
+    return s
+
+
+class Engine:
+    def __init__(self, device: int, jmax: int, nrepres: int, maxdeg: int):
+        L = _lib.load()
+        h = C.c_void_p()
+        check(L.gain_create(C.byref(h), device, jmax, nrepres, maxdeg))
+        self._L, self._h = L, h
+        self.jmax, self.nrepres, self.maxdeg = jmax, nrepres, maxdeg
+        self._keep = []
+        self.blocks: Dict = {}
+
+    # ------------------------------------------------------------ lifecycle
+    def close(self):
+        if self._h:
+            check(self._L.gain_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ------------------------------------------------------------ setup
+    def threej(self) -> np.ndarray:
+        t = np.zeros((self.jmax + 1) ** 2 * 9)
+        check(self._L.gain_get_threej(self._h, _ptr(t)))
+        return t
+
+    def stream(self) -> int:
+        s = C.c_void_p()
+        check(self._L.gain_get_stream(self._h, C.byref(s)))
+        return s.value or 0
+
+    def synchronize(self):
+        check(self._L.gain_synchronize(self._h))
+
+    def set_params(self, p):
+        s = params_struct(p)
+        check(self._L.gain_set_params(self._h, C.byref(s)))
+
+    def gamma_pair(self, g: int) -> int:
+        out = C.c_int()
+        check(self._L.gain_get_gamma_pair(self._h, g, C.byref(out)))
+        return out.value
+
+    def upload_dipole(self, mu: np.ndarray):
+        mu = np.ascontiguousarray(mu, _f64)
+        nc = mu.shape[-1]
+        check(self._L.gain_upload_dipole(self._h, _ptr(mu), nc, 0, nc))
+
+    def upload_basis(self, jind: int, b):
+        arrs = [np.ascontiguousarray(getattr(b, n), _i32)
+                for n in ("kstart", "kblock", "tau", "vib")]
+        check(self._L.gain_upload_basis(self._h, jind, b.J, b.nprim,
+                                        *[_ptr(a) for a in arrs]))
+
+    def upload_inflation(self, jind: int, b):
+        nrep = self.nrepres
+        ij = np.ascontiguousarray(b.ijterms, _i32)
+        ic = np.ascontiguousarray(b.icontr, _i32)
+        ntot = np.array(list(b.ntot) + [0] * (nrep - len(b.ntot)), _i32)
+        sdeg = np.array(list(b.sdeg) + [1] * (nrep - len(b.sdeg)), _i32)
+        Ns = [np.ascontiguousarray(b.N[g], _i32) for g in range(nrep)]
+        Rs = [np.ascontiguousarray(b.repres[g], _f64) if b.repres[g].size
+              else np.zeros(1) for g in range(nrep)]
+        Np = (C.c_void_p * nrep)(*[_ptr(a) for a in Ns])
+        Rp = (C.c_void_p * nrep)(*[_ptr(a) for a in Rs])
+        check(self._L.gain_upload_inflation(
+            self._h, jind, b.maxsymcoeffs, _ptr(ij), _ptr(ic), b.matsize,
+            _ptr(ntot), _ptr(sdeg), C.cast(Np, C.c_void_p),
+            C.cast(Rp, C.c_void_p)))
+
+    def upload_block(self, jind, gamma, energies, vectors=None, rec_len=None):
+        e = np.ascontiguousarray(energies, _f64)
+        if vectors is not None:
+            v = np.ascontiguousarray(vectors, _f64)
+            rl = v.shape[1]
+            vp = _ptr(v)
+        else:
+            rl, vp = int(rec_len), None
+        check(self._L.gain_upload_block(self._h, jind, gamma, e.shape[0], rl,
+                                        _ptr(e), vp))
+        self.blocks[(jind, gamma)] = (e.shape[0], rl)
+
+    def bind_block_device(self, jind, gamma, energies, dev_ptr: int,
+                          rec_len: int, ld: int):
+        e = np.ascontiguousarray(energies, _f64)
+        check(self._L.gain_bind_block_device(self._h, jind, gamma,
+                                             e.shape[0], rec_len, ld,
+                                             _ptr(e), dev_ptr))
+        self.blocks[(jind, gamma)] = (e.shape[0], rec_len)
+
+    @classmethod
+    def from_problem(cls, prob, device: int = 0, upload_vectors=True,
+                     device_blocks: Optional[Dict] = None) -> "Engine":
+        """Upload a synth.Problem.  `device_blocks` maps (jind, gamma) to a
+        (device pointer, ld) of records already in HBM (bound, not copied)."""
+        eng = cls(device, prob.jmax, prob.params.nrepres, prob.params.maxdeg)
+        eng.set_params(prob.params)
+        eng.upload_dipole(prob.mu)
+        for jind, b in enumerate(prob.bases):
+            eng.upload_basis(jind, b)
+            eng.upload_inflation(jind, b)
+        for key, blk in prob.blocks.items():
+            if device_blocks and key in device_blocks:
+                ptr, ld = device_blocks[key]
+                eng.bind_block_device(key[0], key[1], blk.energy, ptr,
+                                      blk.rec_len, ld)
+            elif upload_vectors and blk.vectors is not None:
+                eng.upload_block(key[0], key[1], blk.energy, blk.vectors)
+            else:
+                eng.upload_block(key[0], key[1], blk.energy,
+                                 rec_len=blk.rec_len)
+        return eng
+
+    # ------------------------------------------------------------ hot path
+    def half_ls(self, jindI, gammaI, idegI, cI, jindF, nprimF) -> np.ndarray:
+        c = np.ascontiguousarray(cI, _f64)
+        h = np.zeros(nprimF)
+        check(self._L.gain_half_ls(self._h, jindI, gammaI, idegI, _ptr(c),
+                                   jindF, _ptr(h)))
+        return h
+
+    def _host_lines(self, nI, nF, want_ls=True):
+        md2 = self.maxdeg * self.maxdeg
+        out = {"S": np.zeros((nI, nF)), "A": np.zeros((nI, nF)),
+               "I": np.zeros((nI, nF)),
+               "flags": np.zeros((nI, nF), np.uint8)}
+        if want_ls:
+            out["ls"] = np.zeros((nI, nF, md2))
+        s = _lib.Lines()
+        for k in ("ls", "S", "A", "I", "flags"):
+            setattr(s, k, _ptr(out[k]) if k in out else None)
+        s.on_device = 0
+        return out, s
+
+    def plain_lines(self, jindI, gammaI, levelI, jindF, gammaF, cI=None,
+                    want_ls=True):
+        nF = self.blocks[(jindF, gammaF)][0]
+        out, s = self._host_lines(1, nF, want_ls)
+        c = None if cI is None else np.ascontiguousarray(cI, _f64)
+        check(self._L.gain_plain_lines(self._h, jindI, gammaI, levelI,
+                                       None if c is None else _ptr(c), jindF,
+                                       gammaF, C.byref(s)))
+        return {k: v[0] for k, v in out.items()}
+
+    def batched_lines(self, jindI, gammaI, levels: Sequence[int], jindF,
+                      gammaF, precision=_lib.PREC_FP64, want_ls=True):
+        lv = np.ascontiguousarray(levels, _i32)
+        nF = self.blocks[(jindF, gammaF)][0]
+        out, s = self._host_lines(lv.shape[0], nF, want_ls)
+        check(self._L.gain_batched_lines(self._h, jindI, gammaI, _ptr(lv),
+                                         lv.shape[0], jindF, gammaF,
+                                         precision, C.byref(s)))
+        return out
+
+    def lines_device(self, jindI, gammaI, levels, jindF, gammaF, ptrs: Dict,
+                     precision=_lib.PREC_FP64, plain=False, cI=None):
+        """Write results into caller device buffers (dict of name -> ptr)."""
+        s = _lib.Lines()
+        for k in ("ls", "S", "A", "I", "flags"):
+            setattr(s, k, ptrs.get(k))
+        s.on_device = 1
+        if plain:
+            c = None if cI is None else np.ascontiguousarray(cI, _f64)
+            check(self._L.gain_plain_lines(
+                self._h, jindI, gammaI, int(levels[0]),
+                None if c is None else _ptr(c), jindF, gammaF, C.byref(s)))
+        else:
+            lv = np.ascontiguousarray(levels, _i32)
+            check(self._L.gain_batched_lines(self._h, jindI, gammaI,
+                                             _ptr(lv), lv.shape[0], jindF,
+                                             gammaF, precision, C.byref(s)))
+
+    # ------------------------------------------------------------ timing
+    def set_timing(self, on: bool):
+        check(self._L.gain_set_timing(self._h, int(on)))
+
+    def timing(self, name: str, reset=False):
+        ms, n = C.c_double(), C.c_int64()
+        check(self._L.gain_get_timing(self._h, name.encode(), C.byref(ms),
+                                      C.byref(n), int(reset)))
+        return ms.value, n.value
+
+    def set_tuning(self, key: str, value: int):
+        check(self._L.gain_set_tuning(self._h, key.encode(), int(value)))

@@ -1,0 +1,202 @@
+/*
+ * gain_engine — MI355X (gfx950) line-strength engine, C ABI.
+ *
+ * Replaces the device layer of GAIN-MPI's dipole path: the kernels of
+ * GPU/dipole_GPU.cu (expand, K-block half line strength), the cuBLAS ddot
+ * projection, and the S -> Einstein-A -> intensity stage of
+ * BaseClasses/Output.cpp, behind plain C entry points (no torch types).
+ *
+ * Conventions
+ *   - every function returns a gain_status (0 = OK); on failure
+ *     gain_last_error() returns a message for the calling thread.  Nothing
+ *     aborts the process (the reference calls MPI_Abort,
+ *     GpuManager.cpp:80-93, BaseProcess.cpp:42-52).
+ *   - host arrays are caller-owned and only read (or written) during the
+ *     call; device arrays created by the engine are engine-owned.  Arrays
+ *     marked "device" are caller-owned device memory the engine borrows.
+ *   - a context is bound to one GPU and one HIP stream; calls on one context
+ *     must not overlap (the reference's per-`proc` slots map to separate
+ *     contexts).
+ */
+#ifndef GAIN_ENGINE_H
+#define GAIN_ENGINE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  GAIN_OK = 0,
+  GAIN_ERR_ARG = 1,     /* invalid argument / size / index */
+  GAIN_ERR_HIP = 2,     /* HIP runtime error */
+  GAIN_ERR_STATE = 3,   /* call order: a prerequisite upload is missing */
+  GAIN_ERR_NOMEM = 4,   /* device allocation failed */
+  GAIN_ERR_CONFIG = 5   /* inconsistent selection rules / gns */
+} gain_status;
+
+typedef struct gain_ctx gain_ctx;
+
+#define GAIN_MAX_IRREPS 16
+
+/* Intensity / selection parameters: the Input.h getters consumed by
+ * States (States.cpp:7-38) and Output (Output.cpp:3-31). */
+typedef struct {
+  int nrepres;                      /* number of irreps */
+  int maxdeg;                       /* max irrep degeneracy (<= 3) */
+  int maxJ;                         /* largest J of the J list */
+  int reduced;                      /* SYMMETRY REDUCED */
+  int symmetry_type;                /* 1 generic, 2 C3v/D3h, 3 Td */
+  int sym_degen[GAIN_MAX_IRREPS];
+  int isym_do[GAIN_MAX_IRREPS];
+  int isym_pairs[GAIN_MAX_IRREPS];  /* SELECTION */
+  double gns[GAIN_MAX_IRREPS];      /* GNS */
+  double erange_lower[2];           /* ENERGY LOW a b */
+  double erange_upper[2];           /* ENERGY ... UPPER c d */
+  double freq_window[2];            /* FREQ-WINDOW */
+  double zpe;                       /* ZPE */
+  double temperature;               /* TEMPERATURE */
+  double partition;                 /* QSTAT */
+  double threshold;                 /* THRESH_LINE (applies to A) */
+} gain_params;
+
+/* Per-pair results for a tile of initial states x one final block.
+ * Row i, column F (F = level index inside the final block, 0..nF-1).
+ * Any pointer may be NULL (not written).  `on_device` = 1 means the
+ * pointers are device memory (written without a copy). */
+typedef struct {
+  double* ls;      /* [nI][nF][maxdeg*maxdeg], ls[dI + dF*maxdeg] */
+  double* S;       /* [nI][nF] degeneracy-averaged line strength */
+  double* A;       /* [nI][nF] Einstein A */
+  double* I;       /* [nI][nF] absorption intensity */
+  uint8_t* flags;  /* [nI][nF] bit0: passes the transition filters,
+                      bit1: line emitted (nu >= 1e-40 and A >= threshold) */
+  int on_device;
+} gain_lines;
+
+/* Precision of the contractions of the batched engine. */
+typedef enum {
+  GAIN_PREC_FP64 = 0,       /* FP64 throughout */
+  GAIN_PREC_FP32 = 1,       /* FP32 MFMA products, FP64 accumulation */
+  GAIN_PREC_BF16X3 = 2      /* split-BF16 (3 slices) MFMA, FP32/FP64 acc. */
+} gain_precision;
+
+const char* gain_last_error(void);
+const char* gain_version(void);
+
+/* ---- setup (GpuManager ctor + InitializeAndTransferConstants,
+ *      GpuManager.cpp:127-213: device, 3j table, constants) ------------ */
+int gain_create(gain_ctx** ctx, int device, int jmax, int nrepres,
+                int maxdegen);
+int gain_destroy(gain_ctx* ctx);
+/* The (jmax+1)^2*9 3j table (GpuManager.cpp:164-184 index layout). */
+int gain_get_threej(gain_ctx* ctx, double* host_out);
+/* HIP stream the context launches on (hipStream_t as void*). */
+int gain_get_stream(gain_ctx* ctx, void** stream);
+int gain_synchronize(gain_ctx* ctx);
+
+/* Selection rules and intensity constants; computes the irrep pairs like
+ * TroveInput::ComputeIGammaPair (TroveInput.cpp:202-236) and returns
+ * GAIN_ERR_CONFIG where the reference aborts. */
+int gain_set_params(gain_ctx* ctx, const gain_params* p);
+int gain_get_gamma_pair(gain_ctx* ctx, int gamma, int* pair);
+
+/* ---- data uploads ------------------------------------------------------ */
+/* Dipole mu in the reference piece layout (TroveDipole.cpp:110):
+ * mu[(vF-startF) + vI*(endF-startF) + alpha*(endF-startF)*nc].  The engine
+ * needs the whole matrix (startF=0, endF=nc); GpuManager::TransferDipole,
+ * GpuManager.cpp:499-527. */
+int gain_upload_dipole(gain_ctx* ctx, const double* mu, int nc, int startF,
+                       int endF);
+/* Primitive basis of one J (GpuManager::TransferBasisSet,
+ * GpuManager.cpp:215-265): K-block starts/sizes (J+1 each), tau (= ktau&1)
+ * and vibrational index per primitive, primitives K-ordered. */
+int gain_upload_basis(gain_ctx* ctx, int jind, int J, int nprim,
+                      const int* kstart, const int* kblock, const int* ktau,
+                      const int* vib);
+/* Inflation data of one J (GpuManager::TransferInflation,
+ * GpuManager.cpp:381-422): ijterms[maxsymcoeffs*nrepres], icontr[2*nprim],
+ * and per irrep N[maxsymcoeffs], repres[ntot*sdeg*matsize]. */
+int gain_upload_inflation(gain_ctx* ctx, int jind, int maxsymcoeffs,
+                          const int* ijterms, const int* icontr, int matsize,
+                          const int* ntot, const int* sdeg, const int** N,
+                          const double** repres);
+/* All states of one (J index, irrep) block: energies (host, one per state)
+ * and the eigenvector records (one record of rec_len doubles per state,
+ * EigenVector.cpp:152-178), copied to HBM where they stay resident.
+ * `vectors` may be NULL to register energies only (initial-state blocks
+ * whose vectors are passed per call). */
+int gain_upload_block(gain_ctx* ctx, int jind, int gamma, int nstates,
+                      int rec_len, const double* energies,
+                      const double* vectors);
+/* Same, but the records already live in device memory (row stride `ld`
+ * doubles); the engine borrows the pointer. */
+int gain_bind_block_device(gain_ctx* ctx, int jind, int gamma, int nstates,
+                           int rec_len, int64_t ld, const double* energies,
+                           const double* vectors_device);
+
+/* ---- hot path ---------------------------------------------------------- */
+/* Half line strength of one initial vector, J_I -> J_F, irrep gammaI,
+ * component idegI: TransformHalfLsVector + ExecuteKBlockHalfLs +
+ * GetHalfLineStrengthResult (GpuManager.cpp:612-752).  cI is host memory
+ * of length ntot(jindI, gammaI); h_out receives nprim(jindF) doubles. */
+int gain_half_ls(gain_ctx* ctx, int jindI, int gammaI, int idegI,
+                 const double* cI, int jindF, double* h_out);
+
+/* Plain FP64 drop-in for one initial state: half line strength, projection
+ * of every final state of block (jindF, gammaF) by FP64 GEMV over the
+ * HBM-resident records, and the S/A/I epilogue with the transition
+ * filters (main.cpp:238-338, GpuManager.cpp:733-759, Output.cpp:52-112).
+ * The initial state is level `levelI` of block (jindI, gammaI); cI is its
+ * record in host memory, or NULL to read it from the uploaded block. */
+int gain_plain_lines(gain_ctx* ctx, int jindI, int gammaI, int levelI,
+                     const double* cI, int jindF, int gammaF,
+                     const gain_lines* out);
+
+/* Batched engine: nI initial states of block (jindI, gammaI) (levels
+ * levelsI[0..nI), vectors from the uploaded block) against every final
+ * state of (jindF, gammaF); half line strengths and projection run as
+ * GEMMs at the requested precision.  Rows of `out` follow levelsI. */
+int gain_batched_lines(gain_ctx* ctx, int jindI, int gammaI,
+                       const int* levelsI, int nI, int jindF, int gammaF,
+                       int precision, const gain_lines* out);
+
+/* ---- per-kernel timing (HIP events around each launch) ------------------ */
+int gain_set_timing(gain_ctx* ctx, int enable);
+/* Accumulated time (ms) and launch count of a named kernel family:
+ * "gemv", "gemm_proj", "half_ls", "epilogue". Resets the counters if
+ * reset != 0. */
+int gain_get_timing(gain_ctx* ctx, const char* name, double* total_ms,
+                    int64_t* launches, int reset);
+
+/* ---- kernel-level drop-ins for GPU/dipole_GPU.h ------------------------- *
+ * Same argument meaning as the reference's free functions; pointers are
+ * device memory, `stream` is a hipStream_t (NULL = default stream).
+ * The __constant__ setters mirror copy_*_constant (dipole_GPU.cu:119-134). */
+int gain_copy_symmetry_constants(int sym_nrepres, int maxdeg);
+int gain_copy_jmax_constant(int jmax);
+int gain_copy_dipole_constant(int dip_stride);
+/* transform_vector_primitive, dipole_GPU.h:19 */
+int gain_transform_vector_primitive(int dimenI, int igammaI, int maxcoeff,
+                                    int idegI, int sdeg, int Ntot,
+                                    const int* ijterms, const int* icontr,
+                                    const int* N, const double* repres,
+                                    const double* vecI, double* vec,
+                                    void* stream);
+/* compute_gpu_half_linestrength_, dipole_GPU.h:23-26 (one K_F block) */
+int gain_compute_half_linestrength(
+    int dimenF, int dimenI, int jI, int jF, int kF, const int* tauI,
+    const int* tauF, const int* icorrI, const int* icorrF, int startF_idx,
+    int startI_idx, int startFblock, int endFblock, int ncontrF,
+    int kFBlocksize, const int* kblock_size, const double* dipole_me,
+    const double* vector, const double* threej, double* half_ls,
+    void* stream);
+/* cublasDdot(handle, n, x, 1, y, 1, result) in DEVICE pointer mode,
+ * GpuManager.cpp:741 */
+int gain_ddot(int n, const double* x, const double* y, double* result,
+              void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

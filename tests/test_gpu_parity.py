@@ -1,0 +1,175 @@
+"""GPU parity: the HIP engine against the CPU oracle on the same seeded
+inputs (transition lists bit-exact, FP64 values within 1e-12 relative on
+strong lines).  Runs through the C ABI; needs an MI355X."""
+import numpy as np
+import pytest
+
+from gain_engine import synth
+from gain_engine.engine import Engine
+from oracle import oracle as orc
+
+from helpers import (compare_lines, engine_lines, irregular_basis,
+                     oracle_map, problem_with_bases)
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_both(prob, initials, mode="plain", precision=0):
+    tr, rows = orc.run(prob, initials)
+    with Engine.from_problem(prob) as eng:
+        got = engine_lines(eng, prob, initials, mode, precision)
+    return got, oracle_map(tr)
+
+
+def test_threej_table_matches_oracle():
+    with Engine(0, 31, 1, 1) as eng:
+        t = eng.threej()
+    ref = orc.threej_table(31)
+    np.testing.assert_allclose(t, ref, rtol=2e-16, atol=1e-300)
+
+
+def test_config0_plain_j0_to_j1():
+    """J_I=0 -> J_F=1, Nc=500, C1: the 50 lowest J=0 states x all finals."""
+    prob = synth.make_problem([0, 1], 500, "c1")
+    rows = prob.state_table()
+    initials = [i for i, r in enumerate(rows) if r[1] == 0][:50]
+    got, ref = _run_both(prob, initials)
+    assert len(ref) > 60000
+    compare_lines(got, ref)
+
+
+@pytest.mark.parametrize("scheme,jvals,nc", [("c2v4", [2, 3], 16),
+                                             ("c1", [1, 2], 10),
+                                             ("pair2", [3, 4], 8),
+                                             ("c3v3", [1, 2], 12)])
+def test_plain_every_direction(scheme, jvals, nc):
+    """All state pairs of a two-J problem (dJ = -1, 0, +1, same-block pairs,
+    degenerate irreps for c3v3)."""
+    prob = synth.make_problem(jvals, nc, scheme)
+    p = prob.params
+    p.erange_lower = (0.0, 1e9)
+    rows = prob.state_table()
+    initials = list(range(0, len(rows), max(1, len(rows) // 40)))
+    got, ref = _run_both(prob, initials)
+    assert len(ref) > 0
+    compare_lines(got, ref)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_plain_irregular_basis(seed):
+    """Random K-block sizes, tau per primitive and repeated vib indices."""
+    rng = np.random.default_rng(seed)
+    nc = 7
+    bases = [irregular_basis(2, nc, rng, 12), irregular_basis(3, nc, rng, 12)]
+    prob = problem_with_bases(bases, nc, seed)
+    prob.params.erange_lower = (0.0, 1e9)
+    rows = prob.state_table()
+    initials = list(range(0, len(rows), 3))
+    got, ref = _run_both(prob, initials)
+    assert len(ref) > 0
+    compare_lines(got, ref)
+
+
+@pytest.mark.parametrize("scheme,jI,jF", [("c1", 2, 3), ("c2v4", 3, 3),
+                                          ("c3v3", 2, 1)])
+def test_half_ls_matches_oracle(scheme, jI, jF):
+    nc = 12
+    prob = synth.make_problem(sorted({jI, jF}), nc, scheme)
+    iI = prob.jvals.index(jI)
+    iF = prob.jvals.index(jF)
+    bI, bF = prob.bases[iI], prob.bases[iF]
+    rng = np.random.default_rng(4)
+    tab = orc.threej_table(prob.jmax)
+    with Engine.from_problem(prob) as eng:
+        for g in range(len(bI.ntot)):
+            if bI.ntot[g] == 0:
+                continue
+            for d in range(bI.sdeg[g]):
+                c = rng.standard_normal(bI.ntot[g])
+                h = eng.half_ls(iI, g, d, c, iF, bF.nprim)
+                ref = orc.half_ls(prob, bI, bF, orc.expand(bI, g, d, c), tab)
+                np.testing.assert_allclose(h, ref, rtol=0,
+                                           atol=1e-13 * np.abs(ref).max())
+
+
+def test_batched_fp64_matches_oracle():
+    prob = synth.make_problem([3, 4], 16, "c2v4")
+    prob.params.erange_lower = (0.0, 1e9)
+    rows = prob.state_table()
+    initials = list(range(0, len(rows), 2))
+    got, ref = _run_both(prob, initials, mode="batched", precision=0)
+    compare_lines(got, ref)
+
+
+def test_dropin_kernels_match_oracle():
+    """gain_transform_vector_primitive / gain_compute_half_linestrength /
+    gain_ddot with the reference argument meaning and device pointers."""
+    import ctypes as C
+
+    import torch
+
+    from gain_engine import _lib
+    L = _lib.load()
+    dev = torch.device("cuda:0")
+    nc = 10
+    prob = synth.make_problem([2, 3], nc, "c2v4")
+    bI, bF = prob.bases
+    jmax = prob.jmax
+    tab = orc.threej_table(jmax)
+    T = lambda a, dt=None: torch.as_tensor(np.ascontiguousarray(a),
+                                           device=dev)
+    rng = np.random.default_rng(9)
+    g = 1
+    c = rng.standard_normal(bI.ntot[g])
+    d_x = torch.zeros(bI.nprim, dtype=torch.float64, device=dev)
+    _lib.check(L.gain_transform_vector_primitive(
+        bI.nprim, g, bI.maxsymcoeffs, 0, bI.sdeg[g], bI.ntot[g],
+        T(bI.ijterms).data_ptr(), T(bI.icontr).data_ptr(),
+        T(bI.N[g]).data_ptr(), T(bI.repres[g]).data_ptr(),
+        T(c).data_ptr(), d_x.data_ptr(), None))
+    torch.cuda.synchronize()
+    x_ref = orc.expand(bI, g, 0, c)
+    np.testing.assert_allclose(d_x.cpu().numpy(), x_ref, rtol=0, atol=1e-15)
+    # piece [2, 8) of the dipole in the reference piece layout
+    s0, s1 = 2, 8
+    piece = np.ascontiguousarray(prob.mu[:, :, s0:s1]).reshape(-1)
+    _lib.check(L.gain_copy_jmax_constant(jmax))
+    _lib.check(L.gain_copy_dipole_constant(nc))
+    d_h = torch.zeros(bF.nprim, dtype=torch.float64, device=dev)
+    d_tab, d_piece = T(tab), T(piece)
+    d_tauI, d_tauF = T(bI.tau), T(bF.tau)
+    d_vibI, d_vibF = T(bI.vib), T(bF.vib)
+    d_kbI = T(bI.kblock)
+    for kF in range(bF.J + 1):
+        _lib.check(L.gain_compute_half_linestrength(
+            int(bF.kblock[kF]), bI.nprim, bI.J, bF.J, kF,
+            d_tauI.data_ptr(), d_tauF.data_ptr(), d_vibI.data_ptr(),
+            d_vibF.data_ptr(), int(bF.kstart[kF]),
+            int(bI.kstart[max(kF - 1, 0)]), s0, s1, s1 - s0,
+            int(bF.kblock[kF]), d_kbI.data_ptr(), d_piece.data_ptr(),
+            d_x.data_ptr(), d_tab.data_ptr(), d_h.data_ptr(), None))
+    torch.cuda.synchronize()
+    h_ref = orc.half_ls(prob, bI, bF, x_ref, tab, startF=s0, endF=s1)
+    np.testing.assert_allclose(d_h.cpu().numpy(), h_ref, rtol=0,
+                               atol=1e-14 * np.abs(h_ref).max())
+    d_out = torch.zeros(1, dtype=torch.float64, device=dev)
+    y = rng.standard_normal(bF.nprim)
+    _lib.check(L.gain_ddot(bF.nprim, d_h.data_ptr(), T(y).data_ptr(),
+                           d_out.data_ptr(), None))
+    torch.cuda.synchronize()
+    ref = orc.dot(h_ref, y)
+    assert abs(d_out.item() - ref) <= 1e-14 * np.abs(h_ref * y).sum()
+
+
+def test_error_paths_return_codes():
+    from gain_engine import _lib
+    with Engine(0, 4, 1, 1) as eng:
+        with pytest.raises(_lib.GainError) as e:
+            eng.plain_lines(0, 0, 0, 0, 0)
+        assert e.value.code == _lib.ERR_STATE
+        p = synth.Params(nrepres=1, sym_degen=[1], gns=[1.0],
+                         isym_pairs=[1], maxJ=4)
+        eng.set_params(p)
+        with pytest.raises(_lib.GainError):
+            eng.upload_dipole(np.zeros((3, 4, 4)))
+            eng.plain_lines(0, 0, 0, 0, 0)
