@@ -53,6 +53,9 @@ def main():
     ap.add_argument("--ninitial", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--copy-calib-gb", type=float, default=0.0,
+                    help="also run a device-to-device copy of this many GB "
+                         "(reference point for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -98,6 +101,17 @@ def main():
                           blkF.rec_len)
     nF = blkF.n
     stream = torch.cuda.ExternalStream(eng.stream())
+
+    if args.copy_calib_gb > 0:
+        n = int(args.copy_calib_gb * 1e9 / 8)
+        src = torch.ones(n, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {n * 8 / 1e9:.2f} GB read + written")
+        del src, dst
+        torch.cuda.empty_cache()
 
     # ---------------- plain FP64 GEMV drop-in
     pl = {k: torch.zeros(nF, dtype=torch.float64, device=dev)

@@ -211,7 +211,8 @@ struct gain_ctx {
   Work w_c, w_X, w_Y, w_H, w_G, w_part, w_small, w_out;
   PinnedRing ring;
   // tuning
-  int gemv_rows = 4, gemv_ksplit = 8, mu_ksplit = 4;
+  int gemv_rows = 4, gemv_ksplit = 8, mu_ksplit = 0;  // 0: automatic
+  bool use_mfma = true;
   bool timing = false;
   std::map<std::string, TimerSlot> timers;
 };
@@ -483,7 +484,14 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
   const int64_t xstride = (int64_t)bI.ncol * nc;
   double* X = c->w_X.get<double>((size_t)nvecX * xstride);
   const int ncolTot = nvecX * bI.ncol;
-  const int ks = c->mu_ksplit;
+  int ks = c->mu_ksplit;
+  if (ks <= 0) {
+    // enough workgroups to fill 256 CUs: 64-row tiles x column tiles x 3
+    const int64_t tiles = (int64_t)((nc + 63) / 64) *
+                          ((ncolTot + (ncolTot > 64 ? 127 : 63)) /
+                           (ncolTot > 64 ? 128 : 64)) * 3;
+    ks = (int)std::min<int64_t>(16, std::max<int64_t>(1, 2048 / tiles));
+  }
   double* Y = c->w_Y.get<double>((size_t)ks * 3 * ncolTot * nc);
   Timed t(c, "half_ls");
   ScatterArgs sa{};
@@ -506,7 +514,27 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
   sa.X = X;
   sa.xstride = xstride;
   launch_scatter_expand(sa, c->stream);
-  launch_mu_gemm_valu(c->d_mu, nc, X, ncolTot, Y, ks, c->stream);
+  if (c->use_mfma) {
+    GemmArgs g{};
+    g.M = nc;
+    g.N = ncolTot;
+    g.K = nc;
+    g.A = c->d_mu;
+    g.lda = nc;
+    g.sAz = (int64_t)nc * nc;
+    g.a_kcontig = false;
+    g.B = X;
+    g.ldb = nc;
+    g.C = Y;
+    g.ldc = nc;
+    g.sCz = (int64_t)ncolTot * nc;
+    g.sCks = 3 * (int64_t)ncolTot * nc;
+    g.nz = 3;
+    g.ksplit = ks;
+    gemm_f64(g, c->stream);
+  } else {
+    launch_mu_gemm_valu(c->d_mu, nc, X, ncolTot, Y, ks, c->stream);
+  }
   GatherArgs ga{};
   ga.nprimF = bF.nprim;
   ga.nc = nc;
@@ -661,8 +689,14 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
     } else {
       double* LS = c->w_part.get<double>((size_t)nR * nvecG);
       Timed t(c, "gemm_proj");
-      gemm_projection(precision, nR, nvecG, ntotF, Crows + (int64_t)P.Flo * blkF.ld,
-                      blkF.ld, G, gstride, LS, nR, c->stream);
+      if (c->use_mfma || precision != GAIN_PREC_FP64)
+        gemm_projection(precision, nR, nvecG, ntotF,
+                        Crows + (int64_t)P.Flo * blkF.ld, blkF.ld, G, gstride,
+                        LS, nR, c->stream);
+      else
+        launch_gemm_tn_valu(nR, nvecG, ntotF,
+                            Crows + (int64_t)P.Flo * blkF.ld, blkF.ld, G,
+                            gstride, LS, nR, c->stream);
       src = LS;
       src_f = 1;
       src_v = nR;
@@ -1187,8 +1221,10 @@ extern "C" int gain_set_tuning(gain_ctx* c, const char* key, int value) {
       GAIN_CHECK_ARG(value >= 1 && value <= 256, "ksplit 1..256");
       c->gemv_ksplit = value;
     } else if (k == "mu_ksplit") {
-      GAIN_CHECK_ARG(value >= 1 && value <= 64, "ksplit 1..64");
+      GAIN_CHECK_ARG(value >= 0 && value <= 64, "ksplit 0..64");
       c->mu_ksplit = value;
+    } else if (k == "mfma") {
+      c->use_mfma = value != 0;
     } else {
       fail(GAIN_ERR_ARG, "unknown tuning key %s", key);
     }

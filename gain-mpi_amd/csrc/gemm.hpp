@@ -1,6 +1,10 @@
-// Projection GEMM of the batched engine:
-//   LS[n*ldc + m] = sum_k A[m*lda + k] * B[n*ldb + k]
-// A = final eigenvector records (rows), B = projected half line strengths.
+// Dense GEMMs of the engine (gfx950 MFMA).
+//
+//   C[z][ks][n*ldc + m] = sum_{k in split ks} A[z](m,k) * B(n,k)
+//
+// A is either K-contiguous (A[m*lda + k], eigenvector records) or
+// M-contiguous (A[k*lda + m], the dipole mu_alpha with v_F fastest);
+// B is always K-contiguous (B[n*ldb + k]); C is M-contiguous.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -8,6 +12,26 @@
 
 namespace gain {
 
+struct GemmArgs {
+  int M, N, K;
+  const double* A;
+  int64_t lda;
+  int64_t sAz;    // A stride between z slices
+  bool a_kcontig;
+  const double* B;
+  int64_t ldb;
+  double* C;
+  int64_t ldc;
+  int64_t sCz;    // C stride between z slices
+  int64_t sCks;   // C stride between split-K partials
+  int nz;         // number of z slices (e.g. the 3 dipole components)
+  int ksplit;     // split-K partial count (partials are summed by the caller)
+};
+
+void gemm_f64(const GemmArgs& g, hipStream_t s);
+
+// Projection GEMM of the batched engine at the requested precision:
+//   LS[n*ldc + m] = sum_k A[m*lda + k] * B[n*ldb + k]
 void gemm_projection(int precision, int M, int N, int K, const double* A,
                      int64_t lda, const double* B, int64_t ldb, double* C,
                      int64_t ldc, hipStream_t s);

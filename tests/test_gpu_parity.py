@@ -111,7 +111,7 @@ def test_dropin_kernels_match_oracle():
     from gain_engine import _lib
     L = _lib.load()
     dev = torch.device("cuda:0")
-    nc = 10
+    nc = 12
     prob = synth.make_problem([2, 3], nc, "c2v4")
     bI, bF = prob.bases
     jmax = prob.jmax
@@ -173,3 +173,19 @@ def test_error_paths_return_codes():
         with pytest.raises(_lib.GainError):
             eng.upload_dipole(np.zeros((3, 4, 4)))
             eng.plain_lines(0, 0, 0, 0, 0)
+
+
+@pytest.mark.parametrize("mfma", [0, 1])
+@pytest.mark.parametrize("mode", ["plain", "batched"])
+def test_vector_alu_and_mfma_gemms_agree_with_oracle(mfma, mode):
+    """Half line strength and projection GEMMs on the vector ALUs and on
+    the FP64 matrix cores; uneven sizes exercise every tile edge."""
+    prob = synth.make_problem([4, 5], 20, "c2v4")
+    prob.params.erange_lower = (0.0, 1e9)
+    rows = prob.state_table()
+    initials = list(range(1, len(rows), 3))
+    tr, _ = orc.run(prob, initials)
+    with Engine.from_problem(prob) as eng:
+        eng.set_tuning("mfma", mfma)
+        got = engine_lines(eng, prob, initials, mode)
+    compare_lines(got, oracle_map(tr))
