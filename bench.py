@@ -41,6 +41,23 @@ def build_problem(nc, jI, jF):
     return prob
 
 
+def rows_in_window(prob, eF, eIs):
+    """Final rows the engine streams for these initial energies: the union
+    over initials of the states whose energy passes the frequency and upper
+    windows (energies are sorted)."""
+    import numpy as np
+    p = prob.params
+    lo, hi = len(eF), 0
+    for eI in eIs:
+        ok = ((eF - eI) >= p.freq_window[0]) & ((eF - eI) <= p.freq_window[1]) \
+            & ((eF - p.zpe) >= p.erange_upper[0]) & \
+            ((eF - p.zpe) <= p.erange_upper[1])
+        idx = np.nonzero(ok)[0]
+        if idx.size:
+            lo, hi = min(lo, idx[0]), max(hi, idx[-1] + 1)
+    return max(hi - lo, 0)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,9 +158,11 @@ def main():
     hl_ms, hl_n = eng.timing("half_ls", reset=True)
     plain_tps = plain_trans / (plain_ms / 1e3)
     gemv_avg = gemv_ms / max(gemv_n, 1)
-    # algorithmic bytes of one GEMV launch: rows streamed + the vector
-    lo_rows = nF  # every final row is inside the window for these states
-    gemv_bytes = 8.0 * (lo_rows * blkF.rec_len + blkF.rec_len + lo_rows)
+    # algorithmic bytes of one GEMV launch: the final records inside the
+    # energy window (rows), the projected vector and the row results
+    rows = np.mean([rows_in_window(prob, blkF.energy, [blkI.energy[i]])
+                    for i in range(args.plain_states)])
+    gemv_bytes = 8.0 * (rows * blkF.rec_len + blkF.rec_len + rows)
     gemv_gbs = gemv_bytes / (gemv_avg / 1e3) / 1e9
     log(f"plain: {args.plain_states} initial states, {plain_trans} "
         f"transitions in {plain_ms:.1f} ms -> {plain_tps:.3e}/s; "
@@ -186,7 +205,10 @@ def main():
     gp_ms, gp_n = eng.timing("gemm_proj")
     hb_ms, hb_n = eng.timing("half_ls")
     gp_avg = gp_ms / max(gp_n, 1)
-    gemm_flops = 2.0 * nF * blkF.rec_len * B
+    mrows = np.mean([rows_in_window(prob, blkF.energy,
+                                    [blkI.energy[l] for l in levels_of(st)])
+                     for st in range(nsteps)])
+    gemm_flops = 2.0 * mrows * blkF.rec_len * B
     gemm_tfs = gemm_flops / (gp_avg / 1e3) / 1e12
     log(f"batched: {nsteps} x {B} initial states, {trans} transitions in "
         f"{batched_ms:.1f} ms (wall {wall * 1e3:.1f}) -> {tps:.3e}/s; "
