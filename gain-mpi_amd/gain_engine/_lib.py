@@ -65,6 +65,13 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
+    # One HIP runtime per process: torch ships its own libamdhip64.so.7; when
+    # torch is importable it is loaded first so the engine binds to the same
+    # runtime (device pointers are then shared with torch tensors).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `make -C gain-mpi_amd` "
