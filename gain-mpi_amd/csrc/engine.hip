@@ -16,6 +16,7 @@
 #include "common.hpp"
 #include "gemm.hpp"
 #include "kernels.hpp"
+#include "ozaki.hpp"
 
 namespace gain {
 
@@ -174,7 +175,16 @@ struct Basis {
   Irrep irrep[GAIN_MAX_IRREPS];
 };
 
+// Split-BF16 copy of a static operand (final records or the dipole).
+struct Sliced {
+  int S = 0;
+  int64_t Kp = 0;
+  uint16_t* planes = nullptr;
+  int* expo = nullptr;
+};
+
 struct Block {
+  Sliced sl;
   int n = 0, rec_len = 0;
   int64_t ld = 0;
   const double* vec = nullptr;  // device, may be null (energies only)
@@ -205,10 +215,12 @@ struct gain_ctx {
   bool params_set = false;
   double* d_mu = nullptr;
   int nc = 0;
+  Sliced mu_sl;
   std::vector<Basis> bases;
   std::map<std::pair<int, int>, Block> blocks;
   DeviceArena arena;
-  Work w_c, w_X, w_Y, w_H, w_G, w_part, w_small, w_out;
+  Work w_c, w_X, w_Y, w_H, w_G, w_part, w_small, w_out, w_tmp;
+  Work w_xs, w_xe, w_gs, w_ge;
   PinnedRing ring;
   // tuning
   int gemv_rows = 4, gemv_ksplit = 8, mu_ksplit = 0;  // 0: automatic
@@ -312,6 +324,53 @@ void ensure_slots(gain_ctx* c, Basis& b) {
   b.slot_ptr = c->arena.upload(cnt.data(), cnt.size());
   b.slot_prim = c->arena.upload(prim.data(), prim.size());
   b.slot_nc = c->nc;
+}
+
+int slices_of(int precision) {
+  switch (precision) {
+    case GAIN_PREC_BF16X3: return 3;
+    case GAIN_PREC_BF16X4: return 4;
+    default: return 0;
+  }
+}
+
+void release_sliced(gain_ctx* c, Sliced& sl) {
+  if (sl.planes) c->arena.release(sl.planes);
+  if (sl.expo) c->arena.release(sl.expo);
+  sl = Sliced{};
+}
+
+// Sliced copy of the dipole along v_I (rows = (alpha, v_F)), built once per
+// slice count.
+const Sliced& mu_sliced(gain_ctx* c, int S) {
+  if (c->mu_sl.S == S) return c->mu_sl;
+  release_sliced(c, c->mu_sl);
+  const int nc = c->nc;
+  double* muT = c->w_tmp.get<double>((size_t)3 * nc * nc);
+  transpose_mu(c->d_mu, nc, muT, c->stream);
+  Sliced sl;
+  sl.S = S;
+  sl.Kp = slice_kpad(nc);
+  sl.planes = c->arena.alloc<uint16_t>((size_t)S * 3 * nc * sl.Kp);
+  sl.expo = c->arena.alloc<int>((size_t)3 * nc);
+  slice_rows_bf16(S, muT, nc, 3 * nc, nc, sl.planes, sl.expo, c->stream);
+  c->mu_sl = sl;
+  return c->mu_sl;
+}
+
+// Sliced copy of a block's eigenvector records, built once per slice count.
+const Sliced& block_sliced(gain_ctx* c, Block& b, int S) {
+  if (b.sl.S == S) return b.sl;
+  release_sliced(c, b.sl);
+  Sliced sl;
+  sl.S = S;
+  sl.Kp = slice_kpad(b.rec_len);
+  sl.planes = c->arena.alloc<uint16_t>((size_t)S * b.n * sl.Kp);
+  sl.expo = c->arena.alloc<int>(b.n);
+  slice_rows_bf16(S, b.vec, b.ld, b.n, b.rec_len, sl.planes, sl.expo,
+                  c->stream);
+  b.sl = sl;
+  return b.sl;
 }
 
 bool degfilter(const gain_ctx* c, int gI, int gF, int dI, int dF) {
@@ -471,7 +530,7 @@ struct Small {
 // Half line strengths of every (initial, dI) vector: scatter, mu GEMM and
 // gather.  H rows: i * |DI| + k.  Returns the H stride.
 int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
-                    const Small& sm, int nvecX, double* H) {
+                    const Small& sm, int nvecX, double* H, int precision) {
   Basis& bI = *P.bI;
   Basis& bF = *P.bF;
   ensure_slots(c, bI);
@@ -514,7 +573,32 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
   sa.X = X;
   sa.xstride = xstride;
   launch_scatter_expand(sa, c->stream);
-  if (c->use_mfma) {
+  const int S = slices_of(precision);
+  if (S > 0) {
+    ks = 1;
+    const Sliced& ms = mu_sliced(c, S);
+    uint16_t* xs = c->w_xs.get<uint16_t>((size_t)S * ncolTot * ms.Kp);
+    int* xe = c->w_xe.get<int>(ncolTot);
+    slice_rows_bf16(S, X, nc, ncolTot, nc, xs, xe, c->stream);
+    SlicedGemmArgs g{};
+    g.S = S;
+    g.M = nc;
+    g.N = ncolTot;
+    g.Kp = ms.Kp;
+    g.A = ms.planes;
+    g.a_plane = (int64_t)3 * nc * ms.Kp;
+    g.a_z = (int64_t)nc * ms.Kp;
+    g.eA = ms.expo;
+    g.ea_z = nc;
+    g.B = xs;
+    g.b_plane = (int64_t)ncolTot * ms.Kp;
+    g.eB = xe;
+    g.C = Y;
+    g.ldc = nc;
+    g.c_z = (int64_t)ncolTot * nc;
+    g.nz = 3;
+    gemm_sliced_bf16(g, c->stream);
+  } else if (c->use_mfma) {
     GemmArgs g{};
     g.M = nc;
     g.N = ncolTot;
@@ -642,7 +726,7 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
     const int64_t hstride = bF.nprim;
     double* H = c->w_H.get<double>((size_t)(nvecX + 1) * hstride);
     launch_fill_zero(H + (size_t)nvecX * hstride, hstride, c->stream);
-    run_half_ls(c, P, cbase, sm, nvecX, H);
+    run_half_ls(c, P, cbase, sm, nvecX, H, plain ? GAIN_PREC_FP64 : precision);
     double* G = c->w_G.get<double>((size_t)nvecG * gstride);
     {
       Timed t(c, "expand_T");
@@ -689,7 +773,28 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
     } else {
       double* LS = c->w_part.get<double>((size_t)nR * nvecG);
       Timed t(c, "gemm_proj");
-      if (c->use_mfma || precision != GAIN_PREC_FP64)
+      const int S = slices_of(precision);
+      if (S > 0) {
+        const Sliced& cs = block_sliced(c, blkF, S);
+        uint16_t* gs = c->w_gs.get<uint16_t>((size_t)S * nvecG * cs.Kp);
+        int* ge = c->w_ge.get<int>(nvecG);
+        slice_rows_bf16(S, G, gstride, nvecG, ntotF, gs, ge, c->stream);
+        SlicedGemmArgs g{};
+        g.S = S;
+        g.M = nR;
+        g.N = nvecG;
+        g.Kp = cs.Kp;
+        g.A = cs.planes + (int64_t)P.Flo * cs.Kp;
+        g.a_plane = (int64_t)blkF.n * cs.Kp;
+        g.eA = cs.expo + P.Flo;
+        g.B = gs;
+        g.b_plane = (int64_t)nvecG * cs.Kp;
+        g.eB = ge;
+        g.C = LS;
+        g.ldc = nR;
+        g.nz = 1;
+        gemm_sliced_bf16(g, c->stream);
+      } else if (c->use_mfma || precision != GAIN_PREC_FP64)
         gemm_projection(precision, nR, nvecG, ntotF,
                         Crows + (int64_t)P.Flo * blkF.ld, blkF.ld, G, gstride,
                         LS, nR, c->stream);
@@ -938,6 +1043,7 @@ extern "C" int gain_upload_dipole(gain_ctx* c, const double* mu, int nc,
            "the engine keeps the whole dipole resident: need startF=0, "
            "endF=nc (got %d,%d of %d)", startF, endF, nc);
     DeviceGuard g(c->device);
+    release_sliced(c, c->mu_sl);
     if (c->d_mu) c->arena.release(c->d_mu);
     c->d_mu = c->arena.upload(mu, (size_t)3 * nc * nc);
     c->nc = nc;
@@ -1057,6 +1163,7 @@ static void register_block(gain_ctx* c, int jind, int gamma, int nstates,
   DeviceGuard g(c->device);
   auto it = c->blocks.find({jind, gamma});
   if (it != c->blocks.end()) {
+    release_sliced(c, it->second.sl);
     if (it->second.owned && it->second.vec)
       c->arena.release((void*)it->second.vec);
     if (it->second.d_energy) c->arena.release(it->second.d_energy);
@@ -1142,7 +1249,7 @@ extern "C" int gain_half_ls(gain_ctx* c, int jindI, int gammaI, int idegI,
     sm.c_off = (int64_t*)ds;
     sm.ideg = (int*)(ds + sizeof(int64_t));
     double* H = c->w_H.get<double>(P.bF->nprim);
-    run_half_ls(c, P, dc, sm, 1, H);
+    run_half_ls(c, P, dc, sm, 1, H, GAIN_PREC_FP64);
     GAIN_HIP(hipMemcpyAsync(h_out, H, sizeof(double) * P.bF->nprim,
                             hipMemcpyDeviceToHost, c->stream));
     GAIN_HIP(hipStreamSynchronize(c->stream));
@@ -1168,7 +1275,7 @@ extern "C" int gain_batched_lines(gain_ctx* c, int jindI, int gammaI,
   return guarded([&] {
     need(c);
     GAIN_CHECK_ARG(out && levelsI && nI > 0, "bad batch");
-    GAIN_CHECK_ARG(precision >= GAIN_PREC_FP64 && precision <= GAIN_PREC_BF16X3,
+    GAIN_CHECK_ARG(precision >= GAIN_PREC_FP64 && precision <= GAIN_PREC_BF16X4,
                    "unknown precision %d", precision);
     DeviceGuard g(c->device);
     Plan P = make_plan(c, jindI, gammaI, levelsI, nI, jindF, gammaF);

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(_HERE, "libgain_engine.so")
 MAX_IRREPS = 16
 
 OK, ERR_ARG, ERR_HIP, ERR_STATE, ERR_NOMEM, ERR_CONFIG = range(6)
-PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
+PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_BF16X4 = 0, 1, 2, 3
 
 # Every symbol include/gain_engine.h declares.
 EXPORTS = (

@@ -78,7 +78,8 @@ typedef struct {
 typedef enum {
   GAIN_PREC_FP64 = 0,       /* FP64 throughout */
   GAIN_PREC_FP32 = 1,       /* FP32 MFMA products, FP64 accumulation */
-  GAIN_PREC_BF16X3 = 2      /* split-BF16 (3 slices) MFMA, FP32/FP64 acc. */
+  GAIN_PREC_BF16X3 = 2,     /* split-BF16, 3 slices (24 bits of the row max) */
+  GAIN_PREC_BF16X4 = 3      /* split-BF16, 4 slices (32 bits of the row max) */
 } gain_precision;
 
 const char* gain_last_error(void);

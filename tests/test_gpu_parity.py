@@ -189,3 +189,36 @@ def test_vector_alu_and_mfma_gemms_agree_with_oracle(mfma, mode):
         eng.set_tuning("mfma", mfma)
         got = engine_lines(eng, prob, initials, mode)
     compare_lines(got, oracle_map(tr))
+
+
+def _strong_weak_err(got, ref):
+    keys = list(ref)
+    r = np.array([ref[k]["S"] for k in keys])
+    g = np.array([got[k]["S"] for k in keys])
+    strong = r > 1e-6 * r.max()
+    rel = np.abs(g - r) / np.where(r > 0, r, 1.0)
+    return rel[strong].max(), (rel[~strong].max() if (~strong).any()
+                               else 0.0), strong.sum(), (~strong).sum()
+
+
+@pytest.mark.parametrize("precision,bound", [(3, 1e-6), (2, 1e-3)])
+def test_batched_split_bf16_error_vs_fp64_oracle(precision, bound):
+    """Split-BF16 contractions (4 and 3 slices): transition list identical,
+    per-line S error against the FP64 oracle within `bound` on strong lines
+    (S > 1e-6 S_max); the FP64 stages (3j, S accumulation, A, intensity)
+    are unchanged."""
+    prob = synth.make_problem([5, 6], 32, "c2v4")
+    prob.params.erange_lower = (0.0, 1e9)
+    rows = prob.state_table()
+    initials = list(range(0, len(rows), 2))
+    tr, _ = orc.run(prob, initials)
+    ref = oracle_map(tr)
+    with Engine.from_problem(prob) as eng:
+        got = engine_lines(eng, prob, initials, "batched", precision)
+    assert set(got) == set(ref)
+    assert {k for k in got if got[k]["emitted"]} == \
+        {k for k in ref if ref[k]["emitted"]}
+    es, ew, ns, nw = _strong_weak_err(got, ref)
+    print(f"precision {precision}: strong lines {ns} max rel err {es:.3e}; "
+          f"weak lines {nw} max rel err {ew:.3e}")
+    assert es <= bound
