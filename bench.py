@@ -27,7 +27,12 @@ sys.path[:0] = [ROOT, os.path.join(ROOT, "gain-mpi_amd")]
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
-PRECISION_NAMES = {0: "f64", 1: "f32-mfma/f64-acc", 2: "bf16x3-mfma/f64-acc"}
+PRECISION_NAMES = {0: "f64", 1: "f32-mfma/f64-acc",
+                   2: "bf16x3-ozaki-mfma/f32-f64-acc",
+                   3: "bf16x4-ozaki-mfma/f32-f64-acc"}
+PEAKS = {0: ("TFLOP/s", 78.6), 1: ("TFLOP/s", 157.3), 2: ("TFLOP/s", 2500.0),
+         3: ("TFLOP/s", 2500.0)}
+PRODUCTS = {0: 1, 1: 1, 2: 6, 3: 10}  # hardware products per algorithmic FMA
 
 
 def log(msg):
@@ -70,6 +75,8 @@ def main():
     ap.add_argument("--ninitial", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--variants", type=str, default="",
+                    help="comma-separated extra precisions to time")
     ap.add_argument("--copy-calib-gb", type=float, default=0.0,
                     help="also run a device-to-device copy of this many GB "
                          "(reference point for HBM byte counters)")
@@ -179,41 +186,75 @@ def main():
     def levels_of(step):
         return [(step * B + i) % nI_all for i in range(B)]
 
-    for w in range(args.warmup):
-        eng.lines_device(0, 0, levels_of(w), 1, 1,
-                         {k: v.data_ptr() for k, v in outs.items()} |
-                         {"flags": bflags[0].data_ptr()},
-                         precision=args.precision)
-    eng.synchronize()
-    for name in ("gemv", "half_ls", "expand_T", "epilogue", "gemm_proj"):
-        eng.timing(name, reset=True)
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    ev0.record(stream)
-    for s in range(nsteps):
-        eng.lines_device(0, 0, levels_of(s), 1, 1,
-                         {k: v.data_ptr() for k, v in outs.items()} |
-                         {"flags": bflags[s].data_ptr()},
-                         precision=args.precision)
-    ev1.record(stream)
-    eng.synchronize()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t_start
-    batched_ms = ev0.elapsed_time(ev1)
-    trans = int((bflags & 1).sum().item())
-    tps = trans / (batched_ms / 1e3)
-    gp_ms, gp_n = eng.timing("gemm_proj")
-    hb_ms, hb_n = eng.timing("half_ls")
-    gp_avg = gp_ms / max(gp_n, 1)
-    mrows = np.mean([rows_in_window(prob, blkF.energy,
-                                    [blkI.energy[l] for l in levels_of(st)])
-                     for st in range(nsteps)])
-    gemm_flops = 2.0 * mrows * blkF.rec_len * B
-    gemm_tfs = gemm_flops / (gp_avg / 1e3) / 1e12
-    log(f"batched: {nsteps} x {B} initial states, {trans} transitions in "
-        f"{batched_ms:.1f} ms (wall {wall * 1e3:.1f}) -> {tps:.3e}/s; "
-        f"gemm avg {gp_avg:.2f} ms = {gemm_tfs:.1f} TF/s; half_ls avg "
-        f"{hb_ms / max(hb_n, 1):.2f} ms")
+    def run_batched(prec):
+        for w in range(max(args.warmup, 1)):
+            eng.lines_device(0, 0, levels_of(w), 1, 1,
+                             {k: v.data_ptr() for k, v in outs.items()} |
+                             {"flags": bflags[0].data_ptr()}, precision=prec)
+        eng.synchronize()
+        for name in ("gemv", "half_ls", "expand_T", "epilogue",
+                     "gemm_proj"):
+            eng.timing(name, reset=True)
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        for st in range(nsteps):
+            eng.lines_device(0, 0, levels_of(st), 1, 1,
+                             {k: v.data_ptr() for k, v in outs.items()} |
+                             {"flags": bflags[st].data_ptr()}, precision=prec)
+        ev1.record(stream)
+        eng.synchronize()
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1)
+        trans = int((bflags & 1).sum().item())
+        gp_ms, gp_n = eng.timing("gemm_proj")
+        hb_ms, hb_n = eng.timing("half_ls")
+        gp_avg = gp_ms / max(gp_n, 1)
+        mrows = np.mean([rows_in_window(prob, blkF.energy,
+                                        [blkI.energy[l] for l in
+                                         levels_of(st)])
+                         for st in range(nsteps)])
+        flops = 2.0 * mrows * blkF.rec_len * B
+        tfs = flops / (gp_avg / 1e3) / 1e12
+        r = {"precision": prec, "dtype": PRECISION_NAMES[prec],
+             "transitions": trans, "ms": ms, "tps": trans / (ms / 1e3),
+             "gemm_proj_ms": gp_avg, "gemm_proj_tflops": tfs,
+             "half_ls_ms": hb_ms / max(hb_n, 1)}
+        log(f"batched[{PRECISION_NAMES[prec]}]: {nsteps} x {B} initial "
+            f"states, {trans} transitions in {ms:.1f} ms -> "
+            f"{r['tps']:.3e}/s; projection avg {gp_avg:.2f} ms = {tfs:.1f} "
+            f"algorithmic TF/s; half_ls avg {r['half_ls_ms']:.2f} ms")
+        return r
+
+    def s_of_batch0(prec):
+        eng.lines_device(0, 0, levels_of(0), 1, 1,
+                         {"S": outs["S"].data_ptr(),
+                          "flags": bflags[0].data_ptr()}, precision=prec)
+        eng.synchronize()
+        return outs["S"].cpu().numpy().copy(), \
+            bflags[0].cpu().numpy().copy()
+
+    head = run_batched(args.precision)
+    variants = [run_batched(int(v)) for v in args.variants.split(",")
+                if v.strip() and int(v) != args.precision]
+    # per-line error of the measured precision against FP64 on one batch
+    s64, f64 = s_of_batch0(0)
+    sp, fp = s_of_batch0(args.precision)
+    lines = (f64 & 1).astype(bool)
+    ref = s64[lines]
+    rel = np.abs(sp[lines] - ref) / np.where(ref > 0, ref, 1.0)
+    strong = ref > 1e-6 * ref.max()
+    prec_check = {"vs": "batched FP64 (checked against the oracle at 1e-12 "
+                        "in the GPU tests)",
+                  "transition_list_identical": bool(np.array_equal(f64, fp)),
+                  "strong_lines": int(strong.sum()),
+                  "strong_max_rel_err": float(rel[strong].max()),
+                  "weak_lines": int((~strong).sum()),
+                  "weak_max_rel_err": float(rel[~strong].max())
+                  if (~strong).any() else 0.0}
+    log(f"precision check: {prec_check}")
+    tps = head["tps"]
+    batched_ms = head["ms"]
+    gemm_tfs = head["gemm_proj_tflops"]
 
     # ---------------- CPU baseline (oracle, bounded sample)
     cpu = None
@@ -260,9 +301,18 @@ def main():
                   "ms_per_initial": plain_ms / args.plain_states},
         "speedup_vs_plain": tps / plain_tps,
         "roofline": {"bound": "mfma", "kernel": "gemm_proj",
-                     "achieved": gemm_tfs, "peak": FP64_PEAK_TFS,
-                     "unit": "TFLOP/s", "frac": gemm_tfs / FP64_PEAK_TFS,
+                     "achieved": gemm_tfs,
+                     "peak": PEAKS[args.precision][1] /
+                     PRODUCTS[args.precision],
+                     "unit": "TFLOP/s",
+                     "frac": gemm_tfs * PRODUCTS[args.precision] /
+                     PEAKS[args.precision][1],
+                     "note": "achieved = algorithmic 2*M*N*K flops / avg "
+                             "kernel time; peak = dense MFMA peak of the "
+                             "datatype / hardware products per FMA",
                      "traffic": None},
+        "precision_check": prec_check,
+        "variants": variants,
         "roofline_plain": {"bound": "hbm", "kernel": "gemv",
                            "achieved": gemv_gbs, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": gemv_gbs / HBM_PEAK_GBS,

@@ -121,12 +121,17 @@ def test_dropin_kernels_match_oracle():
     rng = np.random.default_rng(9)
     g = 1
     c = rng.standard_normal(bI.ntot[g])
+    # every device buffer is held by a name until the kernels have run: a
+    # temporary tensor would be freed as soon as data_ptr() returns and its
+    # block handed to the next allocation
+    d_ijt, d_icontr = T(bI.ijterms), T(bI.icontr)
+    d_N, d_rep, d_c = T(bI.N[g]), T(bI.repres[g]), T(c)
     d_x = torch.zeros(bI.nprim, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
     _lib.check(L.gain_transform_vector_primitive(
         bI.nprim, g, bI.maxsymcoeffs, 0, bI.sdeg[g], bI.ntot[g],
-        T(bI.ijterms).data_ptr(), T(bI.icontr).data_ptr(),
-        T(bI.N[g]).data_ptr(), T(bI.repres[g]).data_ptr(),
-        T(c).data_ptr(), d_x.data_ptr(), None))
+        d_ijt.data_ptr(), d_icontr.data_ptr(), d_N.data_ptr(),
+        d_rep.data_ptr(), d_c.data_ptr(), d_x.data_ptr(), None))
     torch.cuda.synchronize()
     x_ref = orc.expand(bI, g, 0, c)
     np.testing.assert_allclose(d_x.cpu().numpy(), x_ref, rtol=0, atol=1e-15)
@@ -140,6 +145,7 @@ def test_dropin_kernels_match_oracle():
     d_tauI, d_tauF = T(bI.tau), T(bF.tau)
     d_vibI, d_vibF = T(bI.vib), T(bF.vib)
     d_kbI = T(bI.kblock)
+    torch.cuda.synchronize()
     for kF in range(bF.J + 1):
         _lib.check(L.gain_compute_half_linestrength(
             int(bF.kblock[kF]), bI.nprim, bI.J, bF.J, kF,
@@ -154,7 +160,9 @@ def test_dropin_kernels_match_oracle():
                                atol=1e-14 * np.abs(h_ref).max())
     d_out = torch.zeros(1, dtype=torch.float64, device=dev)
     y = rng.standard_normal(bF.nprim)
-    _lib.check(L.gain_ddot(bF.nprim, d_h.data_ptr(), T(y).data_ptr(),
+    d_y = T(y)
+    torch.cuda.synchronize()
+    _lib.check(L.gain_ddot(bF.nprim, d_h.data_ptr(), d_y.data_ptr(),
                            d_out.data_ptr(), None))
     torch.cuda.synchronize()
     ref = orc.dot(h_ref, y)
