@@ -1,0 +1,99 @@
+"""The J_I=20 -> J_F=21 workload at full size, resident in HBM.
+
+C2v-like 4-irrep basis (Hadamard quartets), initial block (J=20, A1) with
+N_I = (2J+1) Nc / 4 functions, final block (J=21, A2) with
+N_F = n_F = (2J+1) Nc / 4 states.  The final records (43000 x 43000 FP64 at
+Nc = 4000, 14.8 GB) are the Kronecker product of two QR factors, generated
+directly in HBM; the `ninitial` lowest initial records likewise.
+"""
+from __future__ import annotations
+
+import dataclasses
+from typing import Optional
+
+import numpy as np
+
+from . import synth
+from .engine import Engine
+
+
+@dataclasses.dataclass
+class Workload:
+    prob: synth.Problem
+    eng: Engine
+    CF: object  # torch tensor [n_F, N_F]
+    CI: object  # torch tensor [ninitial, N_I]
+    ninitial: int
+
+    @property
+    def blkI(self):
+        return self.prob.blocks[(0, 0)]
+
+    @property
+    def blkF(self):
+        return self.prob.blocks[(1, 1)]
+
+    def close(self):
+        self.eng.close()
+
+
+def build(nc: int = 4000, jI: int = 20, jF: int = 21, ninitial: int = 2000,
+          device: int = 0, params: Optional[synth.Params] = None) -> Workload:
+    import torch
+
+    prob = synth.make_problem([jI, jF], nc, "c2v4", blocks={(0, 0), (1, 1)},
+                              kron=True, params=params)
+    blkI, blkF = prob.blocks[(0, 0)], prob.blocks[(1, 1)]
+    nI = min(ninitial, blkI.n)
+    dev = torch.device(f"cuda:{device}")
+    qa, qb = (torch.as_tensor(q, device=dev) for q in blkF.kron)
+    CF = torch.kron(qa.T.contiguous(), qb.T.contiguous())  # rows = records
+    del qa, qb
+    qa, qb = blkI.kron
+    b = qb.shape[0]
+    ka = (nI + b - 1) // b
+    CI = torch.kron(torch.as_tensor(qa.T[:ka].copy(), device=dev),
+                    torch.as_tensor(qb.T.copy(), device=dev))[:nI]
+    CI = CI.contiguous()
+    torch.cuda.synchronize()
+    eng = Engine(device, prob.jmax, prob.params.nrepres, prob.params.maxdeg)
+    eng.set_params(prob.params)
+    eng.upload_dipole(prob.mu)
+    for jind, bs in enumerate(prob.bases):
+        eng.upload_basis(jind, bs)
+        eng.upload_inflation(jind, bs)
+    eng.bind_block_device(0, 0, blkI.energy[:nI], CI.data_ptr(),
+                          blkI.rec_len, blkI.rec_len)
+    eng.bind_block_device(1, 1, blkF.energy, CF.data_ptr(), blkF.rec_len,
+                          blkF.rec_len)
+    return Workload(prob, eng, CF, CI, nI)
+
+
+def rows_in_window(prob, eF, eIs) -> int:
+    """Final rows the engine streams for these initial energies: the union
+    over initials of the states whose energy passes the frequency and upper
+    windows (energies sorted)."""
+    p = prob.params
+    lo, hi = len(eF), 0
+    for eI in eIs:
+        ok = ((eF - eI) >= p.freq_window[0]) & ((eF - eI) <= p.freq_window[1]) \
+            & ((eF - p.zpe) >= p.erange_upper[0]) & \
+            ((eF - p.zpe) <= p.erange_upper[1])
+        idx = np.nonzero(ok)[0]
+        if idx.size:
+            lo, hi = min(lo, idx[0]), max(hi, idx[-1] + 1)
+    return max(hi - lo, 0)
+
+
+def expected_flags(prob, eI: float, eF: np.ndarray) -> np.ndarray:
+    """Transition filter of one initial state against the final block, with
+    the same floating-point operations as States::FilterIntensity
+    (States.cpp:54-76); block-level conditions hold for this workload."""
+    p = prob.params
+    nu = eF - eI
+    low_ok = (eI - p.zpe) <= p.erange_lower[1] and \
+        (eI - p.zpe) >= p.erange_lower[0]
+    ok = (nu >= p.freq_window[0]) & (nu <= p.freq_window[1]) & \
+        ((eF - p.zpe) <= p.erange_upper[1]) & \
+        ((eF - p.zpe) >= p.erange_upper[0])
+    return ok & low_ok

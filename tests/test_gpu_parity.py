@@ -170,17 +170,30 @@ def test_dropin_kernels_match_oracle():
 
 
 def test_error_paths_return_codes():
+    """Missing uploads and bad arguments come back as status codes."""
+    import ctypes as C
+
     from gain_engine import _lib
+    L = _lib.load()
     with Engine(0, 4, 1, 1) as eng:
-        with pytest.raises(_lib.GainError) as e:
-            eng.plain_lines(0, 0, 0, 0, 0)
-        assert e.value.code == _lib.ERR_STATE
+        out = _lib.Lines()
+        rc = L.gain_plain_lines(eng._h, 0, 0, 0, None, 0, 0, C.byref(out))
+        assert rc == _lib.ERR_STATE  # no params yet
         p = synth.Params(nrepres=1, sym_degen=[1], gns=[1.0],
                          isym_pairs=[1], maxJ=4)
         eng.set_params(p)
+        rc = L.gain_plain_lines(eng._h, 0, 0, 0, None, 0, 0, C.byref(out))
+        assert rc == _lib.ERR_STATE  # no dipole
+        eng.upload_dipole(np.zeros((3, 4, 4)))
+        rc = L.gain_plain_lines(eng._h, 0, 0, 0, None, 0, 0, C.byref(out))
+        assert rc == _lib.ERR_STATE and b"basis" in L.gain_last_error()
+        bad = synth.Params(nrepres=1, sym_degen=[1], gns=[1.0],
+                           isym_pairs=[1], maxJ=4, temperature=0.0)
+        with pytest.raises(_lib.GainError) as e:
+            eng.set_params(bad)
+        assert e.value.code == _lib.ERR_ARG
         with pytest.raises(_lib.GainError):
-            eng.upload_dipole(np.zeros((3, 4, 4)))
-            eng.plain_lines(0, 0, 0, 0, 0)
+            eng.batched_lines(0, 0, [0], 0, 0, precision=9)
 
 
 @pytest.mark.parametrize("mfma", [0, 1])
