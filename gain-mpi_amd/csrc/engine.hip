@@ -352,7 +352,7 @@ const Sliced& mu_sliced(gain_ctx* c, int S) {
   sl.S = S;
   sl.Kp = slice_kpad(nc);
   sl.planes = c->arena.alloc<uint16_t>((size_t)S * 3 * nc * sl.Kp);
-  sl.expo = c->arena.alloc<int>((size_t)3 * nc);
+  sl.expo = c->arena.alloc<int>((size_t)3 * nc * slice_nchunk(S, nc));
   slice_rows_bf16(S, muT, nc, 3 * nc, nc, sl.planes, sl.expo, c->stream);
   c->mu_sl = sl;
   return c->mu_sl;
@@ -366,7 +366,7 @@ const Sliced& block_sliced(gain_ctx* c, Block& b, int S) {
   sl.S = S;
   sl.Kp = slice_kpad(b.rec_len);
   sl.planes = c->arena.alloc<uint16_t>((size_t)S * b.n * sl.Kp);
-  sl.expo = c->arena.alloc<int>(b.n);
+  sl.expo = c->arena.alloc<int>((size_t)b.n * slice_nchunk(S, b.rec_len));
   slice_rows_bf16(S, b.vec, b.ld, b.n, b.rec_len, sl.planes, sl.expo,
                   c->stream);
   b.sl = sl;
@@ -578,7 +578,8 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
     ks = 1;
     const Sliced& ms = mu_sliced(c, S);
     uint16_t* xs = c->w_xs.get<uint16_t>((size_t)S * ncolTot * ms.Kp);
-    int* xe = c->w_xe.get<int>(ncolTot);
+    const int nch = (int)slice_nchunk(S, nc);
+    int* xe = c->w_xe.get<int>((size_t)ncolTot * nch);
     slice_rows_bf16(S, X, nc, ncolTot, nc, xs, xe, c->stream);
     SlicedGemmArgs g{};
     g.S = S;
@@ -589,7 +590,8 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
     g.a_plane = (int64_t)3 * nc * ms.Kp;
     g.a_z = (int64_t)nc * ms.Kp;
     g.eA = ms.expo;
-    g.ea_z = nc;
+    g.ea_z = (int64_t)nc * nch;
+    g.nch = nch;
     g.B = xs;
     g.b_plane = (int64_t)ncolTot * ms.Kp;
     g.eB = xe;
@@ -777,7 +779,8 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
       if (S > 0) {
         const Sliced& cs = block_sliced(c, blkF, S);
         uint16_t* gs = c->w_gs.get<uint16_t>((size_t)S * nvecG * cs.Kp);
-        int* ge = c->w_ge.get<int>(nvecG);
+        const int nch = (int)slice_nchunk(S, ntotF);
+        int* ge = c->w_ge.get<int>((size_t)nvecG * nch);
         slice_rows_bf16(S, G, gstride, nvecG, ntotF, gs, ge, c->stream);
         SlicedGemmArgs g{};
         g.S = S;
@@ -786,7 +789,8 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
         g.Kp = cs.Kp;
         g.A = cs.planes + (int64_t)P.Flo * cs.Kp;
         g.a_plane = (int64_t)blkF.n * cs.Kp;
-        g.eA = cs.expo + P.Flo;
+        g.eA = cs.expo + (int64_t)P.Flo * nch;
+        g.nch = nch;
         g.B = gs;
         g.b_plane = (int64_t)nvecG * cs.Kp;
         g.eB = ge;

@@ -16,23 +16,31 @@ template <int S>
 __global__ __launch_bounds__(256) void k_slice_rows(
     const double* __restrict__ src, int64_t ld, int rows, int K, int64_t Kp,
     uint16_t* __restrict__ planes, int* __restrict__ expo) {
+  constexpr int CH = (S <= 3 ? 256 : 128);
+  constexpr int WPC = CH / 64;  // waves per chunk
   __shared__ double red[4];
   const int r = blockIdx.x;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, wave = tid >> 6;
   const double* x = src + (int64_t)r * ld;
-  double mx = 0.0;
-  for (int k = tid; k < K; k += 256) mx = fmax(mx, fabs(x[k]));
-  for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
-  if ((tid & 63) == 0) red[tid >> 6] = mx;
-  __syncthreads();
-  mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-  int e = 0;
-  if (mx > 0.0) frexp(mx, &e);  // mx = f 2^e, f in [0.5, 1): |x| 2^-e < 1
-  if (tid == 0) expo[r] = e;
   const int64_t pstride = (int64_t)rows * Kp;
+  const int64_t nch = Kp / CH;
   uint16_t* out = planes + (int64_t)r * Kp;
-  for (int64_t k = tid; k < Kp; k += 256) {
-    double v = k < K ? ldexp(x[k], -e) : 0.0;
+  for (int64_t k0 = 0; k0 < Kp; k0 += 256) {
+    const int64_t k = k0 + tid;
+    double v = k < K ? x[k] : 0.0;
+    double mx = fabs(v);
+    for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+    __syncthreads();
+    if ((tid & 63) == 0) red[wave] = mx;
+    __syncthreads();
+    const int w0 = wave / WPC * WPC;
+    mx = red[w0];
+#pragma unroll
+    for (int w = 1; w < WPC; ++w) mx = fmax(mx, red[w0 + w]);
+    int e = 0;
+    if (mx > 0.0) frexp(mx, &e);  // mx = f 2^e, f in [0.5, 1): |x| 2^-e < 1
+    if (tid % CH == 0) expo[(int64_t)r * nch + k / CH] = e;
+    v = ldexp(v, -e);
 #pragma unroll
     for (int i = 0; i < S; ++i) {
       const double n = rint(ldexp(v, 8 * (i + 1)));
@@ -103,7 +111,7 @@ constexpr int kSBM = 64, kSBN = 64, kSBK = 32, kSRow = kSBK + 8;
 template <int S>
 __global__ __launch_bounds__(256) void k_gemm_sliced(SlicedGemmArgs g,
                                                      int ntn) {
-  // stages per exact FP32 chunk: 256 k for S <= 3, 128 k for S = 4
+  // stages per chunk (exact FP32 sums, one exponent pair per chunk)
   constexpr int kChunk = (S <= 3 ? 256 : 128) / kSBK;
   __shared__ __attribute__((aligned(16))) uint16_t As[S][kSBM][kSRow];
   __shared__ __attribute__((aligned(16))) uint16_t Bs[S][kSBN][kSRow];
@@ -148,21 +156,44 @@ __global__ __launch_bounds__(256) void k_gemm_sliced(SlicedGemmArgs g,
                    : u4{0u, 0u, 0u, 0u};
     }
   };
-  auto flush = [&]() {
+  // rows / columns this lane's accumulators belong to (clamped in range)
+  int mrow[2][4], ncol[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      mrow[t][e] = min(m0 + wm * 32 + t * 16 + 4 * (lane >> 4) + e, g.M - 1);
+    ncol[t] = min(n0 + wn * 32 + t * 16 + (lane & 15), g.N - 1);
+  }
+  auto flush = [&](int ch) {
+    int ea[2][4], eb[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) ea[t][e] = eA[(int64_t)mrow[t][e] * g.nch + ch];
+      eb[t] = g.eB[(int64_t)ncol[t] * g.nch + ch];
+    }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int j = 0; j < 2; ++j)
 #pragma unroll
-        for (int L = S - 1; L >= 0; --L) {
+        for (int e = 0; e < 4; ++e) {
+          double sum = 0.0;
 #pragma unroll
-          for (int e = 0; e < 4; ++e) acc64[i][j][e] += (double)acc[L][i][j][e];
-          acc[L][i][j] = f4{0.f, 0.f, 0.f, 0.f};
+          for (int L = S - 1; L >= 0; --L) sum += (double)acc[L][i][j][e];
+          acc64[i][j][e] += ldexp(sum, ea[i][e] + eb[j]);
         }
+#pragma unroll
+    for (int L = 0; L < S; ++L)
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[L][i][j] = f4{0.f, 0.f, 0.f, 0.f};
   };
 
   load(0);
-  int stage = 0;
+  int stage = 0, chunk = 0;
   for (int64_t k0 = 0; k0 < g.Kp; k0 += kSBK) {
     __syncthreads();
 #pragma unroll
@@ -195,29 +226,27 @@ __global__ __launch_bounds__(256) void k_gemm_sliced(SlicedGemmArgs g,
                 fa[i][tm2], fb[L - i][tn2], acc[L][tm2][tn2], 0, 0, 0);
     if (++stage == kChunk) {
       stage = 0;
-      flush();
+      flush(chunk++);
     }
   }
-  flush();
 #pragma unroll
   for (int tm2 = 0; tm2 < 2; ++tm2)
 #pragma unroll
     for (int tn2 = 0; tn2 < 2; ++tn2) {
       const int n = n0 + wn * 32 + tn2 * 16 + (lane & 15);
       if (n >= g.N) continue;
-      const int en = g.eB[n];
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int m = m0 + wm * 32 + tm2 * 16 + 4 * (lane >> 4) + e;
-        if (m < g.M)
-          C[(int64_t)n * g.ldc + m] = ldexp(acc64[tm2][tn2][e], eA[m] + en);
+        if (m < g.M) C[(int64_t)n * g.ldc + m] = acc64[tm2][tn2][e];
       }
     }
 }
 
 void gemm_sliced_bf16(const SlicedGemmArgs& g, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0) return;
-  GAIN_CHECK_ARG(g.Kp % kSliceKPad == 0, "sliced gemm: Kp not padded");
+  GAIN_CHECK_ARG(g.Kp % 256 == 0 && g.nch == g.Kp / slice_chunk(g.S),
+                 "sliced gemm: Kp / chunk count mismatch");
   const int ntm = (g.M + kSBM - 1) / kSBM, ntn = (g.N + kSBN - 1) / kSBN;
   dim3 grid(ntn, ntm, g.nz);
   switch (g.S) {

@@ -192,8 +192,10 @@ def test_error_paths_return_codes():
         with pytest.raises(_lib.GainError) as e:
             eng.set_params(bad)
         assert e.value.code == _lib.ERR_ARG
-        with pytest.raises(_lib.GainError):
-            eng.batched_lines(0, 0, [0], 0, 0, precision=9)
+        lv = np.zeros(1, np.int32)
+        rc = L.gain_batched_lines(eng._h, 0, 0, lv.ctypes.data, 1, 0, 0, 9,
+                                  C.byref(out))
+        assert rc == _lib.ERR_ARG and b"precision" in L.gain_last_error()
 
 
 @pytest.mark.parametrize("mfma", [0, 1])
