@@ -525,6 +525,17 @@ struct Small {
   double* eI;
   int* levelI;
   uint8_t* okI;
+  int* vptr;  // per H vector: the projection vectors it feeds (CSR)
+  int* vidx;
+  int* vdeg;
+};
+
+// Per-call arrays of one plan, staged to the device.
+struct Staged {
+  Small sm{};
+  const double* cbase = nullptr;
+  int nDI = 0, nvpi = 0, nvecX = 0, nvecG = 0;
+  bool compute = false;
 };
 
 // Half line strengths of every (initial, dI) vector: scatter, mu GEMM and
@@ -643,28 +654,29 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
   return bF.nprim;
 }
 
-// Lines for nI initial states of one block against one final block.
-void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
-               int precision, bool plain, const gain_lines* out) {
-  const gain_params& p = c->params;
-  Basis& bF = *P.bF;
+Staged stage(gain_ctx* c, const Plan& P, const double* cI_host) {
   Block& blkI = *P.blkI;
-  Block& blkF = *P.blkF;
-  const int nI = P.nI, nF = blkF.n;
+  Staged st;
+  const int nI = P.nI;
   const int nDI = (int)P.DI.size();
   const int nvpi = (int)P.vec.size();
-  const int md2 = c->maxdeg * c->maxdeg;
-  const bool compute = P.Fhi > P.Flo && nvpi > 0 && nDI > 0;
   const int nvecX = nI * nDI;
   const int nvecG = nI * nvpi;
-
+  st.nDI = nDI;
+  st.nvpi = nvpi;
+  st.nvecX = nvecX;
+  st.nvecG = nvecG;
+  st.compute = P.Fhi > P.Flo && nvpi > 0 && nDI > 0;
   // ---- small per-call arrays through the pinned ring
   const size_t off_c = 0;
   const size_t off_ideg = off_c + sizeof(int64_t) * std::max(nvecX, 1);
   const size_t off_hv = off_ideg + sizeof(int) * std::max(nvecX, 1);
   const size_t off_dv = off_hv + sizeof(int) * std::max(nvecG, 1);
   const size_t off_sv = off_dv + sizeof(int) * std::max(nvecG, 1);
-  const size_t off_e = (off_sv + sizeof(int) * 16 + 7) / 8 * 8;
+  const size_t off_vp = off_sv + sizeof(int) * 16;
+  const size_t off_vi = off_vp + sizeof(int) * (nvecX + 2);
+  const size_t off_vd = off_vi + sizeof(int) * std::max(nvecG, 1);
+  const size_t off_e = (off_vd + sizeof(int) * std::max(nvecG, 1) + 7) / 8 * 8;
   const size_t off_l = off_e + sizeof(double) * nI;
   const size_t off_ok = off_l + sizeof(int) * nI;
   const size_t total = off_ok + nI + 16;
@@ -675,6 +687,9 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
   int* h_hv = (int*)(hs + off_hv);
   int* h_dv = (int*)(hs + off_dv);
   int* h_sv = (int*)(hs + off_sv);
+  int* h_vp = (int*)(hs + off_vp);
+  int* h_vi = (int*)(hs + off_vi);
+  int* h_vd = (int*)(hs + off_vd);
   for (int i = 0; i < nI; ++i)
     for (int k = 0; k < nDI; ++k) {
       h_c[i * nDI + k] = cI_host ? 0 : (int64_t)P.levels[i] * blkI.ld;
@@ -693,16 +708,30 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
     }
   for (int v = 0; v < nvpi; ++v)
     h_sv[v] = P.vec[v].first + P.vec[v].second * c->maxdeg;
+  // inverse of h_of_vec (for the adjoint): vectors fed by each H vector
+  int q = 0;
+  for (int hv = 0; hv < nvecX; ++hv) {
+    h_vp[hv] = q;
+    for (int v = 0; v < nvecG; ++v)
+      if (h_hv[v] == hv) {
+        h_vi[q] = v;
+        h_vd[q] = h_dv[v];
+        ++q;
+      }
+  }
+  h_vp[nvecX] = q;
   std::memcpy(hs + off_e, P.eI.data(), sizeof(double) * nI);
   std::memcpy(hs + off_l, P.levels.data(), sizeof(int) * nI);
   std::memcpy(hs + off_ok, P.okI.data(), nI);
   char* ds = c->w_small.get<char>(total);
   GAIN_HIP(hipMemcpyAsync(ds, hs, total, hipMemcpyHostToDevice, c->stream));
-  Small sm{(int64_t*)(ds + off_c), (int*)(ds + off_ideg),
-           (int*)(ds + off_hv),    (int*)(ds + off_dv),
-           (int*)(ds + off_sv),    (double*)(ds + off_e),
-           (int*)(ds + off_l),     (uint8_t*)(ds + off_ok)};
-  const double* cbase = blkI.vec;
+  st.sm = Small{(int64_t*)(ds + off_c), (int*)(ds + off_ideg),
+                (int*)(ds + off_hv),    (int*)(ds + off_dv),
+                (int*)(ds + off_sv),    (double*)(ds + off_e),
+                (int*)(ds + off_l),     (uint8_t*)(ds + off_ok),
+                (int*)(ds + off_vp),    (int*)(ds + off_vi),
+                (int*)(ds + off_vd)};
+  st.cbase = blkI.vec;
   int cslot = -1;
   if (cI_host) {
     const size_t nb = sizeof(double) * blkI.rec_len;
@@ -710,12 +739,28 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
     std::memcpy(hc, cI_host, nb);
     double* dc = c->w_c.get<double>(blkI.rec_len);
     GAIN_HIP(hipMemcpyAsync(dc, hc, nb, hipMemcpyHostToDevice, c->stream));
-    cbase = dc;
+    st.cbase = dc;
   }
   c->ring.release(slot, c->stream);
   if (cslot >= 0) c->ring.release(cslot, c->stream);
-  if (!cbase && compute)
+  if (!st.cbase && st.compute)
     fail(GAIN_ERR_STATE, "initial block has no eigenvectors and no cI given");
+  return st;
+}
+
+// Lines for nI initial states of one block against one final block.
+void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
+               int precision, bool plain, const gain_lines* out) {
+  const gain_params& p = c->params;
+  Basis& bF = *P.bF;
+  Block& blkF = *P.blkF;
+  const int nI = P.nI, nF = blkF.n;
+  const int md2 = c->maxdeg * c->maxdeg;
+  const Staged st = stage(c, P, cI_host);
+  const Small& sm = st.sm;
+  const double* cbase = st.cbase;
+  const int nvpi = st.nvpi, nvecX = st.nvecX, nvecG = st.nvecG;
+  const bool compute = st.compute;
 
   const int nR = P.Fhi - P.Flo;
   const Irrep& irF = bF.irrep[P.gF];
@@ -895,6 +940,185 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
   }
 }
 
+
+// Adjoint of the batched FP64 path: recomputes the forward, then
+//   dls = dS * dS/dls,  dG = C_F^T dls,  dH = P_F dG,  dY = gather^T dH,
+//   dmu_alpha = dY_alpha X^T  (a rank-k update, k = columns of X).
+void run_adjoint(gain_ctx* c, const Plan& P, const double* dS_dev,
+                 double* dmu_work) {
+  const gain_params& p = c->params;
+  Basis& bI = *P.bI;
+  Basis& bF = *P.bF;
+  Block& blkF = *P.blkF;
+  const int nI = P.nI, nF = blkF.n, nc = c->nc;
+  const Staged st = stage(c, P, nullptr);
+  const Small& sm = st.sm;
+  launch_fill_zero(dmu_work, (int64_t)3 * nc * nc, c->stream);
+  if (!st.compute) return;
+  const int nvpi = st.nvpi, nvecX = st.nvecX, nvecG = st.nvecG;
+  const int nR = P.Fhi - P.Flo;
+  const Irrep& irF = bF.irrep[P.gF];
+  const int ntotF = irF.ntot;
+  const int64_t gstride = (ntotF + 1) / 2 * 2;
+  const int64_t hstride = bF.nprim;
+  // ---- forward (FP64)
+  double* H = c->w_H.get<double>((size_t)(nvecX + 1) * hstride);
+  launch_fill_zero(H + (size_t)nvecX * hstride, hstride, c->stream);
+  run_half_ls(c, P, st.cbase, sm, nvecX, H, GAIN_PREC_FP64);
+  const double* X = (const double*)c->w_X.p;
+  double* G = c->w_G.get<double>((size_t)nvecG * gstride);
+  ExpandTArgs ea{};
+  ea.ntot = ntotF;
+  ea.nprim = bF.nprim;
+  ea.sdeg = irF.sdeg;
+  ea.tcase = irF.tcase;
+  ea.case_ptr = bF.case_ptr;
+  ea.case_prim = bF.case_prim;
+  ea.icontr = bF.icontr;
+  ea.repres = irF.repres;
+  ea.H = H;
+  ea.hstride = hstride;
+  ea.h_of_vec = sm.h_of_vec;
+  ea.ideg_of_vec = sm.ideg_of_vec;
+  ea.nvec = nvecG;
+  ea.G = G;
+  ea.gstride = gstride;
+  launch_expand_T(ea, c->stream);
+  const double* Crows = blkF.vec + (int64_t)P.Flo * blkF.ld;
+  // adjoint work buffers: LS and dLS share w_part, flags in w_out
+  double* LS = c->w_part.get<double>((size_t)2 * nR * nvecG);
+  double* dLS = LS + (size_t)nR * nvecG;
+  gemm_projection(GAIN_PREC_FP64, nR, nvecG, ntotF, Crows, blkF.ld, G,
+                  gstride, LS, nR, c->stream);
+  uint8_t* flags = c->w_out.get<uint8_t>((size_t)nI * nF);
+  LinesArgs la{};
+  la.nI = nI;
+  la.nF = nF;
+  la.Flo = P.Flo;
+  la.Fhi = P.Fhi;
+  la.maxdeg = c->maxdeg;
+  la.nvec_per_i = nvpi;
+  la.slot_of_vec = sm.slot_of_vec;
+  la.src = LS;
+  la.ksplit = 1;
+  la.src_f = 1;
+  la.src_v = nR;
+  la.eI = sm.eI;
+  la.levelI = sm.levelI;
+  la.okI = sm.okI;
+  la.eF = blkF.d_energy;
+  la.same_block = (P.jindI == P.jindF && P.gI == P.gF);
+  la.block_ok = P.block_ok;
+  la.jI = bI.J;
+  la.jF = bF.J;
+  la.ndegI = p.sym_degen[P.gI];
+  la.ndegF = p.sym_degen[P.gF];
+  la.reduced = p.reduced;
+  la.gnsF = p.gns[P.gF];
+  la.zpe = p.zpe;
+  la.threshold = p.threshold;
+  la.beta = kPlanck * kVellgt / (kBoltz * p.temperature);
+  la.partition = p.partition;
+  la.fw0 = p.freq_window[0];
+  la.fw1 = p.freq_window[1];
+  la.eu0 = p.erange_upper[0];
+  la.eu1 = p.erange_upper[1];
+  la.isym_do_F = p.isym_do[P.gF];
+  la.flags = flags;
+  launch_lines(la, c->stream);
+  // ---- backward
+  Timed t(c, "adjoint");
+  const int ndegI = p.sym_degen[P.gI], ndegF = p.sym_degen[P.gF];
+  DlsArgs da{};
+  da.nI = nI;
+  da.nF = nF;
+  da.Flo = P.Flo;
+  da.nR = nR;
+  da.nvpi = nvpi;
+  da.dS = dS_dev;
+  da.flags = flags;
+  da.ls = LS;
+  da.scale = (p.reduced && ndegF != 1 && ndegI != 1) ? 2.0
+                                                     : 2.0 / (double)ndegI;
+  da.dls = dLS;
+  launch_dls(da, c->stream);
+  // dG[v][t] = sum_f C[Flo+f][t] dLS[v][f]
+  double* dG = c->w_G.get<double>((size_t)nvecG * gstride);
+  {
+    GemmArgs g{};
+    g.M = ntotF;
+    g.N = nvecG;
+    g.K = nR;
+    g.A = Crows;
+    g.lda = blkF.ld;
+    g.a_kcontig = false;
+    g.B = dLS;
+    g.ldb = nR;
+    g.C = dG;
+    g.ldc = gstride;
+    g.nz = 1;
+    g.ksplit = 1;
+    gemm_f64(g, c->stream);
+  }
+  double* dH = c->w_H.get<double>((size_t)(nvecX + 1) * hstride);
+  {
+    ExpandAccArgs xa{};
+    xa.nprim = bF.nprim;
+    xa.ntot = ntotF;
+    xa.sdeg = irF.sdeg;
+    xa.icontr = bF.icontr;
+    xa.ijt = bF.ijterms + (int64_t)P.gF * bF.maxsymcoeffs;
+    xa.N = irF.N;
+    xa.repres = irF.repres;
+    xa.dG = dG;
+    xa.gstride = gstride;
+    xa.vptr = sm.vptr;
+    xa.vidx = sm.vidx;
+    xa.vdeg = sm.vdeg;
+    xa.nH = nvecX;
+    xa.dH = dH;
+    xa.hstride = hstride;
+    launch_expand_acc(xa, c->stream);
+  }
+  ensure_slots(c, bF);
+  const int ncolTot = nvecX * bI.ncol;
+  double* dY = c->w_Y.get<double>((size_t)3 * ncolTot * nc);
+  {
+    GatherTArgs ga{};
+    ga.nc = nc;
+    ga.jI = bI.J;
+    ga.jF = bF.J;
+    ga.jmax = c->jmax;
+    ga.ncolI = bI.ncol;
+    ga.ncolF = bF.ncol;
+    ga.nvec = nvecX;
+    ga.slot_ptrF = bF.slot_ptr;
+    ga.slot_primF = bF.slot_prim;
+    ga.threej = c->d_threej;
+    ga.dH = dH;
+    ga.hstride = hstride;
+    ga.dY = dY;
+    launch_gather_T(ga, c->stream);
+  }
+  double* XT = c->w_tmp.get<double>((size_t)ncolTot * nc);
+  launch_transpose(X, ncolTot, nc, XT, c->stream);
+  GemmArgs g{};
+  g.M = nc;
+  g.N = nc;
+  g.K = ncolTot;
+  g.A = dY;
+  g.lda = nc;
+  g.sAz = (int64_t)ncolTot * nc;
+  g.a_kcontig = false;
+  g.B = XT;
+  g.ldb = ncolTot;
+  g.C = dmu_work;
+  g.ldc = nc;
+  g.sCz = (int64_t)nc * nc;
+  g.nz = 3;
+  g.ksplit = 1;
+  gemm_f64(g, c->stream);
+}
 }  // namespace
 }  // namespace gain
 
@@ -1338,6 +1562,102 @@ extern "C" int gain_set_tuning(gain_ctx* c, const char* key, int value) {
       c->use_mfma = value != 0;
     } else {
       fail(GAIN_ERR_ARG, "unknown tuning key %s", key);
+    }
+  });
+}
+
+extern "C" int gain_adjoint_dipole(gain_ctx* c, int jindI, int gammaI,
+                                   const int* levelsI, int nI, int jindF,
+                                   int gammaF, const double* dS, double* dmu,
+                                   int on_device, int accumulate) {
+  return guarded([&] {
+    need(c);
+    GAIN_CHECK_ARG(levelsI && nI > 0 && dS && dmu, "bad adjoint arguments");
+    DeviceGuard g(c->device);
+    Plan P = make_plan(c, jindI, gammaI, levelsI, nI, jindF, gammaF);
+    if (!P.blkI->vec)
+      fail(GAIN_ERR_STATE, "adjoint needs the initial block's vectors "
+                           "resident");
+    const int nc = c->nc;
+    const size_t nF = P.blkF->n;
+    const size_t nmu = (size_t)3 * nc * nc;
+    const double* dS_dev = dS;
+    if (!on_device) {
+      double* d = c->w_c.get<double>((size_t)nI * nF);
+      GAIN_HIP(hipMemcpyAsync(d, dS, sizeof(double) * nI * nF,
+                              hipMemcpyHostToDevice, c->stream));
+      dS_dev = d;
+    }
+    double* work = c->w_xs.get<double>(nmu);
+    run_adjoint(c, P, dS_dev, work);
+    if (on_device) {
+      launch_axpy1(work, dmu, (int64_t)nmu, accumulate, c->stream);
+    } else {
+      std::vector<double> h(nmu);
+      GAIN_HIP(hipMemcpyAsync(h.data(), work, sizeof(double) * nmu,
+                              hipMemcpyDeviceToHost, c->stream));
+      GAIN_HIP(hipStreamSynchronize(c->stream));
+      for (size_t i = 0; i < nmu; ++i)
+        dmu[i] = accumulate ? dmu[i] + h[i] : h[i];
+    }
+    GAIN_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+extern "C" int gain_set_dipole_combination(gain_ctx* c, int P,
+                                           const double* coef,
+                                           const double* M_dev, int nc) {
+  return guarded([&] {
+    need(c);
+    GAIN_CHECK_ARG(P > 0 && coef && M_dev && nc > 0, "bad combination");
+    DeviceGuard g(c->device);
+    if (c->d_mu && c->nc != nc)
+      fail(GAIN_ERR_ARG, "dipole dimension %d differs from %d", nc, c->nc);
+    release_sliced(c, c->mu_sl);
+    if (!c->d_mu) {
+      c->d_mu = c->arena.alloc<double>((size_t)3 * nc * nc);
+      c->nc = nc;
+    }
+    double* dcoef = c->w_c.get<double>(P);
+    GAIN_HIP(hipMemcpyAsync(dcoef, coef, sizeof(double) * P,
+                            hipMemcpyHostToDevice, c->stream));
+    launch_combine(P, (int64_t)3 * nc * nc, dcoef, M_dev, c->d_mu,
+                   c->stream);
+    GAIN_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+extern "C" int gain_dipole_coeff_grad(gain_ctx* c, int P, const double* M_dev,
+                                      const double* dmu_dev, double* dc) {
+  return guarded([&] {
+    need(c);
+    GAIN_CHECK_ARG(P > 0 && M_dev && dmu_dev && dc, "bad arguments");
+    DeviceGuard g(c->device);
+    const int64_t n = (int64_t)3 * c->nc * c->nc;
+    GAIN_CHECK_ARG(n > 0 && n < (int64_t)1 << 31, "dipole too large");
+    const int ks = 256;
+    double* part = c->w_part.get<double>((size_t)ks * P);
+    GemvArgs a{};
+    a.C = M_dev;
+    a.ld = n;
+    a.ntot = (int)n;
+    a.F0 = 0;
+    a.nF = P;
+    a.G = dmu_dev;
+    a.gstride = n;
+    a.nvec = 1;
+    a.partial = part;
+    a.ksplit = ks;
+    a.rows = 4;
+    launch_gemv(a, c->stream);
+    std::vector<double> h((size_t)ks * P);
+    GAIN_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * ks * P,
+                            hipMemcpyDeviceToHost, c->stream));
+    GAIN_HIP(hipStreamSynchronize(c->stream));
+    for (int q = 0; q < P; ++q) {
+      double s = 0.0;
+      for (int k = 0; k < ks; ++k) s += h[(size_t)k * P + q];
+      dc[q] = s;
     }
   });
 }

@@ -467,3 +467,181 @@ void launch_gemm_tn_valu(int M, int N, int K, const double* A, int64_t lda,
 }
 
 }  // namespace gain
+
+namespace gain {
+
+// ---------------------------------------------------------------------------
+// Adjoint kernels.
+__global__ __launch_bounds__(256) void k_dls(DlsArgs a) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = blockIdx.y;
+  if (f >= a.nR) return;
+  const int i = v / a.nvpi;
+  const int F = a.Flo + f;
+  const int64_t o = (int64_t)i * a.nF + F;
+  const double g = (a.flags[o] & 1) ? a.dS[o] : 0.0;
+  const int64_t p = (int64_t)v * a.nR + f;
+  a.dls[p] = g * a.scale * a.ls[p];
+}
+
+void launch_dls(const DlsArgs& a, hipStream_t s) {
+  if (a.nR <= 0) return;
+  dim3 grid((a.nR + 255) / 256, a.nI * a.nvpi);
+  hipLaunchKernelGGL(k_dls, grid, dim3(256), 0, s, a);
+  check_launch("dls");
+}
+
+__global__ __launch_bounds__(256) void k_expand_acc(ExpandAccArgs a) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int hv = blockIdx.y;
+  if (r >= a.nprim) return;
+  const int icase = a.icontr[r];
+  const int lam = a.icontr[r + a.nprim];
+  const int t0 = a.ijt[icase];
+  const int ne = a.N[icase];
+  double acc = 0.0;
+  for (int q = a.vptr[hv]; q < a.vptr[hv + 1]; ++q) {
+    const double* g = a.dG + (int64_t)a.vidx[q] * a.gstride;
+    const double* rp = a.repres + (int64_t)a.vdeg[q] * a.ntot +
+                       (int64_t)lam * a.sdeg * a.ntot;
+    for (int k = 0; k < ne; ++k) acc = fma(g[t0 + k], rp[t0 + k], acc);
+  }
+  a.dH[(int64_t)hv * a.hstride + r] = acc;
+}
+
+void launch_expand_acc(const ExpandAccArgs& a, hipStream_t s) {
+  if (a.nprim <= 0 || a.nH <= 0) return;
+  dim3 grid((a.nprim + 255) / 256, a.nH);
+  hipLaunchKernelGGL(k_expand_acc, grid, dim3(256), 0, s, a);
+  check_launch("expand_acc");
+}
+
+// dY[alpha][b, 2KI+tI][v] = sum_{KF in KI-1..KI+1, tF} [link alpha]
+//   f3j * s * m * (-1)^{(KF mod 3) tF} * sum_{r' in slot(KF, tF, v)} dH[r']
+__global__ __launch_bounds__(256) void k_gather_T(GatherTArgs a) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  const int colI = blockIdx.y;  // 2 KI + tI
+  const int b = blockIdx.z;
+  if (v >= a.nc) return;
+  const int KI = colI >> 1, tI = colI & 1;
+  const int n1 = a.jmax + 1;
+  const double* dh = a.dH + (int64_t)b * a.hstride;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (KI <= a.jI) {
+    for (int KF = max(KI - 1, 0); KF <= min(KI + 1, a.jF); ++KF) {
+      const int dk = KF - KI;
+      const double f3j = a.threej[a.jI + KI * n1 + (a.jF - a.jI + 1) * n1 * n1 +
+                                  (dk + 1) * n1 * n1 * 3];
+      const double sq = (dk != 0 && KI * KF != 0) ? kSqrt2Ref : 1.0;
+      for (int tF = 0; tF < 2; ++tF) {
+        int alpha, m;
+        if (dk == 0) {
+          alpha = 2;
+          m = tF - tI;
+        } else if (tF != tI) {
+          alpha = 0;
+          m = (tF - tI) * dk;
+        } else {
+          alpha = 1;
+          m = -1;
+        }
+        if (m == 0) continue;
+        const int slot = (2 * KF + tF) * a.nc + v;
+        double sum = 0.0;
+        for (int q = a.slot_ptrF[slot]; q < a.slot_ptrF[slot + 1]; ++q)
+          sum += dh[a.slot_primF[q]];
+        const double sgn = (((KF % 3) * tF) & 1) ? -1.0 : 1.0;
+        acc[alpha] = fma(f3j * sq * (double)m * sgn, sum, acc[alpha]);
+      }
+    }
+  }
+  const int64_t col = (int64_t)b * a.ncolI + colI;
+  const int64_t zs = (int64_t)a.nvec * a.ncolI * a.nc;
+#pragma unroll
+  for (int al = 0; al < 3; ++al) a.dY[al * zs + col * a.nc + v] = acc[al];
+}
+
+void launch_gather_T(const GatherTArgs& a, hipStream_t s) {
+  if (a.nc <= 0 || a.nvec <= 0) return;
+  dim3 grid((a.nc + 255) / 256, a.ncolI, a.nvec);
+  hipLaunchKernelGGL(k_gather_T, grid, dim3(256), 0, s, a);
+  check_launch("gather_T");
+}
+
+__global__ __launch_bounds__(256) void k_transpose(const double* __restrict__ in,
+                                                   int rows, int cols,
+                                                   double* __restrict__ out) {
+  __shared__ double t[32][33];
+  const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int y = ty; y < 32; y += 8) {
+    const int r = r0 + y, c = c0 + tx;
+    t[y][tx] = (r < rows && c < cols) ? in[(int64_t)r * cols + c] : 0.0;
+  }
+  __syncthreads();
+  for (int y = ty; y < 32; y += 8) {
+    const int c = c0 + y, r = r0 + tx;
+    if (r < rows && c < cols) out[(int64_t)c * rows + r] = t[tx][y];
+  }
+}
+
+void launch_transpose(const double* in, int rows, int cols, double* out,
+                      hipStream_t s) {
+  if (rows <= 0 || cols <= 0) return;
+  dim3 grid((cols + 31) / 32, (rows + 31) / 32);
+  hipLaunchKernelGGL(k_transpose, grid, dim3(256), 0, s, in, rows, cols, out);
+  check_launch("transpose");
+}
+
+__global__ __launch_bounds__(256) void k_axpy1(const double* __restrict__ x,
+                                               double* __restrict__ y,
+                                               int64_t n, int accumulate) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = accumulate ? y[i] + x[i] : x[i];
+}
+
+void launch_axpy1(const double* x, double* y, int64_t n, int accumulate,
+                  hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_axpy1, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     s, x, y, n, accumulate);
+  check_launch("axpy1");
+}
+
+// mu[i] = sum_p c[p] M[p][i]: one pass over the P basis matrices, two
+// elements per thread with 16-byte loads.
+__global__ __launch_bounds__(256) void k_combine(int P, int64_t n,
+                                                 const double* __restrict__ c,
+                                                 const double* __restrict__ M,
+                                                 double* __restrict__ mu) {
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 2;
+  if (i >= n) return;
+  double a0 = 0.0, a1 = 0.0;
+  if (i + 1 < n && (n % 2 == 0)) {
+    for (int p = 0; p < P; ++p) {
+      const dbl2 m = __builtin_nontemporal_load((const dbl2*)(M + p * n + i));
+      a0 = fma(c[p], m.x, a0);
+      a1 = fma(c[p], m.y, a1);
+    }
+    mu[i] = a0;
+    mu[i + 1] = a1;
+  } else {
+    for (int p = 0; p < P; ++p) a0 = fma(c[p], M[p * n + i], a0);
+    mu[i] = a0;
+    if (i + 1 < n) {
+      for (int p = 0; p < P; ++p) a1 = fma(c[p], M[p * n + i + 1], a1);
+      mu[i + 1] = a1;
+    }
+  }
+}
+
+void launch_combine(int P, int64_t n, const double* c, const double* M,
+                    double* mu, hipStream_t s) {
+  if (n <= 0) return;
+  const int64_t threads = (n + 1) / 2;
+  hipLaunchKernelGGL(k_combine, dim3((unsigned)((threads + 255) / 256)),
+                     dim3(256), 0, s, P, n, c, M, mu);
+  check_launch("combine");
+}
+
+}  // namespace gain

@@ -127,3 +127,61 @@ void launch_gemm_tn_valu(int M, int N, int K, const double* A, int64_t lda,
                          int64_t ldc, hipStream_t s);
 
 }  // namespace gain
+
+namespace gain {
+
+// ---- adjoint (backward of the batched path) ------------------------------
+// dLS[v][F-Flo] = dS[i][F] * dS/dls for every pair that passes the filters
+// (S = sum ls^2 / ndegI, or sum ls^2 in the reduced degenerate case).
+struct DlsArgs {
+  int nI, nF, Flo, nR, nvpi;
+  const double* dS;       // [nI][nF]
+  const uint8_t* flags;   // [nI][nF]
+  const double* ls;       // [nvec][nR] forward projection
+  double scale;           // 2 / ndegI (or 2)
+  double* dls;            // [nvec][nR]
+};
+void launch_dls(const DlsArgs& a, hipStream_t s);
+
+// dH[hv][r] = sum_{v in list(hv)} (P_{dF(v)} dG_v)[r]   (expansion, batched)
+struct ExpandAccArgs {
+  int nprim, ntot, sdeg;
+  const int* icontr;
+  const int* ijt;         // ijterms + gamma * maxsymcoeffs
+  const int* N;
+  const double* repres;
+  const double* dG;
+  int64_t gstride;
+  const int* vptr;        // [nH+1] CSR of projection vectors per H vector
+  const int* vidx;        // vector index
+  const int* vdeg;        // final-side component of each listed vector
+  int nH;
+  double* dH;
+  int64_t hstride;
+};
+void launch_expand_acc(const ExpandAccArgs& a, hipStream_t s);
+
+// Transpose of the gather: dY[alpha][b*ncolI + 2K_I+tau_I][v] from dH over
+// the final primitives with vibrational index v (final-basis slot CSR).
+struct GatherTArgs {
+  int nc, jI, jF, jmax, ncolI, ncolF, nvec;
+  const int* slot_ptrF;   // [(2(J_F+1)) * nc + 1]
+  const int* slot_primF;
+  const double* threej;
+  const double* dH;
+  int64_t hstride;
+  double* dY;             // [3][nvec*ncolI][nc]
+};
+void launch_gather_T(const GatherTArgs& a, hipStream_t s);
+
+// out[c][r] = in[r][c] for an rows x cols FP64 matrix
+void launch_transpose(const double* in, int rows, int cols, double* out,
+                      hipStream_t s);
+// y[i] (+)= x[i]
+void launch_axpy1(const double* x, double* y, int64_t n, int accumulate,
+                  hipStream_t s);
+// mu[i] = sum_p c[p] M[p][i]
+void launch_combine(int P, int64_t n, const double* c, const double* M,
+                    double* mu, hipStream_t s);
+
+}  // namespace gain

@@ -25,7 +25,8 @@ EXPORTS = (
     "gain_batched_lines", "gain_set_timing", "gain_get_timing",
     "gain_copy_symmetry_constants", "gain_copy_jmax_constant",
     "gain_copy_dipole_constant", "gain_transform_vector_primitive",
-    "gain_compute_half_linestrength", "gain_ddot",
+    "gain_compute_half_linestrength", "gain_ddot", "gain_adjoint_dipole",
+    "gain_set_dipole_combination", "gain_dipole_coeff_grad",
 )
 
 
@@ -110,6 +111,10 @@ def load():
         "gain_compute_half_linestrength": (
             _ip, [_ip] * 5 + [_vp] * 4 + [_ip] * 6 + [_vp] * 5 + [_vp]),
         "gain_ddot": (_ip, [_ip, _vp, _vp, _vp, _vp]),
+        "gain_adjoint_dipole": (_ip, [_vp, _ip, _ip, _vp, _ip, _ip, _ip, _vp,
+                                      _vp, _ip, _ip]),
+        "gain_set_dipole_combination": (_ip, [_vp, _ip, _vp, _vp, _ip]),
+        "gain_dipole_coeff_grad": (_ip, [_vp, _ip, _vp, _vp, _vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

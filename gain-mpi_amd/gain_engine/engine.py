@@ -236,3 +236,40 @@ class Engine:
 
     def set_tuning(self, key: str, value: int):
         check(self._L.gain_set_tuning(self._h, key.encode(), int(value)))
+
+
+def _attach_adjoint():
+    """Adjoint / dipole-parameterisation methods of Engine."""
+
+    def adjoint(self, jindI, gammaI, levels, jindF, gammaF, dS, dmu,
+                on_device=True, accumulate=False):
+        """dL/dmu for one batch given dS = dL/dS ([nI][nF]); dS and dmu are
+        device pointers (ints) when on_device, else numpy arrays."""
+        lv = np.ascontiguousarray(levels, _i32)
+        if on_device:
+            p_dS, p_dmu = int(dS), int(dmu)
+        else:
+            dS = np.ascontiguousarray(dS, _f64)
+            p_dS, p_dmu = _ptr(dS), _ptr(dmu)
+        check(self._L.gain_adjoint_dipole(self._h, jindI, gammaI, _ptr(lv),
+                                          lv.shape[0], jindF, gammaF, p_dS,
+                                          p_dmu, int(on_device),
+                                          int(accumulate)))
+
+    def set_dipole_combination(self, coef, M_ptr: int, nc: int):
+        cf = np.ascontiguousarray(coef, _f64)
+        check(self._L.gain_set_dipole_combination(self._h, cf.shape[0],
+                                                  _ptr(cf), M_ptr, nc))
+
+    def dipole_coeff_grad(self, P: int, M_ptr: int, dmu_ptr: int):
+        dc = np.zeros(P)
+        check(self._L.gain_dipole_coeff_grad(self._h, P, M_ptr, dmu_ptr,
+                                             _ptr(dc)))
+        return dc
+
+    Engine.adjoint = adjoint
+    Engine.set_dipole_combination = set_dipole_combination
+    Engine.dipole_coeff_grad = dipole_coeff_grad
+
+
+_attach_adjoint()

@@ -162,6 +162,26 @@ int gain_batched_lines(gain_ctx* ctx, int jindI, int gammaI,
                        const int* levelsI, int nI, int jindF, int gammaF,
                        int precision, const gain_lines* out);
 
+/* ---- adjoint ------------------------------------------------------------- *
+ * No counterpart in the reference (it has no derivatives).  Gradient of a
+ * loss L with respect to the dipole, given dS = dL/dS for the pairs of a
+ * batch (same layout as gain_lines.S): recomputes the FP64 forward, then
+ * dls = dS dS/dls, dG = C_F^T dls, dH = P_F dG, dY = (gather)^T dH and
+ * dL/dmu_alpha = dY_alpha X^T (a rank-k update).  dmu has the dipole layout
+ * of gain_upload_dipole; `accumulate` adds into it.  dS and dmu are device
+ * memory if on_device, else host memory. */
+int gain_adjoint_dipole(gain_ctx* ctx, int jindI, int gammaI,
+                        const int* levelsI, int nI, int jindF, int gammaF,
+                        const double* dS, double* dmu, int on_device,
+                        int accumulate);
+/* mu = sum_{p<P} coef[p] M_p (coef host, M device [P][3][nc][nc]); sets the
+ * context's dipole. */
+int gain_set_dipole_combination(gain_ctx* ctx, int P, const double* coef,
+                                const double* M_device, int nc);
+/* dc[p] = <M_p, dmu> over all 3 nc^2 entries (M, dmu device; dc host). */
+int gain_dipole_coeff_grad(gain_ctx* ctx, int P, const double* M_device,
+                           const double* dmu_device, double* dc);
+
 /* ---- per-kernel timing (HIP events around each launch) ------------------ */
 int gain_set_timing(gain_ctx* ctx, int enable);
 /* Accumulated time (ms) and launch count of a named kernel family:
