@@ -52,6 +52,7 @@ def main():
     ap.add_argument("--nc", type=int, default=4000)
     ap.add_argument("--ninitial", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--copy-calib-gb", type=float, default=0.0,
                     help="also run a device-to-device copy of this many GB "
@@ -131,6 +132,8 @@ def main():
         f"({rows:.0f} rows); half_ls avg {hl_ms / max(hl_n, 1):.3f} ms")
 
     # ---------------- batched engine (headline)
+    if not args.no_autotune:
+        eng.set_tuning("autotune", 1)  # tile shapes chosen during warmup
     B = args.batch
     nsteps = args.steps
     outs = {k: torch.zeros((B, nF), dtype=torch.float64, device=dev)
@@ -188,6 +191,9 @@ def main():
             bflags[0].cpu().numpy().copy()
 
     head = run_batched(args.precision)
+    tune_log = eng.tuning_report()
+    if tune_log:
+        log("autotuner:\n" + tune_log.rstrip())
     variants = [run_batched(int(v)) for v in args.variants.split(",")
                 if v.strip() and int(v) != args.precision]
     # per-line error of each measured precision against FP64 on one batch
@@ -273,6 +279,7 @@ def main():
                      "traffic": None},
         "precision_check": prec_check,
         "variants": variants,
+        "autotune": tune_log.strip().splitlines(),
         "roofline_plain": {"bound": "hbm", "kernel": "gemv",
                            "achieved": gemv_gbs, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": gemv_gbs / HBM_PEAK_GBS,

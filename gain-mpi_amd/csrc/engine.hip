@@ -11,6 +11,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "common.hpp"
@@ -225,6 +226,11 @@ struct gain_ctx {
   // tuning
   int gemv_rows = 4, gemv_ksplit = 8, mu_ksplit = 0;  // 0: automatic
   bool use_mfma = true;
+  // GEMM tile autotuner: chosen configuration per (kind, M, N, K)
+  bool autotune = false;
+  int force_cfg[2] = {-1, -1};  // 0: projection, 1: half line strength
+  std::map<std::tuple<int, int, int, int>, int> tuned;
+  std::string tune_log;
   bool timing = false;
   std::map<std::string, TimerSlot> timers;
 };
@@ -540,6 +546,50 @@ struct Staged {
 
 // Half line strengths of every (initial, dI) vector: scatter, mu GEMM and
 // gather.  H rows: i * |DI| + k.  Returns the H stride.
+// Tile configuration for a GEMM of one kind and shape: forced, tuned (the
+// autotuner times every configuration on the real operands once per shape,
+// on the context's stream), or the default.
+int gemm_cfg(gain_ctx* c, int kind, GemmArgs g) {
+  if (c->force_cfg[kind] >= 0) return c->force_cfg[kind];
+  // large M only changes the number of row tiles: one choice per N, K
+  const auto key = std::make_tuple(kind, g.M >= 8192 ? -1 : g.M, g.N, g.K);
+  auto it = c->tuned.find(key);
+  if (it != c->tuned.end()) return it->second;
+  if (!c->autotune) return -1;
+  hipEvent_t e0, e1;
+  GAIN_HIP(hipEventCreate(&e0));
+  GAIN_HIP(hipEventCreate(&e1));
+  int best = -1;
+  float best_ms = 1e30f;
+  char line[160];
+  snprintf(line, sizeof(line), "%s M=%d N=%d K=%d:",
+           kind == 0 ? "projection" : "half_ls", g.M, g.N, g.K);
+  std::string rec = line;
+  for (int cfg = 0; cfg < kGemmConfigs; ++cfg) {
+    g.cfg = cfg;
+    gemm_f64(g, c->stream);  // warm
+    GAIN_HIP(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < 2; ++r) gemm_f64(g, c->stream);
+    GAIN_HIP(hipEventRecord(e1, c->stream));
+    GAIN_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    GAIN_HIP(hipEventElapsedTime(&ms, e0, e1));
+    ms *= 0.5f;
+    snprintf(line, sizeof(line), " %s=%.3fms", gemm_config_name(cfg), ms);
+    rec += line;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = cfg;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  rec += std::string(" -> ") + gemm_config_name(best) + "\n";
+  c->tune_log += rec;
+  c->tuned[key] = best;
+  return best;
+}
+
 int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
                     const Small& sm, int nvecX, double* H, int precision) {
   Basis& bI = *P.bI;
@@ -628,6 +678,7 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
     g.sCks = 3 * (int64_t)ncolTot * nc;
     g.nz = 3;
     g.ksplit = ks;
+    g.cfg = gemm_cfg(c, 1, g);
     gemm_f64(g, c->stream);
   } else {
     launch_mu_gemm_valu(c->d_mu, nc, X, ncolTot, Y, ks, c->stream);
@@ -843,7 +894,23 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
         g.ldc = nR;
         g.nz = 1;
         gemm_sliced_bf16(g, c->stream);
-      } else if (c->use_mfma || precision != GAIN_PREC_FP64)
+      } else if (c->use_mfma) {
+        GemmArgs g{};
+        g.M = nR;
+        g.N = nvecG;
+        g.K = ntotF;
+        g.A = Crows + (int64_t)P.Flo * blkF.ld;
+        g.lda = blkF.ld;
+        g.a_kcontig = true;
+        g.B = G;
+        g.ldb = gstride;
+        g.C = LS;
+        g.ldc = nR;
+        g.nz = 1;
+        g.ksplit = 1;
+        g.cfg = gemm_cfg(c, 0, g);
+        gemm_f64(g, c->stream);
+      } else if (precision != GAIN_PREC_FP64)
         gemm_projection(precision, nR, nvecG, ntotF,
                         Crows + (int64_t)P.Flo * blkF.ld, blkF.ld, G, gstride,
                         LS, nR, c->stream);
@@ -1560,6 +1627,13 @@ extern "C" int gain_set_tuning(gain_ctx* c, const char* key, int value) {
       c->mu_ksplit = value;
     } else if (k == "mfma") {
       c->use_mfma = value != 0;
+    } else if (k == "autotune") {
+      c->autotune = value != 0;
+      if (value == 2) c->tuned.clear();
+    } else if (k == "proj_cfg" || k == "half_ls_cfg") {
+      GAIN_CHECK_ARG(value >= -1 && value < kGemmConfigs, "cfg -1..%d",
+                     kGemmConfigs - 1);
+      c->force_cfg[k == "proj_cfg" ? 0 : 1] = value;
     } else {
       fail(GAIN_ERR_ARG, "unknown tuning key %s", key);
     }
@@ -1659,5 +1733,13 @@ extern "C" int gain_dipole_coeff_grad(gain_ctx* c, int P, const double* M_dev,
       for (int k = 0; k < ks; ++k) s += h[(size_t)k * P + q];
       dc[q] = s;
     }
+  });
+}
+
+extern "C" int gain_tuning_report(gain_ctx* c, char* buf, int len) {
+  return guarded([&] {
+    need(c);
+    GAIN_CHECK_ARG(buf && len > 0, "bad buffer");
+    snprintf(buf, len, "%s", c->tune_log.c_str());
   });
 }

@@ -1,6 +1,7 @@
 // FP64 GEMM on gfx950 matrix cores (v_mfma_f64_16x16x4_f64).
 //
-// Workgroup tile 64 (M) x BN (N), K step 16, 4 waves splitting N.  Operand
+// Tile shapes are template parameters chosen per GEMM shape by the engine's
+// autotuner (kCfgs).  Operand
 // fragments follow the instruction's lane layout: A(m,k) with m = lane%16,
 // k = lane/16; B(k,n) with n = lane%16, k = lane/16; accumulator element e
 // of lane l holds D(m = l/16 + 4e, n = l%16).  Global tiles are prefetched
@@ -19,7 +20,7 @@ namespace gain {
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef double dbl2 __attribute__((ext_vector_type(2)));
 
-constexpr int kBM = 64, kBK = 16, kPad = 1;
+constexpr int kPad = 1;
 
 // Bijective remap of the linear workgroup id so that each of the 8 XCDs
 // (dispatch is round-robin over XCDs) receives a contiguous range of tiles.
@@ -28,105 +29,118 @@ __device__ inline int xcd_remap(int lin, int total) {
   return (lin % 8) * (total / 8) + lin / 8;
 }
 
-template <int BN, bool AK>
+// Workgroup tile BM x BN, K step BK, 4 waves arranged WM x WN; each wave
+// owns (BM/WM) x (BN/WN) = MT x NT MFMA tiles of 16 x 16.
+template <int BM, int BN, int BK, int WM, int WN, bool AK>
 __global__ __launch_bounds__(256) void k_gemm_f64(GemmArgs g, int kchunk,
                                                   int ntn, bool vecA,
                                                   bool vecB) {
-  constexpr int NT = BN / 64;  // 16-wide column tiles per wave
-  __shared__ double As[kBM][kBK + kPad];
-  __shared__ double Bs[BN][kBK + kPad];
+  static_assert(WM * WN == 4, "4 waves");
+  constexpr int MT = BM / WM / 16, NT = BN / WN / 16;
+  constexpr int QA = BM * BK / 4 / 256, QB = BN * BK / 4 / 256;
+  static_assert(QA >= 1 && QB >= 1, "tile too small for 256 threads");
+  __shared__ double As[BM][BK + kPad];
+  __shared__ double Bs[BN][BK + kPad];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wmi = wave % WM, wni = wave / WM;
   // tiles: n fastest so neighbours share the A tile
   const int nxy = ntn * gridDim.y;
   const int t = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, nxy);
   const int tn = t % ntn, tm = t / ntn;
-  const int m0 = tm * kBM, n0 = tn * BN;
+  const int m0 = tm * BM, n0 = tn * BN;
   const int z = blockIdx.z % g.nz, ks = blockIdx.z / g.nz;
   const double* A = g.A + (int64_t)z * g.sAz;
   double* C = g.C + (int64_t)z * g.sCz + (int64_t)ks * g.sCks;
   const int kbeg = ks * kchunk;
   const int kend = min(g.K, kbeg + kchunk);
 
-  d4 acc[4][NT];
+  d4 acc[MT][NT];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < MT; ++i)
 #pragma unroll
     for (int j = 0; j < NT; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
-  double ra[4], rb[4 * NT];
-  // A load coordinates
-  const int ar = AK ? (tid >> 2) : ((tid & 15) * 4);  // m (AK) / m base (!AK)
-  const int ak = AK ? ((tid & 3) * 4) : (tid >> 4);   // k base / k
-  const int br = tid >> 2, bk = (tid & 3) * 4;
-
+  double ra[QA][4], rb[QB][4];
   auto load = [&](int k0) {
-    const bool fullk = k0 + kBK <= kend;
-    if (AK) {
-      const int m = m0 + ar;
-      const double* p = A + (int64_t)m * g.lda + k0 + ak;
-      if (vecA && fullk && m < g.M) {
-        const dbl2 v0 = *(const dbl2*)p, v1 = *(const dbl2*)(p + 2);
-        ra[0] = v0.x; ra[1] = v0.y; ra[2] = v1.x; ra[3] = v1.y;
-      } else {
+    const bool fullk = k0 + BK <= kend;
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-          ra[q] = (m < g.M && k0 + ak + q < kend) ? p[q] : 0.0;
-      }
-    } else {
-      const int k = k0 + ak;
-      const double* p = A + (int64_t)k * g.lda + m0 + ar;
-      if (vecA && k < kend && m0 + ar + 3 < g.M) {
-        const dbl2 v0 = *(const dbl2*)p, v1 = *(const dbl2*)(p + 2);
-        ra[0] = v0.x; ra[1] = v0.y; ra[2] = v1.x; ra[3] = v1.y;
-      } else {
+    for (int q = 0; q < QA; ++q) {
+      const int gi = tid + 256 * q;
+      if (AK) {  // 4 consecutive k of row m
+        const int m = m0 + gi / (BK / 4), k = k0 + (gi % (BK / 4)) * 4;
+        const double* p = A + (int64_t)m * g.lda + k;
+        if (vecA && fullk && m < g.M) {
+          const dbl2 v0 = *(const dbl2*)p, v1 = *(const dbl2*)(p + 2);
+          ra[q][0] = v0.x; ra[q][1] = v0.y; ra[q][2] = v1.x; ra[q][3] = v1.y;
+        } else {
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-          ra[q] = (k < kend && m0 + ar + q < g.M) ? p[q] : 0.0;
+          for (int e = 0; e < 4; ++e)
+            ra[q][e] = (m < g.M && k + e < kend) ? p[e] : 0.0;
+        }
+      } else {  // 4 consecutive m of row k
+        const int k = k0 + gi / (BM / 4), m = m0 + (gi % (BM / 4)) * 4;
+        const double* p = A + (int64_t)k * g.lda + m;
+        if (vecA && k < kend && m + 3 < g.M) {
+          const dbl2 v0 = *(const dbl2*)p, v1 = *(const dbl2*)(p + 2);
+          ra[q][0] = v0.x; ra[q][1] = v0.y; ra[q][2] = v1.x; ra[q][3] = v1.y;
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            ra[q][e] = (k < kend && m + e < g.M) ? p[e] : 0.0;
+        }
       }
     }
 #pragma unroll
-    for (int pp = 0; pp < NT; ++pp) {
-      const int n = n0 + pp * 64 + br;
-      const double* p = g.B + (int64_t)n * g.ldb + k0 + bk;
+    for (int q = 0; q < QB; ++q) {
+      const int gi = tid + 256 * q;
+      const int n = n0 + gi / (BK / 4), k = k0 + (gi % (BK / 4)) * 4;
+      const double* p = g.B + (int64_t)n * g.ldb + k;
       if (vecB && fullk && n < g.N) {
         const dbl2 v0 = *(const dbl2*)p, v1 = *(const dbl2*)(p + 2);
-        rb[pp * 4 + 0] = v0.x; rb[pp * 4 + 1] = v0.y;
-        rb[pp * 4 + 2] = v1.x; rb[pp * 4 + 3] = v1.y;
+        rb[q][0] = v0.x; rb[q][1] = v0.y; rb[q][2] = v1.x; rb[q][3] = v1.y;
       } else {
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-          rb[pp * 4 + q] = (n < g.N && k0 + bk + q < kend) ? p[q] : 0.0;
+        for (int e = 0; e < 4; ++e)
+          rb[q][e] = (n < g.N && k + e < kend) ? p[e] : 0.0;
       }
     }
   };
 
   if (kbeg < kend) load(kbeg);
-  for (int k0 = kbeg; k0 < kend; k0 += kBK) {
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (AK)
-        As[ar][ak + q] = ra[q];
-      else
-        As[ar + q][ak] = ra[q];
+    for (int q = 0; q < QA; ++q) {
+      const int gi = tid + 256 * q;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if (AK)
+          As[gi / (BK / 4)][(gi % (BK / 4)) * 4 + e] = ra[q][e];
+        else
+          As[(gi % (BM / 4)) * 4 + e][gi / (BM / 4)] = ra[q][e];
+      }
     }
 #pragma unroll
-    for (int pp = 0; pp < NT; ++pp)
+    for (int q = 0; q < QB; ++q) {
+      const int gi = tid + 256 * q;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) Bs[pp * 64 + br][bk + q] = rb[pp * 4 + q];
+      for (int e = 0; e < 4; ++e)
+        Bs[gi / (BK / 4)][(gi % (BK / 4)) * 4 + e] = rb[q][e];
+    }
     __syncthreads();
-    if (k0 + kBK < kend) load(k0 + kBK);
+    if (k0 + BK < kend) load(k0 + BK);
 #pragma unroll
-    for (int kk = 0; kk < 4; ++kk) {
+    for (int kk = 0; kk < BK / 4; ++kk) {
       const int kc = kk * 4 + (lane >> 4);
-      double a[4], b[NT];
+      double a[MT], b[NT];
 #pragma unroll
-      for (int mt = 0; mt < 4; ++mt) a[mt] = As[mt * 16 + (lane & 15)][kc];
+      for (int mt = 0; mt < MT; ++mt)
+        a[mt] = As[wmi * (BM / WM) + mt * 16 + (lane & 15)][kc];
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt)
-        b[nt] = Bs[wave * (BN / 4) + nt * 16 + (lane & 15)][kc];
+        b[nt] = Bs[wni * (BN / WN) + nt * 16 + (lane & 15)][kc];
 #pragma unroll
-      for (int mt = 0; mt < 4; ++mt)
+      for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt)
           acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(
@@ -134,50 +148,73 @@ __global__ __launch_bounds__(256) void k_gemm_f64(GemmArgs g, int kchunk,
     }
   }
 #pragma unroll
-  for (int mt = 0; mt < 4; ++mt)
+  for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
-      const int n = n0 + wave * (BN / 4) + nt * 16 + (lane & 15);
+      const int n = n0 + wni * (BN / WN) + nt * 16 + (lane & 15);
       if (n >= g.N) continue;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int m = m0 + mt * 16 + (lane >> 4) + 4 * e;
+        const int m = m0 + wmi * (BM / WM) + mt * 16 + (lane >> 4) + 4 * e;
         if (m < g.M) C[(int64_t)n * g.ldc + m] = acc[mt][nt][e];
       }
     }
 }
 
-void gemm_f64(const GemmArgs& g, hipStream_t s) {
-  if (g.M <= 0 || g.N <= 0) return;
-  GAIN_CHECK_ARG(g.nz >= 1 && g.ksplit >= 1, "gemm: bad nz/ksplit");
-  int kchunk = (g.K + g.ksplit - 1) / g.ksplit;
-  kchunk = (kchunk + kBK - 1) / kBK * kBK;
-  const bool vecA = (g.lda % 2 == 0) && ((uintptr_t)g.A % 16 == 0) &&
-                    (g.sAz % 2 == 0);
-  const bool vecB = (g.ldb % 2 == 0) && ((uintptr_t)g.B % 16 == 0);
-  const int ntm = (g.M + kBM - 1) / kBM;
-  const bool wide = g.N > 64;
-  const int BN = wide ? 128 : 64;
-  const int ntn = (g.N + BN - 1) / BN;
+struct TileCfg {
+  int bm, bn, bk;
+};
+// The tile shapes the autotuner chooses from (index = configuration id).
+constexpr TileCfg kCfgs[kGemmConfigs] = {
+    {64, 64, 16},  {64, 128, 16}, {128, 128, 16}, {64, 256, 16},
+    {128, 64, 16}, {64, 128, 32}, {128, 128, 32}};
+
+template <int BM, int BN, int BK, int WM, int WN>
+static void launch_cfg(const GemmArgs& g, hipStream_t s, int kchunk,
+                       bool vecA, bool vecB) {
+  const int ntm = (g.M + BM - 1) / BM, ntn = (g.N + BN - 1) / BN;
   dim3 grid(ntn, ntm, g.nz * g.ksplit);
   GAIN_CHECK_ARG((int64_t)grid.z < 65536 && (int64_t)ntm < 65536,
                  "gemm: grid too large");
-  if (wide) {
-    if (g.a_kcontig)
-      hipLaunchKernelGGL((k_gemm_f64<128, true>), grid, dim3(256), 0, s, g,
-                         kchunk, ntn, vecA, vecB);
-    else
-      hipLaunchKernelGGL((k_gemm_f64<128, false>), grid, dim3(256), 0, s, g,
-                         kchunk, ntn, vecA, vecB);
-  } else {
-    if (g.a_kcontig)
-      hipLaunchKernelGGL((k_gemm_f64<64, true>), grid, dim3(256), 0, s, g,
-                         kchunk, ntn, vecA, vecB);
-    else
-      hipLaunchKernelGGL((k_gemm_f64<64, false>), grid, dim3(256), 0, s, g,
-                         kchunk, ntn, vecA, vecB);
+  if (g.a_kcontig)
+    hipLaunchKernelGGL((k_gemm_f64<BM, BN, BK, WM, WN, true>), grid,
+                       dim3(256), 0, s, g, kchunk, ntn, vecA, vecB);
+  else
+    hipLaunchKernelGGL((k_gemm_f64<BM, BN, BK, WM, WN, false>), grid,
+                       dim3(256), 0, s, g, kchunk, ntn, vecA, vecB);
+}
+
+int gemm_default_config(int N) { return N > 64 ? 1 : 0; }
+
+void gemm_f64(const GemmArgs& g, hipStream_t s) {
+  if (g.M <= 0 || g.N <= 0) return;
+  GAIN_CHECK_ARG(g.nz >= 1 && g.ksplit >= 1, "gemm: bad nz/ksplit");
+  const int cfg = g.cfg < 0 ? gemm_default_config(g.N) : g.cfg;
+  GAIN_CHECK_ARG(cfg < kGemmConfigs, "gemm: bad configuration %d", cfg);
+  const int bk = kCfgs[cfg].bk;
+  int kchunk = (g.K + g.ksplit - 1) / g.ksplit;
+  kchunk = (kchunk + bk - 1) / bk * bk;
+  const bool vecA = (g.lda % 2 == 0) && ((uintptr_t)g.A % 16 == 0) &&
+                    (g.sAz % 2 == 0);
+  const bool vecB = (g.ldb % 2 == 0) && ((uintptr_t)g.B % 16 == 0);
+  switch (cfg) {
+    case 0: launch_cfg<64, 64, 16, 2, 2>(g, s, kchunk, vecA, vecB); break;
+    case 1: launch_cfg<64, 128, 16, 1, 4>(g, s, kchunk, vecA, vecB); break;
+    case 2: launch_cfg<128, 128, 16, 2, 2>(g, s, kchunk, vecA, vecB); break;
+    case 3: launch_cfg<64, 256, 16, 1, 4>(g, s, kchunk, vecA, vecB); break;
+    case 4: launch_cfg<128, 64, 16, 2, 2>(g, s, kchunk, vecA, vecB); break;
+    case 5: launch_cfg<64, 128, 32, 1, 4>(g, s, kchunk, vecA, vecB); break;
+    case 6: launch_cfg<128, 128, 32, 2, 2>(g, s, kchunk, vecA, vecB); break;
   }
   check_launch("gemm_f64");
+}
+
+const char* gemm_config_name(int cfg) {
+  static char buf[kGemmConfigs][32];
+  if (cfg < 0 || cfg >= kGemmConfigs) return "default";
+  snprintf(buf[cfg], sizeof(buf[cfg]), "%dx%dx%d", kCfgs[cfg].bm,
+           kCfgs[cfg].bn, kCfgs[cfg].bk);
+  return buf[cfg];
 }
 
 // ---------------------------------------------------------------------------

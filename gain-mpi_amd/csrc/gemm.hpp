@@ -26,9 +26,13 @@ struct GemmArgs {
   int64_t sCks;   // C stride between split-K partials
   int nz;         // number of z slices (e.g. the 3 dipole components)
   int ksplit;     // split-K partial count (partials are summed by the caller)
+  int cfg = -1;   // tile configuration (-1: default for the shape)
 };
 
+constexpr int kGemmConfigs = 7;
 void gemm_f64(const GemmArgs& g, hipStream_t s);
+int gemm_default_config(int N);
+const char* gemm_config_name(int cfg);
 
 // Projection GEMM of the batched engine at the requested precision:
 //   LS[n*ldc + m] = sum_k A[m*lda + k] * B[n*ldb + k]

@@ -27,6 +27,7 @@ EXPORTS = (
     "gain_copy_dipole_constant", "gain_transform_vector_primitive",
     "gain_compute_half_linestrength", "gain_ddot", "gain_adjoint_dipole",
     "gain_set_dipole_combination", "gain_dipole_coeff_grad",
+    "gain_set_tuning", "gain_tuning_report",
 )
 
 
@@ -104,6 +105,7 @@ def load():
         "gain_get_timing": (_ip, [_vp, C.c_char_p, C.POINTER(_dp),
                                   C.POINTER(C.c_int64), _ip]),
         "gain_set_tuning": (_ip, [_vp, C.c_char_p, _ip]),
+        "gain_tuning_report": (_ip, [_vp, C.c_char_p, _ip]),
         "gain_copy_symmetry_constants": (_ip, [_ip, _ip]),
         "gain_copy_jmax_constant": (_ip, [_ip]),
         "gain_copy_dipole_constant": (_ip, [_ip]),

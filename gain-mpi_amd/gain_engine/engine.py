@@ -237,6 +237,11 @@ class Engine:
     def set_tuning(self, key: str, value: int):
         check(self._L.gain_set_tuning(self._h, key.encode(), int(value)))
 
+    def tuning_report(self) -> str:
+        buf = C.create_string_buffer(1 << 16)
+        check(self._L.gain_tuning_report(self._h, buf, 1 << 16))
+        return buf.value.decode()
+
 
 def _attach_adjoint():
     """Adjoint / dipole-parameterisation methods of Engine."""

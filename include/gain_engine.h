@@ -190,6 +190,16 @@ int gain_set_timing(gain_ctx* ctx, int enable);
 int gain_get_timing(gain_ctx* ctx, const char* name, double* total_ms,
                     int64_t* launches, int reset);
 
+/* ---- tuning -------------------------------------------------------------- *
+ * Keys: "autotune" (1: time every GEMM tile configuration on the first call
+ * of each shape and keep the fastest; 2: forget earlier choices),
+ * "proj_cfg" / "half_ls_cfg" (force a configuration, -1 = automatic),
+ * "gemv_rows" (2/4/8), "gemv_ksplit", "mu_ksplit" (0 = automatic),
+ * "mfma" (0: vector-ALU GEMMs, for cross-checks). */
+int gain_set_tuning(gain_ctx* ctx, const char* key, int value);
+/* The autotuner's timings and choices, one line per GEMM shape. */
+int gain_tuning_report(gain_ctx* ctx, char* buf, int len);
+
 /* ---- kernel-level drop-ins for GPU/dipole_GPU.h ------------------------- *
  * Same argument meaning as the reference's free functions; pointers are
  * device memory, `stream` is a hipStream_t (NULL = default stream).

@@ -245,3 +245,35 @@ def test_batched_split_bf16_error_vs_fp64_oracle(precision, bound):
     print(f"precision {precision}: strong lines {ns} max rel err {es:.3e}; "
           f"weak lines {nw} max rel err {ew:.3e}")
     assert es <= bound
+
+
+@pytest.mark.parametrize("cfg", list(range(7)))
+def test_every_gemm_tile_configuration(cfg):
+    """Each tile shape the autotuner can pick, for both GEMM kinds, on
+    sizes that leave partial tiles in M, N and K."""
+    prob = synth.make_problem([4, 5], 20, "c2v4")
+    prob.params.erange_lower = (0.0, 1e9)
+    rows = prob.state_table()
+    initials = list(range(1, len(rows), 2))
+    tr, _ = orc.run(prob, initials)
+    with Engine.from_problem(prob) as eng:
+        eng.set_tuning("proj_cfg", cfg)
+        eng.set_tuning("half_ls_cfg", cfg)
+        got = engine_lines(eng, prob, initials, "batched")
+        eng.set_tuning("mu_ksplit", 1)
+        got1 = engine_lines(eng, prob, initials, "plain")
+    compare_lines(got, oracle_map(tr))
+    compare_lines(got1, oracle_map(tr))
+
+
+def test_autotuner_picks_and_reports():
+    prob = synth.make_problem([4, 5], 20, "c2v4")
+    rows = prob.state_table()
+    initials = list(range(0, len(rows), 2))
+    tr, _ = orc.run(prob, initials)
+    with Engine.from_problem(prob) as eng:
+        eng.set_tuning("autotune", 1)
+        got = engine_lines(eng, prob, initials, "batched")
+        rep = eng.tuning_report()
+    assert "projection" in rep and "half_ls" in rep and "->" in rep
+    compare_lines(got, oracle_map(tr))
