@@ -1,0 +1,291 @@
+// Engine internals shared by the C-ABI translation units (engine.hip,
+// sweep.hip): device buffers of one context, plans of one call, and the
+// helpers that build them.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "common.hpp"
+#include "gemm.hpp"
+#include "kernels.hpp"
+#include "ozaki.hpp"
+
+namespace gain {
+
+// ---------------------------------------------------------------------------
+struct DeviceArena {
+  std::vector<void*> ptrs;
+  template <class T>
+  T* alloc(size_t n) {
+    if (n == 0) n = 1;
+    void* p = nullptr;
+    if (hipMalloc(&p, n * sizeof(T)) != hipSuccess)
+      fail(GAIN_ERR_NOMEM, "hipMalloc of %zu bytes failed", n * sizeof(T));
+    ptrs.push_back(p);
+    return (T*)p;
+  }
+  template <class T>
+  T* upload(const T* h, size_t n) {
+    T* d = alloc<T>(n);
+    if (n) GAIN_HIP(hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+  }
+  void release(void* p) {
+    auto it = std::find(ptrs.begin(), ptrs.end(), p);
+    if (it != ptrs.end()) {
+      (void)hipFree(*it);
+      ptrs.erase(it);
+    }
+  }
+  ~DeviceArena() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+};
+
+// Grow-only device work buffer.
+struct Work {
+  void* p = nullptr;
+  size_t bytes = 0;
+  template <class T>
+  T* get(size_t n) {
+    const size_t need = std::max<size_t>(n, 1) * sizeof(T);
+    if (need > bytes) {
+      if (p) {
+        // earlier launches may still read the old buffer
+        (void)hipDeviceSynchronize();
+        (void)hipFree(p);
+      }
+      p = nullptr;
+      size_t nb = std::max(need, bytes * 3 / 2);
+      if (hipMalloc(&p, nb) != hipSuccess) {
+        bytes = 0;
+        fail(GAIN_ERR_NOMEM, "work buffer allocation of %zu bytes failed", nb);
+      }
+      bytes = nb;
+    }
+    return (T*)p;
+  }
+  ~Work() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+// Pinned host staging ring for per-call parameter arrays: a slot is reused
+// only after the copy that read it has executed.
+struct PinnedRing {
+  static constexpr int kSlots = 4;
+  void* h[kSlots] = {};
+  size_t bytes[kSlots] = {};
+  hipEvent_t ev[kSlots] = {};
+  bool used[kSlots] = {};
+  int next = 0;
+  void* acquire(size_t n, int* slot) {
+    const int s = next;
+    next = (next + 1) % kSlots;
+    if (used[s]) GAIN_HIP(hipEventSynchronize(ev[s]));
+    if (n > bytes[s]) {
+      if (h[s]) (void)hipHostFree(h[s]);
+      h[s] = nullptr;
+      size_t nb = std::max<size_t>(n, 1 << 16);
+      GAIN_HIP(hipHostMalloc(&h[s], nb, hipHostMallocDefault));
+      bytes[s] = nb;
+    }
+    if (!ev[s]) GAIN_HIP(hipEventCreateWithFlags(&ev[s], hipEventDisableTiming));
+    *slot = s;
+    return h[s];
+  }
+  void release(int s, hipStream_t st) {
+    GAIN_HIP(hipEventRecord(ev[s], st));
+    used[s] = true;
+  }
+  ~PinnedRing() {
+    for (int s = 0; s < kSlots; ++s) {
+      if (ev[s]) {
+        (void)hipEventSynchronize(ev[s]);
+        (void)hipEventDestroy(ev[s]);
+      }
+      if (h[s]) (void)hipHostFree(h[s]);
+    }
+  }
+};
+
+struct Irrep {
+  int ntot = 0, sdeg = 0;
+  int* N = nullptr;
+  double* repres = nullptr;
+  int* tcase = nullptr;
+};
+
+struct Basis {
+  bool present = false, has_infl = false;
+  int J = 0, nprim = 0, ncol = 0, nvib = 0;
+  std::vector<int> kstart, kblock, hk, htau, hvib;
+  int *kval = nullptr, *tau = nullptr, *vib = nullptr;
+  // scatter CSR over (col, v) slots, built for a given nc
+  int slot_nc = -1;
+  int *slot_ptr = nullptr, *slot_prim = nullptr;
+  // inflation
+  int maxsymcoeffs = 0, matsize = 0;
+  int *ijterms = nullptr, *icontr = nullptr;
+  int *case_ptr = nullptr, *case_prim = nullptr;
+  Irrep irrep[GAIN_MAX_IRREPS];
+};
+
+// Split-BF16 copy of a static operand (final records or the dipole).
+struct Sliced {
+  int S = 0;
+  int64_t Kp = 0;
+  uint16_t* planes = nullptr;
+  int* expo = nullptr;
+};
+
+struct Block {
+  Sliced sl;
+  int n = 0, rec_len = 0;
+  int64_t ld = 0;
+  const double* vec = nullptr;  // device, may be null (energies only)
+  bool owned = false;
+  std::vector<double> energy;
+  double* d_energy = nullptr;
+  bool sorted = true;
+};
+
+struct TimerSlot {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending, pool;
+  double total_ms = 0.0;
+  int64_t launches = 0;
+};
+
+}  // namespace gain
+
+// The opaque handle of the C ABI (global namespace, declared in
+// gain_engine.h).
+using namespace gain;
+
+struct gain_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int jmax = 0, nrepres = 0, maxdeg = 0;
+  std::vector<double> threej;
+  double* d_threej = nullptr;
+  gain_params params{};
+  int pair[GAIN_MAX_IRREPS] = {};
+  bool params_set = false;
+  double* d_mu = nullptr;
+  int nc = 0;
+  Sliced mu_sl;
+  std::vector<Basis> bases;
+  std::map<std::pair<int, int>, Block> blocks;
+  DeviceArena arena;
+  Work w_c, w_X, w_Y, w_H, w_G, w_part, w_small, w_out, w_tmp;
+  Work w_xs, w_xe, w_gs, w_ge;
+  PinnedRing ring;
+  // tuning
+  int gemv_rows = 4, gemv_ksplit = 8, mu_ksplit = 0;  // 0: automatic
+  bool use_mfma = true;
+  // GEMM tile autotuner: chosen configuration per (kind, M, N, K)
+  bool autotune = false;
+  int force_cfg[2] = {-1, -1};  // 0: projection, 1: half line strength
+  std::map<std::tuple<int, int, int, int>, int> tuned;
+  std::string tune_log;
+  bool timing = false;
+  std::map<std::string, TimerSlot> timers;
+};
+
+namespace gain {
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    GAIN_HIP(hipGetDevice(&prev));
+    if (prev != dev) GAIN_HIP(hipSetDevice(dev));
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+// RAII event pair around the launches of one kernel family.
+struct Timed {
+  gain_ctx* c;
+  TimerSlot* slot = nullptr;
+  std::pair<hipEvent_t, hipEvent_t> ev{};
+  Timed(gain_ctx* ctx, const char* name) : c(ctx) {
+    if (!c->timing) return;
+    slot = &c->timers[name];
+    if (!slot->pool.empty()) {
+      ev = slot->pool.back();
+      slot->pool.pop_back();
+    } else {
+      GAIN_HIP(hipEventCreate(&ev.first));
+      GAIN_HIP(hipEventCreate(&ev.second));
+    }
+    GAIN_HIP(hipEventRecord(ev.first, c->stream));
+  }
+  ~Timed() {
+    if (!slot) return;
+    (void)hipEventRecord(ev.second, c->stream);
+    slot->pending.push_back(ev);
+  }
+};
+
+struct Plan {
+  int jindI, gI, jindF, gF;
+  Basis* bI;
+  Basis* bF;
+  Block* blkI;
+  Block* blkF;
+  int nI;
+  std::vector<int> levels;
+  std::vector<double> eI;
+  std::vector<uint8_t> okI;
+  std::vector<int> DI;                   // initial components computed
+  std::vector<std::pair<int, int>> vec;  // (dI, dF) in Output.cpp order
+  bool block_ok = false;
+  int Flo = 0, Fhi = 0;
+};
+
+// Device scratch laid out from one buffer of small per-call arrays.
+struct Small {
+  int64_t* c_off;
+  int* ideg;
+  int* h_of_vec;
+  int* ideg_of_vec;
+  int* slot_of_vec;
+  double* eI;
+  int* levelI;
+  uint8_t* okI;
+  int* vptr;  // per H vector: the projection vectors it feeds (CSR)
+  int* vidx;
+  int* vdeg;
+};
+
+// Per-call arrays of one plan, staged to the device.
+struct Staged {
+  Small sm{};
+  const double* cbase = nullptr;
+  int nDI = 0, nvpi = 0, nvecX = 0, nvecG = 0;
+  bool compute = false;
+};
+
+// Half line strengths of every (initial, dI) vector: scatter, mu GEMM and
+// gather.  H rows: i * |DI| + k.  Returns the H stride.
+gain_ctx* need(gain_ctx* c);
+Basis& basis_of(gain_ctx* c, int jind, bool need_infl);
+Block& block_of(gain_ctx* c, int jind, int gamma);
+void ensure_slots(gain_ctx* c, Basis& b);
+int slices_of(int precision);
+bool degfilter(const gain_ctx* c, int gI, int gF, int dI, int dF);
+Plan make_plan(gain_ctx* c, int jindI, int gI, const int* levels, int nI,
+               int jindF, int gF);
+int gemm_cfg(gain_ctx* c, int kind, GemmArgs g);
+void resolve_timers(gain_ctx* c, TimerSlot& t);
+
+}  // namespace gain

@@ -29,31 +29,21 @@ __device__ inline int xcd_remap(int lin, int total) {
   return (lin % 8) * (total / 8) + lin / 8;
 }
 
-// Workgroup tile BM x BN, K step BK, 4 waves arranged WM x WN; each wave
-// owns (BM/WM) x (BN/WN) = MT x NT MFMA tiles of 16 x 16.
+// One BM x BN output tile, K range [kbeg, kend), 4 waves arranged WM x WN;
+// each wave owns (BM/WM) x (BN/WN) = MT x NT MFMA tiles of 16 x 16.
 template <int BM, int BN, int BK, int WM, int WN, bool AK>
-__global__ __launch_bounds__(256) void k_gemm_f64(GemmArgs g, int kchunk,
-                                                  int ntn, bool vecA,
-                                                  bool vecB) {
+__device__ __forceinline__ void gemm_tile(const GemmArgs& g, const double* A,
+                                          double* C, int m0, int n0,
+                                          int kbeg, int kend, bool vecA,
+                                          bool vecB,
+                                          double (&As)[BM][BK + kPad],
+                                          double (&Bs)[BN][BK + kPad]) {
   static_assert(WM * WN == 4, "4 waves");
   constexpr int MT = BM / WM / 16, NT = BN / WN / 16;
   constexpr int QA = BM * BK / 4 / 256, QB = BN * BK / 4 / 256;
   static_assert(QA >= 1 && QB >= 1, "tile too small for 256 threads");
-  __shared__ double As[BM][BK + kPad];
-  __shared__ double Bs[BN][BK + kPad];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wmi = wave % WM, wni = wave / WM;
-  // tiles: n fastest so neighbours share the A tile
-  const int nxy = ntn * gridDim.y;
-  const int t = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, nxy);
-  const int tn = t % ntn, tm = t / ntn;
-  const int m0 = tm * BM, n0 = tn * BN;
-  const int z = blockIdx.z % g.nz, ks = blockIdx.z / g.nz;
-  const double* A = g.A + (int64_t)z * g.sAz;
-  double* C = g.C + (int64_t)z * g.sCz + (int64_t)ks * g.sCks;
-  const int kbeg = ks * kchunk;
-  const int kend = min(g.K, kbeg + kchunk);
-
   d4 acc[MT][NT];
 #pragma unroll
   for (int i = 0; i < MT; ++i)
@@ -161,13 +151,55 @@ __global__ __launch_bounds__(256) void k_gemm_f64(GemmArgs g, int kchunk,
     }
 }
 
+template <int BM, int BN, int BK, int WM, int WN, bool AK>
+__global__ __launch_bounds__(256) void k_gemm_f64(GemmArgs g, int kchunk,
+                                                  int ntn, bool vecA,
+                                                  bool vecB) {
+  __shared__ double As[BM][BK + kPad];
+  __shared__ double Bs[BN][BK + kPad];
+  // tiles: n fastest so neighbours share the A tile
+  const int nxy = ntn * gridDim.y;
+  const int t = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, nxy);
+  const int tn = t % ntn, tm = t / ntn;
+  const int z = blockIdx.z % g.nz, ks = blockIdx.z / g.nz;
+  const double* A = g.A + (int64_t)z * g.sAz;
+  double* C = g.C + (int64_t)z * g.sCz + (int64_t)ks * g.sCks;
+  const int kbeg = ks * kchunk;
+  const int kend = min(g.K, kbeg + kchunk);
+  gemm_tile<BM, BN, BK, WM, WN, AK>(g, A, C, tm * BM, tn * BN, kbeg, kend,
+                                    vecA, vecB, As, Bs);
+}
+
+// Persistent launch over the tiles of several GEMM problems (one workgroup
+// walks the tile list with a fixed stride, so the assignment is static and
+// the launch needs no counters).
+template <int BM, int BN, int BK, int WM, int WN, bool AK>
+__global__ __launch_bounds__(256) void k_gemm_f64_grouped(
+    const GemmArgs* __restrict__ probs, const GroupTile* __restrict__ tiles,
+    int ntiles) {
+  __shared__ double As[BM][BK + kPad];
+  __shared__ double Bs[BN][BK + kPad];
+  for (int it = blockIdx.x; it < ntiles; it += gridDim.x) {
+    const GroupTile tl = tiles[it];
+    const GemmArgs& g = probs[tl.prob];
+    const double* A = g.A + (int64_t)tl.z * g.sAz;
+    double* C = g.C + (int64_t)tl.z * g.sCz;
+    const bool vecA = (g.lda % 2 == 0) && ((uintptr_t)A % 16 == 0);
+    const bool vecB = (g.ldb % 2 == 0) && ((uintptr_t)g.B % 16 == 0);
+    __syncthreads();  // LDS of the previous tile fully consumed
+    gemm_tile<BM, BN, BK, WM, WN, AK>(g, A, C, tl.tm * BM, tl.tn * BN, 0,
+                                      g.K, vecA, vecB, As, Bs);
+  }
+}
+
 struct TileCfg {
   int bm, bn, bk;
 };
 // The tile shapes the autotuner chooses from (index = configuration id).
 constexpr TileCfg kCfgs[kGemmConfigs] = {
-    {64, 64, 16},  {64, 128, 16}, {128, 128, 16}, {64, 256, 16},
-    {128, 64, 16}, {64, 128, 32}, {128, 128, 32}};
+    {64, 64, 16},   {64, 128, 16},  {128, 128, 16}, {64, 256, 16},
+    {128, 64, 16},  {64, 128, 32},  {128, 128, 32}, {128, 256, 16},
+    {256, 128, 16}, {128, 256, 32}};
 
 template <int BM, int BN, int BK, int WM, int WN>
 static void launch_cfg(const GemmArgs& g, hipStream_t s, int kchunk,
@@ -205,9 +237,47 @@ void gemm_f64(const GemmArgs& g, hipStream_t s) {
     case 4: launch_cfg<128, 64, 16, 2, 2>(g, s, kchunk, vecA, vecB); break;
     case 5: launch_cfg<64, 128, 32, 1, 4>(g, s, kchunk, vecA, vecB); break;
     case 6: launch_cfg<128, 128, 32, 2, 2>(g, s, kchunk, vecA, vecB); break;
+    case 7: launch_cfg<128, 256, 16, 2, 2>(g, s, kchunk, vecA, vecB); break;
+    case 8: launch_cfg<256, 128, 16, 2, 2>(g, s, kchunk, vecA, vecB); break;
+    case 9: launch_cfg<128, 256, 32, 2, 2>(g, s, kchunk, vecA, vecB); break;
   }
   check_launch("gemm_f64");
 }
+
+template <int BM, int BN, int BK, int WM, int WN>
+static void launch_grouped_cfg(const GemmArgs* probs, const GroupTile* tiles,
+                               int ntiles, bool ak, int grid, hipStream_t s) {
+  if (ak)
+    hipLaunchKernelGGL((k_gemm_f64_grouped<BM, BN, BK, WM, WN, true>),
+                       dim3(grid), dim3(256), 0, s, probs, tiles, ntiles);
+  else
+    hipLaunchKernelGGL((k_gemm_f64_grouped<BM, BN, BK, WM, WN, false>),
+                       dim3(grid), dim3(256), 0, s, probs, tiles, ntiles);
+}
+
+void gemm_f64_grouped(const GemmArgs* probs, const GroupTile* tiles,
+                      int ntiles, bool a_kcontig, int cfg, int grid,
+                      hipStream_t s) {
+  if (ntiles <= 0) return;
+  GAIN_CHECK_ARG(cfg >= 0 && cfg < kGemmConfigs, "gemm: bad configuration");
+  grid = std::max(1, std::min(grid, ntiles));
+  switch (cfg) {
+    case 0: launch_grouped_cfg<64, 64, 16, 2, 2>(probs, tiles, ntiles, a_kcontig, grid, s); break;
+    case 1: launch_grouped_cfg<64, 128, 16, 1, 4>(probs, tiles, ntiles, a_kcontig, grid, s); break;
+    case 2: launch_grouped_cfg<128, 128, 16, 2, 2>(probs, tiles, ntiles, a_kcontig, grid, s); break;
+    case 3: launch_grouped_cfg<64, 256, 16, 1, 4>(probs, tiles, ntiles, a_kcontig, grid, s); break;
+    case 4: launch_grouped_cfg<128, 64, 16, 2, 2>(probs, tiles, ntiles, a_kcontig, grid, s); break;
+    case 5: launch_grouped_cfg<64, 128, 32, 1, 4>(probs, tiles, ntiles, a_kcontig, grid, s); break;
+    case 6: launch_grouped_cfg<128, 128, 32, 2, 2>(probs, tiles, ntiles, a_kcontig, grid, s); break;
+    case 7: launch_grouped_cfg<128, 256, 16, 2, 2>(probs, tiles, ntiles, a_kcontig, grid, s); break;
+    case 8: launch_grouped_cfg<256, 128, 16, 2, 2>(probs, tiles, ntiles, a_kcontig, grid, s); break;
+    case 9: launch_grouped_cfg<128, 256, 32, 2, 2>(probs, tiles, ntiles, a_kcontig, grid, s); break;
+  }
+  check_launch("gemm_f64_grouped");
+}
+
+int gemm_tile_m(int cfg) { return kCfgs[cfg].bm; }
+int gemm_tile_n(int cfg) { return kCfgs[cfg].bn; }
 
 const char* gemm_config_name(int cfg) {
   static char buf[kGemmConfigs][32];

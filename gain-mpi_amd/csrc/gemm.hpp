@@ -29,7 +29,20 @@ struct GemmArgs {
   int cfg = -1;   // tile configuration (-1: default for the shape)
 };
 
-constexpr int kGemmConfigs = 7;
+constexpr int kGemmConfigs = 10;
+
+// One output tile of one problem in a grouped (persistent) launch.
+struct GroupTile {
+  int prob, tm, tn, z;
+};
+// Tiles of several problems whose A operands share one layout, computed by
+// `grid` persistent workgroups; probs and tiles are device arrays.  Split-K
+// is not used (ksplit ignored).
+void gemm_f64_grouped(const GemmArgs* probs, const GroupTile* tiles,
+                      int ntiles, bool a_kcontig, int cfg, int grid,
+                      hipStream_t s);
+int gemm_tile_m(int cfg);
+int gemm_tile_n(int cfg);
 void gemm_f64(const GemmArgs& g, hipStream_t s);
 int gemm_default_config(int N);
 const char* gemm_config_name(int cfg);

@@ -1,0 +1,401 @@
+// Mixed-J sweeps: many (initial batch -> final block) tasks in one launch
+// sequence captured as a HIP graph.
+//
+// Tasks with the same initial batch (J index, irrep, levels) share X and
+// Y = mu X; both GEMM stages (mu X for every batch, C_F G for every task)
+// run as single persistent grouped launches over the tiles of all problems.
+// Everything a replay touches is owned by the sweep (sized at creation), so
+// replays stay valid whatever the context does in between.  The launch
+// sequence contains kernels only (no memset or memcpy nodes).
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "engine_internal.hpp"
+
+using namespace gain;
+
+struct gain_sweep {
+  gain_ctx* c = nullptr;
+  DeviceArena arena;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  struct Group {
+    Plan P;
+    int nvecX = 0, ncolTot = 0;
+    double* X = nullptr;
+    double* Y = nullptr;
+    int64_t* c_off = nullptr;
+    int* ideg = nullptr;
+  };
+  struct Task {
+    Plan P;
+    int group = 0;
+    int nvpi = 0, nvecG = 0, nR = 0;
+    int64_t gstride = 0, hstride = 0;
+    bool compute = false;
+    double* H = nullptr;
+    double* G = nullptr;
+    double* LS = nullptr;
+    int* h_of_vec = nullptr;
+    int* ideg_of_vec = nullptr;
+    int* slot_of_vec = nullptr;
+    double* eI = nullptr;
+    int* levelI = nullptr;
+    uint8_t* okI = nullptr;
+    gain_lines out{};
+  };
+  std::vector<Group> groups;
+  std::vector<Task> tasks;
+  GemmArgs* d_hl_probs = nullptr;
+  GroupTile* d_hl_tiles = nullptr;
+  int n_hl_tiles = 0, hl_cfg = 2;
+  GemmArgs* d_pj_probs = nullptr;
+  GroupTile* d_pj_tiles = nullptr;
+  int n_pj_tiles = 0, pj_cfg = 5;
+  int grid = 1024;
+};
+
+namespace {
+
+template <class T>
+T* up(gain_sweep* s, const std::vector<T>& v) {
+  return s->arena.upload(v.data(), v.size());
+}
+
+void add_tiles(std::vector<GroupTile>& tiles, int prob, int M, int N, int nz,
+               int cfg) {
+  const int bm = gemm_tile_m(cfg), bn = gemm_tile_n(cfg);
+  const int ntm = (M + bm - 1) / bm, ntn = (N + bn - 1) / bn;
+  for (int z = 0; z < nz; ++z)
+    for (int tm = 0; tm < ntm; ++tm)
+      for (int tn = 0; tn < ntn; ++tn) tiles.push_back({prob, tm, tn, z});
+}
+
+void build(gain_sweep* s, const gain_task* tasks, int ntasks) {
+  gain_ctx* c = s->c;
+  const gain_params& p = c->params;
+  const int nc = c->nc;
+  std::map<std::tuple<int, int, std::vector<int>>, int> gidx;
+  for (int t = 0; t < ntasks; ++t) {
+    const gain_task& tk = tasks[t];
+    GAIN_CHECK_ARG(tk.levelsI && tk.nI > 0, "task %d: no initial states", t);
+    GAIN_CHECK_ARG(tk.out.on_device, "task %d: outputs must be device "
+                   "memory (replays write them)", t);
+    gain_sweep::Task T;
+    T.P = make_plan(c, tk.jindI, tk.gammaI, tk.levelsI, tk.nI, tk.jindF,
+                    tk.gammaF);
+    if (!T.P.blkI->vec)
+      fail(GAIN_ERR_STATE, "task %d: initial block has no resident vectors",
+           t);
+    T.out = tk.out;
+    const auto key = std::make_tuple(tk.jindI, tk.gammaI, T.P.levels);
+    auto it = gidx.find(key);
+    if (it == gidx.end()) {
+      gain_sweep::Group G;
+      G.P = T.P;
+      gidx[key] = (int)s->groups.size();
+      s->groups.push_back(G);
+      it = gidx.find(key);
+    }
+    T.group = it->second;
+    // the d_I set depends on the final irrep only through the pair rule,
+    // which is the same for every final block of a valid task
+    if (T.P.DI != s->groups[T.group].P.DI)
+      fail(GAIN_ERR_ARG, "task %d: initial components differ within a batch",
+           t);
+    s->tasks.push_back(T);
+  }
+  // ---- groups: X, Y, scatter arrays
+  std::vector<GemmArgs> hl;
+  std::vector<GroupTile> hlt;
+  for (size_t g = 0; g < s->groups.size(); ++g) {
+    gain_sweep::Group& G = s->groups[g];
+    Basis& bI = *G.P.bI;
+    ensure_slots(c, bI);
+    const int nDI = (int)G.P.DI.size();
+    G.nvecX = G.P.nI * nDI;
+    G.ncolTot = G.nvecX * bI.ncol;
+    std::vector<int64_t> off;
+    std::vector<int> deg;
+    for (int i = 0; i < G.P.nI; ++i)
+      for (int k = 0; k < nDI; ++k) {
+        off.push_back((int64_t)G.P.levels[i] * G.P.blkI->ld);
+        deg.push_back(G.P.DI[k]);
+      }
+    if (G.nvecX == 0) continue;
+    G.c_off = up(s, off);
+    G.ideg = up(s, deg);
+    G.X = s->arena.alloc<double>((size_t)G.ncolTot * nc);
+    G.Y = s->arena.alloc<double>((size_t)3 * G.ncolTot * nc);
+    GemmArgs a{};
+    a.M = nc;
+    a.N = G.ncolTot;
+    a.K = nc;
+    a.A = c->d_mu;
+    a.lda = nc;
+    a.sAz = (int64_t)nc * nc;
+    a.a_kcontig = false;
+    a.B = G.X;
+    a.ldb = nc;
+    a.C = G.Y;
+    a.ldc = nc;
+    a.sCz = (int64_t)G.ncolTot * nc;
+    a.nz = 3;
+    a.ksplit = 1;
+    add_tiles(hlt, (int)hl.size(), a.M, a.N, 3, s->hl_cfg);
+    hl.push_back(a);
+  }
+  // ---- tasks: H, G, LS, per-pair arrays, projection problems
+  std::vector<GemmArgs> pj;
+  std::vector<GroupTile> pjt;
+  for (gain_sweep::Task& T : s->tasks) {
+    const gain_sweep::Group& G = s->groups[T.group];
+    Basis& bF = *T.P.bF;
+    const Irrep& irF = bF.irrep[T.P.gF];
+    const int nDI = (int)T.P.DI.size();
+    T.nvpi = (int)T.P.vec.size();
+    T.nvecG = T.P.nI * T.nvpi;
+    T.nR = T.P.Fhi - T.P.Flo;
+    T.compute = T.nR > 0 && T.nvpi > 0 && nDI > 0;
+    T.gstride = (irF.ntot + 1) / 2 * 2;
+    T.hstride = bF.nprim;
+    std::vector<int> hv, dv, sv;
+    for (int i = 0; i < T.P.nI; ++i)
+      for (int v = 0; v < T.nvpi; ++v) {
+        const int dI = T.P.vec[v].first;
+        int pos = -1;
+        for (int k = 0; k < nDI; ++k)
+          if (T.P.DI[k] == dI) pos = k;
+        hv.push_back(pos >= 0 ? i * nDI + pos : G.nvecX);
+        dv.push_back(T.P.vec[v].second);
+      }
+    for (int v = 0; v < T.nvpi; ++v)
+      sv.push_back(T.P.vec[v].first + T.P.vec[v].second * c->maxdeg);
+    sv.push_back(0);
+    T.slot_of_vec = up(s, sv);
+    T.eI = up(s, T.P.eI);
+    T.levelI = up(s, T.P.levels);
+    T.okI = up(s, T.P.okI);
+    if (!T.compute) continue;
+    T.h_of_vec = up(s, hv);
+    T.ideg_of_vec = up(s, dv);
+    T.H = s->arena.alloc<double>((size_t)(G.nvecX + 1) * T.hstride);
+    // the trailing zero vector is constant: cleared once, outside the graph
+    GAIN_HIP(hipMemset(T.H + (size_t)G.nvecX * T.hstride, 0,
+                       sizeof(double) * T.hstride));
+    T.G = s->arena.alloc<double>((size_t)T.nvecG * T.gstride);
+    T.LS = s->arena.alloc<double>((size_t)T.nvecG * T.nR);
+    GemmArgs a{};
+    a.M = T.nR;
+    a.N = T.nvecG;
+    a.K = irF.ntot;
+    a.A = T.P.blkF->vec + (int64_t)T.P.Flo * T.P.blkF->ld;
+    a.lda = T.P.blkF->ld;
+    a.a_kcontig = true;
+    a.B = T.G;
+    a.ldb = T.gstride;
+    a.C = T.LS;
+    a.ldc = T.nR;
+    a.nz = 1;
+    a.ksplit = 1;
+    add_tiles(pjt, (int)pj.size(), a.M, a.N, 1, s->pj_cfg);
+    pj.push_back(a);
+  }
+  (void)p;
+  if (!hl.empty()) {
+    s->d_hl_probs = up(s, hl);
+    s->d_hl_tiles = up(s, hlt);
+    s->n_hl_tiles = (int)hlt.size();
+  }
+  if (!pj.empty()) {
+    s->d_pj_probs = up(s, pj);
+    s->d_pj_tiles = up(s, pjt);
+    s->n_pj_tiles = (int)pjt.size();
+  }
+}
+
+// The launch sequence of one sweep (kernels only).
+void enqueue(gain_sweep* s, hipStream_t st) {
+  gain_ctx* c = s->c;
+  const gain_params& p = c->params;
+  const int nc = c->nc;
+  for (gain_sweep::Group& G : s->groups) {
+    if (G.nvecX == 0) continue;
+    Basis& bI = *G.P.bI;
+    const Irrep& ir = bI.irrep[G.P.gI];
+    ScatterArgs sa{};
+    sa.nslot = bI.ncol * nc;
+    sa.nprim = bI.nprim;
+    sa.slot_ptr = bI.slot_ptr;
+    sa.slot_prim = bI.slot_prim;
+    sa.kval = bI.kval;
+    sa.tau = bI.tau;
+    sa.icontr = bI.icontr;
+    sa.ijt = bI.ijterms + (int64_t)G.P.gI * bI.maxsymcoeffs;
+    sa.N = ir.N;
+    sa.repres = ir.repres;
+    sa.sdeg = ir.sdeg;
+    sa.ntot = ir.ntot;
+    sa.cbase = G.P.blkI->vec;
+    sa.c_off = G.c_off;
+    sa.ideg = G.ideg;
+    sa.nvec = G.nvecX;
+    sa.X = G.X;
+    sa.xstride = (int64_t)bI.ncol * nc;
+    launch_scatter_expand(sa, st);
+  }
+  gemm_f64_grouped(s->d_hl_probs, s->d_hl_tiles, s->n_hl_tiles, false,
+                   s->hl_cfg, s->grid, st);
+  for (gain_sweep::Task& T : s->tasks) {
+    if (!T.compute) continue;
+    const gain_sweep::Group& G = s->groups[T.group];
+    Basis& bF = *T.P.bF;
+    const Irrep& irF = bF.irrep[T.P.gF];
+    GatherArgs ga{};
+    ga.nprimF = bF.nprim;
+    ga.nc = nc;
+    ga.jI = T.P.bI->J;
+    ga.jF = bF.J;
+    ga.jmax = c->jmax;
+    ga.ncolI = T.P.bI->ncol;
+    ga.nvec = G.nvecX;
+    ga.ksplit = 1;
+    ga.kvalF = bF.kval;
+    ga.tauF = bF.tau;
+    ga.vibF = bF.vib;
+    ga.threej = c->d_threej;
+    ga.Y = G.Y;
+    ga.ys_alpha = (int64_t)G.ncolTot * nc;
+    ga.ys_ks = 3 * (int64_t)G.ncolTot * nc;
+    ga.H = T.H;
+    ga.hstride = T.hstride;
+    launch_gather_h(ga, st);
+    ExpandTArgs ea{};
+    ea.ntot = irF.ntot;
+    ea.nprim = bF.nprim;
+    ea.sdeg = irF.sdeg;
+    ea.tcase = irF.tcase;
+    ea.case_ptr = bF.case_ptr;
+    ea.case_prim = bF.case_prim;
+    ea.icontr = bF.icontr;
+    ea.repres = irF.repres;
+    ea.H = T.H;
+    ea.hstride = T.hstride;
+    ea.h_of_vec = T.h_of_vec;
+    ea.ideg_of_vec = T.ideg_of_vec;
+    ea.nvec = T.nvecG;
+    ea.G = T.G;
+    ea.gstride = T.gstride;
+    launch_expand_T(ea, st);
+  }
+  gemm_f64_grouped(s->d_pj_probs, s->d_pj_tiles, s->n_pj_tiles, true,
+                   s->pj_cfg, s->grid, st);
+  for (gain_sweep::Task& T : s->tasks) {
+    LinesArgs la{};
+    la.nI = T.P.nI;
+    la.nF = T.P.blkF->n;
+    la.Flo = T.compute ? T.P.Flo : 0;
+    la.Fhi = T.compute ? T.P.Fhi : 0;
+    la.maxdeg = c->maxdeg;
+    la.nvec_per_i = T.nvpi;
+    la.slot_of_vec = T.slot_of_vec;
+    la.src = T.LS;
+    la.ksplit = 1;
+    la.src_f = 1;
+    la.src_v = T.nR;
+    la.eI = T.eI;
+    la.levelI = T.levelI;
+    la.okI = T.okI;
+    la.eF = T.P.blkF->d_energy;
+    la.same_block = (T.P.jindI == T.P.jindF && T.P.gI == T.P.gF);
+    la.block_ok = T.P.block_ok;
+    la.jI = T.P.bI->J;
+    la.jF = T.P.bF->J;
+    la.ndegI = p.sym_degen[T.P.gI];
+    la.ndegF = p.sym_degen[T.P.gF];
+    la.reduced = p.reduced;
+    la.gnsF = p.gns[T.P.gF];
+    la.zpe = p.zpe;
+    la.threshold = p.threshold;
+    la.beta = kPlanck * kVellgt / (kBoltz * p.temperature);
+    la.partition = p.partition;
+    la.fw0 = p.freq_window[0];
+    la.fw1 = p.freq_window[1];
+    la.eu0 = p.erange_upper[0];
+    la.eu1 = p.erange_upper[1];
+    la.isym_do_F = p.isym_do[T.P.gF];
+    la.ls = T.out.ls;
+    la.S = T.out.S;
+    la.A = T.out.A;
+    la.I = T.out.I;
+    la.flags = T.out.flags;
+    launch_lines(la, st);
+  }
+}
+
+}  // namespace
+
+extern "C" int gain_sweep_create(gain_ctx* c, const gain_task* tasks,
+                                 int ntasks, int precision,
+                                 gain_sweep** out) {
+  gain_sweep* s = nullptr;
+  int rc = guarded([&] {
+    need(c);
+    GAIN_CHECK_ARG(out && tasks && ntasks > 0, "bad sweep arguments");
+    *out = nullptr;
+    GAIN_CHECK_ARG(precision == GAIN_PREC_FP64,
+                   "sweeps run the FP64 grouped GEMMs (precision %d)",
+                   precision);
+    if (!c->params_set) fail(GAIN_ERR_STATE, "gain_set_params not called");
+    if (!c->d_mu) fail(GAIN_ERR_STATE, "dipole not uploaded");
+    DeviceGuard g(c->device);
+    s = new gain_sweep();
+    s->c = c;
+    if (c->force_cfg[1] >= 0) s->hl_cfg = c->force_cfg[1];
+    if (c->force_cfg[0] >= 0) s->pj_cfg = c->force_cfg[0];
+    build(s, tasks, ntasks);
+    GAIN_HIP(hipStreamSynchronize(c->stream));
+    enqueue(s, c->stream);  // eager run: validates every launch
+    GAIN_HIP(hipStreamSynchronize(c->stream));
+    GAIN_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue(s, c->stream);
+    } catch (...) {
+      hipGraph_t dead = nullptr;
+      (void)hipStreamEndCapture(c->stream, &dead);
+      if (dead) (void)hipGraphDestroy(dead);
+      throw;
+    }
+    GAIN_HIP(hipStreamEndCapture(c->stream, &s->graph));
+    GAIN_HIP(hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0));
+    *out = s;
+  });
+  if (rc != GAIN_OK && s) {
+    if (s->exec) (void)hipGraphExecDestroy(s->exec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    delete s;
+  }
+  return rc;
+}
+
+extern "C" int gain_sweep_run(gain_sweep* s) {
+  return guarded([&] {
+    GAIN_CHECK_ARG(s && s->exec, "null sweep");
+    DeviceGuard g(s->c->device);
+    GAIN_HIP(hipGraphLaunch(s->exec, s->c->stream));
+  });
+}
+
+extern "C" int gain_sweep_destroy(gain_sweep* s) {
+  return guarded([&] {
+    if (!s) return;
+    DeviceGuard g(s->c->device);
+    (void)hipStreamSynchronize(s->c->stream);
+    if (s->exec) (void)hipGraphExecDestroy(s->exec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    delete s;
+  });
+}

@@ -27,7 +27,8 @@ EXPORTS = (
     "gain_copy_dipole_constant", "gain_transform_vector_primitive",
     "gain_compute_half_linestrength", "gain_ddot", "gain_adjoint_dipole",
     "gain_set_dipole_combination", "gain_dipole_coeff_grad",
-    "gain_set_tuning", "gain_tuning_report",
+    "gain_set_tuning", "gain_tuning_report", "gain_sweep_create",
+    "gain_sweep_run", "gain_sweep_destroy",
 )
 
 
@@ -55,6 +56,12 @@ class Lines(C.Structure):
     _fields_ = [("ls", C.c_void_p), ("S", C.c_void_p), ("A", C.c_void_p),
                 ("I", C.c_void_p), ("flags", C.c_void_p),
                 ("on_device", C.c_int)]
+
+
+class Task(C.Structure):
+    _fields_ = [("jindI", C.c_int), ("gammaI", C.c_int),
+                ("levelsI", C.c_void_p), ("nI", C.c_int),
+                ("jindF", C.c_int), ("gammaF", C.c_int), ("out", Lines)]
 
 
 _lib = None
@@ -106,6 +113,9 @@ def load():
                                   C.POINTER(C.c_int64), _ip]),
         "gain_set_tuning": (_ip, [_vp, C.c_char_p, _ip]),
         "gain_tuning_report": (_ip, [_vp, C.c_char_p, _ip]),
+        "gain_sweep_create": (_ip, [_vp, _vp, _ip, _ip, C.POINTER(_vp)]),
+        "gain_sweep_run": (_ip, [_vp]),
+        "gain_sweep_destroy": (_ip, [_vp]),
         "gain_copy_symmetry_constants": (_ip, [_ip, _ip]),
         "gain_copy_jmax_constant": (_ip, [_ip]),
         "gain_copy_dipole_constant": (_ip, [_ip]),

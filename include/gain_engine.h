@@ -162,6 +162,32 @@ int gain_batched_lines(gain_ctx* ctx, int jindI, int gammaI,
                        const int* levelsI, int nI, int jindF, int gammaF,
                        int precision, const gain_lines* out);
 
+/* ---- mixed-J sweeps (one HIP graph per sweep) ---------------------------- *
+ * A sweep is a fixed list of tasks, each an initial batch of one
+ * (J index, irrep) block against one final block (the work of many
+ * gain_batched_lines calls, e.g. every J_I -> J_I-1, J_I, J_I+1 pair of a J
+ * list).  At creation the engine plans every task, runs it once, and
+ * captures the launch sequence in a HIP graph: the half line strengths of
+ * all batches (tasks sharing an initial batch share them) and the
+ * projections of all tasks each run as one persistent grouped FP64 MFMA
+ * launch.  gain_sweep_run replays the graph on the context's stream
+ * (asynchronously; gain_synchronize waits).  Task outputs must be device
+ * memory.  Re-uploading the dipole, a basis or a block used by a sweep
+ * invalidates it.  No counterpart in the reference (its loop is
+ * main.cpp:238-338). */
+typedef struct {
+  int jindI, gammaI;
+  const int* levelsI;  /* host, nI levels of block (jindI, gammaI) */
+  int nI;
+  int jindF, gammaF;
+  gain_lines out;      /* [nI][nF] results, device memory */
+} gain_task;
+typedef struct gain_sweep gain_sweep;
+int gain_sweep_create(gain_ctx* ctx, const gain_task* tasks, int ntasks,
+                      int precision, gain_sweep** sweep);
+int gain_sweep_run(gain_sweep* sweep);
+int gain_sweep_destroy(gain_sweep* sweep);
+
 /* ---- adjoint ------------------------------------------------------------- *
  * No counterpart in the reference (it has no derivatives).  Gradient of a
  * loss L with respect to the dipole, given dS = dL/dS for the pairs of a
