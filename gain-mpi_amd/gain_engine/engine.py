@@ -243,45 +243,7 @@ class Engine:
         return buf.value.decode()
 
 
-def _attach_adjoint()
-
-
-class Sweep:
-    """A captured mixed-J sweep (gain_sweep_*): tasks are tuples
-    (jindI, gammaI, levels, jindF, gammaF, {name: device_ptr}) whose
-    outputs are device buffers written by every run()."""
-
-    def __init__(self, eng: Engine, tasks, precision=_lib.PREC_FP64):
-        self._eng = eng
-        self._keep = []
-        arr = (_lib.Task * len(tasks))()
-        for k, (jI, gI, levels, jF, gF, ptrs) in enumerate(tasks):
-            lv = np.ascontiguousarray(levels, _i32)
-            self._keep.append(lv)
-            t = arr[k]
-            t.jindI, t.gammaI, t.jindF, t.gammaF = jI, gI, jF, gF
-            t.levelsI, t.nI = _ptr(lv), lv.shape[0]
-            for name in ("ls", "S", "A", "I", "flags"):
-                setattr(t.out, name, ptrs.get(name))
-            t.out.on_device = 1
-        h = C.c_void_p()
-        check(eng._L.gain_sweep_create(eng._h, C.cast(arr, C.c_void_p),
-                                       len(tasks), precision, C.byref(h)))
-        self._h = h
-
-    def run(self):
-        check(self._eng._L.gain_sweep_run(self._h))
-
-    def close(self):
-        if self._h:
-            check(self._eng._L.gain_sweep_destroy(self._h))
-            self._h = None
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass:
+def _attach_adjoint():
     """Adjoint / dipole-parameterisation methods of Engine."""
 
     def adjoint(self, jindI, gammaI, levels, jindF, gammaF, dS, dmu,
@@ -326,6 +288,7 @@ class Sweep:
     def __init__(self, eng: Engine, tasks, precision=_lib.PREC_FP64):
         self._eng = eng
         self._keep = []
+        self._h = None
         arr = (_lib.Task * len(tasks))()
         for k, (jI, gI, levels, jF, gF, ptrs) in enumerate(tasks):
             lv = np.ascontiguousarray(levels, _i32)

@@ -41,3 +41,10 @@ def test_version_and_error_codes_without_gpu():
     assert L.gain_destroy(None) == _lib.OK
     assert L.gain_set_timing(None, 1) == _lib.ERR_ARG
     assert L.gain_create(None, 0, 1, 1, 1) == _lib.ERR_ARG
+
+
+def test_python_front_end_imports():
+    import importlib
+    for m in ("gain_engine.synth", "gain_engine.engine", "gain_engine.workload",
+              "gain_engine.autograd"):
+        importlib.import_module(m)
