@@ -31,17 +31,20 @@ __device__ inline int xcd_remap(int lin, int total) {
 
 // One BM x BN output tile, K range [kbeg, kend), 4 waves arranged WM x WN;
 // each wave owns (BM/WM) x (BN/WN) = MT x NT MFMA tiles of 16 x 16.
-template <int BM, int BN, int BK, int WM, int WN, bool AK>
+template <int BM, int BN, int BK, int WM, int WN, bool AK,
+          int TH = WM * WN * 64>
 __device__ __forceinline__ void gemm_tile(const GemmArgs& g, const double* A,
                                           double* C, int m0, int n0,
                                           int kbeg, int kend, bool vecA,
                                           bool vecB,
                                           double (&As)[BM][BK + kPad],
                                           double (&Bs)[BN][BK + kPad]) {
-  static_assert(WM * WN == 4, "4 waves");
+  static_assert(WM * WN * 64 == TH, "waves x 64 threads");
   constexpr int MT = BM / WM / 16, NT = BN / WN / 16;
-  constexpr int QA = BM * BK / 4 / 256, QB = BN * BK / 4 / 256;
-  static_assert(QA >= 1 && QB >= 1, "tile too small for 256 threads");
+  constexpr int QA = BM * BK / 4 / TH, QB = BN * BK / 4 / TH;
+  static_assert(QA >= 1 && QB >= 1, "tile too small for the thread count");
+  static_assert(QA * 4 * TH == BM * BK && QB * 4 * TH == BN * BK,
+                "tile not a multiple of the thread count");
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wmi = wave % WM, wni = wave / WM;
   d4 acc[MT][NT];
@@ -55,7 +58,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, const double* A,
     const bool fullk = k0 + BK <= kend;
 #pragma unroll
     for (int q = 0; q < QA; ++q) {
-      const int gi = tid + 256 * q;
+      const int gi = tid + TH * q;
       if (AK) {  // 4 consecutive k of row m
         const int m = m0 + gi / (BK / 4), k = k0 + (gi % (BK / 4)) * 4;
         const double* p = A + (int64_t)m * g.lda + k;
@@ -82,7 +85,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, const double* A,
     }
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
-      const int gi = tid + 256 * q;
+      const int gi = tid + TH * q;
       const int n = n0 + gi / (BK / 4), k = k0 + (gi % (BK / 4)) * 4;
       const double* p = g.B + (int64_t)n * g.ldb + k;
       if (vecB && fullk && n < g.N) {
@@ -101,7 +104,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, const double* A,
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < QA; ++q) {
-      const int gi = tid + 256 * q;
+      const int gi = tid + TH * q;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         if (AK)
@@ -112,7 +115,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, const double* A,
     }
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
-      const int gi = tid + 256 * q;
+      const int gi = tid + TH * q;
 #pragma unroll
       for (int e = 0; e < 4; ++e)
         Bs[gi / (BK / 4)][(gi % (BK / 4)) * 4 + e] = rb[q][e];
@@ -152,7 +155,8 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, const double* A,
 }
 
 template <int BM, int BN, int BK, int WM, int WN, bool AK>
-__global__ __launch_bounds__(256) void k_gemm_f64(GemmArgs g, int kchunk,
+__global__ __launch_bounds__(WM * WN * 64) void k_gemm_f64(GemmArgs g,
+                                                           int kchunk,
                                                   int ntn, bool vecA,
                                                   bool vecB) {
   __shared__ double As[BM][BK + kPad];
@@ -174,7 +178,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(GemmArgs g, int kchunk,
 // walks the tile list with a fixed stride, so the assignment is static and
 // the launch needs no counters).
 template <int BM, int BN, int BK, int WM, int WN, bool AK>
-__global__ __launch_bounds__(256) void k_gemm_f64_grouped(
+__global__ __launch_bounds__(WM * WN * 64) void k_gemm_f64_grouped(
     const GemmArgs* __restrict__ probs, const GroupTile* __restrict__ tiles,
     int ntiles) {
   __shared__ double As[BM][BK + kPad];
@@ -193,13 +197,15 @@ __global__ __launch_bounds__(256) void k_gemm_f64_grouped(
 }
 
 struct TileCfg {
-  int bm, bn, bk;
+  int bm, bn, bk, waves;
 };
 // The tile shapes the autotuner chooses from (index = configuration id).
 constexpr TileCfg kCfgs[kGemmConfigs] = {
-    {64, 64, 16},   {64, 128, 16},  {128, 128, 16}, {64, 256, 16},
-    {128, 64, 16},  {64, 128, 32},  {128, 128, 32}, {128, 256, 16},
-    {256, 128, 16}, {128, 256, 32}};
+    {64, 64, 16, 4},    {64, 128, 16, 4},   {128, 128, 16, 4},
+    {64, 256, 16, 4},   {128, 64, 16, 4},   {64, 128, 32, 4},
+    {128, 128, 32, 4},  {128, 256, 16, 4},  {256, 128, 16, 4},
+    {128, 256, 32, 4},  {128, 256, 16, 8},  {256, 128, 16, 8},
+    {128, 128, 16, 8},  {128, 256, 32, 8},  {64, 256, 32, 8}};
 
 template <int BM, int BN, int BK, int WM, int WN>
 static void launch_cfg(const GemmArgs& g, hipStream_t s, int kchunk,
@@ -210,10 +216,10 @@ static void launch_cfg(const GemmArgs& g, hipStream_t s, int kchunk,
                  "gemm: grid too large");
   if (g.a_kcontig)
     hipLaunchKernelGGL((k_gemm_f64<BM, BN, BK, WM, WN, true>), grid,
-                       dim3(256), 0, s, g, kchunk, ntn, vecA, vecB);
+                       dim3(WM * WN * 64), 0, s, g, kchunk, ntn, vecA, vecB);
   else
     hipLaunchKernelGGL((k_gemm_f64<BM, BN, BK, WM, WN, false>), grid,
-                       dim3(256), 0, s, g, kchunk, ntn, vecA, vecB);
+                       dim3(WM * WN * 64), 0, s, g, kchunk, ntn, vecA, vecB);
 }
 
 int gemm_default_config(int N) { return N > 64 ? 1 : 0; }
@@ -240,6 +246,11 @@ void gemm_f64(const GemmArgs& g, hipStream_t s) {
     case 7: launch_cfg<128, 256, 16, 2, 2>(g, s, kchunk, vecA, vecB); break;
     case 8: launch_cfg<256, 128, 16, 2, 2>(g, s, kchunk, vecA, vecB); break;
     case 9: launch_cfg<128, 256, 32, 2, 2>(g, s, kchunk, vecA, vecB); break;
+    case 10: launch_cfg<128, 256, 16, 2, 4>(g, s, kchunk, vecA, vecB); break;
+    case 11: launch_cfg<256, 128, 16, 4, 2>(g, s, kchunk, vecA, vecB); break;
+    case 12: launch_cfg<128, 128, 16, 2, 4>(g, s, kchunk, vecA, vecB); break;
+    case 13: launch_cfg<128, 256, 32, 2, 4>(g, s, kchunk, vecA, vecB); break;
+    case 14: launch_cfg<64, 256, 32, 1, 8>(g, s, kchunk, vecA, vecB); break;
   }
   check_launch("gemm_f64");
 }
@@ -249,10 +260,12 @@ static void launch_grouped_cfg(const GemmArgs* probs, const GroupTile* tiles,
                                int ntiles, bool ak, int grid, hipStream_t s) {
   if (ak)
     hipLaunchKernelGGL((k_gemm_f64_grouped<BM, BN, BK, WM, WN, true>),
-                       dim3(grid), dim3(256), 0, s, probs, tiles, ntiles);
+                       dim3(grid), dim3(WM * WN * 64), 0, s, probs, tiles,
+                       ntiles);
   else
     hipLaunchKernelGGL((k_gemm_f64_grouped<BM, BN, BK, WM, WN, false>),
-                       dim3(grid), dim3(256), 0, s, probs, tiles, ntiles);
+                       dim3(grid), dim3(WM * WN * 64), 0, s, probs, tiles,
+                       ntiles);
 }
 
 void gemm_f64_grouped(const GemmArgs* probs, const GroupTile* tiles,
@@ -272,6 +285,11 @@ void gemm_f64_grouped(const GemmArgs* probs, const GroupTile* tiles,
     case 7: launch_grouped_cfg<128, 256, 16, 2, 2>(probs, tiles, ntiles, a_kcontig, grid, s); break;
     case 8: launch_grouped_cfg<256, 128, 16, 2, 2>(probs, tiles, ntiles, a_kcontig, grid, s); break;
     case 9: launch_grouped_cfg<128, 256, 32, 2, 2>(probs, tiles, ntiles, a_kcontig, grid, s); break;
+    case 10: launch_grouped_cfg<128, 256, 16, 2, 4>(probs, tiles, ntiles, a_kcontig, grid, s); break;
+    case 11: launch_grouped_cfg<256, 128, 16, 4, 2>(probs, tiles, ntiles, a_kcontig, grid, s); break;
+    case 12: launch_grouped_cfg<128, 128, 16, 2, 4>(probs, tiles, ntiles, a_kcontig, grid, s); break;
+    case 13: launch_grouped_cfg<128, 256, 32, 2, 4>(probs, tiles, ntiles, a_kcontig, grid, s); break;
+    case 14: launch_grouped_cfg<64, 256, 32, 1, 8>(probs, tiles, ntiles, a_kcontig, grid, s); break;
   }
   check_launch("gemm_f64_grouped");
 }
@@ -282,8 +300,8 @@ int gemm_tile_n(int cfg) { return kCfgs[cfg].bn; }
 const char* gemm_config_name(int cfg) {
   static char buf[kGemmConfigs][32];
   if (cfg < 0 || cfg >= kGemmConfigs) return "default";
-  snprintf(buf[cfg], sizeof(buf[cfg]), "%dx%dx%d", kCfgs[cfg].bm,
-           kCfgs[cfg].bn, kCfgs[cfg].bk);
+  snprintf(buf[cfg], sizeof(buf[cfg]), "%dx%dx%d/%dw", kCfgs[cfg].bm,
+           kCfgs[cfg].bn, kCfgs[cfg].bk, kCfgs[cfg].waves);
   return buf[cfg];
 }
 

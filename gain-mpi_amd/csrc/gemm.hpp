@@ -29,7 +29,7 @@ struct GemmArgs {
   int cfg = -1;   // tile configuration (-1: default for the shape)
 };
 
-constexpr int kGemmConfigs = 10;
+constexpr int kGemmConfigs = 15;
 
 // One output tile of one problem in a grouped (persistent) launch.
 struct GroupTile {

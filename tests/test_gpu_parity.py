@@ -247,7 +247,7 @@ def test_batched_split_bf16_error_vs_fp64_oracle(precision, bound):
     assert es <= bound
 
 
-@pytest.mark.parametrize("cfg", list(range(10)))
+@pytest.mark.parametrize("cfg", list(range(15)))
 def test_every_gemm_tile_configuration(cfg):
     """Each tile shape the autotuner can pick, for both GEMM kinds, on
     sizes that leave partial tiles in M, N and K."""
