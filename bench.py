@@ -81,14 +81,23 @@ def main():
     ev0, ev1 = torch.cuda.Event(enable_timing=True), \
         torch.cuda.Event(enable_timing=True)
 
+    copy_gbs = None
     if args.copy_calib_gb > 0:
         n = int(args.copy_calib_gb * 1e9 / 8)
         src = torch.ones(n, dtype=torch.float64, device=dev)
         dst = torch.empty_like(src)
-        for _ in range(2):
-            dst.copy_(src)
+        dst.copy_(src)
         torch.cuda.synchronize()
-        log(f"copy calibration: {n * 8 / 1e9:.2f} GB read + written")
+        c0, c1 = torch.cuda.Event(enable_timing=True), \
+            torch.cuda.Event(enable_timing=True)
+        c0.record()
+        for _ in range(3):
+            dst.copy_(src)
+        c1.record()
+        torch.cuda.synchronize()
+        copy_gbs = 3 * 2 * n * 8 / (c0.elapsed_time(c1) / 1e3) / 1e9
+        log(f"copy calibration: {n * 8 / 1e9:.2f} GB read + written, "
+            f"{copy_gbs:.0f} GB/s (read + write)")
         del src, dst
         torch.cuda.empty_cache()
 
@@ -102,6 +111,8 @@ def main():
         return {k: v.data_ptr() for k, v in pl.items()} | \
             {"flags": pflags[i].data_ptr()}
 
+    if not args.no_autotune:
+        eng.set_tuning("autotune", 1)  # GEMV rows/split chosen in warmup
     for w in range(max(args.warmup, 1)):
         eng.lines_device(0, 0, [w], 1, 1, ptrs(0), plain=True)
     eng.synchronize()
@@ -131,9 +142,7 @@ def main():
         f"gemv avg {gemv_avg:.3f} ms = {gemv_gbs:.0f} GB/s "
         f"({rows:.0f} rows); half_ls avg {hl_ms / max(hl_n, 1):.3f} ms")
 
-    # ---------------- batched engine (headline)
-    if not args.no_autotune:
-        eng.set_tuning("autotune", 1)  # tile shapes chosen during warmup
+    # ---------------- batched engine (headline); GEMM tiles chosen in warmup
     B = args.batch
     nsteps = args.steps
     outs = {k: torch.zeros((B, nF), dtype=torch.float64, device=dev)
@@ -283,6 +292,7 @@ def main():
         "roofline_plain": {"bound": "hbm", "kernel": "gemv",
                            "achieved": gemv_gbs, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": gemv_gbs / HBM_PEAK_GBS,
+                           "device_copy_gbs": copy_gbs,
                            "traffic": None},
         "cpu_baseline": cpu,
     }

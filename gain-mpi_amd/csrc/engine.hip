@@ -539,6 +539,62 @@ Staged stage(gain_ctx* c, const Plan& P, const double* cI_host) {
   return st;
 }
 
+// GEMV autotuner: rows per wave x split-K count, timed on the real final
+// block once per (row count bucket, record length, vector count).
+void gemv_tune(gain_ctx* c, const double* C, int64_t ld, int ntot, int F0,
+               int nR, const double* G, int64_t gstride, int nvec) {
+  const auto key = std::make_tuple(2, nR >= 8192 ? -1 : nR, nvec, ntot);
+  if (c->tuned.count(key)) return;
+  const int rows_opt[3] = {2, 4, 8};
+  const int ks_opt[5] = {2, 4, 8, 16, 32};
+  hipEvent_t e0, e1;
+  GAIN_HIP(hipEventCreate(&e0));
+  GAIN_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  int br = c->gemv_rows, bk = c->gemv_ksplit;
+  char line[96];
+  snprintf(line, sizeof(line), "gemv rows=%d N=%d nvec=%d:", nR, ntot, nvec);
+  std::string rec = line;
+  for (int r : rows_opt)
+    for (int k : ks_opt) {
+      double* part = c->w_part.get<double>((size_t)k * nR * nvec);
+      GemvArgs a{};
+      a.C = C;
+      a.ld = ld;
+      a.ntot = ntot;
+      a.F0 = F0;
+      a.nF = nR;
+      a.G = G;
+      a.gstride = gstride;
+      a.nvec = nvec;
+      a.partial = part;
+      a.ksplit = k;
+      a.rows = r;
+      launch_gemv(a, c->stream);
+      GAIN_HIP(hipEventRecord(e0, c->stream));
+      for (int q = 0; q < 3; ++q) launch_gemv(a, c->stream);
+      GAIN_HIP(hipEventRecord(e1, c->stream));
+      GAIN_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      GAIN_HIP(hipEventElapsedTime(&ms, e0, e1));
+      ms /= 3.f;
+      snprintf(line, sizeof(line), " %dx%d=%.3fms", r, k, ms);
+      rec += line;
+      if (ms < best) {
+        best = ms;
+        br = r;
+        bk = k;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  snprintf(line, sizeof(line), " -> rows %d ksplit %d\n", br, bk);
+  c->tune_log += rec + line;
+  c->gemv_rows = br;
+  c->gemv_ksplit = bk;
+  c->tuned[key] = br * 100 + bk;
+}
+
 // Lines for nI initial states of one block against one final block.
 void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
                int precision, bool plain, const gain_lines* out) {
@@ -588,6 +644,8 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
     }
     const double* Crows = blkF.vec;
     if (plain && nI == 1) {
+      if (c->autotune) gemv_tune(c, Crows, blkF.ld, ntotF, P.Flo, nR, G,
+                                 gstride, nvpi);
       ksplit = c->gemv_ksplit;
       double* part = c->w_part.get<double>((size_t)ksplit * nR * nvpi);
       Timed t(c, "gemv");
