@@ -290,253 +290,75 @@ Plan make_plan(gain_ctx* c, int jindI, int gI, const int* levels, int nI,
 // autotuner times every configuration on the real operands once per shape,
 // on the context's stream), or the default.
 int gemm_cfg(gain_ctx* c, int kind, GemmArgs g) {
-  if (c->force_cfg[kind] >= 0) return c->force_cfg[kind];
+  int ks = 0;
+  return gemm_cfg_split(c, kind, g, &ks, false);
+}
+
+// Same, and for kind 0 (projection) also the split-K count when
+// `tune_split` (the caller then sums the partials).
+int gemm_cfg_split(gain_ctx* c, int kind, GemmArgs g, int* ksplit,
+                   bool tune_split) {
+  if (c->force_cfg[kind] >= 0) {
+    if (tune_split) *ksplit = std::max(1, c->force_split);
+    return c->force_cfg[kind];
+  }
   // large M only changes the number of row tiles: one choice per N, K
   const auto key = std::make_tuple(kind, g.M >= 8192 ? -1 : g.M, g.N, g.K);
   auto it = c->tuned.find(key);
-  if (it != c->tuned.end()) return it->second;
-  if (!c->autotune) return -1;
+  if (it != c->tuned.end()) {
+    if (tune_split) *ksplit = it->second / 100;
+    return it->second % 100;
+  }
+  if (!c->autotune) {
+    if (tune_split) *ksplit = std::max(1, c->force_split);
+    return -1;
+  }
   hipEvent_t e0, e1;
   GAIN_HIP(hipEventCreate(&e0));
   GAIN_HIP(hipEventCreate(&e1));
-  int best = -1;
+  int best = -1, best_ks = 1;
   float best_ms = 1e30f;
   char line[160];
   snprintf(line, sizeof(line), "%s M=%d N=%d K=%d:",
            kind == 0 ? "projection" : "half_ls", g.M, g.N, g.K);
   std::string rec = line;
-  for (int cfg = 0; cfg < kGemmConfigs; ++cfg) {
-    g.cfg = cfg;
-    gemm_f64(g, c->stream);  // warm
-    GAIN_HIP(hipEventRecord(e0, c->stream));
-    for (int r = 0; r < 2; ++r) gemm_f64(g, c->stream);
-    GAIN_HIP(hipEventRecord(e1, c->stream));
-    GAIN_HIP(hipEventSynchronize(e1));
-    float ms = 0.f;
-    GAIN_HIP(hipEventElapsedTime(&ms, e0, e1));
-    ms *= 0.5f;
-    snprintf(line, sizeof(line), " %s=%.3fms", gemm_config_name(cfg), ms);
-    rec += line;
-    if (ms < best_ms) {
-      best_ms = ms;
-      best = cfg;
+  const int splits[4] = {1, 2, 4, 8};
+  const int nsplit = tune_split ? 4 : 1;
+  for (int cfg = 0; cfg < kGemmConfigs; ++cfg)
+    for (int si = 0; si < nsplit; ++si) {
+      g.cfg = cfg;
+      if (tune_split) {
+        g.ksplit = splits[si];
+        g.sCks = (int64_t)g.ldc * g.N;
+      }
+      gemm_f64(g, c->stream);  // warm
+      GAIN_HIP(hipEventRecord(e0, c->stream));
+      for (int r = 0; r < 2; ++r) gemm_f64(g, c->stream);
+      GAIN_HIP(hipEventRecord(e1, c->stream));
+      GAIN_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      GAIN_HIP(hipEventElapsedTime(&ms, e0, e1));
+      ms *= 0.5f;
+      if (tune_split)
+        snprintf(line, sizeof(line), " %s/k%d=%.3fms", gemm_config_name(cfg),
+                 g.ksplit, ms);
+      else
+        snprintf(line, sizeof(line), " %s=%.3fms", gemm_config_name(cfg), ms);
+      rec += line;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = cfg;
+        best_ks = tune_split ? g.ksplit : 1;
+      }
     }
-  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  rec += std::string(" -> ") + gemm_config_name(best) + "\n";
-  c->tune_log += rec;
-  c->tuned[key] = best;
+  snprintf(line, sizeof(line), " -> %s split %d\n", gemm_config_name(best),
+           best_ks);
+  c->tune_log += rec + line;
+  c->tuned[key] = best_ks * 100 + best;
+  if (tune_split) *ksplit = best_ks;
   return best;
-}
-
-int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
-                    const Small& sm, int nvecX, double* H, int precision) {
-  Basis& bI = *P.bI;
-  Basis& bF = *P.bF;
-  ensure_slots(c, bI);
-  if (bF.nvib > c->nc)
-    fail(GAIN_ERR_ARG,
-         "vibrational index %d of J=%d exceeds the dipole dimension %d",
-         bF.nvib - 1, bF.J, c->nc);
-  const Irrep& ir = bI.irrep[P.gI];
-  const int nc = c->nc;
-  const int64_t xstride = (int64_t)bI.ncol * nc;
-  double* X = c->w_X.get<double>((size_t)nvecX * xstride);
-  const int ncolTot = nvecX * bI.ncol;
-  int ks = c->mu_ksplit;
-  if (ks <= 0) {
-    // enough workgroups to fill 256 CUs: 64-row tiles x column tiles x 3
-    const int64_t tiles = (int64_t)((nc + 63) / 64) *
-                          ((ncolTot + (ncolTot > 64 ? 127 : 63)) /
-                           (ncolTot > 64 ? 128 : 64)) * 3;
-    ks = (int)std::min<int64_t>(16, std::max<int64_t>(1, 2048 / tiles));
-  }
-  double* Y = c->w_Y.get<double>((size_t)ks * 3 * ncolTot * nc);
-  Timed t(c, "half_ls");
-  ScatterArgs sa{};
-  sa.nslot = bI.ncol * nc;
-  sa.nprim = bI.nprim;
-  sa.slot_ptr = bI.slot_ptr;
-  sa.slot_prim = bI.slot_prim;
-  sa.kval = bI.kval;
-  sa.tau = bI.tau;
-  sa.icontr = bI.icontr;
-  sa.ijt = bI.ijterms + (int64_t)P.gI * bI.maxsymcoeffs;
-  sa.N = ir.N;
-  sa.repres = ir.repres;
-  sa.sdeg = ir.sdeg;
-  sa.ntot = ir.ntot;
-  sa.cbase = cbase;
-  sa.c_off = sm.c_off;
-  sa.ideg = sm.ideg;
-  sa.nvec = nvecX;
-  sa.X = X;
-  sa.xstride = xstride;
-  launch_scatter_expand(sa, c->stream);
-  const int S = slices_of(precision);
-  if (S > 0) {
-    ks = 1;
-    const Sliced& ms = mu_sliced(c, S);
-    uint16_t* xs = c->w_xs.get<uint16_t>((size_t)S * ncolTot * ms.Kp);
-    const int nch = (int)slice_nchunk(S, nc);
-    int* xe = c->w_xe.get<int>((size_t)ncolTot * nch);
-    slice_rows_bf16(S, X, nc, ncolTot, nc, xs, xe, c->stream);
-    SlicedGemmArgs g{};
-    g.S = S;
-    g.M = nc;
-    g.N = ncolTot;
-    g.Kp = ms.Kp;
-    g.A = ms.planes;
-    g.a_plane = (int64_t)3 * nc * ms.Kp;
-    g.a_z = (int64_t)nc * ms.Kp;
-    g.eA = ms.expo;
-    g.ea_z = (int64_t)nc * nch;
-    g.nch = nch;
-    g.B = xs;
-    g.b_plane = (int64_t)ncolTot * ms.Kp;
-    g.eB = xe;
-    g.C = Y;
-    g.ldc = nc;
-    g.c_z = (int64_t)ncolTot * nc;
-    g.nz = 3;
-    gemm_sliced_bf16(g, c->stream);
-  } else if (c->use_mfma) {
-    GemmArgs g{};
-    g.M = nc;
-    g.N = ncolTot;
-    g.K = nc;
-    g.A = c->d_mu;
-    g.lda = nc;
-    g.sAz = (int64_t)nc * nc;
-    g.a_kcontig = false;
-    g.B = X;
-    g.ldb = nc;
-    g.C = Y;
-    g.ldc = nc;
-    g.sCz = (int64_t)ncolTot * nc;
-    g.sCks = 3 * (int64_t)ncolTot * nc;
-    g.nz = 3;
-    g.ksplit = ks;
-    g.cfg = gemm_cfg(c, 1, g);
-    gemm_f64(g, c->stream);
-  } else {
-    launch_mu_gemm_valu(c->d_mu, nc, X, ncolTot, Y, ks, c->stream);
-  }
-  GatherArgs ga{};
-  ga.nprimF = bF.nprim;
-  ga.nc = nc;
-  ga.jI = bI.J;
-  ga.jF = bF.J;
-  ga.jmax = c->jmax;
-  ga.ncolI = bI.ncol;
-  ga.nvec = nvecX;
-  ga.ksplit = ks;
-  ga.kvalF = bF.kval;
-  ga.tauF = bF.tau;
-  ga.vibF = bF.vib;
-  ga.threej = c->d_threej;
-  ga.Y = Y;
-  ga.ys_alpha = (int64_t)ncolTot * nc;
-  ga.ys_ks = 3 * (int64_t)ncolTot * nc;
-  ga.H = H;
-  ga.hstride = bF.nprim;
-  launch_gather_h(ga, c->stream);
-  return bF.nprim;
-}
-
-Staged stage(gain_ctx* c, const Plan& P, const double* cI_host) {
-  Block& blkI = *P.blkI;
-  Staged st;
-  const int nI = P.nI;
-  const int nDI = (int)P.DI.size();
-  const int nvpi = (int)P.vec.size();
-  const int nvecX = nI * nDI;
-  const int nvecG = nI * nvpi;
-  st.nDI = nDI;
-  st.nvpi = nvpi;
-  st.nvecX = nvecX;
-  st.nvecG = nvecG;
-  st.compute = P.Fhi > P.Flo && nvpi > 0 && nDI > 0;
-  // ---- small per-call arrays through the pinned ring
-  const size_t off_c = 0;
-  const size_t off_ideg = off_c + sizeof(int64_t) * std::max(nvecX, 1);
-  const size_t off_hv = off_ideg + sizeof(int) * std::max(nvecX, 1);
-  const size_t off_dv = off_hv + sizeof(int) * std::max(nvecG, 1);
-  const size_t off_sv = off_dv + sizeof(int) * std::max(nvecG, 1);
-  const size_t off_vp = off_sv + sizeof(int) * 16;
-  const size_t off_vi = off_vp + sizeof(int) * (nvecX + 2);
-  const size_t off_vd = off_vi + sizeof(int) * std::max(nvecG, 1);
-  const size_t off_e = (off_vd + sizeof(int) * std::max(nvecG, 1) + 7) / 8 * 8;
-  const size_t off_l = off_e + sizeof(double) * nI;
-  const size_t off_ok = off_l + sizeof(int) * nI;
-  const size_t total = off_ok + nI + 16;
-  int slot;
-  char* hs = (char*)c->ring.acquire(total, &slot);
-  int64_t* h_c = (int64_t*)(hs + off_c);
-  int* h_ideg = (int*)(hs + off_ideg);
-  int* h_hv = (int*)(hs + off_hv);
-  int* h_dv = (int*)(hs + off_dv);
-  int* h_sv = (int*)(hs + off_sv);
-  int* h_vp = (int*)(hs + off_vp);
-  int* h_vi = (int*)(hs + off_vi);
-  int* h_vd = (int*)(hs + off_vd);
-  for (int i = 0; i < nI; ++i)
-    for (int k = 0; k < nDI; ++k) {
-      h_c[i * nDI + k] = cI_host ? 0 : (int64_t)P.levels[i] * blkI.ld;
-      h_ideg[i * nDI + k] = P.DI[k];
-    }
-  for (int i = 0; i < nI; ++i)
-    for (int v = 0; v < nvpi; ++v) {
-      const int dI = P.vec[v].first, dF = P.vec[v].second;
-      int pos = -1;
-      for (int k = 0; k < nDI; ++k)
-        if (P.DI[k] == dI) pos = k;
-      // a (dI,dF) pair whose dI has no half line strength reads a zero
-      // vector (the reference reads its memset-zero buffer)
-      h_hv[i * nvpi + v] = pos >= 0 ? i * nDI + pos : nvecX;
-      h_dv[i * nvpi + v] = dF;
-    }
-  for (int v = 0; v < nvpi; ++v)
-    h_sv[v] = P.vec[v].first + P.vec[v].second * c->maxdeg;
-  // inverse of h_of_vec (for the adjoint): vectors fed by each H vector
-  int q = 0;
-  for (int hv = 0; hv < nvecX; ++hv) {
-    h_vp[hv] = q;
-    for (int v = 0; v < nvecG; ++v)
-      if (h_hv[v] == hv) {
-        h_vi[q] = v;
-        h_vd[q] = h_dv[v];
-        ++q;
-      }
-  }
-  h_vp[nvecX] = q;
-  std::memcpy(hs + off_e, P.eI.data(), sizeof(double) * nI);
-  std::memcpy(hs + off_l, P.levels.data(), sizeof(int) * nI);
-  std::memcpy(hs + off_ok, P.okI.data(), nI);
-  char* ds = c->w_small.get<char>(total);
-  GAIN_HIP(hipMemcpyAsync(ds, hs, total, hipMemcpyHostToDevice, c->stream));
-  st.sm = Small{(int64_t*)(ds + off_c), (int*)(ds + off_ideg),
-                (int*)(ds + off_hv),    (int*)(ds + off_dv),
-                (int*)(ds + off_sv),    (double*)(ds + off_e),
-                (int*)(ds + off_l),     (uint8_t*)(ds + off_ok),
-                (int*)(ds + off_vp),    (int*)(ds + off_vi),
-                (int*)(ds + off_vd)};
-  st.cbase = blkI.vec;
-  int cslot = -1;
-  if (cI_host) {
-    const size_t nb = sizeof(double) * blkI.rec_len;
-    void* hc = c->ring.acquire(nb, &cslot);
-    std::memcpy(hc, cI_host, nb);
-    double* dc = c->w_c.get<double>(blkI.rec_len);
-    GAIN_HIP(hipMemcpyAsync(dc, hc, nb, hipMemcpyHostToDevice, c->stream));
-    st.cbase = dc;
-  }
-  c->ring.release(slot, c->stream);
-  if (cslot >= 0) c->ring.release(cslot, c->stream);
-  if (!st.cbase && st.compute)
-    fail(GAIN_ERR_STATE, "initial block has no eigenvectors and no cI given");
-  return st;
 }
 
 // GEMV autotuner: rows per wave x split-K count, timed on the real final
@@ -702,12 +524,19 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
         g.a_kcontig = true;
         g.B = G;
         g.ldb = gstride;
-        g.C = LS;
         g.ldc = nR;
+        g.sCks = (int64_t)nR * nvecG;
         g.nz = 1;
         g.ksplit = 1;
-        g.cfg = gemm_cfg(c, 0, g);
+        // the tuner may write every split count: room for the largest
+        LS = c->w_part.get<double>((size_t)8 * nR * nvecG);
+        g.C = LS;
+        int ks = 1;
+        g.cfg = gemm_cfg_split(c, 0, g, &ks, true);
+        g.ksplit = ks;
         gemm_f64(g, c->stream);
+        ksplit = ks;
+        src_ks = (int64_t)nR * nvecG;
       } else if (precision != GAIN_PREC_FP64)
         gemm_projection(precision, nR, nvecG, ntotF,
                         Crows + (int64_t)P.Flo * blkF.ld, blkF.ld, G, gstride,
@@ -1429,6 +1258,9 @@ extern "C" int gain_set_tuning(gain_ctx* c, const char* key, int value) {
     } else if (k == "autotune") {
       c->autotune = value != 0;
       if (value == 2) c->tuned.clear();
+    } else if (k == "proj_split") {
+      GAIN_CHECK_ARG(value >= 1 && value <= 64, "split 1..64");
+      c->force_split = value;
     } else if (k == "proj_cfg" || k == "half_ls_cfg") {
       GAIN_CHECK_ARG(value >= -1 && value < kGemmConfigs, "cfg -1..%d",
                      kGemmConfigs - 1);

@@ -193,6 +193,7 @@ struct gain_ctx {
   // GEMM tile autotuner: chosen configuration per (kind, M, N, K)
   bool autotune = false;
   int force_cfg[2] = {-1, -1};  // 0: projection, 1: half line strength
+  int force_split = 1;          // projection split-K when not tuned
   std::map<std::tuple<int, int, int, int>, int> tuned;
   std::string tune_log;
   bool timing = false;
@@ -286,6 +287,8 @@ bool degfilter(const gain_ctx* c, int gI, int gF, int dI, int dF);
 Plan make_plan(gain_ctx* c, int jindI, int gI, const int* levels, int nI,
                int jindF, int gF);
 int gemm_cfg(gain_ctx* c, int kind, GemmArgs g);
+int gemm_cfg_split(gain_ctx* c, int kind, GemmArgs g, int* ksplit,
+                   bool tune_split);
 void resolve_timers(gain_ctx* c, TimerSlot& t);
 
 }  // namespace gain

@@ -277,3 +277,17 @@ def test_autotuner_picks_and_reports():
         rep = eng.tuning_report()
     assert "projection" in rep and "half_ls" in rep and "->" in rep
     compare_lines(got, oracle_map(tr))
+
+
+@pytest.mark.parametrize("split", [2, 3, 8])
+def test_projection_split_k(split):
+    prob = synth.make_problem([4, 5], 20, "c2v4")
+    prob.params.erange_lower = (0.0, 1e9)
+    rows = prob.state_table()
+    initials = list(range(0, len(rows), 2))
+    tr, _ = orc.run(prob, initials)
+    with Engine.from_problem(prob) as eng:
+        eng.set_tuning("proj_cfg", 5)
+        eng.set_tuning("proj_split", split)
+        got = engine_lines(eng, prob, initials, "batched")
+    compare_lines(got, oracle_map(tr))
