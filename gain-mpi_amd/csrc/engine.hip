@@ -361,6 +361,215 @@ int gemm_cfg_split(gain_ctx* c, int kind, GemmArgs g, int* ksplit,
   return best;
 }
 
+int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
+                    const Small& sm, int nvecX, double* H, int precision) {
+  Basis& bI = *P.bI;
+  Basis& bF = *P.bF;
+  ensure_slots(c, bI);
+  if (bF.nvib > c->nc)
+    fail(GAIN_ERR_ARG,
+         "vibrational index %d of J=%d exceeds the dipole dimension %d",
+         bF.nvib - 1, bF.J, c->nc);
+  const Irrep& ir = bI.irrep[P.gI];
+  const int nc = c->nc;
+  const int64_t xstride = (int64_t)bI.ncol * nc;
+  double* X = c->w_X.get<double>((size_t)nvecX * xstride);
+  const int ncolTot = nvecX * bI.ncol;
+  int ks = c->mu_ksplit;
+  if (ks <= 0) {
+    // enough workgroups to fill 256 CUs: 64-row tiles x column tiles x 3
+    const int64_t tiles = (int64_t)((nc + 63) / 64) *
+                          ((ncolTot + (ncolTot > 64 ? 127 : 63)) /
+                           (ncolTot > 64 ? 128 : 64)) * 3;
+    ks = (int)std::min<int64_t>(16, std::max<int64_t>(1, 2048 / tiles));
+  }
+  double* Y = c->w_Y.get<double>((size_t)ks * 3 * ncolTot * nc);
+  Timed t(c, "half_ls");
+  ScatterArgs sa{};
+  sa.nslot = bI.ncol * nc;
+  sa.nprim = bI.nprim;
+  sa.slot_ptr = bI.slot_ptr;
+  sa.slot_prim = bI.slot_prim;
+  sa.kval = bI.kval;
+  sa.tau = bI.tau;
+  sa.icontr = bI.icontr;
+  sa.ijt = bI.ijterms + (int64_t)P.gI * bI.maxsymcoeffs;
+  sa.N = ir.N;
+  sa.repres = ir.repres;
+  sa.sdeg = ir.sdeg;
+  sa.ntot = ir.ntot;
+  sa.cbase = cbase;
+  sa.c_off = sm.c_off;
+  sa.ideg = sm.ideg;
+  sa.nvec = nvecX;
+  sa.X = X;
+  sa.xstride = xstride;
+  launch_scatter_expand(sa, c->stream);
+  const int S = slices_of(precision);
+  if (S > 0) {
+    ks = 1;
+    const Sliced& ms = mu_sliced(c, S);
+    uint16_t* xs = c->w_xs.get<uint16_t>((size_t)S * ncolTot * ms.Kp);
+    const int nch = (int)slice_nchunk(S, nc);
+    int* xe = c->w_xe.get<int>((size_t)ncolTot * nch);
+    slice_rows_bf16(S, X, nc, ncolTot, nc, xs, xe, c->stream);
+    SlicedGemmArgs g{};
+    g.S = S;
+    g.M = nc;
+    g.N = ncolTot;
+    g.Kp = ms.Kp;
+    g.A = ms.planes;
+    g.a_plane = (int64_t)3 * nc * ms.Kp;
+    g.a_z = (int64_t)nc * ms.Kp;
+    g.eA = ms.expo;
+    g.ea_z = (int64_t)nc * nch;
+    g.nch = nch;
+    g.B = xs;
+    g.b_plane = (int64_t)ncolTot * ms.Kp;
+    g.eB = xe;
+    g.C = Y;
+    g.ldc = nc;
+    g.c_z = (int64_t)ncolTot * nc;
+    g.nz = 3;
+    gemm_sliced_bf16(g, c->stream);
+  } else if (c->use_mfma) {
+    GemmArgs g{};
+    g.M = nc;
+    g.N = ncolTot;
+    g.K = nc;
+    g.A = c->d_mu;
+    g.lda = nc;
+    g.sAz = (int64_t)nc * nc;
+    g.a_kcontig = false;
+    g.B = X;
+    g.ldb = nc;
+    g.C = Y;
+    g.ldc = nc;
+    g.sCz = (int64_t)ncolTot * nc;
+    g.sCks = 3 * (int64_t)ncolTot * nc;
+    g.nz = 3;
+    g.ksplit = ks;
+    g.cfg = gemm_cfg(c, 1, g);
+    gemm_f64(g, c->stream);
+  } else {
+    launch_mu_gemm_valu(c->d_mu, nc, X, ncolTot, Y, ks, c->stream);
+  }
+  GatherArgs ga{};
+  ga.nprimF = bF.nprim;
+  ga.nc = nc;
+  ga.jI = bI.J;
+  ga.jF = bF.J;
+  ga.jmax = c->jmax;
+  ga.ncolI = bI.ncol;
+  ga.nvec = nvecX;
+  ga.ksplit = ks;
+  ga.kvalF = bF.kval;
+  ga.tauF = bF.tau;
+  ga.vibF = bF.vib;
+  ga.threej = c->d_threej;
+  ga.Y = Y;
+  ga.ys_alpha = (int64_t)ncolTot * nc;
+  ga.ys_ks = 3 * (int64_t)ncolTot * nc;
+  ga.H = H;
+  ga.hstride = bF.nprim;
+  launch_gather_h(ga, c->stream);
+  return bF.nprim;
+}
+
+Staged stage(gain_ctx* c, const Plan& P, const double* cI_host) {
+  Block& blkI = *P.blkI;
+  Staged st;
+  const int nI = P.nI;
+  const int nDI = (int)P.DI.size();
+  const int nvpi = (int)P.vec.size();
+  const int nvecX = nI * nDI;
+  const int nvecG = nI * nvpi;
+  st.nDI = nDI;
+  st.nvpi = nvpi;
+  st.nvecX = nvecX;
+  st.nvecG = nvecG;
+  st.compute = P.Fhi > P.Flo && nvpi > 0 && nDI > 0;
+  // ---- small per-call arrays through the pinned ring
+  const size_t off_c = 0;
+  const size_t off_ideg = off_c + sizeof(int64_t) * std::max(nvecX, 1);
+  const size_t off_hv = off_ideg + sizeof(int) * std::max(nvecX, 1);
+  const size_t off_dv = off_hv + sizeof(int) * std::max(nvecG, 1);
+  const size_t off_sv = off_dv + sizeof(int) * std::max(nvecG, 1);
+  const size_t off_vp = off_sv + sizeof(int) * 16;
+  const size_t off_vi = off_vp + sizeof(int) * (nvecX + 2);
+  const size_t off_vd = off_vi + sizeof(int) * std::max(nvecG, 1);
+  const size_t off_e = (off_vd + sizeof(int) * std::max(nvecG, 1) + 7) / 8 * 8;
+  const size_t off_l = off_e + sizeof(double) * nI;
+  const size_t off_ok = off_l + sizeof(int) * nI;
+  const size_t total = off_ok + nI + 16;
+  int slot;
+  char* hs = (char*)c->ring.acquire(total, &slot);
+  int64_t* h_c = (int64_t*)(hs + off_c);
+  int* h_ideg = (int*)(hs + off_ideg);
+  int* h_hv = (int*)(hs + off_hv);
+  int* h_dv = (int*)(hs + off_dv);
+  int* h_sv = (int*)(hs + off_sv);
+  int* h_vp = (int*)(hs + off_vp);
+  int* h_vi = (int*)(hs + off_vi);
+  int* h_vd = (int*)(hs + off_vd);
+  for (int i = 0; i < nI; ++i)
+    for (int k = 0; k < nDI; ++k) {
+      h_c[i * nDI + k] = cI_host ? 0 : (int64_t)P.levels[i] * blkI.ld;
+      h_ideg[i * nDI + k] = P.DI[k];
+    }
+  for (int i = 0; i < nI; ++i)
+    for (int v = 0; v < nvpi; ++v) {
+      const int dI = P.vec[v].first, dF = P.vec[v].second;
+      int pos = -1;
+      for (int k = 0; k < nDI; ++k)
+        if (P.DI[k] == dI) pos = k;
+      // a (dI,dF) pair whose dI has no half line strength reads a zero
+      // vector (the reference reads its memset-zero buffer)
+      h_hv[i * nvpi + v] = pos >= 0 ? i * nDI + pos : nvecX;
+      h_dv[i * nvpi + v] = dF;
+    }
+  for (int v = 0; v < nvpi; ++v)
+    h_sv[v] = P.vec[v].first + P.vec[v].second * c->maxdeg;
+  // inverse of h_of_vec (for the adjoint): vectors fed by each H vector
+  int q = 0;
+  for (int hv = 0; hv < nvecX; ++hv) {
+    h_vp[hv] = q;
+    for (int v = 0; v < nvecG; ++v)
+      if (h_hv[v] == hv) {
+        h_vi[q] = v;
+        h_vd[q] = h_dv[v];
+        ++q;
+      }
+  }
+  h_vp[nvecX] = q;
+  std::memcpy(hs + off_e, P.eI.data(), sizeof(double) * nI);
+  std::memcpy(hs + off_l, P.levels.data(), sizeof(int) * nI);
+  std::memcpy(hs + off_ok, P.okI.data(), nI);
+  char* ds = c->w_small.get<char>(total);
+  GAIN_HIP(hipMemcpyAsync(ds, hs, total, hipMemcpyHostToDevice, c->stream));
+  st.sm = Small{(int64_t*)(ds + off_c), (int*)(ds + off_ideg),
+                (int*)(ds + off_hv),    (int*)(ds + off_dv),
+                (int*)(ds + off_sv),    (double*)(ds + off_e),
+                (int*)(ds + off_l),     (uint8_t*)(ds + off_ok),
+                (int*)(ds + off_vp),    (int*)(ds + off_vi),
+                (int*)(ds + off_vd)};
+  st.cbase = blkI.vec;
+  int cslot = -1;
+  if (cI_host) {
+    const size_t nb = sizeof(double) * blkI.rec_len;
+    void* hc = c->ring.acquire(nb, &cslot);
+    std::memcpy(hc, cI_host, nb);
+    double* dc = c->w_c.get<double>(blkI.rec_len);
+    GAIN_HIP(hipMemcpyAsync(dc, hc, nb, hipMemcpyHostToDevice, c->stream));
+    st.cbase = dc;
+  }
+  c->ring.release(slot, c->stream);
+  if (cslot >= 0) c->ring.release(cslot, c->stream);
+  if (!st.cbase && st.compute)
+    fail(GAIN_ERR_STATE, "initial block has no eigenvectors and no cI given");
+  return st;
+}
+
 // GEMV autotuner: rows per wave x split-K count, timed on the real final
 // block once per (row count bucket, record length, vector count).
 void gemv_tune(gain_ctx* c, const double* C, int64_t ld, int ntot, int F0,
