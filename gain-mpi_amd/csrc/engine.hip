@@ -432,6 +432,24 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
     g.c_z = (int64_t)ncolTot * nc;
     g.nz = 3;
     gemm_sliced_bf16(g, c->stream);
+  } else if (precision == GAIN_PREC_FP32) {
+    ks = 1;
+    GemmArgs g{};
+    g.M = nc;
+    g.N = ncolTot;
+    g.K = nc;
+    g.A = c->d_mu;
+    g.lda = nc;
+    g.sAz = (int64_t)nc * nc;
+    g.a_kcontig = false;
+    g.B = X;
+    g.ldb = nc;
+    g.C = Y;
+    g.ldc = nc;
+    g.sCz = (int64_t)ncolTot * nc;
+    g.nz = 3;
+    g.ksplit = 1;
+    gemm_f32acc(g, c->stream);
   } else if (c->use_mfma) {
     GemmArgs g{};
     g.M = nc;
@@ -723,7 +741,7 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
         g.ldc = nR;
         g.nz = 1;
         gemm_sliced_bf16(g, c->stream);
-      } else if (c->use_mfma) {
+      } else if (c->use_mfma && precision == GAIN_PREC_FP64) {
         GemmArgs g{};
         g.M = nR;
         g.N = nvecG;

@@ -339,6 +339,23 @@ void gemm_projection(int precision, int M, int N, int K, const double* A,
       gemm_f64(g, s);
       return;
     }
+    case GAIN_PREC_FP32: {
+      GemmArgs g{};
+      g.M = M;
+      g.N = N;
+      g.K = K;
+      g.A = A;
+      g.lda = lda;
+      g.a_kcontig = true;
+      g.B = B;
+      g.ldb = ldb;
+      g.C = C;
+      g.ldc = ldc;
+      g.nz = 1;
+      g.ksplit = 1;
+      gemm_f32acc(g, s);
+      return;
+    }
     default:
       fail(GAIN_ERR_ARG, "precision %d not available in this build",
            precision);

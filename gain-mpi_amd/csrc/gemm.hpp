@@ -41,6 +41,8 @@ struct GroupTile {
 void gemm_f64_grouped(const GemmArgs* probs, const GroupTile* tiles,
                       int ntiles, bool a_kcontig, int cfg, int grid,
                       hipStream_t s);
+// FP32 matrix-core products with FP64 accumulation (ksplit, cfg ignored).
+void gemm_f32acc(const GemmArgs& g, hipStream_t s);
 int gemm_tile_m(int cfg);
 int gemm_tile_n(int cfg);
 void gemm_f64(const GemmArgs& g, hipStream_t s);

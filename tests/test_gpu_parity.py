@@ -291,3 +291,21 @@ def test_projection_split_k(split):
         eng.set_tuning("proj_split", split)
         got = engine_lines(eng, prob, initials, "batched")
     compare_lines(got, oracle_map(tr))
+
+
+def test_batched_fp32_mfma_error_vs_fp64_oracle():
+    """FP32 matrix-core contractions with FP64 accumulation: transition list
+    identical; S error reported (operands rounded to FP32)."""
+    prob = synth.make_problem([5, 6], 32, "c2v4")
+    prob.params.erange_lower = (0.0, 1e9)
+    rows = prob.state_table()
+    initials = list(range(0, len(rows), 2))
+    tr, _ = orc.run(prob, initials)
+    ref = oracle_map(tr)
+    with Engine.from_problem(prob) as eng:
+        got = engine_lines(eng, prob, initials, "batched", 1)
+    assert set(got) == set(ref)
+    es, ew, ns, nw = _strong_weak_err(got, ref)
+    print(f"precision 1 (fp32): strong lines {ns} max rel err {es:.3e}; "
+          f"weak lines {nw} max rel err {ew:.3e}")
+    assert es <= 1e-3
