@@ -33,6 +33,25 @@ PRECISION_NAMES = {0: "f64", 1: "f32-mfma/f64-acc",
 PEAKS = {0: 78.6, 1: 157.3, 2: 2500.0, 3: 2500.0}
 PRODUCTS = {0: 1, 1: 1, 2: 6, 3: 10}  # hardware products per algorithmic FMA
 TIMERS = ("gemv", "half_ls", "expand_T", "epilogue", "gemm_proj")
+# rocprofv3 FETCH_SIZE / WRITE_SIZE per launch for this workload, scaled by
+# the same counters' reading of a 4 GB device copy (profiles/ holds the
+# commands that produced it)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles",
+                            "hbm_traffic_j20_j21_nc4000.json")
+
+
+def measured_traffic(label, grid):
+    """HBM bytes of one launch of kernel `label` ("64x128x32/4w" tiles of
+    the FP64 projection GEMM, or "gemv r4") over `grid` work-items, from
+    the committed counter summary; None when that launch was not profiled."""
+    if grid is None or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        table = json.load(f)
+    for e in table["launches"]:
+        if e["label"] == label and e["grid_items"] == grid:
+            return e["bytes"]
+    return None
 
 
 def log(msg):
@@ -137,6 +156,13 @@ def main():
                     for i in range(args.plain_states)])
     gemv_bytes = 8.0 * (rows * blkF.rec_len + blkF.rec_len + rows)
     gemv_gbs = gemv_bytes / (gemv_avg / 1e3) / 1e9
+    # the launch the tuner chose, on initial state 0 (the profiled shape)
+    eng.lines_device(0, 0, [0], 1, 1, ptrs(0), plain=True)
+    eng.synchronize()
+    gemv_traffic = measured_traffic(f"gemv r{eng.get_tuning('gemv_rows')}",
+                                    eng.get_tuning("gemv_grid"))
+    eng.timing("gemv", reset=True)
+    eng.timing("half_ls", reset=True)
     log(f"plain: {args.plain_states} initial states, {plain_trans} "
         f"transitions in {plain_ms:.1f} ms -> {plain_tps:.3e}/s; "
         f"gemv avg {gemv_avg:.3f} ms = {gemv_gbs:.0f} GB/s "
@@ -224,6 +250,14 @@ def main():
                 "weak_max_rel_err": float(rel[~strong].max())
                 if (~strong).any() else 0.0}
     prec_check = check(args.precision)
+    proj_traffic = None
+    if args.precision == 0:  # check() ran batch 0 last: the tuned shape
+        chosen = [ln for ln in tune_log.splitlines()
+                  if ln.startswith("projection") and "->" in ln]
+        if chosen:
+            label = chosen[-1].split("->")[1].split()[0]
+            proj_traffic = measured_traffic(label,
+                                            eng.get_tuning("proj_grid"))
     for v in variants:
         v["precision_check"] = check(v["precision"])
         log(f"precision check {v['dtype']}: {v['precision_check']}")
@@ -284,8 +318,10 @@ def main():
                      "frac": gemm_tfs * prod / PEAKS[args.precision],
                      "note": "achieved = algorithmic 2*M*N*K flops / avg "
                              "kernel time; peak = dense MFMA peak of the "
-                             "datatype / hardware products per FMA",
-                     "traffic": None},
+                             "datatype / hardware products per FMA; "
+                             "traffic = counter HBM bytes of the chosen "
+                             "launch on batch 0",
+                     "traffic": proj_traffic},
         "precision_check": prec_check,
         "variants": variants,
         "autotune": tune_log.strip().splitlines(),
@@ -293,7 +329,7 @@ def main():
                            "achieved": gemv_gbs, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": gemv_gbs / HBM_PEAK_GBS,
                            "device_copy_gbs": copy_gbs,
-                           "traffic": None},
+                           "traffic": gemv_traffic},
         "cpu_baseline": cpu,
     }
     print(json.dumps(res))

@@ -469,6 +469,9 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
     g.ksplit = ks;
     g.cfg = gemm_cfg(c, 1, g);
     gemm_f64(g, c->stream);
+    c->last["half_ls_cfg"] = g.cfg < 0 ? gemm_default_config(g.N) : g.cfg;
+    c->last["half_ls_split"] = ks;
+    c->last["half_ls_grid"] = gemm_grid_items(g);
   } else {
     launch_mu_gemm_valu(c->d_mu, nc, X, ncolTot, Y, ks, c->stream);
   }
@@ -711,6 +714,7 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
       a.ksplit = ksplit;
       a.rows = c->gemv_rows;
       launch_gemv(a, c->stream);
+      c->last["gemv_grid"] = gemv_grid_items(a);
       src = part;
       src_ks = (int64_t)nR * nvpi;
       src_f = nvpi;
@@ -762,6 +766,9 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
         g.cfg = gemm_cfg_split(c, 0, g, &ks, true);
         g.ksplit = ks;
         gemm_f64(g, c->stream);
+        c->last["proj_cfg"] = g.cfg < 0 ? gemm_default_config(g.N) : g.cfg;
+        c->last["proj_split"] = ks;
+        c->last["proj_grid"] = gemm_grid_items(g);
         ksplit = ks;
         src_ks = (int64_t)nR * nvecG;
       } else if (precision != GAIN_PREC_FP64)
@@ -1590,6 +1597,31 @@ extern "C" int gain_dipole_coeff_grad(gain_ctx* c, int P, const double* M_dev,
       double s = 0.0;
       for (int k = 0; k < ks; ++k) s += h[(size_t)k * P + q];
       dc[q] = s;
+    }
+  });
+}
+
+extern "C" int gain_get_tuning(gain_ctx* c, const char* key,
+                               long long* value) {
+  return guarded([&] {
+    need(c);
+    GAIN_CHECK_ARG(key && value, "null argument");
+    const std::string k(key);
+    if (k == "gemv_rows") {
+      *value = c->gemv_rows;
+    } else if (k == "gemv_ksplit") {
+      *value = c->gemv_ksplit;
+    } else if (k == "mu_ksplit") {
+      *value = c->mu_ksplit;
+    } else if (k == "mfma") {
+      *value = c->use_mfma;
+    } else if (k == "autotune") {
+      *value = c->autotune;
+    } else {
+      auto it = c->last.find(k);
+      if (it == c->last.end())
+        fail(GAIN_ERR_ARG, "unknown tuning key %s, or no launch yet", key);
+      *value = it->second;
     }
   });
 }

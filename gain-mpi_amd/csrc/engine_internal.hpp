@@ -196,6 +196,8 @@ struct gain_ctx {
   int force_split = 1;          // projection split-K when not tuned
   std::map<std::tuple<int, int, int, int>, int> tuned;
   std::string tune_log;
+  // what the most recent launch of each tuned kernel used (tuning getter)
+  std::map<std::string, int64_t> last;
   bool timing = false;
   std::map<std::string, TimerSlot> timers;
 };

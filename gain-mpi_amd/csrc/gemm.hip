@@ -22,12 +22,8 @@ typedef double dbl2 __attribute__((ext_vector_type(2)));
 
 constexpr int kPad = 1;
 
-// Bijective remap of the linear workgroup id so that each of the 8 XCDs
-// (dispatch is round-robin over XCDs) receives a contiguous range of tiles.
-__device__ inline int xcd_remap(int lin, int total) {
-  if (total % 8) return lin;
-  return (lin % 8) * (total / 8) + lin / 8;
-}
+// Rows of output tiles walked together (see tile_order).
+constexpr int kGroupRows = 8;
 
 // One BM x BN output tile, K range [kbeg, kend), 4 waves arranged WM x WN;
 // each wave owns (BM/WM) x (BN/WN) = MT x NT MFMA tiles of 16 x 16.
@@ -161,11 +157,16 @@ __global__ __launch_bounds__(WM * WN * 64) void k_gemm_f64(GemmArgs g,
                                                   bool vecB) {
   __shared__ double As[BM][BK + kPad];
   __shared__ double Bs[BN][BK + kPad];
-  // tiles: n fastest so neighbours share the A tile
+  // linear id -> (z slice, split, tile): each XCD takes a contiguous run
+  // of the grouped tile order, so the workgroups resident on one XCD cover
+  // a compact block of rows and columns and share operand slabs in its L2
   const int nxy = ntn * gridDim.y;
-  const int t = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, nxy);
-  const int tn = t % ntn, tm = t / ntn;
-  const int z = blockIdx.z % g.nz, ks = blockIdx.z / g.nz;
+  const int lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const int p = xcd_remap(lin, nxy * gridDim.z);
+  const int zk = p / nxy;
+  int tm, tn;
+  tile_order(p - zk * nxy, gridDim.y, ntn, kGroupRows, &tm, &tn);
+  const int z = zk % g.nz, ks = zk / g.nz;
   const double* A = g.A + (int64_t)z * g.sAz;
   double* C = g.C + (int64_t)z * g.sCz + (int64_t)ks * g.sCks;
   const int kbeg = ks * kchunk;
@@ -212,7 +213,8 @@ static void launch_cfg(const GemmArgs& g, hipStream_t s, int kchunk,
                        bool vecA, bool vecB) {
   const int ntm = (g.M + BM - 1) / BM, ntn = (g.N + BN - 1) / BN;
   dim3 grid(ntn, ntm, g.nz * g.ksplit);
-  GAIN_CHECK_ARG((int64_t)grid.z < 65536 && (int64_t)ntm < 65536,
+  GAIN_CHECK_ARG((int64_t)grid.z < 65536 && (int64_t)ntm < 65536 &&
+                     (int64_t)ntn * ntm * grid.z < (1ll << 31),
                  "gemm: grid too large");
   if (g.a_kcontig)
     hipLaunchKernelGGL((k_gemm_f64<BM, BN, BK, WM, WN, true>), grid,
@@ -292,6 +294,13 @@ void gemm_f64_grouped(const GemmArgs* probs, const GroupTile* tiles,
     case 14: launch_grouped_cfg<64, 256, 32, 1, 8>(probs, tiles, ntiles, a_kcontig, grid, s); break;
   }
   check_launch("gemm_f64_grouped");
+}
+
+int64_t gemm_grid_items(const GemmArgs& g) {
+  const int cfg = g.cfg < 0 ? gemm_default_config(g.N) : g.cfg;
+  const TileCfg& t = kCfgs[cfg];
+  return (int64_t)((g.M + t.bm - 1) / t.bm) * ((g.N + t.bn - 1) / t.bn) *
+         g.nz * g.ksplit * t.waves * 64;
 }
 
 int gemm_tile_m(int cfg) { return kCfgs[cfg].bm; }

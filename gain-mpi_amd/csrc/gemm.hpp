@@ -31,6 +31,28 @@ struct GemmArgs {
 
 constexpr int kGemmConfigs = 15;
 
+// Bijective remap of a linear workgroup id in [0, total): the hardware deals
+// ids round-robin over the 8 XCDs (XCD = lin % 8); XCD x receives the
+// contiguous position range starting at x*(total/8) + min(x, total%8).
+__host__ __device__ inline int xcd_remap(int lin, int total) {
+  const int q = total / 8, r = total % 8, x = lin % 8;
+  return x * q + (x < r ? x : r) + lin / 8;
+}
+
+// Position p in [0, ntm*ntn) -> output tile (tm, tn): groups of `gm` tile
+// rows, walked column by column inside a group, so consecutive positions
+// share A rows (within a column) and B columns (across the group).
+__host__ __device__ inline void tile_order(int p, int ntm, int ntn, int gm,
+                                           int* tm, int* tn) {
+  const int per = gm * ntn;
+  const int grp = p / per;
+  const int first = grp * gm;
+  const int rows = ntm - first < gm ? ntm - first : gm;
+  const int r = p - grp * per;
+  *tm = first + r % rows;
+  *tn = r / rows;
+}
+
 // One output tile of one problem in a grouped (persistent) launch.
 struct GroupTile {
   int prob, tm, tn, z;
@@ -43,6 +65,8 @@ void gemm_f64_grouped(const GemmArgs* probs, const GroupTile* tiles,
                       hipStream_t s);
 // FP32 matrix-core products with FP64 accumulation (ksplit, cfg ignored).
 void gemm_f32acc(const GemmArgs& g, hipStream_t s);
+// Work-items of the launch gemm_f64 makes for g (as profilers report it).
+int64_t gemm_grid_items(const GemmArgs& g);
 int gemm_tile_m(int cfg);
 int gemm_tile_n(int cfg);
 void gemm_f64(const GemmArgs& g, hipStream_t s);

@@ -301,6 +301,12 @@ int gemv_kchunk(int ntot, int ksplit) {
   return (kc + 127) / 128 * 128;
 }
 
+int64_t gemv_grid_items(const GemvArgs& a) {
+  const int R = a.rows == 2 || a.rows == 8 ? a.rows : 4;
+  const int64_t waves = (int64_t)((a.nF + R - 1) / R) * a.ksplit;
+  return (waves + 3) / 4 * 256;
+}
+
 template <int R, int NV>
 static void gemv_dispatch_vec(const GemvArgs& a, bool vec, hipStream_t s) {
   const int kc = gemv_kchunk(a.ntot, a.ksplit);

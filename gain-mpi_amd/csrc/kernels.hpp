@@ -94,6 +94,8 @@ struct GemvArgs {
 };
 void launch_gemv(const GemvArgs& a, hipStream_t s);
 int gemv_kchunk(int ntot, int ksplit);
+// Work-items of the launch launch_gemv makes for a.
+int64_t gemv_grid_items(const GemvArgs& a);
 
 struct LinesArgs {
   int nI, nF, Flo, Fhi, maxdeg;

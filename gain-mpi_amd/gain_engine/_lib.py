@@ -27,7 +27,8 @@ EXPORTS = (
     "gain_copy_dipole_constant", "gain_transform_vector_primitive",
     "gain_compute_half_linestrength", "gain_ddot", "gain_adjoint_dipole",
     "gain_set_dipole_combination", "gain_dipole_coeff_grad",
-    "gain_set_tuning", "gain_tuning_report", "gain_sweep_create",
+    "gain_set_tuning", "gain_get_tuning", "gain_tuning_report",
+    "gain_sweep_create",
     "gain_sweep_run", "gain_sweep_destroy",
 )
 
@@ -112,6 +113,8 @@ def load():
         "gain_get_timing": (_ip, [_vp, C.c_char_p, C.POINTER(_dp),
                                   C.POINTER(C.c_int64), _ip]),
         "gain_set_tuning": (_ip, [_vp, C.c_char_p, _ip]),
+        "gain_get_tuning": (_ip, [_vp, C.c_char_p,
+                                 C.POINTER(C.c_longlong)]),
         "gain_tuning_report": (_ip, [_vp, C.c_char_p, _ip]),
         "gain_sweep_create": (_ip, [_vp, _vp, _ip, _ip, C.POINTER(_vp)]),
         "gain_sweep_run": (_ip, [_vp]),

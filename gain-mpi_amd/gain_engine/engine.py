@@ -237,6 +237,11 @@ class Engine:
     def set_tuning(self, key: str, value: int):
         check(self._L.gain_set_tuning(self._h, key.encode(), int(value)))
 
+    def get_tuning(self, key: str) -> int:
+        v = C.c_longlong(0)
+        check(self._L.gain_get_tuning(self._h, key.encode(), C.byref(v)))
+        return int(v.value)
+
     def tuning_report(self) -> str:
         buf = C.create_string_buffer(1 << 16)
         check(self._L.gain_tuning_report(self._h, buf, 1 << 16))

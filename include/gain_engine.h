@@ -223,6 +223,12 @@ int gain_get_timing(gain_ctx* ctx, const char* name, double* total_ms,
  * "gemv_rows" (2/4/8), "gemv_ksplit", "mu_ksplit" (0 = automatic),
  * "mfma" (0: vector-ALU GEMMs, for cross-checks). */
 int gain_set_tuning(gain_ctx* ctx, const char* key, int value);
+/* Current value of a settable key, or what the most recent launch used:
+ * "proj_cfg", "proj_split", "proj_grid" (FP64 MFMA projection GEMM),
+ * "half_ls_cfg", "half_ls_split", "half_ls_grid" (mu GEMM), "gemv_grid"
+ * (plain GEMV); *_grid = work-items of that launch. No reference
+ * counterpart: the reference has no tunable kernels. */
+int gain_get_tuning(gain_ctx* ctx, const char* key, long long* value);
 /* The autotuner's timings and choices, one line per GEMM shape. */
 int gain_tuning_report(gain_ctx* ctx, char* buf, int len);
 
