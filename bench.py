@@ -73,6 +73,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-initials", type=int, default=3,
+                    help="initial states in the CPU baseline sample")
     ap.add_argument("--copy-calib-gb", type=float, default=0.0,
                     help="also run a device-to-device copy of this many GB "
                          "(reference point for HBM byte counters)")
@@ -250,14 +252,16 @@ def main():
                 "weak_max_rel_err": float(rel[~strong].max())
                 if (~strong).any() else 0.0}
     prec_check = check(args.precision)
-    proj_traffic = None
+    proj_traffic, proj_launch = None, None
     if args.precision == 0:  # check() ran batch 0 last: the tuned shape
         chosen = [ln for ln in tune_log.splitlines()
                   if ln.startswith("projection") and "->" in ln]
-        if chosen:
-            label = chosen[-1].split("->")[1].split()[0]
-            proj_traffic = measured_traffic(label,
-                                            eng.get_tuning("proj_grid"))
+        label = chosen[-1].split("->")[1].split()[0] if chosen else None
+        proj_launch = {"tiles": label,
+                       "split_k": eng.get_tuning("proj_split"),
+                       "grid_items_batch0": eng.get_tuning("proj_grid")}
+        proj_traffic = measured_traffic(label, proj_launch[
+            "grid_items_batch0"])
     for v in variants:
         v["precision_check"] = check(v["precision"])
         log(f"precision check {v['dtype']}: {v['precision_check']}")
@@ -273,19 +277,22 @@ def main():
         from oracle import oracle as orc
         rows_t = prob.state_table()
         gidx = {(r[1], r[2], r[3]): i for i, r in enumerate(rows_t)}
-        I0 = gidx[(0, 0, 0)]
         finals = [gidx[(1, 1, f)] for f in range(nF)]
         thr = min(args.cpu_threads, os.cpu_count() or 1)
         del wl
         torch.cuda.empty_cache()
-        tc = time.perf_counter()
-        tr, _ = orc.run(prob, [I0], nthreads=thr, finals=finals)
-        cpu_s = time.perf_counter() - tc
+        sample = [gidx[(0, 0, lv)] for lv in range(args.cpu_initials)]
+        st = {}
+        tr, _ = orc.run(prob, sample, nthreads=thr, finals=finals,
+                        plain_fp64=True, stats=st)
+        cpu_s = st["seconds"]
         cpu = {"value": len(tr) / cpu_s, "unit": "transitions/s",
                "cores": thr, "kind": "port",
-               "sample": f"1 initial state (lowest J=20 A1) x all {nF} "
-                         f"J=21 A2 finals, {len(tr)} transitions, "
-                         f"{cpu_s:.1f} s incl. record staging"}
+               "sample": f"{len(sample)} lowest J=20 A1 initial states x all "
+                         f"{nF} J=21 A2 finals, {len(tr)} transitions in "
+                         f"{cpu_s:.1f} s: the reference's per-pair loop "
+                         f"(expand + half line strength per initial, expand "
+                         f"+ dot per final) in plain FP64 with OpenMP"}
         log(f"cpu baseline: {cpu['value']:.3e} transitions/s on {thr} "
             f"threads ({cpu_s:.1f}s)")
 
@@ -321,7 +328,7 @@ def main():
                              "datatype / hardware products per FMA; "
                              "traffic = counter HBM bytes of the chosen "
                              "launch on batch 0",
-                     "traffic": proj_traffic},
+                     "traffic": proj_traffic, "launch": proj_launch},
         "precision_check": prec_check,
         "variants": variants,
         "autotune": tune_log.strip().splitlines(),

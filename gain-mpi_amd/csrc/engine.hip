@@ -175,6 +175,17 @@ bool degfilter(const gain_ctx* c, int gI, int gF, int dI, int dF) {
   return false;
 }
 
+// Block-level selection rules of a (J_I, gamma_I) -> (J_F, gamma_F) pair:
+// gns > 0 on both sides, not both J = maxJ, |dJ| <= 1, J_I + J_F >= 1
+// (States.cpp:54-76) and the irrep pair (TroveInput.cpp:202-236).
+bool block_pair_ok(const gain_ctx* c, int jI, int gI, int jF, int gF) {
+  const gain_params& p = c->params;
+  return p.gns[gI] > 0.0 && p.gns[gF] > 0.0 &&
+         (jF != p.maxJ || jI != p.maxJ) &&
+         p.isym_pairs[gI] == p.isym_pairs[gF] && c->pair[gI] == gF &&
+         std::abs(jF - jI) <= 1 && jI + jF >= 1;
+}
+
 bool filter_lower(const gain_ctx* c, double e, int g) {
   const gain_params& p = c->params;
   return (e - p.zpe) <= p.erange_lower[1] && (e - p.zpe) >= p.erange_lower[0] &&
@@ -271,11 +282,7 @@ Plan make_plan(gain_ctx* c, int jindI, int gI, const int* levels, int nI,
     P.eI.push_back(e);
     P.okI.push_back(filter_lower(c, e, gI) ? 1 : 0);
   }
-  const int jI = P.bI->J, jF = P.bF->J;
-  P.block_ok = p.gns[gI] > 0.0 && p.gns[gF] > 0.0 &&
-               (jF != p.maxJ || jI != p.maxJ) &&
-               p.isym_pairs[gI] == p.isym_pairs[gF] && c->pair[gI] == gF &&
-               std::abs(jF - jI) <= 1 && jI + jF >= 1;
+  P.block_ok = block_pair_ok(c, P.bI->J, gI, P.bF->J, gF);
   const int ndegI = p.sym_degen[gI], ndegF = p.sym_degen[gF];
   for (int dI = 0; dI < ndegI; ++dI)
     if (degfilter(c, gI, c->pair[gI], dI, 0)) P.DI.push_back(dI);

@@ -286,6 +286,8 @@ Block& block_of(gain_ctx* c, int jind, int gamma);
 void ensure_slots(gain_ctx* c, Basis& b);
 int slices_of(int precision);
 bool degfilter(const gain_ctx* c, int gI, int gF, int dI, int dF);
+bool filter_lower(const gain_ctx* c, double e, int g);
+bool block_pair_ok(const gain_ctx* c, int jI, int gI, int jF, int gF);
 Plan make_plan(gain_ctx* c, int jindI, int gI, const int* levels, int nI,
                int jindF, int gF);
 int gemm_cfg(gain_ctx* c, int kind, GemmArgs g);

@@ -25,16 +25,35 @@ constexpr int kPad = 1;
 // Rows of output tiles walked together (see tile_order).
 constexpr int kGroupRows = 8;
 
-// One BM x BN output tile, K range [kbeg, kend), 4 waves arranged WM x WN;
-// each wave owns (BM/WM) x (BN/WN) = MT x NT MFMA tiles of 16 x 16.
+// LDS layout of one stage: A as [BM][BK+1] when A is k-contiguous in
+// memory, else k-major [BK][BM+16] (rows 16 doubles apart in bank space, so
+// the 4 k rows a fragment read touches fall in different bank halves); B as
+// [BN][BK+1].  Two stages (one barrier per k step) when they fit in 80 KB,
+// so two workgroups share a CU.
+template <int BM, int BN, int BK, bool AK>
+struct TileLds {
+  static constexpr int kPadT = 16;
+  static constexpr int SA = AK ? BM * (BK + kPad) : BK * (BM + kPadT);
+  static constexpr int SB = BN * (BK + kPad);
+  static constexpr bool DB = 2 * (SA + SB) * 8 <= 80 * 1024;
+  static constexpr int NB = DB ? 2 : 1;
+  static constexpr int SIZE = NB * (SA + SB);
+  __device__ static int a_at(int m, int k) {
+    return AK ? m * (BK + kPad) + k : k * (BM + kPadT) + m;
+  }
+  __device__ static int b_at(int n, int k) { return n * (BK + kPad) + k; }
+};
+
+// One BM x BN output tile, K range [kbeg, kend), WM x WN waves; each wave
+// owns (BM/WM) x (BN/WN) = MT x NT MFMA tiles of 16 x 16.  Global tiles are
+// loaded into registers one k step ahead.
 template <int BM, int BN, int BK, int WM, int WN, bool AK,
           int TH = WM * WN * 64>
 __device__ __forceinline__ void gemm_tile(const GemmArgs& g, const double* A,
                                           double* C, int m0, int n0,
                                           int kbeg, int kend, bool vecA,
-                                          bool vecB,
-                                          double (&As)[BM][BK + kPad],
-                                          double (&Bs)[BN][BK + kPad]) {
+                                          bool vecB, double* lds) {
+  using L = TileLds<BM, BN, BK, AK>;
   static_assert(WM * WN * 64 == TH, "waves x 64 threads");
   constexpr int MT = BM / WM / 16, NT = BN / WN / 16;
   constexpr int QA = BM * BK / 4 / TH, QB = BN * BK / 4 / TH;
@@ -94,19 +113,18 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, const double* A,
       }
     }
   };
-
-  if (kbeg < kend) load(kbeg);
-  for (int k0 = kbeg; k0 < kend; k0 += BK) {
-    __syncthreads();
+  auto store = [&](int buf) {
+    double* As = lds + buf * (L::SA + L::SB);
+    double* Bs = As + L::SA;
 #pragma unroll
     for (int q = 0; q < QA; ++q) {
       const int gi = tid + TH * q;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         if (AK)
-          As[gi / (BK / 4)][(gi % (BK / 4)) * 4 + e] = ra[q][e];
+          As[L::a_at(gi / (BK / 4), (gi % (BK / 4)) * 4 + e)] = ra[q][e];
         else
-          As[(gi % (BM / 4)) * 4 + e][gi / (BM / 4)] = ra[q][e];
+          As[L::a_at((gi % (BM / 4)) * 4 + e, gi / (BM / 4))] = ra[q][e];
       }
     }
 #pragma unroll
@@ -114,26 +132,56 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, const double* A,
       const int gi = tid + TH * q;
 #pragma unroll
       for (int e = 0; e < 4; ++e)
-        Bs[gi / (BK / 4)][(gi % (BK / 4)) * 4 + e] = rb[q][e];
+        Bs[L::b_at(gi / (BK / 4), (gi % (BK / 4)) * 4 + e)] = rb[q][e];
     }
-    __syncthreads();
-    if (k0 + BK < kend) load(k0 + BK);
+  };
+  auto compute = [&](int buf) {
+    const double* As = lds + buf * (L::SA + L::SB);
+    const double* Bs = As + L::SA;
 #pragma unroll
     for (int kk = 0; kk < BK / 4; ++kk) {
       const int kc = kk * 4 + (lane >> 4);
       double a[MT], b[NT];
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt)
-        a[mt] = As[wmi * (BM / WM) + mt * 16 + (lane & 15)][kc];
+        a[mt] = As[L::a_at(wmi * (BM / WM) + mt * 16 + (lane & 15), kc)];
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt)
-        b[nt] = Bs[wni * (BN / WN) + nt * 16 + (lane & 15)][kc];
+        b[nt] = Bs[L::b_at(wni * (BN / WN) + nt * 16 + (lane & 15), kc)];
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt)
           acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(
               a[mt], b[nt], acc[mt][nt], 0, 0, 0);
+    }
+  };
+
+  if (L::DB) {
+    // stage s is read while stage s^1 is filled; the one barrier per step
+    // orders this step's reads before the next step's overwrite
+    if (kbeg < kend) {
+      load(kbeg);
+      store(0);
+    }
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+      const bool more = k0 + BK < kend;
+      if (more) load(k0 + BK);
+      compute(buf);
+      if (more) store(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+    }
+  } else {
+    if (kbeg < kend) load(kbeg);
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+      __syncthreads();
+      store(0);
+      __syncthreads();
+      if (k0 + BK < kend) load(k0 + BK);
+      compute(0);
     }
   }
 #pragma unroll
@@ -155,8 +203,7 @@ __global__ __launch_bounds__(WM * WN * 64) void k_gemm_f64(GemmArgs g,
                                                            int kchunk,
                                                   int ntn, bool vecA,
                                                   bool vecB) {
-  __shared__ double As[BM][BK + kPad];
-  __shared__ double Bs[BN][BK + kPad];
+  __shared__ double lds[TileLds<BM, BN, BK, AK>::SIZE];
   // linear id -> (z slice, split, tile): each XCD takes a contiguous run
   // of the grouped tile order, so the workgroups resident on one XCD cover
   // a compact block of rows and columns and share operand slabs in its L2
@@ -172,7 +219,7 @@ __global__ __launch_bounds__(WM * WN * 64) void k_gemm_f64(GemmArgs g,
   const int kbeg = ks * kchunk;
   const int kend = min(g.K, kbeg + kchunk);
   gemm_tile<BM, BN, BK, WM, WN, AK>(g, A, C, tm * BM, tn * BN, kbeg, kend,
-                                    vecA, vecB, As, Bs);
+                                    vecA, vecB, lds);
 }
 
 // Persistent launch over the tiles of several GEMM problems (one workgroup
@@ -182,8 +229,7 @@ template <int BM, int BN, int BK, int WM, int WN, bool AK>
 __global__ __launch_bounds__(WM * WN * 64) void k_gemm_f64_grouped(
     const GemmArgs* __restrict__ probs, const GroupTile* __restrict__ tiles,
     int ntiles) {
-  __shared__ double As[BM][BK + kPad];
-  __shared__ double Bs[BN][BK + kPad];
+  __shared__ double lds[TileLds<BM, BN, BK, AK>::SIZE];
   for (int it = blockIdx.x; it < ntiles; it += gridDim.x) {
     const GroupTile tl = tiles[it];
     const GemmArgs& g = probs[tl.prob];
@@ -193,7 +239,7 @@ __global__ __launch_bounds__(WM * WN * 64) void k_gemm_f64_grouped(
     const bool vecB = (g.ldb % 2 == 0) && ((uintptr_t)g.B % 16 == 0);
     __syncthreads();  // LDS of the previous tile fully consumed
     gemm_tile<BM, BN, BK, WM, WN, AK>(g, A, C, tl.tm * BM, tl.tn * BN, 0,
-                                      g.K, vecA, vecB, As, Bs);
+                                      g.K, vecA, vecB, lds);
   }
 }
 

@@ -28,7 +28,7 @@ EXPORTS = (
     "gain_compute_half_linestrength", "gain_ddot", "gain_adjoint_dipole",
     "gain_set_dipole_combination", "gain_dipole_coeff_grad",
     "gain_set_tuning", "gain_get_tuning", "gain_tuning_report",
-    "gain_sweep_create",
+    "gain_sweep_create", "gain_format_line", "gain_write_transitions",
     "gain_sweep_run", "gain_sweep_destroy",
 )
 
@@ -57,6 +57,11 @@ class Lines(C.Structure):
     _fields_ = [("ls", C.c_void_p), ("S", C.c_void_p), ("A", C.c_void_p),
                 ("I", C.c_void_p), ("flags", C.c_void_p),
                 ("on_device", C.c_int)]
+
+
+class OutputOpts(C.Structure):
+    _fields_ = [("full_line", C.c_int), ("compute_intensity", C.c_int),
+                ("batch", C.c_int), ("precision", C.c_int)]
 
 
 class Task(C.Structure):
@@ -110,6 +115,11 @@ def load():
         "gain_batched_lines": (_ip, [_vp, _ip, _ip, _vp, _ip, _ip, _ip, _ip,
                                      C.POINTER(Lines)]),
         "gain_set_timing": (_ip, [_vp, _ip]),
+        "gain_format_line": (_ip, [C.c_char_p, _ip, _dp, _ip, _ip, _ip, _ip,
+                                   _ip, _ip, _dp, _ip, _dp, _vp, _ip]),
+        "gain_write_transitions": (_ip, [_vp, C.POINTER(OutputOpts),
+                                         C.c_char_p,
+                                         C.POINTER(C.c_int64)]),
         "gain_get_timing": (_ip, [_vp, C.c_char_p, C.POINTER(_dp),
                                   C.POINTER(C.c_int64), _ip]),
         "gain_set_tuning": (_ip, [_vp, C.c_char_p, _ip]),

@@ -237,6 +237,18 @@ class Engine:
     def set_tuning(self, key: str, value: int):
         check(self._L.gain_set_tuning(self._h, key.encode(), int(value)))
 
+    def write_transitions(self, path, full_line=False, intensity=True,
+                          batch=64, precision=_lib.PREC_FP64) -> int:
+        """Every line of the uploaded blocks in the reference's output format
+        (main.cpp:238-338, Output.cpp:52-112); returns the line count."""
+        o = _lib.OutputOpts(int(full_line), int(intensity), int(batch),
+                            int(precision))
+        n = C.c_int64(0)
+        check(self._L.gain_write_transitions(
+            self._h, C.byref(o), None if path is None else str(path).encode(),
+            C.byref(n)))
+        return int(n.value)
+
     def get_tuning(self, key: str) -> int:
         v = C.c_longlong(0)
         check(self._L.gain_get_tuning(self._h, key.encode(), C.byref(v)))

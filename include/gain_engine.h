@@ -232,6 +232,30 @@ int gain_get_tuning(gain_ctx* ctx, const char* key, long long* value);
 /* The autotuner's timings and choices, one line per GEMM shape. */
 int gain_tuning_report(gain_ctx* ctx, char* buf, int len);
 
+/* ---- transition list (main.cpp:238-338 + Output.cpp:52-112) ----------- */
+typedef struct {
+  int full_line;          /* print the components "[ ... ] " (Output.cpp:64,105) */
+  int compute_intensity;  /* print the intensity column (Output.cpp:100-103) */
+  int batch;              /* initial states per batched call (0: 64) */
+  int precision;          /* gain_precision of the contractions */
+} gain_output_opts;
+/* One line exactly as Output::OutputLinestrength prints it
+ * ("%12.6f %8d %4d %4d <- %8d %4d %4d %16.9E " ... " || \n"); levels and
+ * irreps are 0-based and printed +1 as the reference does.  nls = -1: no
+ * components block; nls >= 0: "[ " + nls x " %16.9E" + " ] ". */
+int gain_format_line(char* buf, int len, double nu, int levelF, int jF,
+                     int gammaF, int levelI, int jI, int gammaI, double A,
+                     int with_intensity, double intensity, const double* ls,
+                     int nls);
+/* Every line of every uploaded (J, gamma) block against every block the
+ * selection rules allow (the main.cpp:238-338 loop: FilterLowerState,
+ * irrep pair, |dJ| <= 1, FilterIntensity, threshold), written to `path`
+ * (NULL: stdout).  Initial states in energy order, each one's finals in
+ * energy order.  All participating blocks need their eigenvectors
+ * resident.  *nlines = lines written. */
+int gain_write_transitions(gain_ctx* ctx, const gain_output_opts* opts,
+                           const char* path, int64_t* nlines);
+
 /* ---- kernel-level drop-ins for GPU/dipole_GPU.h ------------------------- *
  * Same argument meaning as the reference's free functions; pointers are
  * device memory, `stream` is a hipStream_t (NULL = default stream).

@@ -27,7 +27,18 @@ static inline void dd_add(dd_t* a, double b) {
   a->lo += err;
 }
 
+/* Arithmetic of the accumulations: 0 = double-double (the checker), 1 =
+ * plain FP64 multiply-adds, the arithmetic of the reference's own kernels
+ * and of cuBLAS (used only to time a CPU baseline). */
+static int g_plain_fp64 = 0;
+
+void orc_set_plain_fp64(int plain) { g_plain_fp64 = plain != 0; }
+
 static inline void dd_add_prod(dd_t* a, double x, double y) {
+  if (g_plain_fp64) {
+    a->hi += x * y;
+    return;
+  }
   double p = x * y;
   double e = fma(x, y, -p);
   dd_add(a, p);
@@ -215,6 +226,14 @@ void orc_half_ls(int jI, int dimenI, const int* kstartI, const int* kblockI,
 
 /* cublasDdot replacement (GpuManager.cpp:741), compensated. */
 double orc_dot(int64_t n, const double* x, const double* y) {
+  if (g_plain_fp64) { /* eight independent FP64 partial sums */
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t i = 0;
+    for (; i + 8 <= n; i += 8)
+      for (int q = 0; q < 8; q++) s[q] += x[i + q] * y[i + q];
+    for (; i < n; i++) s[0] += x[i] * y[i];
+    return ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+  }
   dd_t acc = {0.0, 0.0};
   for (int64_t i = 0; i < n; i++) dd_add_prod(&acc, x[i], y[i]);
   return dd_val(acc);

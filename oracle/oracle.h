@@ -61,6 +61,8 @@ void orc_half_ls(int jI, int dimenI, const int* kstartI, const int* kblockI,
                  double* h);
 
 double orc_dot(int64_t n, const double* x, const double* y);
+/* 1: plain FP64 accumulations instead of double-double (CPU timing only). */
+void orc_set_plain_fp64(int plain);
 
 /* Intensity / filter parameters (Input.h getters used by States/Output). */
 typedef struct {

@@ -9,6 +9,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 import subprocess
+import time
 
 import numpy as np
 
@@ -95,6 +96,8 @@ def lib():
             [C.c_int, C.c_int] + [_i32p] * 4 + [C.c_int, C.c_int] +
             [_i32p] * 4 + [_f64p, C.c_int, C.c_int, C.c_int, _f64p, C.c_int,
                            _f64p, _f64p])
+        L.orc_set_plain_fp64.restype = None
+        L.orc_set_plain_fp64.argtypes = [C.c_int]
         L.orc_dot.restype = C.c_double
         L.orc_dot.argtypes = [C.c_int64, _f64p, _f64p]
         L.orc_gamma_pairs.restype = C.c_int
@@ -235,10 +238,14 @@ def basis_struct(b, keep) -> OrcBasis:
     return s
 
 
-def run(prob, initial_states, nthreads: int = 0, finals=None):
+def run(prob, initial_states, nthreads: int = 0, finals=None,
+        plain_fp64: bool = False, stats=None):
     """main.cpp:238-338 over `initial_states` (indices into the energy-sorted
     state table).  `finals` (optional) restricts the state table to those
     indices plus the initials (used to bound CPU time on large problems).
+    `plain_fp64` switches the accumulations from double-double to plain FP64
+    (CPU timing only); `stats` (a dict) receives the seconds spent in the
+    compute call, without the record staging.
     Returns (structured transitions array, state table rows)."""
     rows = prob.state_table()
     keep_idx = None
@@ -278,10 +285,17 @@ def run(prob, initial_states, nthreads: int = 0, finals=None):
     cap = max(1, len(init_k) * n)
     out = np.zeros(cap, TRANSITION_DTYPE)
     assert out.dtype.itemsize == C.sizeof(OrcTransition)
-    cnt = lib().orc_run(C.byref(ps), bases, len(prob.bases), C.byref(st),
-                        np.ascontiguousarray(prob.mu).reshape(-1), prob.nc,
-                        prob.jmax, init, len(init_k),
-                        out.ctypes.data_as(C.c_void_p), cap, nthreads)
+    mu = np.ascontiguousarray(prob.mu).reshape(-1)
+    lib().orc_set_plain_fp64(1 if plain_fp64 else 0)
+    t0 = time.perf_counter()
+    try:
+        cnt = lib().orc_run(C.byref(ps), bases, len(prob.bases), C.byref(st),
+                            mu, prob.nc, prob.jmax, init, len(init_k),
+                            out.ctypes.data_as(C.c_void_p), cap, nthreads)
+    finally:
+        lib().orc_set_plain_fp64(0)
+    if stats is not None:
+        stats["seconds"] = time.perf_counter() - t0
     if cnt < 0:
         raise RuntimeError("oracle run failed")
     out = out[:cnt].copy()

@@ -360,3 +360,22 @@ def test_format_line_literal():
                           "    1 ")
     assert f"{A:16.9E} " in txt
     assert txt.endswith("[   5.000000000E-01 ]  || \n")
+
+
+def test_plain_fp64_timing_mode_computes_the_same_lines():
+    """The plain-FP64 accumulation mode (used only to time the CPU baseline)
+    runs the same loop: same transitions, S within FP64 rounding."""
+    prob = synth.make_problem([2, 3], 12, "c2v4")
+    prob.params.erange_lower = (0.0, 1e9)
+    init = list(range(0, 40, 3))
+    dd, _ = orc.run(prob, init)
+    st = {}
+    pl, _ = orc.run(prob, init, plain_fp64=True, stats=st)
+    assert st["seconds"] >= 0.0
+    assert np.array_equal(dd[["iI", "iF", "emitted"]], pl[["iI", "iF",
+                                                           "emitted"]])
+    big = dd["S"] > 1e-6 * dd["S"].max()
+    np.testing.assert_allclose(pl["S"][big], dd["S"][big], rtol=1e-10)
+    # and the checker mode is back on afterwards
+    again, _ = orc.run(prob, init)
+    assert np.array_equal(again["S"], dd["S"])
