@@ -58,6 +58,100 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def run_mixed_sweep(nreplay):
+    """Mixed-J sweep (256 initial states from J_I in [5, 30], Nc = 3000,
+    2000 states per block, every dJ in {-1, 0, +1}): one captured sweep
+    replayed `nreplay` times, and the same tasks through the unbatched plain
+    FP64 path; transition flags and S compared between the two."""
+    import numpy as np
+    import torch
+
+    from gain_engine import workload
+    from gain_engine.engine import Sweep
+
+    t0 = time.time()
+    w = workload.build_mixed()
+    eng = w.eng
+    tasks = w.tasks()
+    dev = torch.device("cuda:0")
+    sw_out, pl_out, full = [], [], []
+    for (jI, gI, levels, jF, gF) in tasks:
+        nF = w.prob.blocks[(jF, gF)].n
+        o = {k: torch.zeros((len(levels), nF), dtype=torch.float64,
+                            device=dev) for k in ("S", "A", "I")}
+        o["flags"] = torch.zeros((len(levels), nF), dtype=torch.uint8,
+                                 device=dev)
+        q = {k: torch.zeros_like(v) for k, v in o.items()}
+        sw_out.append(o)
+        pl_out.append(q)
+        full.append((jI, gI, levels, jF, gF,
+                     {k: v.data_ptr() for k, v in o.items()}))
+    torch.cuda.synchronize()
+    build_s = time.time() - t0
+    t1 = time.time()
+    sw = Sweep(eng, full)
+    create_s = time.time() - t1
+    stream = torch.cuda.ExternalStream(eng.stream())
+    e0, e1 = torch.cuda.Event(enable_timing=True), \
+        torch.cuda.Event(enable_timing=True)
+    sw.run()
+    eng.synchronize()
+    e0.record(stream)
+    for _ in range(nreplay):
+        sw.run()
+    e1.record(stream)
+    eng.synchronize()
+    sweep_ms = e0.elapsed_time(e1) / nreplay
+    trans = int(sum(int((o["flags"] & 1).sum().item()) for o in sw_out))
+    # the same tasks unbatched: one initial state per plain call
+    e0.record(stream)
+    for (jI, gI, levels, jF, gF), q in zip(tasks, pl_out):
+        for i, lv in enumerate(levels):
+            eng.lines_device(jI, gI, [lv], jF, gF,
+                             {k: v[i].data_ptr() for k, v in q.items()},
+                             plain=True)
+    e1.record(stream)
+    eng.synchronize()
+    plain_ms = e0.elapsed_time(e1)
+    same_flags, worst = True, 0.0
+    for o, q in zip(sw_out, pl_out):
+        same_flags &= bool(torch.equal(o["flags"], q["flags"]))
+        S, R = o["S"], q["S"]
+        rmax = R.max(dim=1, keepdim=True).values
+        bound = 2e-12 * R + 2e-14 * torch.sqrt(R * rmax)
+        d = (S - R).abs()
+        ok = d <= bound
+        if not bool(ok.all()):
+            worst = max(worst, float((d / bound.clamp_min(1e-300)).max()))
+    sw.close()
+    nI = len(w.initials)
+    r = {"workload": "256 initial states drawn from J_I in [5,30] (4 irreps, "
+                     "N_J=(2J+1)*750), Nc=3000, 2000 states per block, "
+                     "every dJ in {-1,0,+1} paired block",
+         "tasks": len(tasks), "initial_states": nI,
+         "final_record_gb": sum(v.numel() for v in w.records.values()) * 8
+         / 1e9,
+         "transitions": trans, "ms_per_sweep": sweep_ms,
+         "value": trans / (sweep_ms / 1e3), "unit": "transitions/s",
+         "plain_unbatched_ms": plain_ms,
+         "plain_unbatched_tps": trans / (plain_ms / 1e3),
+         "speedup_vs_plain": plain_ms / sweep_ms,
+         "graph_capture_s": create_s, "build_s": build_s,
+         "transition_flags_identical": same_flags,
+         "s_within_fp64_bound": worst == 0.0,
+         "note": "one hipGraph replay per sweep; the bound is |dS| <= "
+                 "2e-12 S + 2e-14 sqrt(S S_max) per initial state"}
+    log(f"mixed-J sweep: {len(tasks)} tasks, {trans} transitions, "
+        f"{sweep_ms:.2f} ms per replay -> {r['value']:.3e}/s; unbatched "
+        f"plain {plain_ms:.1f} ms; flags identical {same_flags}, S within "
+        f"bound {worst == 0.0}")
+    del sw_out, pl_out
+    w.close()
+    del w
+    torch.cuda.empty_cache()
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,8 +166,10 @@ def main():
     ap.add_argument("--ninitial", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="skip the mixed-J sweep measurement")
     ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--cpu-initials", type=int, default=3,
+    ap.add_argument("--cpu-initials", type=int, default=8,
                     help="initial states in the CPU baseline sample")
     ap.add_argument("--copy-calib-gb", type=float, default=0.0,
                     help="also run a device-to-device copy of this many GB "
@@ -270,17 +366,20 @@ def main():
     batched_ms = head["ms"]
     gemm_tfs = head["gemm_proj_tflops"]
 
-    # ---------------- CPU baseline (oracle, bounded sample)
+    # ---------------- mixed-J sweep (hipGraph), after freeing the workload
     cpu = None
     wl.close()
+    del wl, outs, bflags, pl, pflags
+    torch.cuda.empty_cache()
+    sweep = None if args.no_sweep else run_mixed_sweep(max(args.steps, 1))
+
+    # ---------------- CPU baseline (oracle, bounded sample)
     if not args.no_cpu_baseline:
         from oracle import oracle as orc
         rows_t = prob.state_table()
         gidx = {(r[1], r[2], r[3]): i for i, r in enumerate(rows_t)}
         finals = [gidx[(1, 1, f)] for f in range(nF)]
         thr = min(args.cpu_threads, os.cpu_count() or 1)
-        del wl
-        torch.cuda.empty_cache()
         sample = [gidx[(0, 0, lv)] for lv in range(args.cpu_initials)]
         st = {}
         tr, _ = orc.run(prob, sample, nthreads=thr, finals=finals,
@@ -337,6 +436,7 @@ def main():
                            "unit": "GB/s", "frac": gemv_gbs / HBM_PEAK_GBS,
                            "device_copy_gbs": copy_gbs,
                            "traffic": gemv_traffic},
+        "mixed_j_sweep": sweep,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res))

@@ -4,6 +4,9 @@
 // Tasks with the same initial batch (J index, irrep, levels) share X and
 // Y = mu X; both GEMM stages (mu X for every batch, C_F G for every task)
 // run as single persistent grouped launches over the tiles of all problems.
+// A task that projects at most 9 vectors streams its final records through
+// the FP64 GEMV instead (one read of the records; a 128-wide GEMM tile
+// would be mostly padding).
 // Everything a replay touches is owned by the sweep (sized at creation), so
 // replays stay valid whatever the context does in between.  The launch
 // sequence contains kernels only (no memset or memcpy nodes).
@@ -36,6 +39,8 @@ struct gain_sweep {
     int nvpi = 0, nvecG = 0, nR = 0;
     int64_t gstride = 0, hstride = 0;
     bool compute = false;
+    bool gemv = false;  // projection by GEMV (few vectors), else GEMM tiles
+    int ksplit = 1;
     double* H = nullptr;
     double* G = nullptr;
     double* LS = nullptr;
@@ -187,6 +192,12 @@ void build(gain_sweep* s, const gain_task* tasks, int ntasks) {
     GAIN_HIP(hipMemset(T.H + (size_t)G.nvecX * T.hstride, 0,
                        sizeof(double) * T.hstride));
     T.G = s->arena.alloc<double>((size_t)T.nvecG * T.gstride);
+    if (T.nvecG <= 9) {
+      T.gemv = true;
+      T.ksplit = c->gemv_ksplit;
+      T.LS = s->arena.alloc<double>((size_t)T.ksplit * T.nvecG * T.nR);
+      continue;
+    }
     T.LS = s->arena.alloc<double>((size_t)T.nvecG * T.nR);
     GemmArgs a{};
     a.M = T.nR;
@@ -294,6 +305,22 @@ void enqueue(gain_sweep* s, hipStream_t st) {
   gemm_f64_grouped(s->d_pj_probs, s->d_pj_tiles, s->n_pj_tiles, true,
                    s->pj_cfg, s->grid, st);
   for (gain_sweep::Task& T : s->tasks) {
+    if (!T.compute || !T.gemv) continue;
+    GemvArgs a{};
+    a.C = T.P.blkF->vec;
+    a.ld = T.P.blkF->ld;
+    a.ntot = T.P.bF->irrep[T.P.gF].ntot;
+    a.F0 = T.P.Flo;
+    a.nF = T.nR;
+    a.G = T.G;
+    a.gstride = T.gstride;
+    a.nvec = T.nvecG;
+    a.partial = T.LS;
+    a.ksplit = T.ksplit;
+    a.rows = c->gemv_rows;
+    launch_gemv(a, st);
+  }
+  for (gain_sweep::Task& T : s->tasks) {
     LinesArgs la{};
     la.nI = T.P.nI;
     la.nF = T.P.blkF->n;
@@ -303,9 +330,16 @@ void enqueue(gain_sweep* s, hipStream_t st) {
     la.nvec_per_i = T.nvpi;
     la.slot_of_vec = T.slot_of_vec;
     la.src = T.LS;
-    la.ksplit = 1;
-    la.src_f = 1;
-    la.src_v = T.nR;
+    if (T.gemv) {  // partials [ks][F][vec]
+      la.ksplit = T.ksplit;
+      la.src_ks = (int64_t)T.nR * T.nvecG;
+      la.src_f = T.nvecG;
+      la.src_v = 1;
+    } else {  // [vec][F]
+      la.ksplit = 1;
+      la.src_f = 1;
+      la.src_v = T.nR;
+    }
     la.eI = T.eI;
     la.levelI = T.levelI;
     la.okI = T.okI;

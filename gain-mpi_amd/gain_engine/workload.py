@@ -97,3 +97,73 @@ def expected_flags(prob, eI: float, eF: np.ndarray) -> np.ndarray:
         ((eF - p.zpe) <= p.erange_upper[1]) & \
         ((eF - p.zpe) >= p.erange_upper[0])
     return ok & low_ok
+
+
+@dataclasses.dataclass
+class MixedWorkload:
+    """Every (J, irrep) block of a J range, `nstates` states each, records
+    resident in HBM; `initials` are (jind, gamma, level) triples."""
+    prob: synth.Problem
+    eng: Engine
+    records: dict  # (jind, gamma) -> torch tensor [nstates, N_J]
+    initials: list
+
+    def close(self):
+        self.eng.close()
+
+    def tasks(self):
+        """Sweep tasks: the initial states grouped by block, each group
+        against every final block with |dJ| <= 1 and the paired irrep
+        (main.cpp:238-338 selection).  Returns [(jI, gI, levels, jF, gF)]."""
+        by = {}
+        for jind, g, lv in self.initials:
+            by.setdefault((jind, g), []).append(lv)
+        out = []
+        for (jI, gI), levels in sorted(by.items()):
+            gF = self.eng.gamma_pair(gI)
+            for jF in (jI - 1, jI, jI + 1):
+                if (jF, gF) in self.prob.blocks:
+                    out.append((jI, gI, sorted(levels), jF, gF))
+        return out
+
+
+def build_mixed(nc: int = 3000, jmin: int = 5, jmax: int = 30,
+                nstates: int = 2000, ninitial: int = 256, seed: int = 7,
+                device: int = 0) -> MixedWorkload:
+    """Mixed-J workload: C2v-like 4-irrep bases for J = jmin-1 .. jmax+1
+    (N_J = (2J+1) Nc / 4 per irrep, uneven), `nstates` states per block with
+    orthonormal Kronecker-factor records generated in HBM, and `ninitial`
+    initial states drawn uniformly from the blocks with jmin <= J <= jmax."""
+    import torch
+
+    jvals = list(range(jmin - 1, jmax + 2))
+    nrep = 4
+    ns = {(j, g): nstates for j in range(len(jvals)) for g in range(nrep)}
+    prob = synth.make_problem(jvals, nc, "c2v4", kron=True, nstates=ns)
+    dev = torch.device(f"cuda:{device}")
+    eng = Engine(device, prob.jmax, prob.params.nrepres, prob.params.maxdeg)
+    eng.set_params(prob.params)
+    eng.upload_dipole(prob.mu)
+    for jind, bs in enumerate(prob.bases):
+        eng.upload_basis(jind, bs)
+        eng.upload_inflation(jind, bs)
+    records = {}
+    for key, blk in sorted(prob.blocks.items()):
+        qa, qb = blk.kron
+        b = qb.shape[0]
+        ka = (blk.n + b - 1) // b
+        R = torch.kron(torch.as_tensor(qa.T[:ka].copy(), device=dev),
+                       torch.as_tensor(qb.T.copy(), device=dev))[:blk.n]
+        R = R.contiguous()
+        records[key] = R
+        eng.bind_block_device(key[0], key[1], blk.energy, R.data_ptr(),
+                              blk.rec_len, blk.rec_len)
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(seed)
+    pool = [(j, g) for (j, g) in sorted(prob.blocks)
+            if jmin <= prob.bases[j].J <= jmax]
+    picks = set()
+    while len(picks) < ninitial:
+        j, g = pool[int(rng.integers(len(pool)))]
+        picks.add((j, g, int(rng.integers(nstates))))
+    return MixedWorkload(prob, eng, records, sorted(picks))

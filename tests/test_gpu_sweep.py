@@ -81,3 +81,55 @@ def test_mixed_j_sweep_graph_replays_match_unbatched(cfgs):
                               if blk[0] + 1 < len(prob.bases) else blk[0],
                               eng.gamma_pair(blk[1]))
         sw.close()
+
+
+@pytest.mark.slow
+def test_mixed_j_sweep_full_shape_matches_unbatched():
+    """256 initial states drawn from J_I in [5, 30] (Nc = 3000, uneven
+    blocks N_J = (2J+1) Nc / 4, 2000 states per block, 48 GB of records in
+    HBM), every dJ in {-1, 0, +1} final block, one captured sweep replayed
+    twice, against the unbatched plain FP64 path: transition flags
+    identical; S within the condition-normalised FP64 bound (the two paths
+    sum in different orders)."""
+    import torch
+    from gain_engine import workload
+    w = workload.build_mixed()
+    try:
+        eng = w.eng
+        tasks = w.tasks()
+        outs, full = [], []
+        for (jI, gI, levels, jF, gF) in tasks:
+            nF = w.prob.blocks[(jF, gF)].n
+            o = {k: torch.zeros((len(levels), nF), dtype=torch.float64,
+                                device="cuda:0") for k in ("S",)}
+            o["flags"] = torch.zeros((len(levels), nF), dtype=torch.uint8,
+                                     device="cuda:0")
+            outs.append(o)
+            full.append((jI, gI, levels, jF, gF,
+                         {k: v.data_ptr() for k, v in o.items()}))
+        sw = Sweep(eng, full)
+        for rep in range(2):
+            for o in outs:
+                for v in o.values():
+                    v.zero_()
+            torch.cuda.synchronize()
+            sw.run()
+            eng.synchronize()
+            nlines = 0
+            for (jI, gI, levels, jF, gF), o in zip(tasks, outs):
+                fl = o["flags"].cpu().numpy()
+                S = o["S"].cpu().numpy()
+                for i, lv in enumerate(levels):
+                    r = eng.plain_lines(jI, gI, lv, jF, gF, want_ls=False)
+                    assert np.array_equal(r["flags"], fl[i]), (jI, gI, lv, jF)
+                    ref = r["S"]
+                    if ref.max() <= 0:
+                        continue
+                    d = np.abs(S[i] - ref)
+                    assert (d <= 2e-12 * ref + 2e-14 * np.sqrt(
+                        ref * ref.max())).all(), (jI, gI, lv, jF)
+                    nlines += int((fl[i] & 1).sum())
+            assert nlines > 100000
+        sw.close()
+    finally:
+        w.close()
