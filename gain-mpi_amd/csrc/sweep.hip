@@ -2,16 +2,21 @@
 // sequence captured as a HIP graph.
 //
 // Tasks with the same initial batch (J index, irrep, levels) share X and
-// Y = mu X; both GEMM stages (mu X for every batch, C_F G for every task)
-// run as single persistent grouped launches over the tiles of all problems.
-// A task that projects at most 9 vectors streams its final records through
-// the FP64 GEMV instead (one read of the records; a 128-wide GEMM tile
-// would be mostly padding).
+// Y = mu X.  The X of every batch sits in one column range of a single
+// matrix, so mu X for the whole sweep is one GEMM (tile shape from the
+// autotuner); the projections C_F G of the tasks with many vectors run as
+// one persistent grouped launch over the tiles of all of them.
+// Tasks that project at most 9 vectors go through the FP64 GEMV instead (a
+// 128-wide GEMM tile would be mostly padding); such tasks that target the
+// same final block are packed into one GEMV of up to 9 vectors over the
+// union of their energy windows, so the block's records are read once per
+// pack rather than once per task.
 // Everything a replay touches is owned by the sweep (sized at creation), so
 // replays stay valid whatever the context does in between.  The launch
 // sequence contains kernels only (no memset or memcpy nodes).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <map>
 #include <tuple>
 #include <vector>
@@ -40,7 +45,7 @@ struct gain_sweep {
     int64_t gstride = 0, hstride = 0;
     bool compute = false;
     bool gemv = false;  // projection by GEMV (few vectors), else GEMM tiles
-    int ksplit = 1;
+    int unit = -1, vecoff = 0;  // GEMV pack and first vector inside it
     double* H = nullptr;
     double* G = nullptr;
     double* LS = nullptr;
@@ -52,11 +57,22 @@ struct gain_sweep {
     uint8_t* okI = nullptr;
     gain_lines out{};
   };
+  // one GEMV over one final block for a pack of tasks
+  struct Unit {
+    const Block* blkF = nullptr;
+    int ntot = 0;
+    int64_t gstride = 0;
+    int Flo = 0, Fhi = 0, nvec = 0, ksplit = 1;
+    double* G = nullptr;
+    double* part = nullptr;  // [ksplit][Fhi-Flo][nvec]
+  };
   std::vector<Group> groups;
   std::vector<Task> tasks;
-  GemmArgs* d_hl_probs = nullptr;
-  GroupTile* d_hl_tiles = nullptr;
-  int n_hl_tiles = 0, hl_cfg = 2;
+  std::vector<Unit> units;
+  // mu X of every batch: one GEMM over [X of batch 0 | X of batch 1 | ...]
+  GemmArgs hl{};
+  bool has_hl = false;
+  int64_t ncol_all = 0;
   GemmArgs* d_pj_probs = nullptr;
   GroupTile* d_pj_tiles = nullptr;
   int n_pj_tiles = 0, pj_cfg = 5;
@@ -113,9 +129,8 @@ void build(gain_sweep* s, const gain_task* tasks, int ntasks) {
            t);
     s->tasks.push_back(T);
   }
-  // ---- groups: X, Y, scatter arrays
-  std::vector<GemmArgs> hl;
-  std::vector<GroupTile> hlt;
+  // ---- groups: X, Y (column ranges of one matrix), scatter arrays
+  std::vector<int64_t> col0(s->groups.size(), 0);
   for (size_t g = 0; g < s->groups.size(); ++g) {
     gain_sweep::Group& G = s->groups[g];
     Basis& bI = *G.P.bI;
@@ -133,25 +148,36 @@ void build(gain_sweep* s, const gain_task* tasks, int ntasks) {
     if (G.nvecX == 0) continue;
     G.c_off = up(s, off);
     G.ideg = up(s, deg);
-    G.X = s->arena.alloc<double>((size_t)G.ncolTot * nc);
-    G.Y = s->arena.alloc<double>((size_t)3 * G.ncolTot * nc);
-    GemmArgs a{};
+    col0[g] = s->ncol_all;
+    s->ncol_all += G.ncolTot;
+  }
+  if (s->ncol_all > 0) {
+    GAIN_CHECK_ARG(s->ncol_all < (1ll << 31), "sweep: too many columns");
+    double* X = s->arena.alloc<double>((size_t)s->ncol_all * nc);
+    double* Y = s->arena.alloc<double>((size_t)3 * s->ncol_all * nc);
+    for (size_t g = 0; g < s->groups.size(); ++g) {
+      gain_sweep::Group& G = s->groups[g];
+      if (G.nvecX == 0) continue;
+      G.X = X + col0[g] * nc;
+      G.Y = Y + col0[g] * nc;  // alpha planes are ncol_all * nc apart
+    }
+    GemmArgs& a = s->hl;
     a.M = nc;
-    a.N = G.ncolTot;
+    a.N = (int)s->ncol_all;
     a.K = nc;
     a.A = c->d_mu;
     a.lda = nc;
     a.sAz = (int64_t)nc * nc;
     a.a_kcontig = false;
-    a.B = G.X;
+    a.B = X;
     a.ldb = nc;
-    a.C = G.Y;
+    a.C = Y;
     a.ldc = nc;
-    a.sCz = (int64_t)G.ncolTot * nc;
+    a.sCz = s->ncol_all * nc;
     a.nz = 3;
     a.ksplit = 1;
-    add_tiles(hlt, (int)hl.size(), a.M, a.N, 3, s->hl_cfg);
-    hl.push_back(a);
+    a.cfg = gemm_cfg(c, 1, a);  // tuned here: replays must not time
+    s->has_hl = true;
   }
   // ---- tasks: H, G, LS, per-pair arrays, projection problems
   std::vector<GemmArgs> pj;
@@ -191,13 +217,11 @@ void build(gain_sweep* s, const gain_task* tasks, int ntasks) {
     // the trailing zero vector is constant: cleared once, outside the graph
     GAIN_HIP(hipMemset(T.H + (size_t)G.nvecX * T.hstride, 0,
                        sizeof(double) * T.hstride));
-    T.G = s->arena.alloc<double>((size_t)T.nvecG * T.gstride);
-    if (T.nvecG <= 9) {
+    if (T.nvecG <= 9) {  // G and the partials come from its GEMV pack
       T.gemv = true;
-      T.ksplit = c->gemv_ksplit;
-      T.LS = s->arena.alloc<double>((size_t)T.ksplit * T.nvecG * T.nR);
       continue;
     }
+    T.G = s->arena.alloc<double>((size_t)T.nvecG * T.gstride);
     T.LS = s->arena.alloc<double>((size_t)T.nvecG * T.nR);
     GemmArgs a{};
     a.M = T.nR;
@@ -215,12 +239,45 @@ void build(gain_sweep* s, const gain_task* tasks, int ntasks) {
     add_tiles(pjt, (int)pj.size(), a.M, a.N, 1, s->pj_cfg);
     pj.push_back(a);
   }
-  (void)p;
-  if (!hl.empty()) {
-    s->d_hl_probs = up(s, hl);
-    s->d_hl_tiles = up(s, hlt);
-    s->n_hl_tiles = (int)hlt.size();
+  // ---- GEMV packs: tasks of one final block, in task order, up to 9
+  // vectors each
+  std::map<std::pair<int, int>, int> open_unit;
+  for (gain_sweep::Task& T : s->tasks) {
+    if (!T.compute || !T.gemv) continue;
+    const auto key = std::make_pair(T.P.jindF, T.P.gF);
+    auto it = open_unit.find(key);
+    if (it == open_unit.end() ||
+        s->units[it->second].nvec + T.nvecG > 9) {
+      gain_sweep::Unit U;
+      U.blkF = T.P.blkF;
+      U.ntot = T.P.bF->irrep[T.P.gF].ntot;
+      U.gstride = T.gstride;
+      U.Flo = T.P.Flo;
+      U.Fhi = T.P.Fhi;
+      U.ksplit = c->gemv_ksplit;
+      open_unit[key] = (int)s->units.size();
+      s->units.push_back(U);
+      it = open_unit.find(key);
+    }
+    gain_sweep::Unit& U = s->units[it->second];
+    T.unit = it->second;
+    T.vecoff = U.nvec;
+    U.nvec += T.nvecG;
+    U.Flo = std::min(U.Flo, T.P.Flo);
+    U.Fhi = std::max(U.Fhi, T.P.Fhi);
   }
+  for (gain_sweep::Unit& U : s->units) {
+    U.G = s->arena.alloc<double>((size_t)U.nvec * U.gstride);
+    U.part = s->arena.alloc<double>((size_t)U.ksplit * U.nvec *
+                                    (U.Fhi - U.Flo));
+  }
+  for (gain_sweep::Task& T : s->tasks) {
+    if (T.unit < 0) continue;
+    const gain_sweep::Unit& U = s->units[T.unit];
+    T.G = U.G + (int64_t)T.vecoff * U.gstride;
+    T.LS = U.part + T.vecoff;
+  }
+  (void)p;
   if (!pj.empty()) {
     s->d_pj_probs = up(s, pj);
     s->d_pj_tiles = up(s, pjt);
@@ -258,8 +315,7 @@ void enqueue(gain_sweep* s, hipStream_t st) {
     sa.xstride = (int64_t)bI.ncol * nc;
     launch_scatter_expand(sa, st);
   }
-  gemm_f64_grouped(s->d_hl_probs, s->d_hl_tiles, s->n_hl_tiles, false,
-                   s->hl_cfg, s->grid, st);
+  if (s->has_hl) gemm_f64(s->hl, st);
   for (gain_sweep::Task& T : s->tasks) {
     if (!T.compute) continue;
     const gain_sweep::Group& G = s->groups[T.group];
@@ -279,8 +335,8 @@ void enqueue(gain_sweep* s, hipStream_t st) {
     ga.vibF = bF.vib;
     ga.threej = c->d_threej;
     ga.Y = G.Y;
-    ga.ys_alpha = (int64_t)G.ncolTot * nc;
-    ga.ys_ks = 3 * (int64_t)G.ncolTot * nc;
+    ga.ys_alpha = s->ncol_all * nc;
+    ga.ys_ks = 3 * s->ncol_all * nc;
     ga.H = T.H;
     ga.hstride = T.hstride;
     launch_gather_h(ga, st);
@@ -304,20 +360,21 @@ void enqueue(gain_sweep* s, hipStream_t st) {
   }
   gemm_f64_grouped(s->d_pj_probs, s->d_pj_tiles, s->n_pj_tiles, true,
                    s->pj_cfg, s->grid, st);
-  for (gain_sweep::Task& T : s->tasks) {
-    if (!T.compute || !T.gemv) continue;
+  for (const gain_sweep::Unit& U : s->units) {
     GemvArgs a{};
-    a.C = T.P.blkF->vec;
-    a.ld = T.P.blkF->ld;
-    a.ntot = T.P.bF->irrep[T.P.gF].ntot;
-    a.F0 = T.P.Flo;
-    a.nF = T.nR;
-    a.G = T.G;
-    a.gstride = T.gstride;
-    a.nvec = T.nvecG;
-    a.partial = T.LS;
-    a.ksplit = T.ksplit;
-    a.rows = c->gemv_rows;
+    a.C = U.blkF->vec;
+    a.ld = U.blkF->ld;
+    a.ntot = U.ntot;
+    a.F0 = U.Flo;
+    a.nF = U.Fhi - U.Flo;
+    a.G = U.G;
+    a.gstride = U.gstride;
+    a.nvec = U.nvec;
+    a.partial = U.part;
+    a.ksplit = U.ksplit;
+    // with many vectors per record row, more rows per wave cut the re-reads
+    // of the projected vectors
+    a.rows = U.nvec > 4 ? 8 : c->gemv_rows;
     launch_gemv(a, st);
   }
   for (gain_sweep::Task& T : s->tasks) {
@@ -326,14 +383,19 @@ void enqueue(gain_sweep* s, hipStream_t st) {
     la.nF = T.P.blkF->n;
     la.Flo = T.compute ? T.P.Flo : 0;
     la.Fhi = T.compute ? T.P.Fhi : 0;
+    if (T.unit >= 0) {  // rows of the pack's window
+      la.Flo = s->units[T.unit].Flo;
+      la.Fhi = s->units[T.unit].Fhi;
+    }
     la.maxdeg = c->maxdeg;
     la.nvec_per_i = T.nvpi;
     la.slot_of_vec = T.slot_of_vec;
     la.src = T.LS;
-    if (T.gemv) {  // partials [ks][F][vec]
-      la.ksplit = T.ksplit;
-      la.src_ks = (int64_t)T.nR * T.nvecG;
-      la.src_f = T.nvecG;
+    if (T.unit >= 0) {  // pack partials [ks][F][vec], this task's columns
+      const gain_sweep::Unit& U = s->units[T.unit];
+      la.ksplit = U.ksplit;
+      la.src_ks = (int64_t)(U.Fhi - U.Flo) * U.nvec;
+      la.src_f = U.nvec;
       la.src_v = 1;
     } else {  // [vec][F]
       la.ksplit = 1;
@@ -388,7 +450,6 @@ extern "C" int gain_sweep_create(gain_ctx* c, const gain_task* tasks,
     DeviceGuard g(c->device);
     s = new gain_sweep();
     s->c = c;
-    if (c->force_cfg[1] >= 0) s->hl_cfg = c->force_cfg[1];
     if (c->force_cfg[0] >= 0) s->pj_cfg = c->force_cfg[0];
     build(s, tasks, ntasks);
     GAIN_HIP(hipStreamSynchronize(c->stream));
