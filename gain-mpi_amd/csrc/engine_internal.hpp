@@ -155,6 +155,7 @@ struct Block {
   std::vector<double> energy;
   double* d_energy = nullptr;
   bool sorted = true;
+  std::vector<int> label;  // level number printed per state (default: row)
 };
 
 struct TimerSlot {

@@ -209,9 +209,13 @@ extern "C" int gain_write_transitions(gain_ctx* c,
         std::sort(v.begin(), v.end(), [](const Line& a, const Line& b) {
           return a.final_rank < b.final_rank;
         });
+        const Block& bI = c->blocks.at({s.jind, s.g});
+        const int lvI = bI.label.empty() ? s.level : bI.label[s.level];
         for (const Line& L : v) {
-          format_line(text.data(), (int)text.size(), L.nu, L.levelF,
-                      c->bases[L.jindF].J, L.gF, s.level, c->bases[s.jind].J,
+          const Block& bF = c->blocks.at({L.jindF, L.gF});
+          const int lvF = bF.label.empty() ? L.levelF : bF.label[L.levelF];
+          format_line(text.data(), (int)text.size(), L.nu, lvF,
+                      c->bases[L.jindF].J, L.gF, lvI, c->bases[s.jind].J,
                       s.g, L.A, o.compute_intensity != 0, L.I, L.ls,
                       o.full_line ? L.nls : -1);
           if (fputs(text.data(), out) < 0)

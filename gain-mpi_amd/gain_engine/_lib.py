@@ -29,6 +29,10 @@ EXPORTS = (
     "gain_set_dipole_combination", "gain_dipole_coeff_grad",
     "gain_set_tuning", "gain_get_tuning", "gain_tuning_report",
     "gain_sweep_create", "gain_format_line", "gain_write_transitions",
+    "gain_set_block_levels", "gain_trove_read_dipole", "gain_trove_read_basis",
+    "gain_trove_basis_free", "gain_trove_basis_sizes", "gain_trove_basis_get",
+    "gain_trove_upload_basis", "gain_trove_read_states",
+    "gain_trove_read_records", "gain_trove_load",
     "gain_sweep_run", "gain_sweep_destroy",
 )
 
@@ -115,6 +119,20 @@ def load():
         "gain_batched_lines": (_ip, [_vp, _ip, _ip, _vp, _ip, _ip, _ip, _ip,
                                      C.POINTER(Lines)]),
         "gain_set_timing": (_ip, [_vp, _ip]),
+        "gain_set_block_levels": (_ip, [_vp, _ip, _ip, _vp]),
+        "gain_trove_read_dipole": (_ip, [C.c_char_p, C.POINTER(_ip), _vp]),
+        "gain_trove_read_basis": (_ip, [C.c_char_p, C.c_char_p, _ip, _ip,
+                                        _vp, C.POINTER(_vp)]),
+        "gain_trove_basis_free": (_ip, [_vp]),
+        "gain_trove_basis_sizes": (_ip, [_vp, C.POINTER(_ip), C.POINTER(_ip),
+                                         C.POINTER(_ip), _vp]),
+        "gain_trove_basis_get": (_ip, [_vp, C.c_char_p, _ip, _vp]),
+        "gain_trove_upload_basis": (_ip, [_vp, _ip, _vp]),
+        "gain_trove_read_states": (_ip, [C.c_char_p, _ip, C.POINTER(Params),
+                                         C.POINTER(_ip), C.POINTER(_ip), _vp,
+                                         _vp, _vp]),
+        "gain_trove_read_records": (_ip, [C.c_char_p, _ip, _vp, _ip, _vp]),
+        "gain_trove_load": (_ip, [_vp, C.c_char_p, _vp, _ip]),
         "gain_format_line": (_ip, [C.c_char_p, _ip, _dp, _ip, _ip, _ip, _ip,
                                    _ip, _ip, _dp, _ip, _dp, _vp, _ip]),
         "gain_write_transitions": (_ip, [_vp, C.POINTER(OutputOpts),

@@ -249,6 +249,18 @@ class Engine:
             C.byref(n)))
         return int(n.value)
 
+    def set_block_levels(self, jind, gamma, levels):
+        lv = None if levels is None else np.ascontiguousarray(levels, _i32)
+        check(self._L.gain_set_block_levels(
+            self._h, jind, gamma, None if lv is None else _ptr(lv)))
+
+    def trove_load(self, directory, jvals):
+        """Dipole, bases, states and records from TROVE checkpoint files in
+        `directory` for the J list `jvals` (J index = position)."""
+        jv = np.ascontiguousarray(jvals, _i32)
+        check(self._L.gain_trove_load(self._h, str(directory).encode(),
+                                      _ptr(jv), jv.shape[0]))
+
     def get_tuning(self, key: str) -> int:
         v = C.c_longlong(0)
         check(self._L.gain_get_tuning(self._h, key.encode(), C.byref(v)))

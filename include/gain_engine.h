@@ -256,6 +256,53 @@ int gain_format_line(char* buf, int len, double nu, int levelF, int jF,
 int gain_write_transitions(gain_ctx* ctx, const gain_output_opts* opts,
                            const char* path, int64_t* nlines);
 
+/* ---- TROVE checkpoint files (host-side readers) ------------------------ */
+/* Level number printed for each state of a block (the descr file's ilevel,
+ * 0-based; TroveStates.cpp:94); NULL restores the row index. */
+int gain_set_block_levels(gain_ctx* ctx, int jind, int gamma,
+                          const int* levels);
+/* j0_extfield.chk (TroveDipole.cpp:9-119): Fortran sequential records,
+ * including >2 GiB subrecords (Util.cpp:89-109).  Returns nc; with piece
+ * != NULL also fills the 3*nc*nc piece layout gain_upload_dipole takes
+ * (transposed per component as TroveDipole.cpp:110 does). */
+int gain_trove_read_dipole(const char* path, int* nc, double* piece);
+/* j0eigen_quanta<J>.chk plus the J=0 file (TroveBasisSet.cpp:27-709):
+ * contracted basis, correlation with J=0 (vibrational index per function),
+ * K-blocks, tau, icontr, ijterms, N and repres per irrep. */
+typedef struct gain_trove_basis gain_trove_basis;
+int gain_trove_read_basis(const char* path, const char* path_j0, int J,
+                          int nrepres, const int* sdeg,
+                          gain_trove_basis** out);
+int gain_trove_basis_free(gain_trove_basis* b);
+/* ntot has nrepres entries. */
+int gain_trove_basis_sizes(const gain_trove_basis* b, int* nprim,
+                           int* maxsymcoeffs, int* matsize, int* ntot);
+/* Copy one array out: "kstart", "kblock" (J+1 ints), "tau", "vib"
+ * (nprim ints), "icontr" (2*nprim), "ijterms" (maxsymcoeffs*nrepres),
+ * "N" (maxsymcoeffs ints of irrep gamma), "repres" (doubles of gamma). */
+int gain_trove_basis_get(const gain_trove_basis* b, const char* name,
+                         int gamma, void* dst);
+/* gain_upload_basis + gain_upload_inflation from a read basis. */
+int gain_trove_upload_basis(gain_ctx* ctx, int jind,
+                            const gain_trove_basis* b);
+/* j0eigen_descr<J>_<gamma+1>.chk (TroveStates.cpp:15-131): the states that
+ * pass the state filter (lower or upper window, isym_do; States.cpp:41-53)
+ * in file order.  Arrays may be NULL (count only): irec and ilevel are
+ * 0-based, rec_len = the file's dim_basis. */
+int gain_trove_read_states(const char* path, int gamma, const gain_params* p,
+                           int* nstates, int* rec_len, int* irec,
+                           int* ilevel, double* energy);
+/* j0eigen_vectors<J>_<gamma+1>.chk: record irec[i] (rec_len doubles at
+ * irec*rec_len, EigenVector.cpp:88) into out[i*rec_len]. */
+int gain_trove_read_records(const char* path, int n, const int* irec,
+                            int rec_len, double* out);
+/* Everything the reference's main.cpp:118-212 reads, from directory `dir`,
+ * for J list jvals (J index = position): dipole, bases, and for every irrep
+ * with isym_do the states and their records (uploaded, level numbers
+ * set).  gain_set_params must come first. */
+int gain_trove_load(gain_ctx* ctx, const char* dir, const int* jvals,
+                    int nj);
+
 /* ---- kernel-level drop-ins for GPU/dipole_GPU.h ------------------------- *
  * Same argument meaning as the reference's free functions; pointers are
  * device memory, `stream` is a hipStream_t (NULL = default stream).
