@@ -9,8 +9,12 @@ The 2000 lowest initial states go against every final state with a
 0-15000 cm^-1 frequency window.
 
 A step is one batch of B initial states through the batched engine (half
-line strengths and projection as GEMMs, S/A/I epilogue); the plain FP64
-GEMV drop-in (one initial state per call) is timed beside it.  Kernel times
+line strengths and projection as GEMMs, S/A/I epilogue).  The headline
+contractions run as split-INT8 with 6 slices (exact int32 products on the
+integer matrix cores, FP64 combination; the fastest mode whose S stays
+within 1e-6 of FP64 on strong lines, checked on the first batch); batched
+FP64 is timed as a variant and the plain FP64 GEMV drop-in (one initial
+state per call) beside both.  Kernel times
 come from HIP events the engine records on its own stream around every
 launch.  Prints one JSON line.
 """
@@ -28,10 +32,17 @@ sys.path[:0] = [ROOT, os.path.join(ROOT, "gain-mpi_amd")]
 HBM_PEAK_GBS = 8000.0
 PRECISION_NAMES = {0: "f64", 1: "f32-mfma/f64-acc",
                    2: "bf16x3-ozaki-mfma/f32-f64-acc",
-                   3: "bf16x4-ozaki-mfma/f32-f64-acc"}
-# dense MFMA peak of the datatype the products run in (TFLOP/s)
-PEAKS = {0: 78.6, 1: 157.3, 2: 2500.0, 3: 2500.0}
-PRODUCTS = {0: 1, 1: 1, 2: 6, 3: 10}  # hardware products per algorithmic FMA
+                   3: "bf16x4-ozaki-mfma/f32-f64-acc",
+                   4: "i8x5-ozaki-mfma/i32-f64-acc",
+                   5: "i8x6-ozaki-mfma/i32-f64-acc",
+                   6: "i8x7-ozaki-mfma/i32-f64-acc"}
+# dense MFMA peak of the datatype the products run in (TFLOP/s; int8 in
+# TOP/s)
+PEAKS = {0: 78.6, 1: 157.3, 2: 2500.0, 3: 2500.0, 4: 5000.0, 5: 5000.0,
+         6: 5000.0}
+# hardware products per algorithmic FMA: slice pairs (i, j) with i + j < S
+PRODUCTS = {0: 1, 1: 1, 2: 6, 3: 10, 4: 15, 5: 21, 6: 28}
+PRODUCTS_SLICES = {4: 5, 5: 6, 6: 7}  # split-INT8 slice counts
 TIMERS = ("gemv", "half_ls", "expand_T", "epilogue", "gemm_proj")
 # rocprofv3 FETCH_SIZE / WRITE_SIZE per launch for this workload, scaled by
 # the same counters' reading of a 4 GB device copy (profiles/ holds the
@@ -159,8 +170,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=250)
     ap.add_argument("--plain-states", type=int, default=100)
-    ap.add_argument("--precision", type=int, default=0)
-    ap.add_argument("--variants", type=str, default="",
+    ap.add_argument("--precision", type=int, default=5,
+                    help="contraction precision of the headline line (5: "
+                         "split-INT8 x6, the fastest mode within 1e-6 of "
+                         "FP64 on strong lines; 0: FP64)")
+    ap.add_argument("--variants", type=str, default="0",
                     help="comma-separated extra precisions to time")
     ap.add_argument("--nc", type=int, default=4000)
     ap.add_argument("--ninitial", type=int, default=2000)
@@ -305,10 +319,16 @@ def main():
             for st in range(nsteps)])
         flops = 2.0 * mrows * blkF.rec_len * B
         tfs = flops / (gp_avg / 1e3) / 1e12
+        prod = PRODUCTS[prec]
         r = {"precision": prec, "dtype": PRECISION_NAMES[prec],
              "transitions": trans, "ms": ms, "tps": trans / (ms / 1e3),
+             "speedup_vs_plain": trans / (ms / 1e3) / plain_tps,
              "gemm_proj_ms": gp_avg, "gemm_proj_tflops": tfs,
-             "half_ls_ms": hb_ms / max(hb_n, 1)}
+             "half_ls_ms": hb_ms / max(hb_n, 1),
+             "roofline": {"bound": "mfma", "kernel": "gemm_proj",
+                          "achieved": tfs, "peak": PEAKS[prec] / prod,
+                          "unit": "TFLOP/s",
+                          "frac": tfs * prod / PEAKS[prec]}}
         log(f"batched[{PRECISION_NAMES[prec]}]: {nsteps} x {B} initial "
             f"states, {trans} transitions in {ms:.1f} ms -> "
             f"{r['tps']:.3e}/s; projection avg {gp_avg:.2f} ms = {tfs:.1f} "
@@ -324,9 +344,6 @@ def main():
             bflags[0].cpu().numpy().copy()
 
     head = run_batched(args.precision)
-    tune_log = eng.tuning_report()
-    if tune_log:
-        log("autotuner:\n" + tune_log.rstrip())
     variants = [run_batched(int(v)) for v in args.variants.split(",")
                 if v.strip() and int(v) != args.precision]
     # per-line error of each measured precision against FP64 on one batch
@@ -347,21 +364,27 @@ def main():
                 "weak_lines": int((~strong).sum()),
                 "weak_max_rel_err": float(rel[~strong].max())
                 if (~strong).any() else 0.0}
-    prec_check = check(args.precision)
-    proj_traffic, proj_launch = None, None
-    if args.precision == 0:  # check() ran batch 0 last: the tuned shape
-        chosen = [ln for ln in tune_log.splitlines()
-                  if ln.startswith("projection") and "->" in ln]
-        label = chosen[-1].split("->")[1].split()[0] if chosen else None
-        proj_launch = {"tiles": label,
-                       "split_k": eng.get_tuning("proj_split"),
-                       "grid_items_batch0": eng.get_tuning("proj_grid")}
-        proj_traffic = measured_traffic(label, proj_launch[
-            "grid_items_batch0"])
     for v in variants:
         v["precision_check"] = check(v["precision"])
         log(f"precision check {v['dtype']}: {v['precision_check']}")
+    # the headline precision last: its batch-0 launch is the profiled shape
+    prec_check = check(args.precision)
     log(f"precision check {PRECISION_NAMES[args.precision]}: {prec_check}")
+    tune_log = eng.tuning_report()
+    if tune_log:
+        log("autotuner:\n" + tune_log.rstrip())
+    proj_traffic, proj_launch = None, None
+    prefix = ("projection" if args.precision == 0 else
+              f"int8x{PRODUCTS_SLICES.get(args.precision)} projection")
+    chosen = [ln for ln in tune_log.splitlines()
+              if ln.startswith(prefix) and "->" in ln]
+    label = chosen[-1].split("->")[1].split()[0] if chosen else None
+    if label is not None and args.precision != 0:
+        label = f"int8x{PRODUCTS_SLICES[args.precision]} {label}"
+    proj_launch = {"tiles": label,
+                   "split_k": eng.get_tuning("proj_split"),
+                   "grid_items_batch0": eng.get_tuning("proj_grid")}
+    proj_traffic = measured_traffic(label, proj_launch["grid_items_batch0"])
     tps = head["tps"]
     batched_ms = head["ms"]
     gemm_tfs = head["gemm_proj_tflops"]
@@ -423,10 +446,12 @@ def main():
                      prod, "unit": "TFLOP/s",
                      "frac": gemm_tfs * prod / PEAKS[args.precision],
                      "note": "achieved = algorithmic 2*M*N*K flops / avg "
-                             "kernel time; peak = dense MFMA peak of the "
-                             "datatype / hardware products per FMA; "
-                             "traffic = counter HBM bytes of the chosen "
-                             "launch on batch 0",
+                             "time of the projection (the slicing of the "
+                             "projected vectors included for split modes); "
+                             "peak = dense MFMA peak of the datatype (int8: "
+                             "TOP/s) / hardware products per FMA (split-INT8 "
+                             "x S: S(S+1)/2); traffic = counter HBM bytes of "
+                             "the chosen launch on batch 0",
                      "traffic": proj_traffic, "launch": proj_launch},
         "precision_check": prec_check,
         "variants": variants,

@@ -121,8 +121,18 @@ int slices_of(int precision) {
   }
 }
 
+int i8_slices_of(int precision) {
+  switch (precision) {
+    case GAIN_PREC_I8X5: return 5;
+    case GAIN_PREC_I8X6: return 6;
+    case GAIN_PREC_I8X7: return 7;
+    default: return 0;
+  }
+}
+
 void release_sliced(gain_ctx* c, Sliced& sl) {
   if (sl.planes) c->arena.release(sl.planes);
+  if (sl.p8) c->arena.release(sl.p8);
   if (sl.expo) c->arena.release(sl.expo);
   sl = Sliced{};
 }
@@ -130,7 +140,7 @@ void release_sliced(gain_ctx* c, Sliced& sl) {
 // Sliced copy of the dipole along v_I (rows = (alpha, v_F)), built once per
 // slice count.
 const Sliced& mu_sliced(gain_ctx* c, int S) {
-  if (c->mu_sl.S == S) return c->mu_sl;
+  if (c->mu_sl.S == S && !c->mu_sl.i8) return c->mu_sl;
   release_sliced(c, c->mu_sl);
   const int nc = c->nc;
   double* muT = c->w_tmp.get<double>((size_t)3 * nc * nc);
@@ -147,7 +157,7 @@ const Sliced& mu_sliced(gain_ctx* c, int S) {
 
 // Sliced copy of a block's eigenvector records, built once per slice count.
 const Sliced& block_sliced(gain_ctx* c, Block& b, int S) {
-  if (b.sl.S == S) return b.sl;
+  if (b.sl.S == S && !b.sl.i8) return b.sl;
   release_sliced(c, b.sl);
   Sliced sl;
   sl.S = S;
@@ -158,6 +168,114 @@ const Sliced& block_sliced(gain_ctx* c, Block& b, int S) {
                   c->stream);
   b.sl = sl;
   return b.sl;
+}
+
+// Split-INT8 copy of the dipole along v_I: one tiled matrix per component
+// (rows v_F), built once per slice count.
+const Sliced& mu_sliced_i8(gain_ctx* c, int S) {
+  if (c->mu_sl.S == S && c->mu_sl.i8) return c->mu_sl;
+  release_sliced(c, c->mu_sl);
+  const int nc = c->nc;
+  double* muT = c->w_tmp.get<double>((size_t)3 * nc * nc);
+  transpose_mu(c->d_mu, nc, muT, c->stream);
+  Sliced sl;
+  sl.S = S;
+  sl.i8 = true;
+  sl.Kp = i8_kpad(nc);
+  sl.Rp = i8_rows_pad(nc);
+  sl.plane = 3 * sl.Rp * sl.Kp;
+  sl.p8 = c->arena.alloc<int8_t>((size_t)S * sl.plane);
+  sl.expo = c->arena.alloc<int>((size_t)3 * sl.Rp);
+  for (int a = 0; a < 3; ++a)
+    slice_rows_i8(S, muT + (size_t)a * nc * nc, nc, nc, nc,
+                  sl.p8 + a * sl.Rp * sl.Kp, sl.plane, sl.expo + a * sl.Rp,
+                  c->stream);
+  c->mu_sl = sl;
+  return c->mu_sl;
+}
+
+// Split-INT8 copy of a block's eigenvector records, built once per slice
+// count.
+const Sliced& block_sliced_i8(gain_ctx* c, Block& b, int S) {
+  if (b.sl.S == S && b.sl.i8) return b.sl;
+  release_sliced(c, b.sl);
+  Sliced sl;
+  sl.S = S;
+  sl.i8 = true;
+  sl.Kp = i8_kpad(b.rec_len);
+  sl.Rp = i8_rows_pad(b.n);
+  sl.plane = sl.Rp * sl.Kp;
+  sl.p8 = c->arena.alloc<int8_t>((size_t)S * sl.plane);
+  sl.expo = c->arena.alloc<int>((size_t)sl.Rp);
+  slice_rows_i8(S, b.vec, b.ld, b.n, b.rec_len, sl.p8, sl.plane, sl.expo,
+                c->stream);
+  b.sl = sl;
+  return b.sl;
+}
+
+// Tile configuration and split count of a split-INT8 GEMM (kind 0:
+// projection, 1: half line strength), timed on the real operands once per
+// shape when autotuning (splits from the smallest exact one up to
+// `max_split`; g.C must hold that many partials).  Without tuning: the
+// 64 x 128 tile and the smallest exact split.
+int i8_cfg_split(gain_ctx* c, int kind, I8GemmArgs g, int max_split,
+                 int* ksplit) {
+  const int64_t nkb = g.Kp / 32, maxkb = i8_max_krange(g.S) / 32;
+  const int min_split = (int)((nkb + maxkb - 1) / maxkb);
+  GAIN_CHECK_ARG(max_split >= min_split,
+                 "int8 gemm: K = %lld needs %d splits", (long long)g.Kp,
+                 min_split);
+  if (c->force_i8_cfg >= 0) {
+    *ksplit = std::max(min_split, std::min(max_split, c->force_i8_split));
+    return c->force_i8_cfg;
+  }
+  const auto key = std::make_tuple(100 + 10 * kind + g.S,
+                                   g.M >= 8192 ? -1 : g.M, g.N, (int)g.Kp);
+  auto it = c->tuned.find(key);
+  if (it != c->tuned.end()) {
+    *ksplit = it->second / 100;
+    return it->second % 100;
+  }
+  *ksplit = min_split;
+  if (!c->autotune) return 0;
+  hipEvent_t e0, e1;
+  GAIN_HIP(hipEventCreate(&e0));
+  GAIN_HIP(hipEventCreate(&e1));
+  int best = 0, best_ks = min_split;
+  float best_ms = 1e30f;
+  char line[160];
+  snprintf(line, sizeof(line), "int8x%d %s M=%d N=%d K=%lld:", g.S,
+           kind == 0 ? "projection" : "half_ls", g.M, g.N, (long long)g.Kp);
+  std::string rec = line;
+  for (int cfg = 0; cfg < kI8Configs; ++cfg)
+    for (int ks = min_split; ks <= max_split; ks *= 2) {
+      g.cfg = cfg;
+      g.ksplit = ks;
+      gemm_sliced_i8(g, c->stream);  // warm
+      GAIN_HIP(hipEventRecord(e0, c->stream));
+      for (int r = 0; r < 2; ++r) gemm_sliced_i8(g, c->stream);
+      GAIN_HIP(hipEventRecord(e1, c->stream));
+      GAIN_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      GAIN_HIP(hipEventElapsedTime(&ms, e0, e1));
+      ms *= 0.5f;
+      snprintf(line, sizeof(line), " %s/k%d=%.3fms", i8_config_name(cfg), ks,
+               ms);
+      rec += line;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = cfg;
+        best_ks = ks;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  snprintf(line, sizeof(line), " -> %s split %d\n", i8_config_name(best),
+           best_ks);
+  c->tune_log += rec + line;
+  c->tuned[key] = best_ks * 100 + best;
+  *ksplit = best_ks;
+  return best;
 }
 
 bool degfilter(const gain_ctx* c, int gI, int gF, int dI, int dF) {
@@ -439,6 +557,40 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
     g.c_z = (int64_t)ncolTot * nc;
     g.nz = 3;
     gemm_sliced_bf16(g, c->stream);
+  } else if (i8_slices_of(precision) > 0) {
+    const int S8 = i8_slices_of(precision);
+    const Sliced& ms = mu_sliced_i8(c, S8);
+    const int64_t Rx = i8_rows_pad(ncolTot), plane = Rx * ms.Kp;
+    int8_t* xs = c->w_xs.get<int8_t>((size_t)S8 * plane);
+    int* xe = c->w_xe.get<int>((size_t)Rx);
+    slice_rows_i8(S8, X, nc, ncolTot, nc, xs, plane, xe, c->stream);
+    I8GemmArgs g{};
+    g.S = S8;
+    g.M = nc;
+    g.N = ncolTot;
+    g.row0 = 0;
+    g.Kp = ms.Kp;
+    g.A = ms.p8;
+    g.a_plane = ms.plane;
+    g.a_z = ms.Rp * ms.Kp;
+    g.eA = ms.expo;
+    g.ea_z = ms.Rp;
+    g.B = xs;
+    g.b_plane = plane;
+    g.eB = xe;
+    g.ldc = nc;
+    g.c_z = (int64_t)ncolTot * nc;
+    g.c_ks = 3 * (int64_t)ncolTot * nc;
+    g.nz = 3;
+    // K = Nc is far below the exact range: no split
+    Y = c->w_Y.get<double>((size_t)3 * ncolTot * nc);
+    g.C = Y;
+    g.cfg = i8_cfg_split(c, 1, g, 1, &ks);
+    g.ksplit = ks;
+    gemm_sliced_i8(g, c->stream);
+    c->last["half_ls_cfg"] = g.cfg;
+    c->last["half_ls_split"] = ks;
+    c->last["half_ls_grid"] = gemm_i8_grid_items(g);
   } else if (precision == GAIN_PREC_FP32) {
     ks = 1;
     GemmArgs g{};
@@ -752,6 +904,44 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
         g.ldc = nR;
         g.nz = 1;
         gemm_sliced_bf16(g, c->stream);
+      } else if (i8_slices_of(precision) > 0) {
+        const int S8 = i8_slices_of(precision);
+        const Sliced& cs = block_sliced_i8(c, blkF, S8);
+        const int64_t Rg = i8_rows_pad(nvecG), plane = Rg * cs.Kp;
+        int8_t* gs = c->w_gs.get<int8_t>((size_t)S8 * plane);
+        int* ge = c->w_ge.get<int>((size_t)Rg);
+        slice_rows_i8(S8, G, gstride, nvecG, ntotF, gs, plane, ge,
+                      c->stream);
+        I8GemmArgs g{};
+        g.S = S8;
+        g.M = nR;
+        g.N = nvecG;
+        g.row0 = P.Flo;
+        g.Kp = cs.Kp;
+        g.A = cs.p8;
+        g.a_plane = cs.plane;
+        g.a_z = 0;
+        g.eA = cs.expo;
+        g.ea_z = 0;
+        g.B = gs;
+        g.b_plane = plane;
+        g.eB = ge;
+        g.ldc = nR;
+        g.c_z = 0;
+        g.c_ks = (int64_t)nR * nvecG;
+        g.nz = 1;
+        // the tuner may write up to 4 split partials
+        LS = c->w_part.get<double>((size_t)4 * nR * nvecG);
+        g.C = LS;
+        int ks = 1;
+        g.cfg = i8_cfg_split(c, 0, g, 4, &ks);
+        g.ksplit = ks;
+        gemm_sliced_i8(g, c->stream);
+        c->last["proj_cfg"] = g.cfg;
+        c->last["proj_split"] = ks;
+        c->last["proj_grid"] = gemm_i8_grid_items(g);
+        ksplit = ks;
+        src_ks = (int64_t)nR * nvecG;
       } else if (c->use_mfma && precision == GAIN_PREC_FP64) {
         GemmArgs g{};
         g.M = nR;
@@ -1439,7 +1629,7 @@ extern "C" int gain_batched_lines(gain_ctx* c, int jindI, int gammaI,
   return guarded([&] {
     need(c);
     GAIN_CHECK_ARG(out && levelsI && nI > 0, "bad batch");
-    GAIN_CHECK_ARG(precision >= GAIN_PREC_FP64 && precision <= GAIN_PREC_BF16X4,
+    GAIN_CHECK_ARG(precision >= GAIN_PREC_FP64 && precision <= GAIN_PREC_I8X7,
                    "unknown precision %d", precision);
     DeviceGuard g(c->device);
     Plan P = make_plan(c, jindI, gammaI, levelsI, nI, jindF, gammaF);
@@ -1502,6 +1692,13 @@ extern "C" int gain_set_tuning(gain_ctx* c, const char* key, int value) {
     } else if (k == "proj_split") {
       GAIN_CHECK_ARG(value >= 1 && value <= 64, "split 1..64");
       c->force_split = value;
+    } else if (k == "i8_cfg") {
+      GAIN_CHECK_ARG(value >= -1 && value < kI8Configs, "cfg -1..%d",
+                     kI8Configs - 1);
+      c->force_i8_cfg = value;
+    } else if (k == "i8_split") {
+      GAIN_CHECK_ARG(value >= 0 && value <= 4, "split 0..4");
+      c->force_i8_split = value;
     } else if (k == "proj_cfg" || k == "half_ls_cfg") {
       GAIN_CHECK_ARG(value >= -1 && value < kGemmConfigs, "cfg -1..%d",
                      kGemmConfigs - 1);

@@ -138,11 +138,17 @@ struct Basis {
   Irrep irrep[GAIN_MAX_IRREPS];
 };
 
-// Split-BF16 copy of a static operand (final records or the dipole).
+// Sliced copy of a static operand (final records or the dipole): split-BF16
+// planes with chunk exponents, or tiled split-INT8 planes (`i8`) with one
+// exponent per row.
 struct Sliced {
   int S = 0;
+  bool i8 = false;
   int64_t Kp = 0;
+  int64_t Rp = 0;      // padded rows of one (z) matrix (int8)
+  int64_t plane = 0;   // bytes between int8 slice planes
   uint16_t* planes = nullptr;
+  int8_t* p8 = nullptr;
   int* expo = nullptr;
 };
 
@@ -195,6 +201,7 @@ struct gain_ctx {
   bool autotune = false;
   int force_cfg[2] = {-1, -1};  // 0: projection, 1: half line strength
   int force_split = 1;          // projection split-K when not tuned
+  int force_i8_cfg = -1, force_i8_split = 0;  // split-INT8 GEMMs
   std::map<std::tuple<int, int, int, int>, int> tuned;
   std::string tune_log;
   // what the most recent launch of each tuned kernel used (tuning getter)

@@ -63,4 +63,59 @@ struct SlicedGemmArgs {
 };
 void gemm_sliced_bf16(const SlicedGemmArgs& g, hipStream_t s);
 
+// ---------------------------------------------------------------------------
+// Split-INT8 (Ozaki-style on the integer matrix cores).
+//
+// Every FP64 row x is scaled by 2^-e, e the exponent of the row's largest
+// magnitude (|x| 2^-e < 1), and cut into S slices
+//   x = 2^e sum_{i<S} n_i 2^{-6-7i},   n_i = rint(remainder * 2^{6+7i}),
+// so |n_i| <= 64 (an int8) and |x - slices| <= 2^{e-7S}.  Slice products
+// n_i m_j are exact int32; the products of one level L = i + j are summed
+// by v_mfma_i32_32x32x32_i8 in int32 over a K range, exact while
+// S * range * 64 * 64 < 2^31 (i8_max_krange).  Levels L >= S are dropped;
+// the S level sums are combined in FP64 and scaled by 2^{e_A + e_B - 12}.
+//
+// Planes are stored tiled: plane i of an R x K matrix is [R/32][Kp/32] blocks
+// of 32 rows x 32 bytes (1 KB), so one MFMA operand fragment is one
+// contiguous 1 KB block.  Rows are padded to i8_rows_pad(R) and K to
+// i8_kpad(K) with zeros, so the GEMM never bounds-checks its loads.
+inline int64_t i8_kpad(int64_t K) { return (K + 31) / 32 * 32; }
+inline int64_t i8_rows_pad(int64_t R) { return (R + 31) / 32 * 32 + 256; }
+inline int64_t i8_max_krange(int S) {
+  return ((1ll << 31) - 1) / (4096ll * S) / 32 * 32;
+}
+
+// rows x K FP64 (row r at src + r*ld) -> S tiled int8 planes of
+// i8_rows_pad(rows) x i8_kpad(K) (plane stride `plane` bytes) and one
+// exponent per padded row.
+void slice_rows_i8(int S, const double* src, int64_t ld, int rows, int K,
+                   int8_t* planes, int64_t plane, int* expo, hipStream_t s);
+
+// C[z][ks][n*ldc + m] = 2^{eA + eB - 12} sum_{L<S} 2^{-7L} sum_{i+j=L} A_i B_j
+// over the k blocks of split ks, for output rows m in [0, M) (A rows
+// row0 + m) and columns n in [0, N) (B rows n).
+struct I8GemmArgs {
+  int S;
+  int M, N;
+  int row0;          // A row of output row 0
+  int64_t Kp;        // i8_kpad(K)
+  const int8_t* A;   // tiled planes, Kp bytes per row
+  int64_t a_plane;   // bytes between A slice planes
+  int64_t a_z;       // bytes between z matrices inside a plane
+  const int* eA;     // exponent per A row; z matrices ea_z apart
+  int64_t ea_z;
+  const int8_t* B;
+  int64_t b_plane;
+  const int* eB;
+  double* C;
+  int64_t ldc, c_z, c_ks;
+  int nz = 1;
+  int ksplit = 1;
+  int cfg = 0;
+};
+constexpr int kI8Configs = 6;
+const char* i8_config_name(int cfg);
+void gemm_sliced_i8(const I8GemmArgs& g, hipStream_t s);
+int64_t gemm_i8_grid_items(const I8GemmArgs& g);
+
 }  // namespace gain

@@ -79,7 +79,11 @@ typedef enum {
   GAIN_PREC_FP64 = 0,       /* FP64 throughout */
   GAIN_PREC_FP32 = 1,       /* FP32 MFMA products, FP64 accumulation */
   GAIN_PREC_BF16X3 = 2,     /* split-BF16, 3 slices (24 bits of the row max) */
-  GAIN_PREC_BF16X4 = 3      /* split-BF16, 4 slices (32 bits of the row max) */
+  GAIN_PREC_BF16X4 = 3,     /* split-BF16, 4 slices (32 bits of the row max) */
+  GAIN_PREC_I8X5 = 4,       /* split-INT8, 5 slices (35 bits of the row max),
+                               exact int32 products on the integer MFMA */
+  GAIN_PREC_I8X6 = 5,       /* split-INT8, 6 slices (42 bits) */
+  GAIN_PREC_I8X7 = 6        /* split-INT8, 7 slices (49 bits) */
 } gain_precision;
 
 const char* gain_last_error(void);
@@ -220,6 +224,8 @@ int gain_get_timing(gain_ctx* ctx, const char* name, double* total_ms,
  * Keys: "autotune" (1: time every GEMM tile configuration on the first call
  * of each shape and keep the fastest; 2: forget earlier choices),
  * "proj_cfg" / "half_ls_cfg" (force a configuration, -1 = automatic),
+ * "i8_cfg" / "i8_split" (split-INT8 GEMMs: force a tile configuration
+ * 0..5, -1 = automatic; projection split count, 0 = automatic),
  * "gemv_rows" (2/4/8), "gemv_ksplit", "mu_ksplit" (0 = automatic),
  * "mfma" (0: vector-ALU GEMMs, for cross-checks). */
 int gain_set_tuning(gain_ctx* ctx, const char* key, int value);

@@ -247,6 +247,56 @@ def test_batched_split_bf16_error_vs_fp64_oracle(precision, bound):
     assert es <= bound
 
 
+@pytest.mark.parametrize("precision,bound", [(6, 5e-10), (5, 5e-8),
+                                             (4, 1e-5)])
+def test_batched_split_int8_error_vs_fp64_oracle(precision, bound):
+    """Split-INT8 contractions (7, 6 and 5 slices, exact integer products
+    on the matrix cores): transition list identical, per-line S error
+    against the FP64 oracle within `bound` on strong lines (S > 1e-6
+    S_max)."""
+    prob = synth.make_problem([5, 6], 32, "c2v4")
+    prob.params.erange_lower = (0.0, 1e9)
+    rows = prob.state_table()
+    initials = list(range(0, len(rows), 2))
+    tr, _ = orc.run(prob, initials)
+    ref = oracle_map(tr)
+    with Engine.from_problem(prob) as eng:
+        got = engine_lines(eng, prob, initials, "batched", precision)
+    assert set(got) == set(ref)
+    assert {k for k in got if got[k]["emitted"]} == \
+        {k for k in ref if ref[k]["emitted"]}
+    es, ew, ns, nw = _strong_weak_err(got, ref)
+    print(f"precision {precision}: strong lines {ns} max rel err {es:.3e}; "
+          f"weak lines {nw} max rel err {ew:.3e}")
+    assert es <= bound
+
+
+@pytest.mark.parametrize("cfg", list(range(6)))
+@pytest.mark.parametrize("split", [1, 2])
+def test_every_int8_tile_configuration(cfg, split):
+    """Each split-INT8 tile shape for both GEMM kinds (and split-K 2 on
+    the projection) on sizes with partial tiles; a frequency window starts
+    the projected final rows off a 32-row block boundary.  Seven slices
+    (49 bits of each row maximum) keep S within 1e-9 of the FP64 oracle
+    on strong lines."""
+    prob = synth.make_problem([4, 5], 20, "c2v4")
+    prob.params.erange_lower = (0.0, 1e9)
+    prob.params.freq_window = (1500.0, 1e9)
+    rows = prob.state_table()
+    initials = list(range(1, len(rows), 2))
+    tr, _ = orc.run(prob, initials)
+    ref = oracle_map(tr)
+    with Engine.from_problem(prob) as eng:
+        eng.set_tuning("i8_cfg", cfg)
+        eng.set_tuning("i8_split", split)
+        got = engine_lines(eng, prob, initials, "batched", 6)
+        assert eng.get_tuning("proj_cfg") == cfg
+        assert eng.get_tuning("proj_split") == split
+    assert set(got) == set(ref)
+    es, _, ns, _ = _strong_weak_err(got, ref)
+    assert ns > 0 and es <= 1e-9, es
+
+
 @pytest.mark.parametrize("cfg", list(range(15)))
 def test_every_gemm_tile_configuration(cfg):
     """Each tile shape the autotuner can pick, for both GEMM kinds, on
