@@ -226,6 +226,9 @@ int i8_cfg_split(gain_ctx* c, int kind, I8GemmArgs g, int max_split,
                  "int8 gemm: K = %lld needs %d splits", (long long)g.Kp,
                  min_split);
   if (c->force_i8_cfg >= 0) {
+    GAIN_CHECK_ARG(i8_config_ok(c->force_i8_cfg, g.S),
+                   "int8 configuration %d does not run %d slices",
+                   c->force_i8_cfg, g.S);
     *ksplit = std::max(min_split, std::min(max_split, c->force_i8_split));
     return c->force_i8_cfg;
   }
@@ -248,7 +251,8 @@ int i8_cfg_split(gain_ctx* c, int kind, I8GemmArgs g, int max_split,
            kind == 0 ? "projection" : "half_ls", g.M, g.N, (long long)g.Kp);
   std::string rec = line;
   for (int cfg = 0; cfg < kI8Configs; ++cfg)
-    for (int ks = min_split; ks <= max_split; ks *= 2) {
+    for (int ks = min_split; ks <= max_split && i8_config_ok(cfg, g.S);
+         ks *= 2) {
       g.cfg = cfg;
       g.ksplit = ks;
       gemm_sliced_i8(g, c->stream);  // warm

@@ -113,7 +113,9 @@ struct I8GemmArgs {
   int ksplit = 1;
   int cfg = 0;
 };
-constexpr int kI8Configs = 6;
+constexpr int kI8Configs = 12;
+// whether configuration cfg supports S slices
+bool i8_config_ok(int cfg, int S);
 const char* i8_config_name(int cfg);
 void gemm_sliced_i8(const I8GemmArgs& g, hipStream_t s);
 int64_t gemm_i8_grid_items(const I8GemmArgs& g);

@@ -92,41 +92,63 @@ void slice_rows_i8(int S, const double* src, int64_t ld, int rows, int K,
 }
 
 // ---------------------------------------------------------------------------
-// Tile configurations: wave tile (32 TM) x (32 TN), WM x WN waves.
+// Tile configurations: wave tile (32 TM) x (32 TN), WM x WN waves; `lds`:
+// operands staged through LDS (k_gemm_i8_lds), else read per wave.
 struct I8Cfg {
-  int tm, tn, wm, wn;
+  int tm, tn, wm, wn, lds;
 };
 constexpr I8Cfg kI8Cfg[kI8Configs] = {
-    {1, 2, 2, 2},  // 64 x 128
-    {2, 1, 2, 2},  // 128 x 64
-    {1, 2, 4, 1},  // 128 x 64
-    {1, 2, 1, 4},  // 32 x 256
-    {2, 1, 4, 1},  // 256 x 32
-    {2, 1, 1, 4},  // 64 x 128
+    {1, 2, 2, 2, 0},  // 64 x 128
+    {2, 1, 2, 2, 0},  // 128 x 64
+    {1, 2, 4, 1, 0},  // 128 x 64
+    {1, 2, 1, 4, 0},  // 32 x 256
+    {2, 1, 4, 1, 0},  // 256 x 32
+    {2, 1, 1, 4, 0},  // 64 x 128
+    {1, 2, 2, 2, 1},  // 64 x 128 through LDS
+    {2, 1, 2, 2, 1},  // 128 x 64 through LDS
+    // 8 waves (two per SIMD) of 32 x 32
+    {1, 1, 4, 2, 0},  // 128 x 64
+    {1, 1, 2, 4, 0},  // 64 x 128
+    {1, 1, 4, 2, 1},  // 128 x 64 through LDS
+    {1, 1, 2, 4, 1},  // 64 x 128 through LDS
 };
 
 const char* i8_config_name(int cfg) {
   static const char* names[kI8Configs] = {
-      "64x128/w32x64", "128x64/w64x32", "128x64/w32x64",
-      "32x256/w32x64", "256x32/w64x32", "64x128/w64x32"};
+      "64x128/w32x64", "128x64/w64x32", "128x64/w32x64", "32x256/w32x64",
+      "256x32/w64x32", "64x128/w64x32", "64x128/w32x64/lds",
+      "128x64/w64x32/lds", "128x64/8w", "64x128/8w", "128x64/8w/lds",
+      "64x128/8w/lds"};
   return cfg >= 0 && cfg < kI8Configs ? names[cfg] : "?";
 }
 
+bool i8_config_ok(int cfg, int S) {
+  return cfg >= 0 && cfg < kI8Configs && S >= 5 && S <= 7;
+}
+
+// Workgroup tile -> (z, split, tile row, tile column): XCD-aware remap of
+// the linear id, splits outside tiles, grouped tile order.
+__device__ inline void i8_tile(int ntm, int ntn, int ksplit, int* z, int* ksi,
+                               int* tm, int* tn) {
+  const int ntiles = ntm * ntn;
+  const int per_z = ntiles * ksplit;
+  const int p = xcd_remap(blockIdx.x, gridDim.x);
+  *z = p / per_z;
+  const int rem = p - *z * per_z;
+  *ksi = rem / ntiles;
+  tile_order(rem - *ksi * ntiles, ntm, ntn, 8, tm, tn);
+}
+
 template <int S, int TM, int TN, int WM, int WN>
-__global__ __launch_bounds__(256) void k_gemm_i8(I8GemmArgs g, int ntm,
-                                                 int ntn, int rb0,
-                                                 int kb_per) {
+__global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8(I8GemmArgs g,
+                                                          int ntm, int ntn,
+                                                          int rb0,
+                                                          int kb_per) {
   constexpr int BMB = TM * WM, BNB = TN * WN;  // tile size in 32-row blocks
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave % WM, wn = wave / WM;
-  const int ntiles = ntm * ntn;
-  const int per_z = ntiles * g.ksplit;
-  const int p = xcd_remap(blockIdx.x, gridDim.x);
-  const int z = p / per_z;
-  const int rem = p - z * per_z;
-  const int ksi = rem / ntiles;
-  int tm, tn;
-  tile_order(rem - ksi * ntiles, ntm, ntn, 8, &tm, &tn);
+  int z, ksi, tm, tn;
+  i8_tile(ntm, ntn, g.ksplit, &z, &ksi, &tm, &tn);
   const int64_t nkb = g.Kp >> 5;
   const int kb0 = ksi * kb_per;
   const int kb1 = (int)min((int64_t)kb0 + kb_per, nkb);
@@ -146,7 +168,12 @@ __global__ __launch_bounds__(256) void k_gemm_i8(I8GemmArgs g, int ntm,
 #pragma unroll
         for (int e = 0; e < 16; ++e) acc[L][a][b][e] = 0;
 
-  i4v fa0[S][TM], fb0[S][TN], fa1[S][TM], fb1[S][TN];
+  // PD register buffers: block kb + PD - 1 is requested before the
+  // products of block kb, so each load has PD - 1 blocks of products to
+  // land (the final records stream from HBM in the projection)
+  // (two waves per SIMD with 8-wave workgroups: fewer buffers fit)
+  constexpr int PD = (S <= 6 && WM * WN == 4) ? 3 : 2;
+  i4v fa[PD][S][TM], fb[PD][S][TN];
   auto load = [&](i4v(&fa)[S][TM], i4v(&fb)[S][TN], int kb) {
 #pragma unroll
     for (int i = 0; i < S; ++i) {
@@ -175,20 +202,179 @@ __global__ __launch_bounds__(256) void k_gemm_i8(I8GemmArgs g, int ntm,
                 fa[i][a], fb[j][b], acc[i + j][a][b], 0, 0, 0);
   };
 
+#pragma unroll
+  for (int q = 0; q < PD - 1; ++q)
+    if (kb0 + q < kb1) load(fa[q], fb[q], kb0 + q);
   int kb = kb0;
-  if (kb < kb1) load(fa0, fb0, kb);
-  for (; kb + 1 < kb1; kb += 2) {
-    load(fa1, fb1, kb + 1);
-    mma(fa0, fb0);
-    if (kb + 2 < kb1) load(fa0, fb0, kb + 2);
-    mma(fa1, fb1);
+  for (; kb + PD - 1 < kb1; kb += PD) {
+#pragma unroll
+    for (int q = 0; q < PD; ++q) {
+      // buffer (q + PD - 1) % PD was consumed by the previous step
+      if (kb + q + PD - 1 < kb1)
+        load(fa[(q + PD - 1) % PD], fb[(q + PD - 1) % PD], kb + q + PD - 1);
+      mma(fa[q], fb[q]);
+    }
   }
-  if (kb < kb1) mma(fa0, fb0);
+  // fewer than PD blocks remain, already in buffers 0, 1, ...
+#pragma unroll
+  for (int q = 0; q < PD - 1; ++q)
+    if (kb + q < kb1) mma(fa[q], fb[q]);
 
   // epilogue: lane holds column n = lane%32 of each n-tile and rows
   // 8(e/4) + 4(lane/32) + e%4 of each m-tile
   double* C = g.C + z * g.c_z + ksi * g.c_ks;
   const int* eA = g.eA + z * g.ea_z;
+#pragma unroll
+  for (int b = 0; b < TN; ++b) {
+    const int n = (brb + b) * 32 + (lane & 31);
+    if (n >= g.N) continue;
+    const int eb = g.eB[n] - 12;
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int ar = (arb + a) * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+        const int m = ar - g.row0;
+        if (m < 0 || m >= g.M) continue;
+        double v = (double)acc[S - 1][a][b][e];
+#pragma unroll
+        for (int L = S - 2; L >= 0; --L)
+          v = (double)acc[L][a][b][e] + v * 0.0078125;  // 2^-7
+        C[(int64_t)n * g.ldc + m] = ldexp(v, eA[ar] + eb);
+      }
+  }
+}
+
+// Chunk swizzle inside a 1 KB block: 16-byte chunk c of the block is kept
+// in LDS slot sigma(c) (an involution).  The fragment read of lane l takes
+// chunk 2 (l % 32) + l / 32; with the swizzle every 16 (and every 32)
+// consecutive lanes of a 128-bit LDS read hit distinct banks.
+__device__ inline int i8_sigma(int c) {
+  return c ^ (((c >> 4) ^ (c >> 5)) & 1);
+}
+
+template <int N>
+__device__ inline void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  // vmcnt in bits [3:0] and [15:14]; expcnt and lgkmcnt left at "no wait"
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) |
+                             (15 << 8));
+}
+
+// Operands staged through LDS: workgroup tile (32 TM WM) x (32 TN WN), 4
+// waves of (32 TM) x (32 TN) (the S level accumulators of a wave stay
+// within the 256 accumulation registers), NST k-block stages filled by
+// direct global -> LDS loads (global_load_lds_dwordx4: each wave-wide load
+// copies one 1 KB block, lane l placing 16 bytes at slot l, so lane l
+// fetches chunk sigma(l)).  Stage kb + NST - 1 is requested before the
+// products of stage kb, so the loads have NST - 1 k blocks of products to
+// land and use no registers; one barrier per k block (after it every wave
+// has finished with stage kb - 1, whose buffer is refilled next, and every
+// wave's loads of stage kb have landed).  A fragments of all slices are
+// held per stage, B fragments are read once per slice.
+template <int S, int TM, int TN, int WM, int WN, int NST>
+__global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_lds(
+    I8GemmArgs g, int ntm, int ntn, int rb0, int kb_per) {
+  constexpr int NW = WM * WN;                  // waves
+  constexpr int BMB = TM * WM, BNB = TN * WN;  // 32-row blocks per side
+  constexpr int NBLK = S * (BMB + BNB);        // 1 KB blocks per stage
+  // padded to a multiple of NW so every wave issues the same number of
+  // loads (the padding slots repeat block 0 and are never read)
+  constexpr int NBLKP = (NBLK + NW - 1) / NW * NW;
+  constexpr int DPW = NBLKP / NW;              // block loads per wave
+  static_assert(S * TM * TN * 16 <= 256, "accumulators beyond 256 regs");
+  __shared__ __attribute__((aligned(1024))) int8_t lds[NST * NBLKP * 1024];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wm = wave % WM, wn = wave / WM;
+  int z, ksi, tm, tn;
+  i8_tile(ntm, ntn, g.ksplit, &z, &ksi, &tm, &tn);
+  const int64_t nkb = g.Kp >> 5;
+  const int kb0 = ksi * kb_per;
+  const int kb1 = (int)min((int64_t)kb0 + kb_per, nkb);
+  const int arb0 = rb0 + tm * BMB, brb0 = tn * BNB;
+  const int src_off = 16 * i8_sigma(lane);
+  const int8_t* Az = g.A + z * g.a_z;
+
+  // block q = wave + NW t of a stage: A blocks first (q = BMB i + rb:
+  // slice i, row block rb of the tile), then B blocks
+  auto issue = [&](int stage, int kb) {
+    int8_t* dst = lds + stage * (NBLKP * 1024);
+#pragma unroll
+    for (int t = 0; t < DPW; ++t) {
+      const int q = wave + NW * t;
+      const int8_t* src;
+      if (q < S * BMB || q >= NBLK) {
+        const int qa = q < NBLK ? q : 0;
+        const int i = qa / BMB, rb = qa % BMB;
+        src = Az + i * g.a_plane + ((int64_t)(arb0 + rb) * nkb + kb) * 1024;
+      } else {
+        const int i = (q - S * BMB) / BNB, rb = (q - S * BMB) % BNB;
+        src = g.B + i * g.b_plane + ((int64_t)(brb0 + rb) * nkb + kb) * 1024;
+      }
+      __builtin_amdgcn_global_load_lds(
+          (const void*)(src + src_off),
+          (__attribute__((address_space(3))) void*)(dst + q * 1024), 16, 0,
+          0);
+    }
+  };
+
+  i16v acc[S][TM][TN];
+#pragma unroll
+  for (int L = 0; L < S; ++L)
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+      for (int b = 0; b < TN; ++b)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[L][a][b][e] = 0;
+
+  const int rd_off = 16 * i8_sigma(2 * (lane & 31) + (lane >> 5));
+#pragma unroll
+  for (int st = 0; st < NST - 1; ++st)
+    if (kb0 + st < kb1) issue(st, kb0 + st);
+  int stage = 0;
+  for (int kb = kb0; kb < kb1; ++kb) {
+    // loads of later stages may stay in flight: min(NST - 2, left) of them
+    const int later = min(NST - 2, kb1 - 1 - kb);
+    if (later >= 2)
+      wait_vmcnt<(NST >= 4 ? 2 * DPW : 0)>();
+    else if (later == 1)
+      wait_vmcnt<(NST >= 3 ? DPW : 0)>();
+    else
+      wait_vmcnt<0>();
+    static_assert(NST <= 4, "wait counts cover up to 2 later stages");
+    __builtin_amdgcn_s_barrier();
+    if (kb + NST - 1 < kb1) issue(stage == 0 ? NST - 1 : stage - 1,
+                                  kb + NST - 1);
+    const int8_t* base = lds + stage * (NBLKP * 1024) + rd_off;
+    i4v fa[S][TM];
+#pragma unroll
+    for (int i = 0; i < S; ++i)
+#pragma unroll
+      for (int a = 0; a < TM; ++a)
+        fa[i][a] = *(const i4v*)(base + (BMB * i + TM * wm + a) * 1024);
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      i4v fb[TN];
+#pragma unroll
+      for (int b = 0; b < TN; ++b)
+        fb[b] = *(const i4v*)(base + (S * BMB + BNB * j + TN * wn + b) * 1024);
+#pragma unroll
+      for (int i = 0; i < S - j; ++i)
+#pragma unroll
+        for (int a = 0; a < TM; ++a)
+#pragma unroll
+          for (int b = 0; b < TN; ++b)
+            acc[i + j][a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                fa[i][a], fb[b], acc[i + j][a][b], 0, 0, 0);
+    }
+    stage = stage == NST - 1 ? 0 : stage + 1;
+  }
+
+  double* C = g.C + z * g.c_z + ksi * g.c_ks;
+  const int* eA = g.eA + z * g.ea_z;
+  const int arb = arb0 + TM * wm, brb = brb0 + TN * wn;
 #pragma unroll
   for (int b = 0; b < TN; ++b) {
     const int n = (brb + b) * 32 + (lane & 31);
@@ -234,13 +420,13 @@ I8Launch i8_launch(const I8GemmArgs& g) {
 }  // namespace
 
 int64_t gemm_i8_grid_items(const I8GemmArgs& g) {
-  return i8_launch(g).grid * 256;
+  return i8_launch(g).grid * 64 * kI8Cfg[g.cfg].wm * kI8Cfg[g.cfg].wn;
 }
 
 void gemm_sliced_i8(const I8GemmArgs& g, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0) return;
-  GAIN_CHECK_ARG(g.cfg >= 0 && g.cfg < kI8Configs, "int8 gemm: config %d",
-                 g.cfg);
+  GAIN_CHECK_ARG(i8_config_ok(g.cfg, g.S),
+                 "int8 gemm: configuration %d with %d slices", g.cfg, g.S);
   GAIN_CHECK_ARG(g.Kp > 0 && g.Kp % 32 == 0, "int8 gemm: Kp %lld",
                  (long long)g.Kp);
   GAIN_CHECK_ARG(g.ksplit >= 1 && g.nz >= 1 && g.row0 >= 0,
@@ -254,7 +440,8 @@ void gemm_sliced_i8(const I8GemmArgs& g, hipStream_t s) {
                  g.S);
   GAIN_CHECK_ARG(L.grid < (1ll << 31), "int8 gemm: grid too large");
   const int key = g.S * 100 + g.cfg;
-  const dim3 grid((unsigned)L.grid), block(256);
+  const dim3 grid((unsigned)L.grid),
+      block(64 * kI8Cfg[g.cfg].wm * kI8Cfg[g.cfg].wn);
   switch (key) {
 #define GAIN_I8_CASE(SS, CC)                                                \
   case SS * 100 + CC:                                                       \
@@ -268,14 +455,36 @@ void gemm_sliced_i8(const I8GemmArgs& g, hipStream_t s) {
   GAIN_I8_CASE(SS, 2) \
   GAIN_I8_CASE(SS, 3) \
   GAIN_I8_CASE(SS, 4) \
-  GAIN_I8_CASE(SS, 5)
+  GAIN_I8_CASE(SS, 5) \
+  GAIN_I8_CASE(SS, 8) \
+  GAIN_I8_CASE(SS, 9)
     GAIN_I8_S(5)
     GAIN_I8_S(6)
     GAIN_I8_S(7)
+#define GAIN_I8_LDS(SS, CC, NS)                                             \
+  case SS * 100 + CC:                                                       \
+    hipLaunchKernelGGL((k_gemm_i8_lds<SS, kI8Cfg[CC].tm, kI8Cfg[CC].tn,     \
+                                      kI8Cfg[CC].wm, kI8Cfg[CC].wn, NS>),   \
+                       grid, block, 0, s, g, L.ntm, L.ntn, L.rb0, L.kb_per); \
+    break;
+    GAIN_I8_LDS(5, 6, 4)
+    GAIN_I8_LDS(5, 7, 4)
+    GAIN_I8_LDS(6, 6, 4)
+    GAIN_I8_LDS(6, 7, 4)
+    GAIN_I8_LDS(7, 6, 3)
+    GAIN_I8_LDS(7, 7, 3)
+    GAIN_I8_LDS(5, 10, 2)
+    GAIN_I8_LDS(5, 11, 2)
+    GAIN_I8_LDS(6, 10, 2)
+    GAIN_I8_LDS(6, 11, 2)
+    GAIN_I8_LDS(7, 10, 2)
+    GAIN_I8_LDS(7, 11, 2)
+#undef GAIN_I8_LDS
 #undef GAIN_I8_S
 #undef GAIN_I8_CASE
     default:
-      fail(GAIN_ERR_ARG, "int8 slice count %d not supported (5..7)", g.S);
+      fail(GAIN_ERR_ARG, "int8 gemm: %d slices with configuration %s not "
+           "supported", g.S, i8_config_name(g.cfg));
   }
   check_launch("gemm_sliced_i8");
 }

@@ -271,14 +271,14 @@ def test_batched_split_int8_error_vs_fp64_oracle(precision, bound):
     assert es <= bound
 
 
-@pytest.mark.parametrize("cfg", list(range(6)))
+@pytest.mark.parametrize("cfg", list(range(12)))
 @pytest.mark.parametrize("split", [1, 2])
 def test_every_int8_tile_configuration(cfg, split):
-    """Each split-INT8 tile shape for both GEMM kinds (and split-K 2 on
-    the projection) on sizes with partial tiles; a frequency window starts
-    the projected final rows off a 32-row block boundary.  Seven slices
-    (49 bits of each row maximum) keep S within 1e-9 of the FP64 oracle
-    on strong lines."""
+    """Each split-INT8 tile shape (per-wave reads and LDS-staged tiles)
+    for both GEMM kinds, and split-K 2 on the projection,
+    on sizes with partial tiles; a frequency window starts the projected
+    final rows off a 32-row block boundary.  Six slices keep S within 5e-8
+    of the FP64 oracle on strong lines."""
     prob = synth.make_problem([4, 5], 20, "c2v4")
     prob.params.erange_lower = (0.0, 1e9)
     prob.params.freq_window = (1500.0, 1e9)
@@ -289,12 +289,14 @@ def test_every_int8_tile_configuration(cfg, split):
     with Engine.from_problem(prob) as eng:
         eng.set_tuning("i8_cfg", cfg)
         eng.set_tuning("i8_split", split)
-        got = engine_lines(eng, prob, initials, "batched", 6)
+        got = engine_lines(eng, prob, initials, "batched", 5)
         assert eng.get_tuning("proj_cfg") == cfg
         assert eng.get_tuning("proj_split") == split
     assert set(got) == set(ref)
     es, _, ns, _ = _strong_weak_err(got, ref)
-    assert ns > 0 and es <= 1e-9, es
+    print(f"int8 x6 configuration {cfg} split {split}: {ns} strong lines, "
+          f"max rel err {es:.3e}")
+    assert ns > 0 and es <= 5e-8, es
 
 
 @pytest.mark.parametrize("cfg", list(range(15)))
