@@ -232,6 +232,15 @@ def main():
         del src, dst
         torch.cuda.empty_cache()
 
+    # measured matrix-core ceilings (register-only MFMA chains)
+    mfma_peak = {"i8_top_s": eng.mfma_peak("i8") / 1e12,
+                 "f64_tflop_s": eng.mfma_peak("f64") / 1e12,
+                 "note": "register-only chains of v_mfma_i32_32x32x32_i8 / "
+                         "v_mfma_f64_16x16x4f64 on every CU (no memory "
+                         "traffic); the rooflines use the published peaks"}
+    log(f"measured MFMA ceiling: int8 {mfma_peak['i8_top_s']:.0f} TOP/s, "
+        f"fp64 {mfma_peak['f64_tflop_s']:.1f} TFLOP/s")
+
     # ---------------- plain FP64 GEMV drop-in
     pl = {k: torch.zeros(nF, dtype=torch.float64, device=dev)
           for k in ("S", "A", "I")}
@@ -461,6 +470,7 @@ def main():
                            "unit": "GB/s", "frac": gemv_gbs / HBM_PEAK_GBS,
                            "device_copy_gbs": copy_gbs,
                            "traffic": gemv_traffic},
+        "mfma_peak_measured": mfma_peak,
         "mixed_j_sweep": sweep,
         "cpu_baseline": cpu,
     }

@@ -884,8 +884,24 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
       src_v = 1;
     } else {
       double* LS = c->w_part.get<double>((size_t)nR * nvecG);
-      Timed t(c, "gemm_proj");
       const int S = slices_of(precision);
+      const int S8 = i8_slices_of(precision);
+      // split-INT8: final records sliced once per block, projected vectors
+      // sliced here, outside the timed projection GEMM
+      const Sliced* cs8 = nullptr;
+      int8_t* gs8 = nullptr;
+      int* ge8 = nullptr;
+      int64_t gplane8 = 0;
+      if (S8 > 0) {
+        cs8 = &block_sliced_i8(c, blkF, S8);
+        const int64_t Rg = i8_rows_pad(nvecG);
+        gplane8 = Rg * cs8->Kp;
+        gs8 = c->w_gs.get<int8_t>((size_t)S8 * gplane8);
+        ge8 = c->w_ge.get<int>((size_t)Rg);
+        slice_rows_i8(S8, G, gstride, nvecG, ntotF, gs8, gplane8, ge8,
+                      c->stream);
+      }
+      Timed t(c, "gemm_proj");
       if (S > 0) {
         const Sliced& cs = block_sliced(c, blkF, S);
         uint16_t* gs = c->w_gs.get<uint16_t>((size_t)S * nvecG * cs.Kp);
@@ -908,14 +924,8 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
         g.ldc = nR;
         g.nz = 1;
         gemm_sliced_bf16(g, c->stream);
-      } else if (i8_slices_of(precision) > 0) {
-        const int S8 = i8_slices_of(precision);
-        const Sliced& cs = block_sliced_i8(c, blkF, S8);
-        const int64_t Rg = i8_rows_pad(nvecG), plane = Rg * cs.Kp;
-        int8_t* gs = c->w_gs.get<int8_t>((size_t)S8 * plane);
-        int* ge = c->w_ge.get<int>((size_t)Rg);
-        slice_rows_i8(S8, G, gstride, nvecG, ntotF, gs, plane, ge,
-                      c->stream);
+      } else if (S8 > 0) {
+        const Sliced& cs = *cs8;
         I8GemmArgs g{};
         g.S = S8;
         g.M = nR;
@@ -927,9 +937,9 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
         g.a_z = 0;
         g.eA = cs.expo;
         g.ea_z = 0;
-        g.B = gs;
-        g.b_plane = plane;
-        g.eB = ge;
+        g.B = gs8;
+        g.b_plane = gplane8;
+        g.eB = ge8;
         g.ldc = nR;
         g.c_z = 0;
         g.c_ks = (int64_t)nR * nvecG;

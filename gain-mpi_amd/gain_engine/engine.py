@@ -266,6 +266,12 @@ class Engine:
         check(self._L.gain_get_tuning(self._h, key.encode(), C.byref(v)))
         return int(v.value)
 
+    def mfma_peak(self, kind: str) -> float:
+        """Measured register-only MFMA throughput ("i8" or "f64"), ops/s."""
+        v = C.c_double(0.0)
+        check(self._L.gain_mfma_peak(self._h, kind.encode(), C.byref(v)))
+        return float(v.value)
+
     def tuning_report(self) -> str:
         buf = C.create_string_buffer(1 << 16)
         check(self._L.gain_tuning_report(self._h, buf, 1 << 16))

@@ -237,6 +237,12 @@ int gain_set_tuning(gain_ctx* ctx, const char* key, int value);
 int gain_get_tuning(gain_ctx* ctx, const char* key, long long* value);
 /* The autotuner's timings and choices, one line per GEMM shape. */
 int gain_tuning_report(gain_ctx* ctx, char* buf, int len);
+/* Measured matrix-core ceiling: register-only chains of one MFMA
+ * instruction ("i8": v_mfma_i32_32x32x32_i8, "f64": v_mfma_f64_16x16x4f64)
+ * on every CU, in operations (2 per multiply-add) per second.  No reference
+ * counterpart: it calibrates the rooflines of the split-INT8 and FP64
+ * GEMMs. */
+int gain_mfma_peak(gain_ctx* ctx, const char* kind, double* ops_per_s);
 
 /* ---- transition list (main.cpp:238-338 + Output.cpp:52-112) ----------- */
 typedef struct {
