@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "common.hpp"
 #include "kernels.hpp"
@@ -17,9 +18,8 @@ typedef double dbl2 __attribute__((ext_vector_type(2)));
 //   X[vec][2K+tau][v] = sum_{r in slot} (-1)^{(K mod 3) tau + K} x[r],
 //   x[r] = sum_{i<N[case]} c[t0+i] repres[(t0+i) + d*Ntot + lambda*sdeg*Ntot]
 // The sign is the initial-side factor of dipole_GPU.cu:437-438.
-__global__ __launch_bounds__(256) void k_scatter_expand(ScatterArgs a) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  const int b = blockIdx.y;
+__device__ inline void scatter_body(const ScatterArgs& a, int bx, int b) {
+  const int s = bx * 256 + threadIdx.x;
   if (s >= a.nslot) return;
   const double* c = a.cbase + a.c_off[b];
   const int ideg = a.ideg[b];
@@ -38,6 +38,10 @@ __global__ __launch_bounds__(256) void k_scatter_expand(ScatterArgs a) {
     acc += ((((K % 3) * t + K) & 1) ? -x : x);
   }
   a.X[(int64_t)b * a.xstride + s] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_scatter_expand(ScatterArgs a) {
+  scatter_body(a, blockIdx.x, blockIdx.y);
 }
 
 void launch_scatter_expand(const ScatterArgs& a, hipStream_t s) {
@@ -110,9 +114,8 @@ void launch_mu_gemm_valu(const double* mu, int nc, const double* X,
 //           f3j * s * sum_{tau_I} m * Y_alpha[v(r'), 2 K_I + tau_I]
 // with alpha, m and s as in the reference kernel (s = truncated SQRT2 when
 // K_I != K_F and K_I K_F != 0).
-__global__ __launch_bounds__(256) void k_gather_h(GatherArgs a) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  const int b = blockIdx.y;
+__device__ inline void gather_body(const GatherArgs& a, int bx, int b) {
+  const int r = bx * 256 + threadIdx.x;
   if (r >= a.nprimF) return;
   const int K = a.kvalF[r];
   const int tF = a.tauF[r] & 1;
@@ -151,6 +154,10 @@ __global__ __launch_bounds__(256) void k_gather_h(GatherArgs a) {
   a.H[(int64_t)b * a.hstride + r] = neg ? -acc : acc;
 }
 
+__global__ __launch_bounds__(256) void k_gather_h(GatherArgs a) {
+  gather_body(a, blockIdx.x, blockIdx.y);
+}
+
 void launch_gather_h(const GatherArgs& a, hipStream_t s) {
   if (a.nprimF == 0 || a.nvec == 0) return;
   dim3 grid((a.nprimF + 255) / 256, a.nvec);
@@ -163,9 +170,8 @@ void launch_gather_h(const GatherArgs& a, hipStream_t s) {
 //   repres[t + d*Ntot + lambda(r)*sdeg*Ntot] h[r]
 // so that <P c_F, h> = <c_F, g> for every final record c_F
 // (GpuManager.cpp:733-741 expands each c_F and dots it with h instead).
-__global__ __launch_bounds__(256) void k_expand_T(ExpandTArgs a) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int v = blockIdx.y;
+__device__ inline void expand_T_body(const ExpandTArgs& a, int bx, int v) {
+  const int t = bx * 256 + threadIdx.x;
   if (t >= a.ntot) return;
   const int c = a.tcase[t];
   const int d = a.ideg_of_vec[v];
@@ -178,6 +184,10 @@ __global__ __launch_bounds__(256) void k_expand_T(ExpandTArgs a) {
     acc = fma(rp[(int64_t)lam * a.sdeg * a.ntot], h[r], acc);
   }
   a.G[(int64_t)v * a.gstride + t] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_expand_T(ExpandTArgs a) {
+  expand_T_body(a, blockIdx.x, blockIdx.y);
 }
 
 void launch_expand_T(const ExpandTArgs& a, hipStream_t s) {
@@ -194,9 +204,9 @@ void launch_expand_T(const ExpandTArgs& a, hipStream_t s) {
 // from L2.  Wave-level tree reduction; split-K partials reduced by the
 // epilogue in a fixed order.
 template <int R, int NV, bool VEC>
-__global__ __launch_bounds__(256) void k_gemv(GemvArgs a, int kchunk) {
+__device__ inline void gemv_body(const GemvArgs& a, int kchunk,
+                                 int64_t wave) {
   const int lane = threadIdx.x & 63;
-  const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int ngroups = (a.nF + R - 1) / R;
   if (wave >= (int64_t)ngroups * a.ksplit) return;
   const int grp = (int)(wave / a.ksplit);
@@ -296,6 +306,12 @@ __global__ __launch_bounds__(256) void k_gemv(GemvArgs a, int kchunk) {
   }
 }
 
+template <int R, int NV, bool VEC>
+__global__ __launch_bounds__(256) void k_gemv(GemvArgs a, int kchunk) {
+  gemv_body<R, NV, VEC>(a, kchunk,
+                        (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6));
+}
+
 int gemv_kchunk(int ntot, int ksplit) {
   int kc = (ntot + ksplit - 1) / ksplit;
   return (kc + 127) / 128 * 128;
@@ -347,9 +363,8 @@ void launch_gemv(const GemvArgs& a, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // Epilogue: transition filters (States.cpp:46-76), S from the degenerate
 // components, Einstein A, threshold and intensity (Output.cpp:52-112).
-__global__ __launch_bounds__(256) void k_lines(LinesArgs a) {
-  const int F = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y;
+__device__ inline void lines_body(const LinesArgs& a, int bx, int i) {
+  const int F = bx * 256 + threadIdx.x;
   if (F >= a.nF) return;
   const int64_t o = (int64_t)i * a.nF + F;
   const double eI = a.eI[i];
@@ -399,6 +414,10 @@ __global__ __launch_bounds__(256) void k_lines(LinesArgs a) {
   if (a.A) a.A[o] = A;
   if (a.I) a.I[o] = In;
   if (a.flags) a.flags[o] = fl;
+}
+
+__global__ __launch_bounds__(256) void k_lines(LinesArgs a) {
+  lines_body(a, blockIdx.x, blockIdx.y);
 }
 
 void launch_lines(const LinesArgs& a, hipStream_t s) {
@@ -648,6 +667,191 @@ void launch_combine(int P, int64_t n, const double* c, const double* M,
   hipLaunchKernelGGL(k_combine, dim3((unsigned)((threads + 255) / 256)),
                      dim3(256), 0, s, P, n, c, M, mu);
   check_launch("combine");
+}
+
+
+// ---------------------------------------------------------------------------
+// Grouped launches (mixed-J sweeps): one kernel over a table of argument
+// blocks in device memory; entry t owns workgroups [start[t], start[t+1])
+// of the flattened grid and runs the same body as its single launch.
+namespace {
+
+__device__ inline int find_entry(const int* __restrict__ start, int n,
+                                 int b) {
+  int lo = 0, hi = n - 1;  // last t with start[t] <= b
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+struct ScatterOp {
+  using Args = ScatterArgs;
+  static __host__ __device__ int nbx(const Args& a) {
+    return (a.nslot + 255) / 256;
+  }
+  static int64_t blocks(const Args& a) {
+    return a.nslot == 0 ? 0 : (int64_t)nbx(a) * a.nvec;
+  }
+  static __device__ void run(const Args& a, int local) {
+    const int n = nbx(a);
+    scatter_body(a, local % n, local / n);
+  }
+};
+struct GatherOp {
+  using Args = GatherArgs;
+  static __host__ __device__ int nbx(const Args& a) {
+    return (a.nprimF + 255) / 256;
+  }
+  static int64_t blocks(const Args& a) {
+    return a.nprimF == 0 ? 0 : (int64_t)nbx(a) * a.nvec;
+  }
+  static __device__ void run(const Args& a, int local) {
+    const int n = nbx(a);
+    gather_body(a, local % n, local / n);
+  }
+};
+struct ExpandTOp {
+  using Args = ExpandTArgs;
+  static __host__ __device__ int nbx(const Args& a) {
+    return (a.ntot + 255) / 256;
+  }
+  static int64_t blocks(const Args& a) {
+    return a.ntot == 0 ? 0 : (int64_t)nbx(a) * a.nvec;
+  }
+  static __device__ void run(const Args& a, int local) {
+    const int n = nbx(a);
+    expand_T_body(a, local % n, local / n);
+  }
+};
+struct LinesOp {
+  using Args = LinesArgs;
+  static __host__ __device__ int nbx(const Args& a) {
+    return (a.nF + 255) / 256;
+  }
+  static int64_t blocks(const Args& a) {
+    return a.nF == 0 ? 0 : (int64_t)nbx(a) * a.nI;
+  }
+  static __device__ void run(const Args& a, int local) {
+    const int n = nbx(a);
+    lines_body(a, local % n, local / n);
+  }
+};
+
+template <class Op>
+__global__ __launch_bounds__(256) void k_grouped(
+    const typename Op::Args* __restrict__ args, const int* __restrict__ start,
+    int n) {
+  const int t = find_entry(start, n, blockIdx.x);
+  Op::run(args[t], blockIdx.x - start[t]);
+}
+
+// GEMV entries: R rows per wave and NV vector slots for every entry; the
+// K chunk of entry t is gemv_kchunk(ntot, ksplit) of that entry.
+template <int R, int NV>
+__global__ __launch_bounds__(256) void k_gemv_grouped(
+    const GemvArgs* __restrict__ args, const int* __restrict__ start, int n) {
+  const int t = find_entry(start, n, blockIdx.x);
+  const GemvArgs& a = args[t];
+  int kc = (a.ntot + a.ksplit - 1) / a.ksplit;
+  kc = (kc + 127) / 128 * 128;
+  gemv_body<R, NV, true>(a, kc,
+                         (int64_t)(blockIdx.x - start[t]) * 4 +
+                             (threadIdx.x >> 6));
+}
+
+template <class Op>
+void grouped_table(const std::vector<typename Op::Args>& v,
+                   std::vector<int>* start, std::vector<int>* keep) {
+  int64_t total = 0;
+  start->clear();
+  keep->clear();
+  for (size_t t = 0; t < v.size(); ++t) {
+    const int64_t b = Op::blocks(v[t]);
+    if (b == 0) continue;
+    keep->push_back((int)t);
+    start->push_back((int)total);
+    total += b;
+  }
+  GAIN_CHECK_ARG(total < (1ll << 31), "grouped launch: too many blocks");
+  start->push_back((int)total);
+}
+
+}  // namespace
+
+#define GAIN_GROUPED_TABLE(NAME, OP, ARGS)                                    \
+  GroupedTable NAME(const std::vector<ARGS>& v) {                            \
+    std::vector<int> start, keep;                                            \
+    grouped_table<OP>(v, &start, &keep);                                     \
+    GroupedTable g;                                                          \
+    g.start = start;                                                         \
+    g.keep = keep;                                                           \
+    return g;                                                                \
+  }
+GAIN_GROUPED_TABLE(grouped_scatter_table, ScatterOp, ScatterArgs)
+GAIN_GROUPED_TABLE(grouped_gather_table, GatherOp, GatherArgs)
+GAIN_GROUPED_TABLE(grouped_expand_T_table, ExpandTOp, ExpandTArgs)
+GAIN_GROUPED_TABLE(grouped_lines_table, LinesOp, LinesArgs)
+#undef GAIN_GROUPED_TABLE
+
+GroupedTable grouped_gemv_table(const std::vector<GemvArgs>& v, int R) {
+  GroupedTable g;
+  int64_t total = 0;
+  for (size_t t = 0; t < v.size(); ++t) {
+    const GemvArgs& a = v[t];
+    if (a.nF <= 0) continue;
+    GAIN_CHECK_ARG((a.ld % 2 == 0) && (a.gstride % 2 == 0) &&
+                       ((uintptr_t)a.C % 16 == 0) &&
+                       ((uintptr_t)a.G % 16 == 0),
+                   "grouped gemv: records and vectors must be 16-byte "
+                   "aligned with even strides");
+    const int64_t waves = (int64_t)((a.nF + R - 1) / R) * a.ksplit;
+    g.keep.push_back((int)t);
+    g.start.push_back((int)total);
+    total += (waves + 3) / 4;
+  }
+  GAIN_CHECK_ARG(total < (1ll << 31), "grouped gemv: too many blocks");
+  g.start.push_back((int)total);
+  return g;
+}
+
+void launch_grouped(int kind, const void* d_args, const int* d_start, int n,
+                    int nblocks, hipStream_t s) {
+  if (n <= 0 || nblocks <= 0) return;
+  const dim3 grid((unsigned)nblocks), block(256);
+  switch (kind) {
+    case kGroupedScatter:
+      hipLaunchKernelGGL(k_grouped<ScatterOp>, grid, block, 0, s,
+                         (const ScatterArgs*)d_args, d_start, n);
+      break;
+    case kGroupedGather:
+      hipLaunchKernelGGL(k_grouped<GatherOp>, grid, block, 0, s,
+                         (const GatherArgs*)d_args, d_start, n);
+      break;
+    case kGroupedExpandT:
+      hipLaunchKernelGGL(k_grouped<ExpandTOp>, grid, block, 0, s,
+                         (const ExpandTArgs*)d_args, d_start, n);
+      break;
+    case kGroupedLines:
+      hipLaunchKernelGGL(k_grouped<LinesOp>, grid, block, 0, s,
+                         (const LinesArgs*)d_args, d_start, n);
+      break;
+    case kGroupedGemv4:
+      hipLaunchKernelGGL((k_gemv_grouped<4, 4>), grid, block, 0, s,
+                         (const GemvArgs*)d_args, d_start, n);
+      break;
+    case kGroupedGemv8:
+      hipLaunchKernelGGL((k_gemv_grouped<8, 9>), grid, block, 0, s,
+                         (const GemvArgs*)d_args, d_start, n);
+      break;
+    default:
+      fail(GAIN_ERR_ARG, "grouped launch kind %d", kind);
+  }
+  check_launch("grouped");
 }
 
 }  // namespace gain

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace gain {
 
@@ -185,5 +186,31 @@ void launch_axpy1(const double* x, double* y, int64_t n, int accumulate,
 // mu[i] = sum_p c[p] M[p][i]
 void launch_combine(int P, int64_t n, const double* c, const double* M,
                     double* mu, hipStream_t s);
+
+
+// Grouped launches for mixed-J sweeps: one kernel runs a table of argument
+// blocks (device memory); entry keep[t] of the host vector owns workgroups
+// [start[t], start[t+1]).  The table builders skip empty entries.
+struct GroupedTable {
+  std::vector<int> start;  // size = keep.size() + 1; back() = workgroups
+  std::vector<int> keep;   // indices into the argument vector
+};
+enum {
+  kGroupedScatter = 0,
+  kGroupedGather = 1,
+  kGroupedExpandT = 2,
+  kGroupedLines = 3,
+  kGroupedGemv4 = 4,  // GEMV, 4 rows per wave, up to 4 vectors
+  kGroupedGemv8 = 5   // GEMV, 8 rows per wave, up to 9 vectors
+};
+GroupedTable grouped_scatter_table(const std::vector<ScatterArgs>& v);
+GroupedTable grouped_gather_table(const std::vector<GatherArgs>& v);
+GroupedTable grouped_expand_T_table(const std::vector<ExpandTArgs>& v);
+GroupedTable grouped_lines_table(const std::vector<LinesArgs>& v);
+// R = rows per wave of the GEMV kind the table is for (4 or 8); requires
+// 16-byte aligned records and vectors with even strides.
+GroupedTable grouped_gemv_table(const std::vector<GemvArgs>& v, int R);
+void launch_grouped(int kind, const void* d_args, const int* d_start, int n,
+                    int nblocks, hipStream_t s);
 
 }  // namespace gain

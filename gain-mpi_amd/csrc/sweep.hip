@@ -10,7 +10,10 @@
 // 128-wide GEMM tile would be mostly padding); such tasks that target the
 // same final block are packed into one GEMV of up to 9 vectors over the
 // union of their energy windows, so the block's records are read once per
-// pack rather than once per task.
+// pack rather than once per task, and every pack of the sweep runs in one
+// grouped GEMV launch.  The per-batch and per-task kernels (scatter, gather,
+// transposed expansion, epilogue) also run as one grouped launch per stage
+// over argument tables built at creation.
 // Everything a replay touches is owned by the sweep (sized at creation), so
 // replays stay valid whatever the context does in between.  The launch
 // sequence contains kernels only (no memset or memcpy nodes).
@@ -69,6 +72,16 @@ struct gain_sweep {
   std::vector<Group> groups;
   std::vector<Task> tasks;
   std::vector<Unit> units;
+  // one grouped launch per stage: argument table + workgroup offsets
+  struct Launch {
+    int kind = 0;
+    const void* args = nullptr;
+    const int* start = nullptr;
+    int n = 0, blocks = 0;
+  };
+  std::vector<Launch> stage_scatter, stage_gather, stage_gemv, stage_lines;
+  std::vector<GemvArgs> gemv_single;  // units whose layout the grouped
+                                      // GEMV cannot take
   // mu X of every batch: one GEMM over [X of batch 0 | X of batch 1 | ...]
   GemmArgs hl{};
   bool has_hl = false;
@@ -95,9 +108,10 @@ void add_tiles(std::vector<GroupTile>& tiles, int prob, int M, int N, int nz,
       for (int tn = 0; tn < ntn; ++tn) tiles.push_back({prob, tm, tn, z});
 }
 
+void build_launches(gain_sweep* s);
+
 void build(gain_sweep* s, const gain_task* tasks, int ntasks) {
   gain_ctx* c = s->c;
-  const gain_params& p = c->params;
   const int nc = c->nc;
   std::map<std::tuple<int, int, std::vector<int>>, int> gidx;
   for (int t = 0; t < ntasks; ++t) {
@@ -254,7 +268,9 @@ void build(gain_sweep* s, const gain_task* tasks, int ntasks) {
       U.gstride = T.gstride;
       U.Flo = T.P.Flo;
       U.Fhi = T.P.Fhi;
-      U.ksplit = c->gemv_ksplit;
+      // K split so that every unit's rows spread over many waves (a block
+      // holds a few thousand records); chunks of at least 4096 elements
+      U.ksplit = std::max(1, std::min(16, U.ntot / 4096));
       open_unit[key] = (int)s->units.size();
       s->units.push_back(U);
       it = open_unit.find(key);
@@ -277,19 +293,36 @@ void build(gain_sweep* s, const gain_task* tasks, int ntasks) {
     T.G = U.G + (int64_t)T.vecoff * U.gstride;
     T.LS = U.part + T.vecoff;
   }
-  (void)p;
   if (!pj.empty()) {
     s->d_pj_probs = up(s, pj);
     s->d_pj_tiles = up(s, pjt);
     s->n_pj_tiles = (int)pjt.size();
   }
+  build_launches(s);
 }
 
-// The launch sequence of one sweep (kernels only).
-void enqueue(gain_sweep* s, hipStream_t st) {
+template <class A>
+void add_stage(gain_sweep* s, std::vector<gain_sweep::Launch>& out, int kind,
+               const std::vector<A>& v, const GroupedTable& tab) {
+  if (tab.keep.empty()) return;
+  std::vector<A> kept;
+  for (int t : tab.keep) kept.push_back(v[t]);
+  gain_sweep::Launch L;
+  L.kind = kind;
+  L.args = up(s, kept);
+  L.start = up(s, tab.start);
+  L.n = (int)tab.keep.size();
+  L.blocks = tab.start.back();
+  out.push_back(L);
+}
+
+// Argument tables of every stage of the launch sequence (all buffers are
+// owned by the sweep, so the pointers stay valid for every replay).
+void build_launches(gain_sweep* s) {
   gain_ctx* c = s->c;
   const gain_params& p = c->params;
   const int nc = c->nc;
+  std::vector<ScatterArgs> sc;
   for (gain_sweep::Group& G : s->groups) {
     if (G.nvecX == 0) continue;
     Basis& bI = *G.P.bI;
@@ -313,9 +346,13 @@ void enqueue(gain_sweep* s, hipStream_t st) {
     sa.nvec = G.nvecX;
     sa.X = G.X;
     sa.xstride = (int64_t)bI.ncol * nc;
-    launch_scatter_expand(sa, st);
+    sc.push_back(sa);
   }
-  if (s->has_hl) gemm_f64(s->hl, st);
+  add_stage(s, s->stage_scatter, kGroupedScatter, sc,
+            grouped_scatter_table(sc));
+
+  std::vector<GatherArgs> gv;
+  std::vector<ExpandTArgs> ev;
   for (gain_sweep::Task& T : s->tasks) {
     if (!T.compute) continue;
     const gain_sweep::Group& G = s->groups[T.group];
@@ -339,7 +376,7 @@ void enqueue(gain_sweep* s, hipStream_t st) {
     ga.ys_ks = 3 * s->ncol_all * nc;
     ga.H = T.H;
     ga.hstride = T.hstride;
-    launch_gather_h(ga, st);
+    gv.push_back(ga);
     ExpandTArgs ea{};
     ea.ntot = irF.ntot;
     ea.nprim = bF.nprim;
@@ -356,10 +393,15 @@ void enqueue(gain_sweep* s, hipStream_t st) {
     ea.nvec = T.nvecG;
     ea.G = T.G;
     ea.gstride = T.gstride;
-    launch_expand_T(ea, st);
+    ev.push_back(ea);
   }
-  gemm_f64_grouped(s->d_pj_probs, s->d_pj_tiles, s->n_pj_tiles, true,
-                   s->pj_cfg, s->grid, st);
+  // every gather before any projection input: expand_T reads H of its task
+  // only, so the two stages are ordered launches
+  add_stage(s, s->stage_gather, kGroupedGather, gv, grouped_gather_table(gv));
+  add_stage(s, s->stage_gather, kGroupedExpandT, ev,
+            grouped_expand_T_table(ev));
+
+  std::vector<GemvArgs> g4, g8;
   for (const gain_sweep::Unit& U : s->units) {
     GemvArgs a{};
     a.C = U.blkF->vec;
@@ -372,11 +414,23 @@ void enqueue(gain_sweep* s, hipStream_t st) {
     a.nvec = U.nvec;
     a.partial = U.part;
     a.ksplit = U.ksplit;
+    const bool aligned = (a.ld % 2 == 0) && (a.gstride % 2 == 0) &&
+                         ((uintptr_t)a.C % 16 == 0) &&
+                         ((uintptr_t)a.G % 16 == 0);
     // with many vectors per record row, more rows per wave cut the re-reads
     // of the projected vectors
-    a.rows = U.nvec > 4 ? 8 : c->gemv_rows;
-    launch_gemv(a, st);
+    a.rows = U.nvec > 4 ? 8 : 4;
+    if (!aligned)
+      s->gemv_single.push_back(a);
+    else if (U.nvec > 4)
+      g8.push_back(a);
+    else
+      g4.push_back(a);
   }
+  add_stage(s, s->stage_gemv, kGroupedGemv4, g4, grouped_gemv_table(g4, 4));
+  add_stage(s, s->stage_gemv, kGroupedGemv8, g8, grouped_gemv_table(g8, 8));
+
+  std::vector<LinesArgs> lv;
   for (gain_sweep::Task& T : s->tasks) {
     LinesArgs la{};
     la.nI = T.P.nI;
@@ -428,8 +482,28 @@ void enqueue(gain_sweep* s, hipStream_t st) {
     la.A = T.out.A;
     la.I = T.out.I;
     la.flags = T.out.flags;
-    launch_lines(la, st);
+    lv.push_back(la);
   }
+  add_stage(s, s->stage_lines, kGroupedLines, lv, grouped_lines_table(lv));
+}
+
+void run_stage(const std::vector<gain_sweep::Launch>& v, hipStream_t st) {
+  for (const gain_sweep::Launch& L : v)
+    launch_grouped(L.kind, L.args, L.start, L.n, L.blocks, st);
+}
+
+// The launch sequence of one sweep (kernels only): scatter of every batch,
+// mu X of every batch as one GEMM, gathers, transposed expansions, the
+// grouped projection GEMM and GEMVs, the epilogue of every task.
+void enqueue(gain_sweep* s, hipStream_t st) {
+  run_stage(s->stage_scatter, st);
+  if (s->has_hl) gemm_f64(s->hl, st);
+  run_stage(s->stage_gather, st);
+  gemm_f64_grouped(s->d_pj_probs, s->d_pj_tiles, s->n_pj_tiles, true,
+                   s->pj_cfg, s->grid, st);
+  run_stage(s->stage_gemv, st);
+  for (const GemvArgs& a : s->gemv_single) launch_gemv(a, st);
+  run_stage(s->stage_lines, st);
 }
 
 }  // namespace
