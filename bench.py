@@ -102,6 +102,8 @@ def run_mixed_sweep(nreplay):
     t1 = time.time()
     sw = Sweep(eng, full)
     create_s = time.time() - t1
+    gemv_bytes = eng.get_tuning("sweep_gemv_bytes")
+    gemv_packs = eng.get_tuning("sweep_gemv_packs")
     stream = torch.cuda.ExternalStream(eng.stream())
     e0, e1 = torch.cuda.Event(enable_timing=True), \
         torch.cuda.Event(enable_timing=True)
@@ -135,6 +137,27 @@ def run_mixed_sweep(nreplay):
         if not bool(ok.all()):
             worst = max(worst, float((d / bound.clamp_min(1e-300)).max()))
     sw.close()
+    # the same sweep with split-INT8 x6 mu X products (projections FP64)
+    sw8 = Sweep(eng, full, precision=5)
+    sw8.run()
+    eng.synchronize()
+    e0.record(stream)
+    for _ in range(nreplay):
+        sw8.run()
+    e1.record(stream)
+    eng.synchronize()
+    sweep8_ms = e0.elapsed_time(e1) / nreplay
+    same8, err8 = True, 0.0
+    for o, q in zip(sw_out, pl_out):
+        same8 &= bool(torch.equal(o["flags"], q["flags"]))
+        R = q["S"]
+        strong = R > 1e-6 * R.max(dim=1, keepdim=True).values
+        if bool(strong.any()):
+            rel = ((o["S"] - R).abs() / R.clamp_min(1e-300))[strong]
+            err8 = max(err8, float(rel.max()))
+    sw8.close()
+    log(f"mixed-J sweep, split-INT8 x6 mu X: {sweep8_ms:.2f} ms per replay, "
+        f"flags identical {same8}, strong-line S max rel err {err8:.2e}")
     nI = len(w.initials)
     r = {"workload": "256 initial states drawn from J_I in [5,30] (4 irreps, "
                      "N_J=(2J+1)*750), Nc=3000, 2000 states per block, "
@@ -148,10 +171,17 @@ def run_mixed_sweep(nreplay):
          "plain_unbatched_tps": trans / (plain_ms / 1e3),
          "speedup_vs_plain": plain_ms / sweep_ms,
          "graph_capture_s": create_s, "build_s": build_s,
+         "gemv_packs": gemv_packs, "gemv_bytes": gemv_bytes,
          "transition_flags_identical": same_flags,
          "s_within_fp64_bound": worst == 0.0,
          "note": "one hipGraph replay per sweep; the bound is |dS| <= "
-                 "2e-12 S + 2e-14 sqrt(S S_max) per initial state"}
+                 "2e-12 S + 2e-14 sqrt(S S_max) per initial state",
+         "split_int8_x6": {"ms_per_sweep": sweep8_ms,
+                           "value": trans / (sweep8_ms / 1e3),
+                           "transition_flags_identical": same8,
+                           "strong_max_rel_err_vs_plain_fp64": err8,
+                           "note": "mu X products as split-INT8 x6, "
+                                   "projections FP64"}}
     log(f"mixed-J sweep: {len(tasks)} tasks, {trans} transitions, "
         f"{sweep_ms:.2f} ms per replay -> {r['value']:.3e}/s; unbatched "
         f"plain {plain_ms:.1f} ms; flags identical {same_flags}, S within "

@@ -293,6 +293,9 @@ Basis& basis_of(gain_ctx* c, int jind, bool need_infl);
 Block& block_of(gain_ctx* c, int jind, int gamma);
 void ensure_slots(gain_ctx* c, Basis& b);
 int slices_of(int precision);
+int i8_slices_of(int precision);
+int i8_cfg_split(gain_ctx* c, int kind, I8GemmArgs g, int max_split,
+                 int* ksplit);
 bool degfilter(const gain_ctx* c, int gI, int gF, int dI, int dF);
 bool filter_lower(const gain_ctx* c, double e, int g);
 bool block_pair_ok(const gain_ctx* c, int jI, int gI, int jF, int gF);

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "gemm.hpp"
 #include "kernels.hpp"
 
 namespace gain {
@@ -752,16 +753,19 @@ __global__ __launch_bounds__(256) void k_grouped(
 
 // GEMV entries: R rows per wave and NV vector slots for every entry; the
 // K chunk of entry t is gemv_kchunk(ntot, ksplit) of that entry.
+// Workgroup ids are remapped so that each XCD walks a contiguous range of
+// entries: the projected vectors of the packs one XCD has in flight then
+// fit its L2 (they are re-read once per group of R rows).
 template <int R, int NV>
 __global__ __launch_bounds__(256) void k_gemv_grouped(
     const GemvArgs* __restrict__ args, const int* __restrict__ start, int n) {
-  const int t = find_entry(start, n, blockIdx.x);
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int t = find_entry(start, n, b);
   const GemvArgs& a = args[t];
   int kc = (a.ntot + a.ksplit - 1) / a.ksplit;
   kc = (kc + 127) / 128 * 128;
   gemv_body<R, NV, true>(a, kc,
-                         (int64_t)(blockIdx.x - start[t]) * 4 +
-                             (threadIdx.x >> 6));
+                         (int64_t)(b - start[t]) * 4 + (threadIdx.x >> 6));
 }
 
 template <class Op>

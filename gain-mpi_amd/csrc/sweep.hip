@@ -4,8 +4,9 @@
 // Tasks with the same initial batch (J index, irrep, levels) share X and
 // Y = mu X.  The X of every batch sits in one column range of a single
 // matrix, so mu X for the whole sweep is one GEMM (tile shape from the
-// autotuner); the projections C_F G of the tasks with many vectors run as
-// one persistent grouped launch over the tiles of all of them.
+// autotuner; FP64, or split-INT8 on the sweep's own int8 copy of mu); the
+// projections C_F G of the tasks with many vectors run as one persistent
+// grouped launch over the tiles of all of them.
 // Tasks that project at most 9 vectors go through the FP64 GEMV instead (a
 // 128-wide GEMM tile would be mostly padding); such tasks that target the
 // same final block are packed into one GEMV of up to 9 vectors over the
@@ -86,6 +87,11 @@ struct gain_sweep {
   GemmArgs hl{};
   bool has_hl = false;
   int64_t ncol_all = 0;
+  // split-INT8 mu X (precision GAIN_PREC_I8X*): the sweep's own int8 copy
+  // of mu, X sliced at every replay
+  int slices = 0;
+  I8GemmArgs hl8{};
+  int64_t x8_plane = 0;
   GemmArgs* d_pj_probs = nullptr;
   GroupTile* d_pj_tiles = nullptr;
   int n_pj_tiles = 0, pj_cfg = 5;
@@ -190,8 +196,50 @@ void build(gain_sweep* s, const gain_task* tasks, int ntasks) {
     a.sCz = s->ncol_all * nc;
     a.nz = 3;
     a.ksplit = 1;
-    a.cfg = gemm_cfg(c, 1, a);  // tuned here: replays must not time
     s->has_hl = true;
+    if (s->slices == 0) {
+      a.cfg = gemm_cfg(c, 1, a);  // tuned here: replays must not time
+    } else {
+      const int S = s->slices;
+      double* muT = s->arena.alloc<double>((size_t)3 * nc * nc);
+      transpose_mu(c->d_mu, nc, muT, c->stream);
+      const int64_t Kp = i8_kpad(nc), Rm = i8_rows_pad(nc);
+      int8_t* m8 = s->arena.alloc<int8_t>((size_t)S * 3 * Rm * Kp);
+      int* me = s->arena.alloc<int>((size_t)3 * Rm);
+      for (int al = 0; al < 3; ++al)
+        slice_rows_i8(S, muT + (size_t)al * nc * nc, nc, nc, nc,
+                      m8 + al * Rm * Kp, 3 * Rm * Kp, me + al * Rm,
+                      c->stream);
+      GAIN_HIP(hipStreamSynchronize(c->stream));
+      s->arena.release(muT);
+      const int64_t Rx = i8_rows_pad(s->ncol_all);
+      s->x8_plane = Rx * Kp;
+      I8GemmArgs& g = s->hl8;
+      g.S = S;
+      g.M = nc;
+      g.N = (int)s->ncol_all;
+      g.row0 = 0;
+      g.Kp = Kp;
+      g.A = m8;
+      g.a_plane = 3 * Rm * Kp;
+      g.a_z = Rm * Kp;
+      g.eA = me;
+      g.ea_z = Rm;
+      g.B = s->arena.alloc<int8_t>((size_t)S * s->x8_plane);
+      g.b_plane = s->x8_plane;
+      g.eB = s->arena.alloc<int>((size_t)Rx);
+      g.C = Y;
+      g.ldc = nc;
+      g.c_z = s->ncol_all * nc;
+      g.c_ks = 3 * s->ncol_all * nc;
+      g.nz = 3;
+      int ks = 1;
+      // slice once so the tuner times real operands
+      slice_rows_i8(S, X, nc, (int)s->ncol_all, nc, (int8_t*)g.B,
+                    s->x8_plane, (int*)g.eB, c->stream);
+      g.cfg = i8_cfg_split(c, 1, g, 1, &ks);
+      g.ksplit = ks;
+    }
   }
   // ---- tasks: H, G, LS, per-pair arrays, projection problems
   std::vector<GemmArgs> pj;
@@ -429,6 +477,15 @@ void build_launches(gain_sweep* s) {
   }
   add_stage(s, s->stage_gemv, kGroupedGemv4, g4, grouped_gemv_table(g4, 4));
   add_stage(s, s->stage_gemv, kGroupedGemv8, g8, grouped_gemv_table(g8, 8));
+  // algorithmic bytes of the GEMV stage: record rows inside each pack's
+  // window, its projected vectors and partials (tuning getter
+  // "sweep_gemv_bytes", for bandwidth figures)
+  int64_t gb = 0;
+  for (const gain_sweep::Unit& U : s->units)
+    gb += 8 * ((int64_t)(U.Fhi - U.Flo) * U.ntot + (int64_t)U.nvec * U.ntot +
+               (int64_t)U.ksplit * (U.Fhi - U.Flo) * U.nvec);
+  c->last["sweep_gemv_bytes"] = gb;
+  c->last["sweep_gemv_packs"] = (int64_t)s->units.size();
 
   std::vector<LinesArgs> lv;
   for (gain_sweep::Task& T : s->tasks) {
@@ -497,7 +554,12 @@ void run_stage(const std::vector<gain_sweep::Launch>& v, hipStream_t st) {
 // grouped projection GEMM and GEMVs, the epilogue of every task.
 void enqueue(gain_sweep* s, hipStream_t st) {
   run_stage(s->stage_scatter, st);
-  if (s->has_hl) gemm_f64(s->hl, st);
+  if (s->has_hl && s->slices == 0) gemm_f64(s->hl, st);
+  if (s->has_hl && s->slices > 0) {
+    slice_rows_i8(s->slices, s->hl.B, s->c->nc, (int)s->ncol_all, s->c->nc,
+                  (int8_t*)s->hl8.B, s->x8_plane, (int*)s->hl8.eB, st);
+    gemm_sliced_i8(s->hl8, st);
+  }
   run_stage(s->stage_gather, st);
   gemm_f64_grouped(s->d_pj_probs, s->d_pj_tiles, s->n_pj_tiles, true,
                    s->pj_cfg, s->grid, st);
@@ -516,14 +578,16 @@ extern "C" int gain_sweep_create(gain_ctx* c, const gain_task* tasks,
     need(c);
     GAIN_CHECK_ARG(out && tasks && ntasks > 0, "bad sweep arguments");
     *out = nullptr;
-    GAIN_CHECK_ARG(precision == GAIN_PREC_FP64,
-                   "sweeps run the FP64 grouped GEMMs (precision %d)",
+    GAIN_CHECK_ARG(precision == GAIN_PREC_FP64 ||
+                       i8_slices_of(precision) > 0,
+                   "sweeps run FP64 or split-INT8 products (precision %d)",
                    precision);
     if (!c->params_set) fail(GAIN_ERR_STATE, "gain_set_params not called");
     if (!c->d_mu) fail(GAIN_ERR_STATE, "dipole not uploaded");
     DeviceGuard g(c->device);
     s = new gain_sweep();
     s->c = c;
+    s->slices = i8_slices_of(precision);
     if (c->force_cfg[0] >= 0) s->pj_cfg = c->force_cfg[0];
     build(s, tasks, ntasks);
     GAIN_HIP(hipStreamSynchronize(c->stream));

@@ -172,9 +172,12 @@ int gain_batched_lines(gain_ctx* ctx, int jindI, int gammaI,
  * gain_batched_lines calls, e.g. every J_I -> J_I-1, J_I, J_I+1 pair of a J
  * list).  At creation the engine plans every task, runs it once, and
  * captures the launch sequence in a HIP graph: the half line strengths of
- * all batches (tasks sharing an initial batch share them) and the
- * projections of all tasks each run as one persistent grouped FP64 MFMA
- * launch.  gain_sweep_run replays the graph on the context's stream
+ * all batches (tasks sharing an initial batch share them) run as one GEMM,
+ * the projections of all tasks as one persistent grouped FP64 MFMA launch
+ * and one grouped GEMV launch, and every per-task kernel as one grouped
+ * launch per stage.  `precision` is GAIN_PREC_FP64 or a split-INT8 mode
+ * (GAIN_PREC_I8X5..7: the mu X products only; the projections stay FP64).
+ * gain_sweep_run replays the graph on the context's stream
  * (asynchronously; gain_synchronize waits).  Task outputs must be device
  * memory.  Re-uploading the dipole, a basis or a block used by a sweep
  * invalidates it.  No counterpart in the reference (its loop is
