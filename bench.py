@@ -35,13 +35,18 @@ PRECISION_NAMES = {0: "f64", 1: "f32-mfma/f64-acc",
                    3: "bf16x4-ozaki-mfma/f32-f64-acc",
                    4: "i8x5-ozaki-mfma/i32-f64-acc",
                    5: "i8x6-ozaki-mfma/i32-f64-acc",
-                   6: "i8x7-ozaki-mfma/i32-f64-acc"}
+                   6: "i8x7-ozaki-mfma/i32-f64-acc",
+                   7: "i8crt11-mfma/i32-crt-f64",
+                   8: "i8crt12-mfma/i32-crt-f64",
+                   9: "i8crt14-mfma/i32-crt-f64"}
 # dense MFMA peak of the datatype the products run in (TFLOP/s; int8 in
 # TOP/s)
 PEAKS = {0: 78.6, 1: 157.3, 2: 2500.0, 3: 2500.0, 4: 5000.0, 5: 5000.0,
-         6: 5000.0}
+         6: 5000.0, 7: 5000.0, 8: 5000.0, 9: 5000.0}
 # hardware products per algorithmic FMA: slice pairs (i, j) with i + j < S
-PRODUCTS = {0: 1, 1: 1, 2: 6, 3: 10, 4: 15, 5: 21, 6: 28}
+# (split forms), one per modulus (CRT form)
+PRODUCTS = {0: 1, 1: 1, 2: 6, 3: 10, 4: 15, 5: 21, 6: 28, 7: 11, 8: 12,
+            9: 14}
 PRODUCTS_SLICES = {4: 5, 5: 6, 6: 7}  # split-INT8 slice counts
 TIMERS = ("gemv", "half_ls", "expand_T", "epilogue", "gemm_proj")
 # rocprofv3 FETCH_SIZE / WRITE_SIZE per launch for this workload, scaled by
@@ -413,13 +418,18 @@ def main():
     if tune_log:
         log("autotuner:\n" + tune_log.rstrip())
     proj_traffic, proj_launch = None, None
-    prefix = ("projection" if args.precision == 0 else
-              f"int8x{PRODUCTS_SLICES.get(args.precision)} projection")
+    if args.precision in PRODUCTS_SLICES:
+        tag = f"int8x{PRODUCTS_SLICES[args.precision]}"
+    elif args.precision >= 7:
+        tag = f"crt{PRODUCTS[args.precision]}"
+    else:
+        tag = None
+    prefix = "projection" if tag is None else f"{tag} projection"
     chosen = [ln for ln in tune_log.splitlines()
               if ln.startswith(prefix) and "->" in ln]
     label = chosen[-1].split("->")[1].split()[0] if chosen else None
-    if label is not None and args.precision != 0:
-        label = f"int8x{PRODUCTS_SLICES[args.precision]} {label}"
+    if label is not None and tag is not None:
+        label = f"{tag} {label}"
     proj_launch = {"tiles": label,
                    "split_k": eng.get_tuning("proj_split"),
                    "grid_items_batch0": eng.get_tuning("proj_grid")}

@@ -130,6 +130,15 @@ int i8_slices_of(int precision) {
   }
 }
 
+int crt_moduli_of(int precision) {
+  switch (precision) {
+    case GAIN_PREC_I8CRT11: return 11;
+    case GAIN_PREC_I8CRT12: return 12;
+    case GAIN_PREC_I8CRT14: return 14;
+    default: return 0;
+  }
+}
+
 void release_sliced(gain_ctx* c, Sliced& sl) {
   if (sl.planes) c->arena.release(sl.planes);
   if (sl.p8) c->arena.release(sl.p8);
@@ -140,7 +149,7 @@ void release_sliced(gain_ctx* c, Sliced& sl) {
 // Sliced copy of the dipole along v_I (rows = (alpha, v_F)), built once per
 // slice count.
 const Sliced& mu_sliced(gain_ctx* c, int S) {
-  if (c->mu_sl.S == S && !c->mu_sl.i8) return c->mu_sl;
+  if (c->mu_sl.S == S && !c->mu_sl.i8 && !c->mu_sl.crt) return c->mu_sl;
   release_sliced(c, c->mu_sl);
   const int nc = c->nc;
   double* muT = c->w_tmp.get<double>((size_t)3 * nc * nc);
@@ -157,7 +166,7 @@ const Sliced& mu_sliced(gain_ctx* c, int S) {
 
 // Sliced copy of a block's eigenvector records, built once per slice count.
 const Sliced& block_sliced(gain_ctx* c, Block& b, int S) {
-  if (b.sl.S == S && !b.sl.i8) return b.sl;
+  if (b.sl.S == S && !b.sl.i8 && !b.sl.crt) return b.sl;
   release_sliced(c, b.sl);
   Sliced sl;
   sl.S = S;
@@ -173,7 +182,7 @@ const Sliced& block_sliced(gain_ctx* c, Block& b, int S) {
 // Split-INT8 copy of the dipole along v_I: one tiled matrix per component
 // (rows v_F), built once per slice count.
 const Sliced& mu_sliced_i8(gain_ctx* c, int S) {
-  if (c->mu_sl.S == S && c->mu_sl.i8) return c->mu_sl;
+  if (c->mu_sl.S == S && c->mu_sl.i8 && !c->mu_sl.crt) return c->mu_sl;
   release_sliced(c, c->mu_sl);
   const int nc = c->nc;
   double* muT = c->w_tmp.get<double>((size_t)3 * nc * nc);
@@ -197,7 +206,7 @@ const Sliced& mu_sliced_i8(gain_ctx* c, int S) {
 // Split-INT8 copy of a block's eigenvector records, built once per slice
 // count.
 const Sliced& block_sliced_i8(gain_ctx* c, Block& b, int S) {
-  if (b.sl.S == S && b.sl.i8) return b.sl;
+  if (b.sl.S == S && b.sl.i8 && !b.sl.crt) return b.sl;
   release_sliced(c, b.sl);
   Sliced sl;
   sl.S = S;
@@ -211,6 +220,97 @@ const Sliced& block_sliced_i8(gain_ctx* c, Block& b, int S) {
                 c->stream);
   b.sl = sl;
   return b.sl;
+}
+
+// Int8 residues of the dipole along v_I (CRT form), one tiled matrix per
+// component, built once per moduli count.
+const Sliced& mu_sliced_crt(gain_ctx* c, int nm) {
+  if (c->mu_sl.S == nm && c->mu_sl.crt) return c->mu_sl;
+  release_sliced(c, c->mu_sl);
+  const int nc = c->nc;
+  double* muT = c->w_tmp.get<double>((size_t)3 * nc * nc);
+  transpose_mu(c->d_mu, nc, muT, c->stream);
+  Sliced sl;
+  sl.S = nm;
+  sl.i8 = true;
+  sl.crt = true;
+  sl.Kp = i8_kpad(nc);
+  sl.beta = crt_beta(nm, sl.Kp);
+  sl.Rp = i8_rows_pad(nc);
+  sl.plane = 3 * sl.Rp * sl.Kp;
+  sl.p8 = c->arena.alloc<int8_t>((size_t)nm * sl.plane);
+  sl.expo = c->arena.alloc<int>((size_t)3 * sl.Rp);
+  for (int a = 0; a < 3; ++a)
+    slice_rows_crt(nm, sl.beta, muT + (size_t)a * nc * nc, nc, nc, nc,
+                   sl.p8 + a * sl.Rp * sl.Kp, sl.plane, sl.expo + a * sl.Rp,
+                   c->stream);
+  c->mu_sl = sl;
+  return c->mu_sl;
+}
+
+// Int8 residues of a block's eigenvector records (CRT form), built once per
+// moduli count.
+const Sliced& block_sliced_crt(gain_ctx* c, Block& b, int nm) {
+  if (b.sl.S == nm && b.sl.crt) return b.sl;
+  release_sliced(c, b.sl);
+  Sliced sl;
+  sl.S = nm;
+  sl.i8 = true;
+  sl.crt = true;
+  sl.Kp = i8_kpad(b.rec_len);
+  sl.beta = crt_beta(nm, sl.Kp);
+  sl.Rp = i8_rows_pad(b.n);
+  sl.plane = sl.Rp * sl.Kp;
+  sl.p8 = c->arena.alloc<int8_t>((size_t)nm * sl.plane);
+  sl.expo = c->arena.alloc<int>((size_t)sl.Rp);
+  slice_rows_crt(nm, sl.beta, b.vec, b.ld, b.n, b.rec_len, sl.p8, sl.plane,
+                 sl.expo, c->stream);
+  b.sl = sl;
+  return b.sl;
+}
+
+// Tile configuration of a CRT-form GEMM (kind 0: projection, 1: half line
+// strength), timed on the real operands once per shape when autotuning;
+// without tuning the 64 x 128 tile.
+int crt_cfg_of(gain_ctx* c, int kind, I8GemmArgs g) {
+  if (c->force_crt_cfg >= 0) return c->force_crt_cfg;
+  const auto key = std::make_tuple(200 + 10 * kind + g.S,
+                                   g.M >= 8192 ? -1 : g.M, g.N, (int)g.Kp);
+  auto it = c->tuned.find(key);
+  if (it != c->tuned.end()) return it->second;
+  if (!c->autotune) return 0;
+  hipEvent_t e0, e1;
+  GAIN_HIP(hipEventCreate(&e0));
+  GAIN_HIP(hipEventCreate(&e1));
+  int best = 0;
+  float best_ms = 1e30f;
+  char line[160];
+  snprintf(line, sizeof(line), "crt%d %s M=%d N=%d K=%lld:", g.S,
+           kind == 0 ? "projection" : "half_ls", g.M, g.N, (long long)g.Kp);
+  std::string rec = line;
+  for (int cfg = 0; cfg < kCrtConfigs; ++cfg) {
+    g.cfg = cfg;
+    gemm_crt_i8(g, c->stream);  // warm
+    GAIN_HIP(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < 2; ++r) gemm_crt_i8(g, c->stream);
+    GAIN_HIP(hipEventRecord(e1, c->stream));
+    GAIN_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    GAIN_HIP(hipEventElapsedTime(&ms, e0, e1));
+    ms *= 0.5f;
+    snprintf(line, sizeof(line), " %s=%.3fms", crt_config_name(cfg), ms);
+    rec += line;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = cfg;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  snprintf(line, sizeof(line), " -> %s\n", crt_config_name(best));
+  c->tune_log += rec + line;
+  c->tuned[key] = best;
+  return best;
 }
 
 // Tile configuration and split count of a split-INT8 GEMM (kind 0:
@@ -561,6 +661,40 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
     g.c_z = (int64_t)ncolTot * nc;
     g.nz = 3;
     gemm_sliced_bf16(g, c->stream);
+  } else if (crt_moduli_of(precision) > 0) {
+    const int nm = crt_moduli_of(precision);
+    const Sliced& ms = mu_sliced_crt(c, nm);
+    const int64_t Rx = i8_rows_pad(ncolTot), plane = Rx * ms.Kp;
+    int8_t* xs = c->w_xs.get<int8_t>((size_t)nm * plane);
+    int* xe = c->w_xe.get<int>((size_t)Rx);
+    slice_rows_crt(nm, ms.beta, X, nc, ncolTot, nc, xs, plane, xe,
+                   c->stream);
+    I8GemmArgs g{};
+    g.S = nm;
+    g.beta = ms.beta;
+    g.M = nc;
+    g.N = ncolTot;
+    g.row0 = 0;
+    g.Kp = ms.Kp;
+    g.A = ms.p8;
+    g.a_plane = ms.plane;
+    g.a_z = ms.Rp * ms.Kp;
+    g.eA = ms.expo;
+    g.ea_z = ms.Rp;
+    g.B = xs;
+    g.b_plane = plane;
+    g.eB = xe;
+    g.ldc = nc;
+    g.c_z = (int64_t)ncolTot * nc;
+    g.nz = 3;
+    ks = 1;
+    Y = c->w_Y.get<double>((size_t)3 * ncolTot * nc);
+    g.C = Y;
+    g.cfg = crt_cfg_of(c, 1, g);
+    gemm_crt_i8(g, c->stream);
+    c->last["half_ls_cfg"] = g.cfg;
+    c->last["half_ls_split"] = 1;
+    c->last["half_ls_grid"] = gemm_crt_grid_items(g);
   } else if (i8_slices_of(precision) > 0) {
     const int S8 = i8_slices_of(precision);
     const Sliced& ms = mu_sliced_i8(c, S8);
@@ -888,18 +1022,25 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
       const int S8 = i8_slices_of(precision);
       // split-INT8: final records sliced once per block, projected vectors
       // sliced here, outside the timed projection GEMM
+      const int NMc = crt_moduli_of(precision);
       const Sliced* cs8 = nullptr;
       int8_t* gs8 = nullptr;
       int* ge8 = nullptr;
       int64_t gplane8 = 0;
-      if (S8 > 0) {
-        cs8 = &block_sliced_i8(c, blkF, S8);
+      if (S8 > 0 || NMc > 0) {
+        const int np = S8 > 0 ? S8 : NMc;
+        cs8 = S8 > 0 ? &block_sliced_i8(c, blkF, S8)
+                     : &block_sliced_crt(c, blkF, NMc);
         const int64_t Rg = i8_rows_pad(nvecG);
         gplane8 = Rg * cs8->Kp;
-        gs8 = c->w_gs.get<int8_t>((size_t)S8 * gplane8);
+        gs8 = c->w_gs.get<int8_t>((size_t)np * gplane8);
         ge8 = c->w_ge.get<int>((size_t)Rg);
-        slice_rows_i8(S8, G, gstride, nvecG, ntotF, gs8, gplane8, ge8,
-                      c->stream);
+        if (S8 > 0)
+          slice_rows_i8(S8, G, gstride, nvecG, ntotF, gs8, gplane8, ge8,
+                        c->stream);
+        else
+          slice_rows_crt(NMc, cs8->beta, G, gstride, nvecG, ntotF, gs8,
+                         gplane8, ge8, c->stream);
       }
       Timed t(c, "gemm_proj");
       if (S > 0) {
@@ -924,6 +1065,29 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
         g.ldc = nR;
         g.nz = 1;
         gemm_sliced_bf16(g, c->stream);
+      } else if (NMc > 0) {
+        const Sliced& cs = *cs8;
+        I8GemmArgs g{};
+        g.S = NMc;
+        g.beta = cs.beta;
+        g.M = nR;
+        g.N = nvecG;
+        g.row0 = P.Flo;
+        g.Kp = cs.Kp;
+        g.A = cs.p8;
+        g.a_plane = cs.plane;
+        g.eA = cs.expo;
+        g.B = gs8;
+        g.b_plane = gplane8;
+        g.eB = ge8;
+        g.C = LS;
+        g.ldc = nR;
+        g.nz = 1;
+        g.cfg = crt_cfg_of(c, 0, g);
+        gemm_crt_i8(g, c->stream);
+        c->last["proj_cfg"] = g.cfg;
+        c->last["proj_split"] = 1;
+        c->last["proj_grid"] = gemm_crt_grid_items(g);
       } else if (S8 > 0) {
         const Sliced& cs = *cs8;
         I8GemmArgs g{};
@@ -1643,7 +1807,8 @@ extern "C" int gain_batched_lines(gain_ctx* c, int jindI, int gammaI,
   return guarded([&] {
     need(c);
     GAIN_CHECK_ARG(out && levelsI && nI > 0, "bad batch");
-    GAIN_CHECK_ARG(precision >= GAIN_PREC_FP64 && precision <= GAIN_PREC_I8X7,
+    GAIN_CHECK_ARG(precision >= GAIN_PREC_FP64 &&
+                       precision <= GAIN_PREC_I8CRT14,
                    "unknown precision %d", precision);
     DeviceGuard g(c->device);
     Plan P = make_plan(c, jindI, gammaI, levelsI, nI, jindF, gammaF);
@@ -1710,6 +1875,10 @@ extern "C" int gain_set_tuning(gain_ctx* c, const char* key, int value) {
       GAIN_CHECK_ARG(value >= -1 && value < kI8Configs, "cfg -1..%d",
                      kI8Configs - 1);
       c->force_i8_cfg = value;
+    } else if (k == "crt_cfg") {
+      GAIN_CHECK_ARG(value >= -1 && value < kCrtConfigs, "cfg -1..%d",
+                     kCrtConfigs - 1);
+      c->force_crt_cfg = value;
     } else if (k == "i8_split") {
       GAIN_CHECK_ARG(value >= 0 && value <= 4, "split 0..4");
       c->force_i8_split = value;

@@ -142,8 +142,10 @@ struct Basis {
 // planes with chunk exponents, or tiled split-INT8 planes (`i8`) with one
 // exponent per row.
 struct Sliced {
-  int S = 0;
+  int S = 0;           // slices (split forms) or moduli (CRT form)
   bool i8 = false;
+  bool crt = false;    // int8 residues (CRT form), `beta` integer bits
+  int beta = 0;
   int64_t Kp = 0;
   int64_t Rp = 0;      // padded rows of one (z) matrix (int8)
   int64_t plane = 0;   // bytes between int8 slice planes
@@ -202,6 +204,7 @@ struct gain_ctx {
   int force_cfg[2] = {-1, -1};  // 0: projection, 1: half line strength
   int force_split = 1;          // projection split-K when not tuned
   int force_i8_cfg = -1, force_i8_split = 0;  // split-INT8 GEMMs
+  int force_crt_cfg = -1;                     // CRT-form int8 GEMMs
   std::map<std::tuple<int, int, int, int>, int> tuned;
   std::string tune_log;
   // what the most recent launch of each tuned kernel used (tuning getter)
@@ -294,6 +297,8 @@ Block& block_of(gain_ctx* c, int jind, int gamma);
 void ensure_slots(gain_ctx* c, Basis& b);
 int slices_of(int precision);
 int i8_slices_of(int precision);
+int crt_moduli_of(int precision);
+int crt_cfg_of(gain_ctx* c, int kind, I8GemmArgs g);
 int i8_cfg_split(gain_ctx* c, int kind, I8GemmArgs g, int max_split,
                  int* ksplit);
 bool degfilter(const gain_ctx* c, int gI, int gF, int dI, int dF);

@@ -112,6 +112,7 @@ struct I8GemmArgs {
   int nz = 1;
   int ksplit = 1;
   int cfg = 0;
+  int beta = 0;      // CRT form: integer bits per operand element
 };
 constexpr int kI8Configs = 12;
 // whether configuration cfg supports S slices
@@ -119,5 +120,43 @@ bool i8_config_ok(int cfg, int S);
 const char* i8_config_name(int cfg);
 void gemm_sliced_i8(const I8GemmArgs& g, hipStream_t s);
 int64_t gemm_i8_grid_items(const I8GemmArgs& g);
+
+// ---------------------------------------------------------------------------
+// Split-INT8, CRT form (Ozaki scheme with integer moduli).
+//
+// Every FP64 row x is scaled by 2^{beta-1-e} (e the exponent of the row
+// maximum) and rounded to an integer X, |X| <= 2^{beta-1}; X is stored as
+// its residues modulo nm pairwise coprime odd moduli m_i <= 255, each in
+// [-(m_i-1)/2, (m_i-1)/2] (an int8).  For every modulus the int8 matrix
+// cores form the exact int32 product of the residue matrices (exact while
+// K * 127^2 < 2^31); it is reduced modulo m_i, and Garner's mixed-radix
+// conversion of the nm residues gives the exact integer product
+// sum_k X_A X_B, since beta is chosen so that 2 K 4^{beta-1} < prod m_i
+// (crt_beta).  The only error is the rounding of the operands to beta-bit
+// integers, 2^{-beta} of the row maxima per element; the cost is nm int8
+// products per FMA instead of S(S+1)/2 for the sliced form.
+constexpr int kCrtMaxModuli = 14;
+struct CrtConsts {
+  int nm;
+  int m[kCrtMaxModuli];
+  float minv[kCrtMaxModuli];   // 1 / m_i
+  double minv_d[kCrtMaxModuli];
+  float inv[kCrtMaxModuli];    // (m_0 ... m_{i-1})^{-1} mod m_i
+  float w[kCrtMaxModuli][kCrtMaxModuli];  // (m_0 ... m_{j-1}) mod m_i, j < i
+};
+const CrtConsts& crt_consts(int nm);
+// Largest beta with 2 K 4^{beta-1} < prod_{i<nm} m_i.
+int crt_beta(int nm, int64_t K);
+// rows x K FP64 -> nm tiled int8 residue planes (layout and padding as
+// slice_rows_i8) and one exponent per padded row.
+void slice_rows_crt(int nm, int beta, const double* src, int64_t ld,
+                    int rows, int K, int8_t* planes, int64_t plane,
+                    int* expo, hipStream_t s);
+constexpr int kCrtConfigs = 5;
+const char* crt_config_name(int cfg);
+// C[z][n*ldc + m] = 2^{eA + eB - 2(beta-1)} * (exact integer product of the
+// operands), g.S = number of moduli, g.beta as sliced, no split-K.
+void gemm_crt_i8(const I8GemmArgs& g, hipStream_t s);
+int64_t gemm_crt_grid_items(const I8GemmArgs& g);
 
 }  // namespace gain

@@ -11,6 +11,7 @@
 // k block ahead, so the kernel has no LDS and no barriers.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 
 #include "common.hpp"
@@ -487,6 +488,376 @@ void gemm_sliced_i8(const I8GemmArgs& g, hipStream_t s) {
            "supported", g.S, i8_config_name(g.cfg));
   }
   check_launch("gemm_sliced_i8");
+}
+
+// ---------------------------------------------------------------------------
+// CRT form.  Moduli: odd, pairwise coprime, <= 255.
+namespace {
+
+constexpr int kCrtModuli[kCrtMaxModuli] = {255, 253, 251, 247, 241, 239, 233,
+                                           229, 227, 223, 217, 211, 199, 197};
+
+int64_t inv_mod(int64_t a, int64_t m) {  // a^{-1} mod m, gcd(a, m) = 1
+  int64_t t = 0, nt = 1, r = m, nr = ((a % m) + m) % m;
+  while (nr != 0) {
+    const int64_t q = r / nr;
+    const int64_t tt = t - q * nt;
+    t = nt;
+    nt = tt;
+    const int64_t rr = r - q * nr;
+    r = nr;
+    nr = rr;
+  }
+  return ((t % m) + m) % m;
+}
+
+CrtConsts make_crt(int nm) {
+  CrtConsts c{};
+  c.nm = nm;
+  for (int i = 0; i < nm; ++i) {
+    const int64_t m = kCrtModuli[i];
+    c.m[i] = (int)m;
+    c.minv[i] = 1.0f / (float)m;
+    c.minv_d[i] = 1.0 / (double)m;
+    int64_t prod = 1;  // m_0 ... m_{j-1} mod m_i
+    for (int j = 0; j < i; ++j) {
+      c.w[i][j] = (float)prod;
+      prod = prod * kCrtModuli[j] % m;
+    }
+    c.inv[i] = i == 0 ? 1.0f : (float)inv_mod(prod, m);
+  }
+  return c;
+}
+
+}  // namespace
+
+const CrtConsts& crt_consts(int nm) {
+  static const CrtConsts c11 = make_crt(11), c12 = make_crt(12),
+                         c14 = make_crt(14);
+  if (nm == 11) return c11;
+  if (nm == 12) return c12;
+  if (nm == 14) return c14;
+  fail(GAIN_ERR_ARG, "CRT moduli count %d not supported (11, 12, 14)", nm);
+}
+
+int crt_beta(int nm, int64_t K) {
+  long double lp = 0.0L;
+  for (int i = 0; i < nm; ++i) lp += log2l((long double)kCrtModuli[i]);
+  // 2 K 4^{beta-1} < P  <=>  beta - 1 < (log2 P - 1 - log2 K) / 2
+  const long double lim = (lp - 1.0L - log2l((long double)K)) / 2.0L;
+  int b1 = (int)floorl(lim);
+  if ((long double)b1 >= lim) --b1;  // strict inequality
+  GAIN_CHECK_ARG(b1 >= 8 && b1 <= 52, "CRT: no integer width for K = %lld",
+                 (long long)K);
+  return b1 + 1;
+}
+
+// Residue of an integer-valued double |x| < 2^52 modulo m, in the symmetric
+// range [-(m-1)/2, (m-1)/2] (m odd).
+__device__ inline double sym_mod_d(double x, double m, double minv) {
+  double r = x - rint(x * minv) * m;  // exact; rint may be off by one
+  const double h = 0.5 * (m - 1.0);
+  if (r > h) r -= m;
+  if (r < -h) r += m;
+  return r;
+}
+
+// Same for an integer-valued float |x| < 2^24.
+__device__ inline float sym_mod_f(float x, float m, float minv) {
+  float r = fmaf(-rintf(x * minv), m, x);
+  const float h = 0.5f * (m - 1.0f);
+  if (r > h) r -= m;
+  if (r < -h) r += m;
+  return r;
+}
+
+template <int NM>
+__global__ __launch_bounds__(256) void k_slice_crt(
+    const double* __restrict__ src, int64_t ld, int rows, int K, int64_t Kp,
+    int64_t plane, int8_t* __restrict__ planes, int* __restrict__ expo,
+    int beta, CrtConsts cc) {
+  __shared__ double red[4];
+  const int r = blockIdx.x;
+  const int tid = threadIdx.x;
+  const bool live = r < rows;
+  const double* x = src + (live ? (int64_t)r * ld : 0);
+  double mx = 0.0;
+  if (live)
+    for (int k = tid; k < K; k += 256) mx = fmax(mx, fabs(x[k]));
+  for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+  if ((tid & 63) == 0) red[tid >> 6] = mx;
+  __syncthreads();
+  mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  int e = 0;
+  if (mx > 0.0) frexp(mx, &e);  // |x| < 2^e
+  if (tid == 0) expo[r] = e;
+  int8_t* rowbase = planes + (int64_t)(r >> 5) * (Kp << 5) + (r & 31) * 32;
+  for (int64_t k0 = 4 * tid; k0 < Kp; k0 += 1024) {
+    uint32_t w[NM];
+#pragma unroll
+    for (int i = 0; i < NM; ++i) w[i] = 0u;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int64_t k = k0 + q;
+      // |X| <= 2^{beta-1} <= 2^51: exact in double
+      const double X =
+          (live && k < K) ? rint(ldexp(x[k], beta - 1 - e)) : 0.0;
+#pragma unroll
+      for (int i = 0; i < NM; ++i) {
+        const double v = sym_mod_d(X, (double)cc.m[i], cc.minv_d[i]);
+        w[i] |= (uint32_t)(uint8_t)(int8_t)(int)v << (8 * q);
+      }
+    }
+    int8_t* dst = rowbase + (k0 >> 5) * 1024 + (k0 & 31);
+#pragma unroll
+    for (int i = 0; i < NM; ++i) *(uint32_t*)(dst + i * plane) = w[i];
+  }
+}
+
+void slice_rows_crt(int nm, int beta, const double* src, int64_t ld,
+                    int rows, int K, int8_t* planes, int64_t plane,
+                    int* expo, hipStream_t s) {
+  const int64_t Rp = i8_rows_pad(rows), Kp = i8_kpad(K);
+  GAIN_CHECK_ARG(rows >= 0 && K > 0 && plane >= Rp * Kp,
+                 "slice_rows_crt: plane of %lld bytes < %lld x %lld",
+                 (long long)plane, (long long)Rp, (long long)Kp);
+  GAIN_CHECK_ARG(((uintptr_t)planes & 3) == 0 && (plane & 3) == 0,
+                 "slice_rows_crt: planes must be 4-byte aligned");
+  GAIN_CHECK_ARG(beta >= 9 && beta <= 53, "slice_rows_crt: beta %d", beta);
+  const CrtConsts& cc = crt_consts(nm);
+  switch (nm) {
+#define GAIN_CRT_SLICE(NN)                                                   \
+  case NN:                                                                   \
+    hipLaunchKernelGGL(k_slice_crt<NN>, dim3((unsigned)Rp), dim3(256), 0, s, \
+                       src, ld, rows, K, Kp, plane, planes, expo, beta, cc); \
+    break;
+    GAIN_CRT_SLICE(11)
+    GAIN_CRT_SLICE(12)
+    GAIN_CRT_SLICE(14)
+#undef GAIN_CRT_SLICE
+    default:
+      fail(GAIN_ERR_ARG, "CRT moduli count %d not supported", nm);
+  }
+  check_launch("slice_rows_crt");
+}
+
+// Configurations: wave tile (32 TM) x (32 TN), WM x WN waves.
+// (Larger wave tiles put the nm residue registers of 64 x 64 outputs
+// beyond the register file.)
+constexpr I8Cfg kCrtCfg[kCrtConfigs] = {
+    {1, 2, 2, 2, 0},  // 64 x 128, 4 waves of 32 x 64
+    {2, 1, 2, 2, 0},  // 128 x 64, 4 waves of 64 x 32
+    {1, 1, 2, 2, 0},  // 64 x 64, 4 waves of 32 x 32
+    {1, 1, 4, 2, 0},  // 128 x 64, 8 waves of 32 x 32
+    {1, 1, 2, 4, 0},  // 64 x 128, 8 waves of 32 x 32
+};
+
+const char* crt_config_name(int cfg) {
+  static const char* names[kCrtConfigs] = {
+      "64x128/w32x64", "128x64/w64x32", "64x64/w32x32", "128x64/8w",
+      "64x128/8w"};
+  return cfg >= 0 && cfg < kCrtConfigs ? names[cfg] : "?";
+}
+
+// One workgroup owns an output tile and runs the K loop once per modulus
+// (modulus-major flattened loop, one k block of register prefetch).  After
+// the last k block of a modulus the int32 sums are reduced modulo m_i and
+// pushed, packed as int8, into a register queue of nm entries; the
+// epilogue converts the nm residues of every output to the integer product
+// (Garner: digit i from the digits below it by a dot product with the
+// (m_0...m_{j-1}) mod m_i weights, all exact small integers in FP32) and
+// evaluates the mixed-radix number in FP64.
+template <int NM, int TM, int TN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void k_gemm_crt(
+    I8GemmArgs g, CrtConsts cc, int ntm, int ntn, int rb0, int nkb) {
+  constexpr int BMB = TM * WM, BNB = TN * WN;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  int z, ksi, tm, tn;
+  i8_tile(ntm, ntn, 1, &z, &ksi, &tm, &tn);
+  const int arb = rb0 + tm * BMB + wm * TM;
+  const int brb = tn * BNB + wn * TN;
+  const int lane_off = (lane & 31) * 32 + 16 * (lane >> 5);
+  const int8_t* Ab = g.A + z * g.a_z + (int64_t)arb * nkb * 1024 + lane_off;
+  const int8_t* Bb = g.B + (int64_t)brb * nkb * 1024 + lane_off;
+
+  i16v acc[TM][TN];
+  uint32_t res[NM][TM][TN][4];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b) {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[a][b][e] = 0;
+#pragma unroll
+      for (int i = 0; i < NM; ++i)
+#pragma unroll
+        for (int w = 0; w < 4; ++w) res[i][a][b][w] = 0u;
+    }
+
+  i4v fa0[TM], fb0[TN], fa1[TM], fb1[TN];
+  auto load = [&](i4v(&fa)[TM], i4v(&fb)[TN], int q) {
+    const int i = q / nkb, kb = q - i * nkb;
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+      fa[a] = *(const i4v*)(Ab + i * g.a_plane +
+                            ((int64_t)a * nkb + kb) * 1024);
+#pragma unroll
+    for (int b = 0; b < TN; ++b)
+      fb[b] = *(const i4v*)(Bb + i * g.b_plane +
+                            ((int64_t)b * nkb + kb) * 1024);
+  };
+  auto mma = [&](const i4v(&fa)[TM], const i4v(&fb)[TN]) {
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+      for (int b = 0; b < TN; ++b)
+        acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[a], fb[b],
+                                                          acc[a][b], 0, 0, 0);
+  };
+  // |acc| <= K 127^2 < 2^31: exact in double
+  auto finish = [&](int i) {
+    const double m = (double)cc.m[i], mi = cc.minv_d[i];
+#pragma unroll
+    for (int j = 0; j < NM - 1; ++j)
+#pragma unroll
+      for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b)
+#pragma unroll
+          for (int w = 0; w < 4; ++w) res[j][a][b][w] = res[j + 1][a][b][w];
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+      for (int b = 0; b < TN; ++b)
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+          uint32_t word = 0u;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const double r = sym_mod_d((double)acc[a][b][4 * w + q], m, mi);
+            word |= (uint32_t)(uint8_t)(int8_t)(int)r << (8 * q);
+            acc[a][b][4 * w + q] = 0;
+          }
+          res[NM - 1][a][b][w] = word;
+        }
+  };
+
+  const int total = NM * nkb;
+  load(fa0, fb0, 0);
+  for (int q = 0; q < total; q += 2) {
+    if (q + 1 < total) load(fa1, fb1, q + 1);
+    mma(fa0, fb0);
+    if ((q + 1) % nkb == 0) finish((q + 1) / nkb - 1);
+    if (q + 1 < total) {
+      if (q + 2 < total) load(fa0, fb0, q + 2);
+      mma(fa1, fb1);
+      if ((q + 2) % nkb == 0) finish((q + 2) / nkb - 1);
+    }
+  }
+
+  double* C = g.C + z * g.c_z;
+  const int* eA = g.eA + z * g.ea_z;
+  const int sc = -2 * (g.beta - 1);
+#pragma unroll
+  for (int b = 0; b < TN; ++b) {
+    const int n = (brb + b) * 32 + (lane & 31);
+    if (n >= g.N) continue;
+    const int eb = g.eB[n] + sc;
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int ar = (arb + a) * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+        const int m = ar - g.row0;
+        if (m < 0 || m >= g.M) continue;
+        float v[NM];
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {
+          const float r = (float)(int8_t)(res[i][a][b][e >> 2] >> (8 * (e & 3)));
+          if (i == 0) {
+            v[0] = r;
+          } else {
+            float s = 0.0f;  // |s| < 14 * 127 * 255 < 2^24: exact
+#pragma unroll
+            for (int j = 0; j < i; ++j) s = fmaf(v[j], cc.w[i][j], s);
+            const float mf = (float)cc.m[i];
+            s = sym_mod_f(s, mf, cc.minv[i]);
+            v[i] = sym_mod_f((r - s) * cc.inv[i], mf, cc.minv[i]);
+          }
+        }
+        double X = (double)v[NM - 1];
+#pragma unroll
+        for (int i = NM - 2; i >= 0; --i)
+          X = fma(X, (double)cc.m[i], (double)v[i]);
+        C[(int64_t)n * g.ldc + m] = ldexp(X, eA[ar] + eb);
+      }
+  }
+}
+
+namespace {
+
+I8Launch crt_launch(const I8GemmArgs& g) {
+  const I8Cfg& cf = kCrtCfg[g.cfg];
+  const int bm = 32 * cf.tm * cf.wm, bn = 32 * cf.tn * cf.wn;
+  I8Launch L;
+  L.rb0 = g.row0 / 32;
+  const int rows = g.row0 + g.M - L.rb0 * 32;
+  L.ntm = (rows + bm - 1) / bm;
+  L.ntn = (g.N + bn - 1) / bn;
+  L.kb_per = (int)(g.Kp / 32);
+  L.grid = (int64_t)L.ntm * L.ntn * g.nz;
+  return L;
+}
+
+}  // namespace
+
+int64_t gemm_crt_grid_items(const I8GemmArgs& g) {
+  return crt_launch(g).grid * 64 * kCrtCfg[g.cfg].wm * kCrtCfg[g.cfg].wn;
+}
+
+void gemm_crt_i8(const I8GemmArgs& g, hipStream_t s) {
+  if (g.M <= 0 || g.N <= 0) return;
+  GAIN_CHECK_ARG(g.cfg >= 0 && g.cfg < kCrtConfigs, "CRT gemm: config %d",
+                 g.cfg);
+  GAIN_CHECK_ARG(g.Kp > 0 && g.Kp % 32 == 0 && g.ksplit == 1 && g.nz >= 1 &&
+                     g.row0 >= 0,
+                 "CRT gemm: Kp %lld, split %d", (long long)g.Kp, g.ksplit);
+  // exact int32 sums per modulus, and an integer width the moduli cover
+  GAIN_CHECK_ARG(g.Kp * 127 * 127 < (1ll << 31),
+                 "CRT gemm: K = %lld beyond the exact int32 range",
+                 (long long)g.Kp);
+  GAIN_CHECK_ARG(g.beta >= 9 && g.beta <= crt_beta(g.S, g.Kp),
+                 "CRT gemm: beta %d too wide for K = %lld and %d moduli",
+                 g.beta, (long long)g.Kp, g.S);
+  const I8Launch L = crt_launch(g);
+  GAIN_CHECK_ARG(L.grid < (1ll << 31), "CRT gemm: grid too large");
+  const CrtConsts& cc = crt_consts(g.S);
+  const dim3 grid((unsigned)L.grid),
+      block(64 * kCrtCfg[g.cfg].wm * kCrtCfg[g.cfg].wn);
+  switch (g.S * 100 + g.cfg) {
+#define GAIN_CRT_CASE(NN, CC)                                                \
+  case NN * 100 + CC:                                                        \
+    hipLaunchKernelGGL((k_gemm_crt<NN, kCrtCfg[CC].tm, kCrtCfg[CC].tn,       \
+                                   kCrtCfg[CC].wm, kCrtCfg[CC].wn>),         \
+                       grid, block, 0, s, g, cc, L.ntm, L.ntn, L.rb0,        \
+                       L.kb_per);                                            \
+    break;
+#define GAIN_CRT_N(NN) \
+  GAIN_CRT_CASE(NN, 0) \
+  GAIN_CRT_CASE(NN, 1) \
+  GAIN_CRT_CASE(NN, 2) \
+  GAIN_CRT_CASE(NN, 3) \
+  GAIN_CRT_CASE(NN, 4)
+    GAIN_CRT_N(11)
+    GAIN_CRT_N(12)
+    GAIN_CRT_N(14)
+#undef GAIN_CRT_N
+#undef GAIN_CRT_CASE
+    default:
+      fail(GAIN_ERR_ARG, "CRT gemm: %d moduli not supported", g.S);
+  }
+  check_launch("gemm_crt_i8");
 }
 
 }  // namespace gain

@@ -85,7 +85,7 @@ extern "C" int gain_write_transitions(gain_ctx* c,
     if (opts) o = *opts;
     const int B = o.batch > 0 ? o.batch : 64;
     GAIN_CHECK_ARG(o.precision >= GAIN_PREC_FP64 &&
-                       o.precision <= GAIN_PREC_I8X7,
+                       o.precision <= GAIN_PREC_I8CRT14,
                    "unknown precision %d", o.precision);
     const gain_params& p = c->params;
     const int md = c->maxdeg;

@@ -15,6 +15,7 @@ MAX_IRREPS = 16
 OK, ERR_ARG, ERR_HIP, ERR_STATE, ERR_NOMEM, ERR_CONFIG = range(6)
 PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_BF16X4 = 0, 1, 2, 3
 PREC_I8X5, PREC_I8X6, PREC_I8X7 = 4, 5, 6
+PREC_I8CRT11, PREC_I8CRT12, PREC_I8CRT14 = 7, 8, 9
 
 # Every symbol include/gain_engine.h declares.
 EXPORTS = (

@@ -83,7 +83,12 @@ typedef enum {
   GAIN_PREC_I8X5 = 4,       /* split-INT8, 5 slices (35 bits of the row max),
                                exact int32 products on the integer MFMA */
   GAIN_PREC_I8X6 = 5,       /* split-INT8, 6 slices (42 bits) */
-  GAIN_PREC_I8X7 = 6        /* split-INT8, 7 slices (49 bits) */
+  GAIN_PREC_I8X7 = 6,       /* split-INT8, 7 slices (49 bits) */
+  GAIN_PREC_I8CRT11 = 7,    /* int8 residues modulo 11 coprime moduli
+                               (operands rounded to ~36 bits), exact
+                               integer products by CRT reconstruction */
+  GAIN_PREC_I8CRT12 = 8,    /* 12 moduli (~40 bits) */
+  GAIN_PREC_I8CRT14 = 9     /* 14 moduli (~47 bits) */
 } gain_precision;
 
 const char* gain_last_error(void);
@@ -135,7 +140,9 @@ int gain_upload_block(gain_ctx* ctx, int jind, int gamma, int nstates,
                       int rec_len, const double* energies,
                       const double* vectors);
 /* Same, but the records already live in device memory (row stride `ld`
- * doubles); the engine borrows the pointer. */
+ * doubles); the engine borrows the pointer.  The split precision modes keep
+ * a sliced copy of a block's records, built on first use: after changing
+ * the records in place, bind the block again (which drops that copy). */
 int gain_bind_block_device(gain_ctx* ctx, int jind, int gamma, int nstates,
                            int rec_len, int64_t ld, const double* energies,
                            const double* vectors_device);
@@ -228,7 +235,8 @@ int gain_get_timing(gain_ctx* ctx, const char* name, double* total_ms,
  * of each shape and keep the fastest; 2: forget earlier choices),
  * "proj_cfg" / "half_ls_cfg" (force a configuration, -1 = automatic),
  * "i8_cfg" / "i8_split" (split-INT8 GEMMs: force a tile configuration
- * 0..5, -1 = automatic; projection split count, 0 = automatic),
+ * 0..11, -1 = automatic; projection split count, 0 = automatic),
+ * "crt_cfg" (CRT-form int8 GEMMs: force a tile configuration 0..4),
  * "gemv_rows" (2/4/8), "gemv_ksplit", "mu_ksplit" (0 = automatic),
  * "mfma" (0: vector-ALU GEMMs, for cross-checks). */
 int gain_set_tuning(gain_ctx* ctx, const char* key, int value);

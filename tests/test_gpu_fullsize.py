@@ -121,7 +121,8 @@ def test_fullsize_batched_fp64_matches_plain(wl):
 # bounds pin the measured behaviour with a margin.
 @pytest.mark.parametrize("precision,max_bound,median_bound", [
     (1, 1e-3, 1e-6), (2, 2e-3, 5e-6), (3, 3e-6, 1e-8),
-    (4, 1e-4, 1e-7), (5, 1e-6, 1e-9), (6, 1e-8, 1e-11)])
+    (4, 1e-4, 1e-7), (5, 1e-6, 1e-9), (6, 1e-8, 1e-11),
+    (7, 1e-5, 1e-8), (8, 1e-6, 1e-9), (9, 1e-8, 1e-11)])
 def test_fullsize_reduced_precision_error(wl, precision, max_bound,
                                           median_bound):
     eng = wl.eng

@@ -55,6 +55,53 @@ def test_plain_every_direction(scheme, jvals, nc):
     compare_lines(got, ref)
 
 
+@pytest.mark.parametrize("scheme,jvals,nc", [("c2v4", [2, 3], 16),
+                                             ("c1", [1, 2], 10),
+                                             ("pair2", [3, 4], 8),
+                                             ("c3v3", [1, 2], 12)])
+@pytest.mark.parametrize("precision,bound", [(5, 5e-8), (8, 1e-7)])
+def test_batched_int8_every_direction(scheme, jvals, nc, precision, bound):
+    """Batched split-INT8 x6 and CRT-form (12 moduli) int8 over all state
+    pairs of a two-J problem (dJ = -1, 0, +1, same-block pairs, degenerate
+    irreps for c3v3): the transition list and the emitted lines equal the
+    oracle's, S within `bound` on strong lines."""
+    prob = synth.make_problem(jvals, nc, scheme)
+    prob.params.erange_lower = (0.0, 1e9)
+    rows = prob.state_table()
+    initials = list(range(0, len(rows), max(1, len(rows) // 40)))
+    tr, _ = orc.run(prob, initials)
+    ref = oracle_map(tr)
+    with Engine.from_problem(prob) as eng:
+        got = engine_lines(eng, prob, initials, "batched", precision)
+    assert len(ref) > 0 and set(got) == set(ref)
+    assert {k for k in got if got[k]["emitted"]} == \
+        {k for k in ref if ref[k]["emitted"]}
+    es, _, ns, _ = _strong_weak_err(got, ref)
+    print(f"{scheme} precision {precision}: {ns} strong lines, max rel err "
+          f"{es:.3e}")
+    assert es <= bound
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_batched_split_int8_irregular_basis(seed):
+    """Split-INT8 x6 on random K-block sizes, tau per primitive and
+    repeated vib indices (uneven X columns and sliced rows)."""
+    rng = np.random.default_rng(seed)
+    nc = 7
+    bases = [irregular_basis(2, nc, rng, 12), irregular_basis(3, nc, rng, 12)]
+    prob = problem_with_bases(bases, nc, seed)
+    prob.params.erange_lower = (0.0, 1e9)
+    rows = prob.state_table()
+    initials = list(range(0, len(rows), 3))
+    tr, _ = orc.run(prob, initials)
+    ref = oracle_map(tr)
+    with Engine.from_problem(prob) as eng:
+        got = engine_lines(eng, prob, initials, "batched", 5)
+    assert set(got) == set(ref)
+    es, _, ns, _ = _strong_weak_err(got, ref)
+    assert ns > 0 and es <= 5e-8, es
+
+
 @pytest.mark.parametrize("seed", [1, 2, 3])
 def test_plain_irregular_basis(seed):
     """Random K-block sizes, tau per primitive and repeated vib indices."""
@@ -269,6 +316,54 @@ def test_batched_split_int8_error_vs_fp64_oracle(precision, bound):
     print(f"precision {precision}: strong lines {ns} max rel err {es:.3e}; "
           f"weak lines {nw} max rel err {ew:.3e}")
     assert es <= bound
+
+
+@pytest.mark.parametrize("precision,bound", [(9, 1e-9), (8, 1e-7),
+                                             (7, 1e-6)])
+def test_batched_crt_error_vs_fp64_oracle(precision, bound):
+    """CRT-form int8 contractions (14, 12 and 11 moduli: exact integer
+    products of the operands rounded to ~47, ~40 and ~36 bits): transition
+    list identical, per-line S error against the FP64 oracle within
+    `bound` on strong lines."""
+    prob = synth.make_problem([5, 6], 32, "c2v4")
+    prob.params.erange_lower = (0.0, 1e9)
+    rows = prob.state_table()
+    initials = list(range(0, len(rows), 2))
+    tr, _ = orc.run(prob, initials)
+    ref = oracle_map(tr)
+    with Engine.from_problem(prob) as eng:
+        got = engine_lines(eng, prob, initials, "batched", precision)
+    assert set(got) == set(ref)
+    assert {k for k in got if got[k]["emitted"]} == \
+        {k for k in ref if ref[k]["emitted"]}
+    es, ew, ns, nw = _strong_weak_err(got, ref)
+    print(f"precision {precision}: strong lines {ns} max rel err {es:.3e}; "
+          f"weak lines {nw} max rel err {ew:.3e}")
+    assert es <= bound
+
+
+@pytest.mark.parametrize("cfg", list(range(5)))
+def test_every_crt_tile_configuration(cfg):
+    """Each CRT-form tile shape for both GEMM kinds on sizes with partial
+    tiles; a frequency window starts the projected final rows off a 32-row
+    block boundary.  Twelve moduli keep S within 1e-7 of the FP64 oracle
+    on strong lines."""
+    prob = synth.make_problem([4, 5], 20, "c2v4")
+    prob.params.erange_lower = (0.0, 1e9)
+    prob.params.freq_window = (1500.0, 1e9)
+    rows = prob.state_table()
+    initials = list(range(1, len(rows), 2))
+    tr, _ = orc.run(prob, initials)
+    ref = oracle_map(tr)
+    with Engine.from_problem(prob) as eng:
+        eng.set_tuning("crt_cfg", cfg)
+        got = engine_lines(eng, prob, initials, "batched", 8)
+        assert eng.get_tuning("proj_cfg") == cfg
+    assert set(got) == set(ref)
+    es, _, ns, _ = _strong_weak_err(got, ref)
+    print(f"crt12 configuration {cfg}: {ns} strong lines, max rel err "
+          f"{es:.3e}")
+    assert ns > 0 and es <= 1e-7, es
 
 
 @pytest.mark.parametrize("cfg", list(range(12)))
