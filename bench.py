@@ -10,11 +10,12 @@ The 2000 lowest initial states go against every final state with a
 
 A step is one batch of B initial states through the batched engine (half
 line strengths and projection as GEMMs, S/A/I epilogue).  The headline
-contractions run as split-INT8 with 6 slices (exact int32 products on the
-integer matrix cores, FP64 combination; the fastest mode whose S stays
-within 1e-6 of FP64 on strong lines, checked on the first batch); batched
-FP64 is timed as a variant and the plain FP64 GEMV drop-in (one initial
-state per call) beside both.  Kernel times
+contractions run in the CRT form on the int8 matrix cores: operands rounded
+to ~40-bit integers, their exact products formed modulo 12 coprime moduli
+(one int8 GEMM each) and reconstructed in FP64 (S within 1.1e-8 of FP64 on
+strong lines, checked on the first batch).  Batched FP64, split-INT8 x6 and
+the 11-moduli CRT form are timed as variants, and the plain FP64 GEMV
+drop-in (one initial state per call) beside them.  Kernel times
 come from HIP events the engine records on its own stream around every
 launch.  Prints one JSON line.
 """
@@ -205,11 +206,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=250)
     ap.add_argument("--plain-states", type=int, default=100)
-    ap.add_argument("--precision", type=int, default=5,
-                    help="contraction precision of the headline line (5: "
-                         "split-INT8 x6, the fastest mode within 1e-6 of "
-                         "FP64 on strong lines; 0: FP64)")
-    ap.add_argument("--variants", type=str, default="0",
+    ap.add_argument("--precision", type=int, default=8,
+                    help="contraction precision of the headline line (8: "
+                         "int8 CRT with 12 moduli, within 1.1e-8 of FP64 on "
+                         "strong lines; 5: split-INT8 x6; 0: FP64)")
+    ap.add_argument("--variants", type=str, default="0,5,7",
                     help="comma-separated extra precisions to time")
     ap.add_argument("--nc", type=int, default=4000)
     ap.add_argument("--ninitial", type=int, default=2000)

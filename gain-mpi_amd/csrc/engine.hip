@@ -269,6 +269,16 @@ const Sliced& block_sliced_crt(gain_ctx* c, Block& b, int nm) {
   return b.sl;
 }
 
+// Residue buffer of a CRT-form GEMM, large enough for every configuration.
+int8_t* crt_residues(gain_ctx* c, I8GemmArgs g) {
+  int64_t mx = 0;
+  for (int cfg = 0; cfg < kCrtConfigs; ++cfg) {
+    g.cfg = cfg;
+    mx = std::max(mx, gemm_crt_res_bytes(g));
+  }
+  return c->w_res.get<int8_t>((size_t)mx);
+}
+
 // Tile configuration of a CRT-form GEMM (kind 0: projection, 1: half line
 // strength), timed on the real operands once per shape when autotuning;
 // without tuning the 64 x 128 tile.
@@ -690,6 +700,7 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
     ks = 1;
     Y = c->w_Y.get<double>((size_t)3 * ncolTot * nc);
     g.C = Y;
+    g.R = crt_residues(c, g);
     g.cfg = crt_cfg_of(c, 1, g);
     gemm_crt_i8(g, c->stream);
     c->last["half_ls_cfg"] = g.cfg;
@@ -1083,6 +1094,7 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
         g.C = LS;
         g.ldc = nR;
         g.nz = 1;
+        g.R = crt_residues(c, g);
         g.cfg = crt_cfg_of(c, 0, g);
         gemm_crt_i8(g, c->stream);
         c->last["proj_cfg"] = g.cfg;

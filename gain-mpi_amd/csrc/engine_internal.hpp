@@ -194,7 +194,7 @@ struct gain_ctx {
   std::map<std::pair<int, int>, Block> blocks;
   DeviceArena arena;
   Work w_c, w_X, w_Y, w_H, w_G, w_part, w_small, w_out, w_tmp;
-  Work w_xs, w_xe, w_gs, w_ge;
+  Work w_xs, w_xe, w_gs, w_ge, w_res;
   PinnedRing ring;
   // tuning
   int gemv_rows = 4, gemv_ksplit = 8, mu_ksplit = 0;  // 0: automatic

@@ -113,6 +113,7 @@ struct I8GemmArgs {
   int ksplit = 1;
   int cfg = 0;
   int beta = 0;      // CRT form: integer bits per operand element
+  int8_t* R = nullptr;  // CRT form: residue buffer, gemm_crt_res_bytes(g)
 };
 constexpr int kI8Configs = 12;
 // whether configuration cfg supports S slices
@@ -152,11 +153,15 @@ int crt_beta(int nm, int64_t K);
 void slice_rows_crt(int nm, int beta, const double* src, int64_t ld,
                     int rows, int K, int8_t* planes, int64_t plane,
                     int* expo, hipStream_t s);
-constexpr int kCrtConfigs = 5;
+constexpr int kCrtConfigs = 8;
 const char* crt_config_name(int cfg);
 // C[z][n*ldc + m] = 2^{eA + eB - 2(beta-1)} * (exact integer product of the
-// operands), g.S = number of moduli, g.beta as sliced, no split-K.
+// operands), g.S = number of moduli, g.beta as sliced, no split-K: one
+// int8 GEMM per (operand matrix, modulus) writing int8 residues to g.R,
+// then the reconstruction kernel.
 void gemm_crt_i8(const I8GemmArgs& g, hipStream_t s);
+int64_t gemm_crt_res_bytes(const I8GemmArgs& g);
+// Work-items of the per-modulus GEMM launch.
 int64_t gemm_crt_grid_items(const I8GemmArgs& g);
 
 }  // namespace gain

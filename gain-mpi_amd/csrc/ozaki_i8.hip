@@ -94,7 +94,9 @@ void slice_rows_i8(int S, const double* src, int64_t ld, int rows, int K,
 
 // ---------------------------------------------------------------------------
 // Tile configurations: wave tile (32 TM) x (32 TN), WM x WN waves; `lds`:
-// operands staged through LDS (k_gemm_i8_lds), else read per wave.
+// operands staged through LDS (k_gemm_i8_lds / k_gemm_i8_res_lds; in the
+// CRT table its value is the number of k blocks per stage), else read per
+// wave.
 struct I8Cfg {
   int tm, tn, wm, wn, lds;
 };
@@ -641,71 +643,65 @@ void slice_rows_crt(int nm, int beta, const double* src, int64_t ld,
   check_launch("slice_rows_crt");
 }
 
-// Configurations: wave tile (32 TM) x (32 TN), WM x WN waves.
-// (Larger wave tiles put the nm residue registers of 64 x 64 outputs
-// beyond the register file.)
+// Configurations of the per-modulus GEMM: wave tile (32 TM) x (32 TN), WM x
+// WN waves.  One accumulator set per wave, so the wave tiles can be large
+// (each residue plane enters one product only: the tile size sets the
+// products per byte loaded).
 constexpr I8Cfg kCrtCfg[kCrtConfigs] = {
-    {1, 2, 2, 2, 0},  // 64 x 128, 4 waves of 32 x 64
-    {2, 1, 2, 2, 0},  // 128 x 64, 4 waves of 64 x 32
-    {1, 1, 2, 2, 0},  // 64 x 64, 4 waves of 32 x 32
-    {1, 1, 4, 2, 0},  // 128 x 64, 8 waves of 32 x 32
-    {1, 1, 2, 4, 0},  // 64 x 128, 8 waves of 32 x 32
+    {4, 4, 2, 2, 0},  // 256 x 256, 4 waves of 128 x 128
+    {2, 4, 2, 2, 0},  // 128 x 256, 4 waves of 64 x 128
+    {4, 2, 2, 2, 0},  // 256 x 128, 4 waves of 128 x 64
+    {2, 2, 2, 2, 0},  // 128 x 128, 4 waves of 64 x 64
+    {2, 2, 2, 4, 0},  // 128 x 256, 8 waves of 64 x 64
+    {4, 4, 2, 2, 1},  // 256 x 256 through LDS (4 stages)
+    {2, 4, 2, 2, 1},  // 128 x 256 through LDS
+    {2, 2, 2, 4, 1},  // 128 x 256, 8 waves, through LDS
 };
 
 const char* crt_config_name(int cfg) {
   static const char* names[kCrtConfigs] = {
-      "64x128/w32x64", "128x64/w64x32", "64x64/w32x32", "128x64/8w",
-      "64x128/8w"};
+      "256x256/w128x128", "128x256/w64x128", "256x128/w128x64",
+      "128x128/w64x64", "128x256/8w/w64x64", "256x256/w128x128/lds",
+      "128x256/w64x128/lds", "128x256/8w/lds"};
   return cfg >= 0 && cfg < kCrtConfigs ? names[cfg] : "?";
 }
 
-// One workgroup owns an output tile and runs the K loop once per modulus
-// (modulus-major flattened loop, one k block of register prefetch).  After
-// the last k block of a modulus the int32 sums are reduced modulo m_i and
-// pushed, packed as int8, into a register queue of nm entries; the
-// epilogue converts the nm residues of every output to the integer product
-// (Garner: digit i from the digits below it by a dot product with the
-// (m_0...m_{j-1}) mod m_i weights, all exact small integers in FP32) and
-// evaluates the mixed-radix number in FP64.
-template <int NM, int TM, int TN, int WM, int WN>
-__global__ __launch_bounds__(64 * WM * WN) void k_gemm_crt(
-    I8GemmArgs g, CrtConsts cc, int ntm, int ntn, int rb0, int nkb) {
+// Per-modulus GEMM: workgroup z in [0, nz * nm) takes modulus z % nm of
+// operand matrix z / nm; the int32 products of the residue planes are
+// reduced modulo m and stored as int8 residues R[z][n][r] (r = A row
+// relative to the first row block; 4 consecutive rows per 32-bit store).
+template <int TM, int TN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res(
+    I8GemmArgs g, CrtConsts cc, int ntm, int ntn, int rb0, int nkb,
+    int64_t ldr, int nr) {
   constexpr int BMB = TM * WM, BNB = TN * WN;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave % WM, wn = wave / WM;
   int z, ksi, tm, tn;
   i8_tile(ntm, ntn, 1, &z, &ksi, &tm, &tn);
+  const int za = z / g.S, mi = z - za * g.S;
   const int arb = rb0 + tm * BMB + wm * TM;
   const int brb = tn * BNB + wn * TN;
   const int lane_off = (lane & 31) * 32 + 16 * (lane >> 5);
-  const int8_t* Ab = g.A + z * g.a_z + (int64_t)arb * nkb * 1024 + lane_off;
-  const int8_t* Bb = g.B + (int64_t)brb * nkb * 1024 + lane_off;
-
+  const int8_t* Ab = g.A + za * g.a_z + mi * g.a_plane +
+                     (int64_t)arb * nkb * 1024 + lane_off;
+  const int8_t* Bb = g.B + mi * g.b_plane + (int64_t)brb * nkb * 1024 +
+                     lane_off;
   i16v acc[TM][TN];
-  uint32_t res[NM][TM][TN][4];
 #pragma unroll
   for (int a = 0; a < TM; ++a)
 #pragma unroll
-    for (int b = 0; b < TN; ++b) {
+    for (int b = 0; b < TN; ++b)
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[a][b][e] = 0;
-#pragma unroll
-      for (int i = 0; i < NM; ++i)
-#pragma unroll
-        for (int w = 0; w < 4; ++w) res[i][a][b][w] = 0u;
-    }
-
   i4v fa0[TM], fb0[TN], fa1[TM], fb1[TN];
-  auto load = [&](i4v(&fa)[TM], i4v(&fb)[TN], int q) {
-    const int i = q / nkb, kb = q - i * nkb;
+  auto load = [&](i4v(&fa)[TM], i4v(&fb)[TN], int kb) {
 #pragma unroll
     for (int a = 0; a < TM; ++a)
-      fa[a] = *(const i4v*)(Ab + i * g.a_plane +
-                            ((int64_t)a * nkb + kb) * 1024);
+      fa[a] = *(const i4v*)(Ab + ((int64_t)a * nkb + kb) * 1024);
 #pragma unroll
     for (int b = 0; b < TN; ++b)
-      fb[b] = *(const i4v*)(Bb + i * g.b_plane +
-                            ((int64_t)b * nkb + kb) * 1024);
+      fb[b] = *(const i4v*)(Bb + ((int64_t)b * nkb + kb) * 1024);
   };
   auto mma = [&](const i4v(&fa)[TM], const i4v(&fb)[TN]) {
 #pragma unroll
@@ -715,102 +711,224 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_crt(
         acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[a], fb[b],
                                                           acc[a][b], 0, 0, 0);
   };
-  // |acc| <= K 127^2 < 2^31: exact in double
-  auto finish = [&](int i) {
-    const double m = (double)cc.m[i], mi = cc.minv_d[i];
-#pragma unroll
-    for (int j = 0; j < NM - 1; ++j)
-#pragma unroll
-      for (int a = 0; a < TM; ++a)
-#pragma unroll
-        for (int b = 0; b < TN; ++b)
-#pragma unroll
-          for (int w = 0; w < 4; ++w) res[j][a][b][w] = res[j + 1][a][b][w];
-#pragma unroll
-    for (int a = 0; a < TM; ++a)
-#pragma unroll
-      for (int b = 0; b < TN; ++b)
-#pragma unroll
-        for (int w = 0; w < 4; ++w) {
-          uint32_t word = 0u;
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const double r = sym_mod_d((double)acc[a][b][4 * w + q], m, mi);
-            word |= (uint32_t)(uint8_t)(int8_t)(int)r << (8 * q);
-            acc[a][b][4 * w + q] = 0;
-          }
-          res[NM - 1][a][b][w] = word;
-        }
-  };
-
-  const int total = NM * nkb;
   load(fa0, fb0, 0);
-  for (int q = 0; q < total; q += 2) {
-    if (q + 1 < total) load(fa1, fb1, q + 1);
+  int kb = 0;
+  for (; kb + 1 < nkb; kb += 2) {
+    load(fa1, fb1, kb + 1);
     mma(fa0, fb0);
-    if ((q + 1) % nkb == 0) finish((q + 1) / nkb - 1);
-    if (q + 1 < total) {
-      if (q + 2 < total) load(fa0, fb0, q + 2);
-      mma(fa1, fb1);
-      if ((q + 2) % nkb == 0) finish((q + 2) / nkb - 1);
-    }
+    if (kb + 2 < nkb) load(fa0, fb0, kb + 2);
+    mma(fa1, fb1);
   }
+  if (kb < nkb) mma(fa0, fb0);
 
-  double* C = g.C + z * g.c_z;
-  const int* eA = g.eA + z * g.ea_z;
-  const int sc = -2 * (g.beta - 1);
+  // |acc| <= K 127^2 < 2^31: exact in double
+  const double m = (double)cc.m[mi], minv = cc.minv_d[mi];
+  int8_t* R = g.R + (int64_t)z * nr * ldr;
 #pragma unroll
   for (int b = 0; b < TN; ++b) {
     const int n = (brb + b) * 32 + (lane & 31);
     if (n >= g.N) continue;
-    const int eb = g.eB[n] + sc;
 #pragma unroll
     for (int a = 0; a < TM; ++a)
 #pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        const int ar = (arb + a) * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-        const int m = ar - g.row0;
-        if (m < 0 || m >= g.M) continue;
-        float v[NM];
+      for (int w = 0; w < 4; ++w) {
+        uint32_t word = 0u;
 #pragma unroll
-        for (int i = 0; i < NM; ++i) {
-          const float r = (float)(int8_t)(res[i][a][b][e >> 2] >> (8 * (e & 3)));
-          if (i == 0) {
-            v[0] = r;
-          } else {
-            float s = 0.0f;  // |s| < 14 * 127 * 255 < 2^24: exact
-#pragma unroll
-            for (int j = 0; j < i; ++j) s = fmaf(v[j], cc.w[i][j], s);
-            const float mf = (float)cc.m[i];
-            s = sym_mod_f(s, mf, cc.minv[i]);
-            v[i] = sym_mod_f((r - s) * cc.inv[i], mf, cc.minv[i]);
-          }
+        for (int q = 0; q < 4; ++q) {
+          const double r = sym_mod_d((double)acc[a][b][4 * w + q], m, minv);
+          word |= (uint32_t)(uint8_t)(int8_t)(int)r << (8 * q);
         }
-        double X = (double)v[NM - 1];
-#pragma unroll
-        for (int i = NM - 2; i >= 0; --i)
-          X = fma(X, (double)cc.m[i], (double)v[i]);
-        C[(int64_t)n * g.ldc + m] = ldexp(X, eA[ar] + eb);
+        const int rr = (arb + a - rb0) * 32 + 8 * w + 4 * (lane >> 5);
+        *(uint32_t*)(R + (int64_t)n * ldr + rr) = word;
       }
   }
 }
 
+// Same product through LDS: NST k-block stages of the A and B blocks of the
+// workgroup tile, filled by direct global -> LDS loads (each wave-wide
+// global_load_lds_dwordx4 copies one 1 KB block; lane l fetches chunk
+// sigma(l) so the fragment reads are bank-conflict-free), stage kb + NST - 1
+// requested before the products of stage kb, one barrier per k block.  The
+// deep prefetch costs no registers, which matters here: a k block carries
+// only TM x TN products per wave.
+template <int TM, int TN, int WM, int WN, int NST, int KS>
+__global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res_lds(
+    I8GemmArgs g, CrtConsts cc, int ntm, int ntn, int rb0, int nkb,
+    int64_t ldr, int nr) {
+  // KS k blocks per stage (one barrier per stage; the configurations use
+  // KS = 1: with KS = 2 the 256 x 256 tile spills and the others gain
+  // nothing); k blocks past the end of K in the last stage are skipped
+  constexpr int NW = WM * WN;
+  constexpr int BMB = TM * WM, BNB = TN * WN;
+  constexpr int NBLK = KS * (BMB + BNB);
+  constexpr int NBLKP = (NBLK + NW - 1) / NW * NW;
+  constexpr int DPW = NBLKP / NW;
+  static_assert(NST >= 2 && NST <= 4, "stage count");
+  __shared__ __attribute__((aligned(1024))) int8_t lds[NST * NBLKP * 1024];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wm = wave % WM, wn = wave / WM;
+  int z, ksi, tm, tn;
+  i8_tile(ntm, ntn, 1, &z, &ksi, &tm, &tn);
+  const int za = z / g.S, mi = z - za * g.S;
+  const int arb0 = rb0 + tm * BMB, brb0 = tn * BNB;
+  const int src_off = 16 * i8_sigma(lane);
+  const int8_t* Az = g.A + za * g.a_z + mi * g.a_plane;
+  const int8_t* Bz = g.B + mi * g.b_plane;
+  // stage st holds k blocks KS st .. KS st + KS - 1; slot q = KS-group
+  // (q / (BMB + BNB)) then A blocks, B blocks.  k blocks past the end of K
+  // repeat the last one (loaded, never multiplied).
+  const int nst = (nkb + KS - 1) / KS;
+  auto issue = [&](int stage, int st) {
+    int8_t* dst = lds + stage * (NBLKP * 1024);
+#pragma unroll
+    for (int t = 0; t < DPW; ++t) {
+      const int q = wave + NW * t;
+      const int qq = q < NBLK ? q : 0;  // padding slots repeat block 0
+      const int kq = qq / (BMB + BNB), rq = qq % (BMB + BNB);
+      const int kb = min(KS * st + kq, nkb - 1);
+      const int8_t* src =
+          rq < BMB ? Az + ((int64_t)(arb0 + rq) * nkb + kb) * 1024
+                   : Bz + ((int64_t)(brb0 + rq - BMB) * nkb + kb) * 1024;
+      __builtin_amdgcn_global_load_lds(
+          (const void*)(src + src_off),
+          (__attribute__((address_space(3))) void*)(dst + q * 1024), 16, 0,
+          0);
+    }
+  };
+  i16v acc[TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[a][b][e] = 0;
+  const int rd_off = 16 * i8_sigma(2 * (lane & 31) + (lane >> 5));
+#pragma unroll
+  for (int st = 0; st < NST - 1; ++st)
+    if (st < nst) issue(st, st);
+  int stage = 0;
+  for (int st = 0; st < nst; ++st) {
+    const int later = min(NST - 2, nst - 1 - st);
+    if (later >= 2)
+      wait_vmcnt<(NST >= 4 ? 2 * DPW : 0)>();
+    else if (later == 1)
+      wait_vmcnt<(NST >= 3 ? DPW : 0)>();
+    else
+      wait_vmcnt<0>();
+    __builtin_amdgcn_s_barrier();
+    if (st + NST - 1 < nst)
+      issue(stage == 0 ? NST - 1 : stage - 1, st + NST - 1);
+#pragma unroll
+    for (int kq = 0; kq < KS; ++kq) {
+      if (KS * st + kq >= nkb) break;
+      const int8_t* base =
+          lds + stage * (NBLKP * 1024) + kq * (BMB + BNB) * 1024 + rd_off;
+      i4v fa[TM], fb[TN];
+#pragma unroll
+      for (int a = 0; a < TM; ++a)
+        fa[a] = *(const i4v*)(base + (TM * wm + a) * 1024);
+#pragma unroll
+      for (int b = 0; b < TN; ++b)
+        fb[b] = *(const i4v*)(base + (BMB + TN * wn + b) * 1024);
+#pragma unroll
+      for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+              fa[a], fb[b], acc[a][b], 0, 0, 0);
+    }
+    stage = stage == NST - 1 ? 0 : stage + 1;
+  }
+
+  const double m = (double)cc.m[mi], minv = cc.minv_d[mi];
+  int8_t* R = g.R + (int64_t)z * nr * ldr;
+  const int arb = arb0 + TM * wm, brb = brb0 + TN * wn;
+#pragma unroll
+  for (int b = 0; b < TN; ++b) {
+    const int n = (brb + b) * 32 + (lane & 31);
+    if (n >= g.N) continue;
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        uint32_t word = 0u;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const double r = sym_mod_d((double)acc[a][b][4 * w + q], m, minv);
+          word |= (uint32_t)(uint8_t)(int8_t)(int)r << (8 * q);
+        }
+        const int rr = (arb + a - rb0) * 32 + 8 * w + 4 * (lane >> 5);
+        *(uint32_t*)(R + (int64_t)n * ldr + rr) = word;
+      }
+  }
+}
+
+// Garner's mixed-radix conversion of the nm residues of every output: digit
+// i from the digits below it by a dot product with the (m_0...m_{j-1}) mod
+// m_i weights (exact small integers in FP32), then the mixed-radix number in
+// FP64, scaled by the operand exponents.
+template <int NM>
+__global__ __launch_bounds__(256) void k_crt_combine(I8GemmArgs g,
+                                                     CrtConsts cc, int rb0,
+                                                     int64_t ldr, int nr) {
+  const int m = blockIdx.x * 256 + threadIdx.x;
+  const int n = blockIdx.y;
+  const int za = blockIdx.z;
+  if (m >= g.M) return;
+  const int ar = g.row0 + m;
+  const int8_t* R = g.R + ((int64_t)za * NM * nr + n) * ldr + (ar - rb0 * 32);
+  const int64_t pst = (int64_t)nr * ldr;
+  float v[NM];
+#pragma unroll
+  for (int i = 0; i < NM; ++i) {
+    const float r = (float)R[i * pst];
+    if (i == 0) {
+      v[0] = r;
+    } else {
+      float s = 0.0f;  // |s| < 14 * 127 * 255 < 2^24: exact
+#pragma unroll
+      for (int j = 0; j < i; ++j) s = fmaf(v[j], cc.w[i][j], s);
+      const float mf = (float)cc.m[i];
+      s = sym_mod_f(s, mf, cc.minv[i]);
+      v[i] = sym_mod_f((r - s) * cc.inv[i], mf, cc.minv[i]);
+    }
+  }
+  double X = (double)v[NM - 1];
+#pragma unroll
+  for (int i = NM - 2; i >= 0; --i) X = fma(X, (double)cc.m[i], (double)v[i]);
+  const int e = g.eA[za * g.ea_z + ar] + g.eB[n] - 2 * (g.beta - 1);
+  g.C[za * g.c_z + (int64_t)n * g.ldc + m] = ldexp(X, e);
+}
+
 namespace {
 
-I8Launch crt_launch(const I8GemmArgs& g) {
+struct CrtLaunch {
+  int ntm, ntn, rb0, nkb, nr;
+  int64_t grid, ldr;
+};
+
+CrtLaunch crt_launch(const I8GemmArgs& g) {
   const I8Cfg& cf = kCrtCfg[g.cfg];
   const int bm = 32 * cf.tm * cf.wm, bn = 32 * cf.tn * cf.wn;
-  I8Launch L;
+  CrtLaunch L;
   L.rb0 = g.row0 / 32;
   const int rows = g.row0 + g.M - L.rb0 * 32;
   L.ntm = (rows + bm - 1) / bm;
   L.ntn = (g.N + bn - 1) / bn;
-  L.kb_per = (int)(g.Kp / 32);
-  L.grid = (int64_t)L.ntm * L.ntn * g.nz;
+  L.nkb = (int)(g.Kp / 32);
+  L.grid = (int64_t)L.ntm * L.ntn * g.nz * g.S;
+  L.ldr = (int64_t)L.ntm * bm;  // residue rows per column
+  L.nr = L.ntn * bn;            // residue columns per matrix
   return L;
 }
 
 }  // namespace
+
+int64_t gemm_crt_res_bytes(const I8GemmArgs& g) {
+  const CrtLaunch L = crt_launch(g);
+  return (int64_t)g.nz * g.S * L.nr * L.ldr;
+}
 
 int64_t gemm_crt_grid_items(const I8GemmArgs& g) {
   return crt_launch(g).grid * 64 * kCrtCfg[g.cfg].wm * kCrtCfg[g.cfg].wn;
@@ -821,8 +939,9 @@ void gemm_crt_i8(const I8GemmArgs& g, hipStream_t s) {
   GAIN_CHECK_ARG(g.cfg >= 0 && g.cfg < kCrtConfigs, "CRT gemm: config %d",
                  g.cfg);
   GAIN_CHECK_ARG(g.Kp > 0 && g.Kp % 32 == 0 && g.ksplit == 1 && g.nz >= 1 &&
-                     g.row0 >= 0,
-                 "CRT gemm: Kp %lld, split %d", (long long)g.Kp, g.ksplit);
+                     g.row0 >= 0 && g.R,
+                 "CRT gemm: Kp %lld, split %d, residue buffer %p",
+                 (long long)g.Kp, g.ksplit, (const void*)g.R);
   // exact int32 sums per modulus, and an integer width the moduli cover
   GAIN_CHECK_ARG(g.Kp * 127 * 127 < (1ll << 31),
                  "CRT gemm: K = %lld beyond the exact int32 range",
@@ -830,34 +949,55 @@ void gemm_crt_i8(const I8GemmArgs& g, hipStream_t s) {
   GAIN_CHECK_ARG(g.beta >= 9 && g.beta <= crt_beta(g.S, g.Kp),
                  "CRT gemm: beta %d too wide for K = %lld and %d moduli",
                  g.beta, (long long)g.Kp, g.S);
-  const I8Launch L = crt_launch(g);
+  const CrtLaunch L = crt_launch(g);
   GAIN_CHECK_ARG(L.grid < (1ll << 31), "CRT gemm: grid too large");
   const CrtConsts& cc = crt_consts(g.S);
   const dim3 grid((unsigned)L.grid),
       block(64 * kCrtCfg[g.cfg].wm * kCrtCfg[g.cfg].wn);
-  switch (g.S * 100 + g.cfg) {
-#define GAIN_CRT_CASE(NN, CC)                                                \
-  case NN * 100 + CC:                                                        \
-    hipLaunchKernelGGL((k_gemm_crt<NN, kCrtCfg[CC].tm, kCrtCfg[CC].tn,       \
-                                   kCrtCfg[CC].wm, kCrtCfg[CC].wn>),         \
-                       grid, block, 0, s, g, cc, L.ntm, L.ntn, L.rb0,        \
-                       L.kb_per);                                            \
+  switch (g.cfg) {
+#define GAIN_CRT_CASE(CC)                                                    \
+  case CC:                                                                   \
+    hipLaunchKernelGGL((k_gemm_i8_res<kCrtCfg[CC].tm, kCrtCfg[CC].tn,        \
+                                      kCrtCfg[CC].wm, kCrtCfg[CC].wn>),      \
+                       grid, block, 0, s, g, cc, L.ntm, L.ntn, L.rb0, L.nkb, \
+                       L.ldr, L.nr);                                         \
     break;
-#define GAIN_CRT_N(NN) \
-  GAIN_CRT_CASE(NN, 0) \
-  GAIN_CRT_CASE(NN, 1) \
-  GAIN_CRT_CASE(NN, 2) \
-  GAIN_CRT_CASE(NN, 3) \
-  GAIN_CRT_CASE(NN, 4)
-    GAIN_CRT_N(11)
-    GAIN_CRT_N(12)
-    GAIN_CRT_N(14)
-#undef GAIN_CRT_N
+    GAIN_CRT_CASE(0)
+    GAIN_CRT_CASE(1)
+    GAIN_CRT_CASE(2)
+    GAIN_CRT_CASE(3)
+    GAIN_CRT_CASE(4)
 #undef GAIN_CRT_CASE
+#define GAIN_CRT_LDS(CC)                                                     \
+  case CC:                                                                   \
+    hipLaunchKernelGGL((k_gemm_i8_res_lds<kCrtCfg[CC].tm, kCrtCfg[CC].tn,    \
+                                          kCrtCfg[CC].wm, kCrtCfg[CC].wn, 4, \
+                                          kCrtCfg[CC].lds>),                 \
+                       grid, block, 0, s, g, cc, L.ntm, L.ntn, L.rb0, L.nkb, \
+                       L.ldr, L.nr);                                         \
+    break;
+    GAIN_CRT_LDS(5)
+    GAIN_CRT_LDS(6)
+    GAIN_CRT_LDS(7)
+#undef GAIN_CRT_LDS
+  }
+  check_launch("gemm_i8_res");
+  const dim3 cgrid((unsigned)((g.M + 255) / 256), (unsigned)g.N,
+                   (unsigned)g.nz);
+  switch (g.S) {
+#define GAIN_CRT_COMB(NN)                                                    \
+  case NN:                                                                   \
+    hipLaunchKernelGGL(k_crt_combine<NN>, cgrid, dim3(256), 0, s, g, cc,     \
+                       L.rb0, L.ldr, L.nr);                                  \
+    break;
+    GAIN_CRT_COMB(11)
+    GAIN_CRT_COMB(12)
+    GAIN_CRT_COMB(14)
+#undef GAIN_CRT_COMB
     default:
       fail(GAIN_ERR_ARG, "CRT gemm: %d moduli not supported", g.S);
   }
-  check_launch("gemm_crt_i8");
+  check_launch("crt_combine");
 }
 
 }  // namespace gain

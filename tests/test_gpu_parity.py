@@ -240,7 +240,7 @@ def test_error_paths_return_codes():
             eng.set_params(bad)
         assert e.value.code == _lib.ERR_ARG
         lv = np.zeros(1, np.int32)
-        rc = L.gain_batched_lines(eng._h, 0, 0, lv.ctypes.data, 1, 0, 0, 9,
+        rc = L.gain_batched_lines(eng._h, 0, 0, lv.ctypes.data, 1, 0, 0, 99,
                                   C.byref(out))
         assert rc == _lib.ERR_ARG and b"precision" in L.gain_last_error()
 
@@ -342,7 +342,7 @@ def test_batched_crt_error_vs_fp64_oracle(precision, bound):
     assert es <= bound
 
 
-@pytest.mark.parametrize("cfg", list(range(5)))
+@pytest.mark.parametrize("cfg", list(range(8)))
 def test_every_crt_tile_configuration(cfg):
     """Each CRT-form tile shape for both GEMM kinds on sizes with partial
     tiles; a frequency window starts the projected final rows off a 32-row
