@@ -867,38 +867,54 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res_lds(
 // Garner's mixed-radix conversion of the nm residues of every output: digit
 // i from the digits below it by a dot product with the (m_0...m_{j-1}) mod
 // m_i weights (exact small integers in FP32), then the mixed-radix number in
-// FP64, scaled by the operand exponents.
+// FP64, scaled by the operand exponents.  Four consecutive outputs per
+// thread (one 32-bit load per residue plane; residue rows are 4-aligned).
 template <int NM>
 __global__ __launch_bounds__(256) void k_crt_combine(I8GemmArgs g,
                                                      CrtConsts cc, int rb0,
                                                      int64_t ldr, int nr) {
-  const int m = blockIdx.x * 256 + threadIdx.x;
   const int n = blockIdx.y;
   const int za = blockIdx.z;
-  if (m >= g.M) return;
-  const int ar = g.row0 + m;
-  const int8_t* R = g.R + ((int64_t)za * NM * nr + n) * ldr + (ar - rb0 * 32);
-  const int64_t pst = (int64_t)nr * ldr;
-  float v[NM];
+  // residue row of this thread's first output, rounded down to 4
+  const int r0 = 4 * (blockIdx.x * 256 + threadIdx.x);
+  const int rlo = g.row0 - rb0 * 32;  // residue row of output 0
+  if (r0 >= rlo + g.M || r0 + 4 <= rlo) return;
+  const uint32_t* R = (const uint32_t*)(g.R + ((int64_t)za * NM * nr + n) *
+                                                  ldr + r0);
+  const int64_t pst = (int64_t)nr * ldr / 4;
+  uint32_t w[NM];
+#pragma unroll
+  for (int i = 0; i < NM; ++i) w[i] = R[i * pst];
+  float v[NM][4];
 #pragma unroll
   for (int i = 0; i < NM; ++i) {
-    const float r = (float)R[i * pst];
-    if (i == 0) {
-      v[0] = r;
-    } else {
-      float s = 0.0f;  // |s| < 14 * 127 * 255 < 2^24: exact
+    const float mf = (float)cc.m[i];
 #pragma unroll
-      for (int j = 0; j < i; ++j) s = fmaf(v[j], cc.w[i][j], s);
-      const float mf = (float)cc.m[i];
-      s = sym_mod_f(s, mf, cc.minv[i]);
-      v[i] = sym_mod_f((r - s) * cc.inv[i], mf, cc.minv[i]);
+    for (int q = 0; q < 4; ++q) {
+      const float r = (float)(int8_t)(w[i] >> (8 * q));
+      if (i == 0) {
+        v[0][q] = r;
+      } else {
+        float sum = 0.0f;  // |sum| < 14 * 127 * 255 < 2^24: exact
+#pragma unroll
+        for (int j = 0; j < i; ++j) sum = fmaf(v[j][q], cc.w[i][j], sum);
+        sum = sym_mod_f(sum, mf, cc.minv[i]);
+        v[i][q] = sym_mod_f((r - sum) * cc.inv[i], mf, cc.minv[i]);
+      }
     }
   }
-  double X = (double)v[NM - 1];
+  const int eb = g.eB[n] - 2 * (g.beta - 1);
+  double* C = g.C + za * g.c_z + (int64_t)n * g.ldc;
 #pragma unroll
-  for (int i = NM - 2; i >= 0; --i) X = fma(X, (double)cc.m[i], (double)v[i]);
-  const int e = g.eA[za * g.ea_z + ar] + g.eB[n] - 2 * (g.beta - 1);
-  g.C[za * g.c_z + (int64_t)n * g.ldc + m] = ldexp(X, e);
+  for (int q = 0; q < 4; ++q) {
+    const int m = r0 + q - rlo;
+    if (m < 0 || m >= g.M) continue;
+    double X = (double)v[NM - 1][q];
+#pragma unroll
+    for (int i = NM - 2; i >= 0; --i)
+      X = fma(X, (double)cc.m[i], (double)v[i][q]);
+    C[m] = ldexp(X, g.eA[za * g.ea_z + g.row0 + m] + eb);
+  }
 }
 
 namespace {
@@ -982,7 +998,9 @@ void gemm_crt_i8(const I8GemmArgs& g, hipStream_t s) {
 #undef GAIN_CRT_LDS
   }
   check_launch("gemm_i8_res");
-  const dim3 cgrid((unsigned)((g.M + 255) / 256), (unsigned)g.N,
+  // residue rows [row0 - 32 rb0, row0 - 32 rb0 + M), four per thread
+  const int rspan = g.row0 - L.rb0 * 32 + g.M;
+  const dim3 cgrid((unsigned)((rspan + 1023) / 1024), (unsigned)g.N,
                    (unsigned)g.nz);
   switch (g.S) {
 #define GAIN_CRT_COMB(NN)                                                    \
