@@ -109,6 +109,63 @@ __global__ __launch_bounds__(256) void k_half_ls_kblock(
   }
 }
 
+// Half line strength in the symmetric-top (ROTSYM) basis with the
+// reference's argument meaning (dipole_GPU.cu:641-723): one thread per
+// final primitive; initial primitives are walked in 256-wide tiles whose
+// coefficient, dipole row offset and Wigner offset (level and degeneracy
+// part) are staged in LDS.  The 3-component Wigner factor is indexed
+// n + 3 lvI + 3 nlvI lvF + 3 nlvI nlvF dgI + 3 nlvI nlvF maxdeg dgF and the
+// dipole components lie ncontrF * dip_stride apart.  Only entries whose
+// vibrational index lies in [startFblock, endFblock) are written.
+__global__ __launch_bounds__(256) void k_half_ls_rotsym(
+    int dimenF, int dimenI, int nlevelI, int nlevelF, int maxdeg,
+    const int* __restrict__ degI, const int* __restrict__ degF,
+    const int* __restrict__ levelI, const int* __restrict__ levelF,
+    const int* __restrict__ vibI, const int* __restrict__ vibF,
+    int startFblock, int endFblock, int ncontrF,
+    const double* __restrict__ dip, const double* __restrict__ wigner,
+    const double* __restrict__ x, int dip_stride,
+    double* __restrict__ half_ls) {
+  __shared__ double s_coef[256];
+  __shared__ int64_t s_row[256];
+  __shared__ int64_t s_w[256];
+  const int t = threadIdx.x;
+  const int rF = blockIdx.x * blockDim.x + t;
+  const bool inblk = rF < dimenF;
+  int vF = inblk ? vibF[rF] : startFblock - 1;
+  const bool valid = inblk && vF >= startFblock && vF < endFblock;
+  vF -= startFblock;
+  const int64_t comp = (int64_t)ncontrF * dip_stride;
+  const int64_t sF = 3 * (int64_t)nlevelI, sdI = sF * nlevelF,
+                sdF = sdI * maxdeg;
+  const int64_t wF =
+      valid ? sF * levelF[rF] + sdF * degF[rF] : 0;
+  const double* dF = dip + (valid ? vF : 0);
+  double acc = 0.0;
+  for (int b0 = 0; b0 < dimenI; b0 += 256) {
+    __syncthreads();
+    const int rI = b0 + t;
+    if (rI < dimenI) {
+      s_coef[t] = x[rI];
+      s_row[t] = (int64_t)vibI[rI] * ncontrF;
+      s_w[t] = 3 * (int64_t)levelI[rI] + sdI * degI[rI];
+    }
+    __syncthreads();
+    if (valid) {
+      const int n = min(256, dimenI - b0);
+      for (int j = 0; j < n; ++j) {
+        const double* w = wigner + wF + s_w[j];
+        const double* d = dF + s_row[j];
+        double f = w[0] * d[0];
+        f = fma(w[1], d[comp], f);
+        f = fma(w[2], d[2 * comp], f);
+        acc = fma(s_coef[j], f, acc);
+      }
+    }
+  }
+  if (valid) half_ls[rF] = acc;
+}
+
 // Deterministic single-workgroup dot product (fixed reduction order).
 __global__ __launch_bounds__(1024) void k_ddot(int n,
                                                const double* __restrict__ x,
@@ -205,6 +262,37 @@ extern "C" int gain_compute_half_linestrength(
                        kblock_size, dipole_me, vector, threej, jmax, stride,
                        half_ls);
     check_launch("half_ls_kblock");
+  });
+}
+
+extern "C" int gain_compute_half_linestrength_rotsym(
+    int dimenF, int dimenI, int nlevelI, int nlevelF, int sym_max_degen,
+    const int* kI, const int* kF, const int* ktauI, const int* ktauF,
+    const int* icorrI, const int* icorrF, int startFblock, int endFblock,
+    int ncontrF, const double* dipole_me, const double* wigner,
+    const double* vector, double* half_ls, void* stream) {
+  return guarded([&] {
+    const int stride = g_dip_stride;
+    if (stride <= 0)
+      fail(GAIN_ERR_STATE,
+           "compute_half_linestrength_rotsym: call "
+           "gain_copy_dipole_constant first");
+    GAIN_CHECK_ARG(dimenF >= 0 && dimenI >= 0 && nlevelI > 0 &&
+                       nlevelF > 0 && sym_max_degen > 0 &&
+                       sym_max_degen <= 3,
+                   "compute_half_linestrength_rotsym: bad sizes");
+    GAIN_CHECK_ARG(ncontrF == endFblock - startFblock && ncontrF > 0,
+                   "compute_half_linestrength_rotsym: ncontrF != endF-startF");
+    GAIN_CHECK_ARG(kI && kF && ktauI && ktauF && icorrI && icorrF &&
+                       dipole_me && wigner && vector && half_ls,
+                   "compute_half_linestrength_rotsym: null pointer");
+    if (dimenF == 0) return;
+    hipLaunchKernelGGL(k_half_ls_rotsym, dim3((dimenF + 255) / 256),
+                       dim3(256), 0, (hipStream_t)stream, dimenF, dimenI,
+                       nlevelI, nlevelF, sym_max_degen, kI, kF, ktauI, ktauF,
+                       icorrI, icorrF, startFblock, endFblock, ncontrF,
+                       dipole_me, wigner, vector, stride, half_ls);
+    check_launch("half_ls_rotsym");
   });
 }
 

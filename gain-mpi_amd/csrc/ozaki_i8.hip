@@ -94,9 +94,8 @@ void slice_rows_i8(int S, const double* src, int64_t ld, int rows, int K,
 
 // ---------------------------------------------------------------------------
 // Tile configurations: wave tile (32 TM) x (32 TN), WM x WN waves; `lds`:
-// operands staged through LDS (k_gemm_i8_lds / k_gemm_i8_res_lds; in the
-// CRT table its value is the number of k blocks per stage), else read per
-// wave.
+// operands staged through LDS (k_gemm_i8_lds / k_gemm_i8_res_lds), else read
+// per wave.
 struct I8Cfg {
   int tm, tn, wm, wn, lds;
 };
@@ -751,16 +750,13 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res(
 // requested before the products of stage kb, one barrier per k block.  The
 // deep prefetch costs no registers, which matters here: a k block carries
 // only TM x TN products per wave.
-template <int TM, int TN, int WM, int WN, int NST, int KS>
+template <int TM, int TN, int WM, int WN, int NST>
 __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res_lds(
     I8GemmArgs g, CrtConsts cc, int ntm, int ntn, int rb0, int nkb,
     int64_t ldr, int nr) {
-  // KS k blocks per stage (one barrier per stage; the configurations use
-  // KS = 1: with KS = 2 the 256 x 256 tile spills and the others gain
-  // nothing); k blocks past the end of K in the last stage are skipped
   constexpr int NW = WM * WN;
   constexpr int BMB = TM * WM, BNB = TN * WN;
-  constexpr int NBLK = KS * (BMB + BNB);
+  constexpr int NBLK = BMB + BNB;
   constexpr int NBLKP = (NBLK + NW - 1) / NW * NW;
   constexpr int DPW = NBLKP / NW;
   static_assert(NST >= 2 && NST <= 4, "stage count");
@@ -775,21 +771,16 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res_lds(
   const int src_off = 16 * i8_sigma(lane);
   const int8_t* Az = g.A + za * g.a_z + mi * g.a_plane;
   const int8_t* Bz = g.B + mi * g.b_plane;
-  // stage st holds k blocks KS st .. KS st + KS - 1; slot q = KS-group
-  // (q / (BMB + BNB)) then A blocks, B blocks.  k blocks past the end of K
-  // repeat the last one (loaded, never multiplied).
-  const int nst = (nkb + KS - 1) / KS;
-  auto issue = [&](int stage, int st) {
+  // slot q of a stage: A blocks, then B blocks
+  auto issue = [&](int stage, int kb) {
     int8_t* dst = lds + stage * (NBLKP * 1024);
 #pragma unroll
     for (int t = 0; t < DPW; ++t) {
       const int q = wave + NW * t;
       const int qq = q < NBLK ? q : 0;  // padding slots repeat block 0
-      const int kq = qq / (BMB + BNB), rq = qq % (BMB + BNB);
-      const int kb = min(KS * st + kq, nkb - 1);
       const int8_t* src =
-          rq < BMB ? Az + ((int64_t)(arb0 + rq) * nkb + kb) * 1024
-                   : Bz + ((int64_t)(brb0 + rq - BMB) * nkb + kb) * 1024;
+          qq < BMB ? Az + ((int64_t)(arb0 + qq) * nkb + kb) * 1024
+                   : Bz + ((int64_t)(brb0 + qq - BMB) * nkb + kb) * 1024;
       __builtin_amdgcn_global_load_lds(
           (const void*)(src + src_off),
           (__attribute__((address_space(3))) void*)(dst + q * 1024), 16, 0,
@@ -806,10 +797,10 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res_lds(
   const int rd_off = 16 * i8_sigma(2 * (lane & 31) + (lane >> 5));
 #pragma unroll
   for (int st = 0; st < NST - 1; ++st)
-    if (st < nst) issue(st, st);
+    if (st < nkb) issue(st, st);
   int stage = 0;
-  for (int st = 0; st < nst; ++st) {
-    const int later = min(NST - 2, nst - 1 - st);
+  for (int kb = 0; kb < nkb; ++kb) {
+    const int later = min(NST - 2, nkb - 1 - kb);
     if (later >= 2)
       wait_vmcnt<(NST >= 4 ? 2 * DPW : 0)>();
     else if (later == 1)
@@ -817,27 +808,22 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res_lds(
     else
       wait_vmcnt<0>();
     __builtin_amdgcn_s_barrier();
-    if (st + NST - 1 < nst)
-      issue(stage == 0 ? NST - 1 : stage - 1, st + NST - 1);
+    if (kb + NST - 1 < nkb)
+      issue(stage == 0 ? NST - 1 : stage - 1, kb + NST - 1);
+    const int8_t* base = lds + stage * (NBLKP * 1024) + rd_off;
+    i4v fa[TM], fb[TN];
 #pragma unroll
-    for (int kq = 0; kq < KS; ++kq) {
-      if (KS * st + kq >= nkb) break;
-      const int8_t* base =
-          lds + stage * (NBLKP * 1024) + kq * (BMB + BNB) * 1024 + rd_off;
-      i4v fa[TM], fb[TN];
+    for (int a = 0; a < TM; ++a)
+      fa[a] = *(const i4v*)(base + (TM * wm + a) * 1024);
 #pragma unroll
-      for (int a = 0; a < TM; ++a)
-        fa[a] = *(const i4v*)(base + (TM * wm + a) * 1024);
+    for (int b = 0; b < TN; ++b)
+      fb[b] = *(const i4v*)(base + (BMB + TN * wn + b) * 1024);
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
 #pragma unroll
       for (int b = 0; b < TN; ++b)
-        fb[b] = *(const i4v*)(base + (BMB + TN * wn + b) * 1024);
-#pragma unroll
-      for (int a = 0; a < TM; ++a)
-#pragma unroll
-        for (int b = 0; b < TN; ++b)
-          acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
-              fa[a], fb[b], acc[a][b], 0, 0, 0);
-    }
+        acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[a], fb[b],
+                                                          acc[a][b], 0, 0, 0);
     stage = stage == NST - 1 ? 0 : stage + 1;
   }
 
@@ -987,8 +973,8 @@ void gemm_crt_i8(const I8GemmArgs& g, hipStream_t s) {
 #define GAIN_CRT_LDS(CC)                                                     \
   case CC:                                                                   \
     hipLaunchKernelGGL((k_gemm_i8_res_lds<kCrtCfg[CC].tm, kCrtCfg[CC].tn,    \
-                                          kCrtCfg[CC].wm, kCrtCfg[CC].wn, 4, \
-                                          kCrtCfg[CC].lds>),                 \
+                                          kCrtCfg[CC].wm, kCrtCfg[CC].wn,    \
+                                          4>),                               \
                        grid, block, 0, s, g, cc, L.ntm, L.ntn, L.rb0, L.nkb, \
                        L.ldr, L.nr);                                         \
     break;

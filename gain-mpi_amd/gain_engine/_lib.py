@@ -27,7 +27,8 @@ EXPORTS = (
     "gain_batched_lines", "gain_set_timing", "gain_get_timing",
     "gain_copy_symmetry_constants", "gain_copy_jmax_constant",
     "gain_copy_dipole_constant", "gain_transform_vector_primitive",
-    "gain_compute_half_linestrength", "gain_ddot", "gain_adjoint_dipole",
+    "gain_compute_half_linestrength", "gain_compute_half_linestrength_rotsym",
+    "gain_ddot", "gain_adjoint_dipole",
     "gain_set_dipole_combination", "gain_dipole_coeff_grad",
     "gain_set_tuning", "gain_get_tuning", "gain_tuning_report",
     "gain_mfma_peak",
@@ -157,6 +158,8 @@ def load():
         "gain_transform_vector_primitive": (_ip, [_ip] * 6 + [_vp] * 7),
         "gain_compute_half_linestrength": (
             _ip, [_ip] * 5 + [_vp] * 4 + [_ip] * 6 + [_vp] * 5 + [_vp]),
+        "gain_compute_half_linestrength_rotsym": (
+            _ip, [_ip] * 5 + [_vp] * 6 + [_ip] * 3 + [_vp] * 5),
         "gain_ddot": (_ip, [_ip, _vp, _vp, _vp, _vp]),
         "gain_adjoint_dipole": (_ip, [_vp, _ip, _ip, _vp, _ip, _ip, _ip, _vp,
                                       _vp, _ip, _ip]),

@@ -348,6 +348,18 @@ int gain_compute_half_linestrength(
     int kFBlocksize, const int* kblock_size, const double* dipole_me,
     const double* vector, const double* threej, double* half_ls,
     void* stream);
+/* compute_gpu_half_linestrength_rotsym_, dipole_GPU.h:38-41 (kernel
+ * dipole_GPU.cu:641-723): half line strength in the symmetric-top basis.
+ * kI/kF = degeneracy index, ktauI/ktauF = rotational level, icorrI/icorrF =
+ * vibrational index of each primitive; wigner holds
+ * 3 x nlevelI x nlevelF x sym_max_degen x sym_max_degen doubles
+ * (first index fastest).  Needs gain_copy_dipole_constant. */
+int gain_compute_half_linestrength_rotsym(
+    int dimenF, int dimenI, int nlevelI, int nlevelF, int sym_max_degen,
+    const int* kI, const int* kF, const int* ktauI, const int* ktauF,
+    const int* icorrI, const int* icorrF, int startFblock, int endFblock,
+    int ncontrF, const double* dipole_me, const double* wigner,
+    const double* vector, double* half_ls, void* stream);
 /* cublasDdot(handle, n, x, 1, y, 1, result) in DEVICE pointer mode,
  * GpuManager.cpp:741 */
 int gain_ddot(int n, const double* x, const double* y, double* result,

@@ -224,6 +224,44 @@ void orc_half_ls(int jI, int dimenI, const int* kstartI, const int* kblockI,
   }
 }
 
+/* Half line strength in the ROTSYM (Euler) basis, restating
+ * device_compute_1st_half_ls_flipped_dipole_shared_rotsym_nontrove
+ * (dipole_GPU.cu:641-723): for every final primitive whose vibrational
+ * index lies in [startFblock, endFblock),
+ *   h[rF] = sum_rI x[rI] sum_n W[n, lvI(rI), lvF(rF), dgI(rI), dgF(rF)]
+ *           mu_n[vF - startFblock + vI ncontrF]
+ * with W indexed n + 3 lvI + 3 nlevelI lvF + 3 nlevelI nlevelF dgI
+ * + 3 nlevelI nlevelF maxdeg dgF (:693) and the dipole components
+ * ncontrF * dip_stride apart (:661).  Other entries are left untouched. */
+void orc_half_ls_rotsym(int dimenF, int dimenI, int nlevelI, int nlevelF,
+                        int maxdeg, const int* degI, const int* degF,
+                        const int* levelI, const int* levelF,
+                        const int* vibI, const int* vibF, int startFblock,
+                        int endFblock, int ncontrF, const double* dipole,
+                        int dip_stride, const double* wigner,
+                        const double* x, double* half_ls) {
+  const size_t comp = (size_t)ncontrF * (size_t)dip_stride;
+  const size_t sF = (size_t)3 * nlevelI, sdI = sF * (size_t)nlevelF,
+               sdF = sdI * (size_t)maxdeg;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int rF = 0; rF < dimenF; rF++) {
+    int vF = vibF[rF];
+    if (!(vF >= startFblock && vF < endFblock)) continue; /* :673 */
+    vF -= startFblock;
+    dd_t acc = {0.0, 0.0};
+    for (int rI = 0; rI < dimenI; rI++) {
+      const size_t w = (size_t)3 * levelI[rI] + sF * levelF[rF] +
+                       sdI * degI[rI] + sdF * degF[rF];
+      const size_t d = (size_t)vF + (size_t)vibI[rI] * (size_t)ncontrF;
+      dd_t t = {0.0, 0.0};
+      for (int n = 0; n < 3; n++)
+        dd_add_prod(&t, wigner[w + n], dipole[d + n * comp]);
+      dd_add_prod(&acc, x[rI], dd_val(t));
+    }
+    half_ls[rF] = dd_val(acc);
+  }
+}
+
 /* cublasDdot replacement (GpuManager.cpp:741), compensated. */
 double orc_dot(int64_t n, const double* x, const double* y) {
   if (g_plain_fp64) { /* eight independent FP64 partial sums */

@@ -60,6 +60,14 @@ void orc_half_ls(int jI, int dimenI, const int* kstartI, const int* kblockI,
                  int endF, const double* threej, int jmax, const double* x,
                  double* h);
 
+void orc_half_ls_rotsym(int dimenF, int dimenI, int nlevelI, int nlevelF,
+                        int maxdeg, const int* degI, const int* degF,
+                        const int* levelI, const int* levelF,
+                        const int* vibI, const int* vibF, int startFblock,
+                        int endFblock, int ncontrF, const double* dipole,
+                        int dip_stride, const double* wigner,
+                        const double* x, double* half_ls);
+
 double orc_dot(int64_t n, const double* x, const double* y);
 /* 1: plain FP64 accumulations instead of double-double (CPU timing only). */
 void orc_set_plain_fp64(int plain);

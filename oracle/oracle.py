@@ -96,6 +96,9 @@ def lib():
             [C.c_int, C.c_int] + [_i32p] * 4 + [C.c_int, C.c_int] +
             [_i32p] * 4 + [_f64p, C.c_int, C.c_int, C.c_int, _f64p, C.c_int,
                            _f64p, _f64p])
+        L.orc_half_ls_rotsym.argtypes = (
+            [C.c_int] * 5 + [_i32p] * 6 + [C.c_int] * 3 +
+            [_f64p, C.c_int, _f64p, _f64p, _f64p])
         L.orc_set_plain_fp64.restype = None
         L.orc_set_plain_fp64.argtypes = [C.c_int]
         L.orc_dot.restype = C.c_double
@@ -208,6 +211,28 @@ def half_ls(prob, bI, bF, x, tab=None, startF=0, endF=None) -> np.ndarray:
                       bF.J, bF.nprim, bF.kstart, bF.kblock, bF.tau, bF.vib,
                       np.ascontiguousarray(mu).reshape(-1), nc, startF, endF,
                       tab, jmax, np.ascontiguousarray(x, np.float64), h)
+    return h
+
+
+def half_ls_rotsym(nlevelI, nlevelF, maxdeg, degI, degF, levelI, levelF,
+                   vibI, vibF, startF, endF, dipole, dip_stride, wigner, x,
+                   out=None) -> np.ndarray:
+    """Symmetric-top half line strength (dipole_GPU.cu:641-723).
+
+    dipole: the piece [3, dip_stride, endF-startF] (final index fastest);
+    wigner: [maxdeg, maxdeg, nlevelF, nlevelI, 3] (last index fastest).
+    Entries of `out` whose final vibrational index is outside
+    [startF, endF) keep their value."""
+    i32 = lambda a: np.ascontiguousarray(a, np.int32)
+    degI, degF, levelI, levelF, vibI, vibF = map(
+        i32, (degI, degF, levelI, levelF, vibI, vibF))
+    h = np.zeros(vibF.shape[0]) if out is None else np.array(out, np.float64)
+    lib().orc_half_ls_rotsym(
+        vibF.shape[0], vibI.shape[0], nlevelI, nlevelF, maxdeg, degI, degF,
+        levelI, levelF, vibI, vibF, startF, endF, endF - startF,
+        np.ascontiguousarray(dipole, np.float64).reshape(-1), dip_stride,
+        np.ascontiguousarray(wigner, np.float64).reshape(-1),
+        np.ascontiguousarray(x, np.float64), h)
     return h
 
 

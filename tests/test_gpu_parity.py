@@ -216,6 +216,54 @@ def test_dropin_kernels_match_oracle():
     assert abs(d_out.item() - ref) <= 1e-14 * np.abs(h_ref * y).sum()
 
 
+@pytest.mark.parametrize("nI,nF,nlI,nlF,md,piece", [
+    (700, 500, 5, 7, 2, (3, 17)), (256, 1, 1, 1, 1, (0, 20)),
+    (1500, 2300, 9, 4, 3, (0, 20))])
+def test_dropin_rotsym_matches_oracle(nI, nF, nlI, nlF, md, piece):
+    """gain_compute_half_linestrength_rotsym against the oracle on random
+    symmetric-top data; entries outside the dipole piece keep their value."""
+    import torch
+
+    from gain_engine import _lib
+    L = _lib.load()
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(nI + nF)
+    nc = 20
+    s0, s1 = piece
+    mu = rng.standard_normal((3, nc, nc))
+    vibI = rng.integers(0, nc, nI).astype(np.int32)
+    vibF = rng.integers(0, nc, nF).astype(np.int32)
+    degI = rng.integers(0, md, nI).astype(np.int32)
+    degF = rng.integers(0, md, nF).astype(np.int32)
+    lvI = rng.integers(0, nlI, nI).astype(np.int32)
+    lvF = rng.integers(0, nlF, nF).astype(np.int32)
+    W = rng.standard_normal((md, md, nlF, nlI, 3))
+    x = rng.standard_normal(nI)
+    piece_mu = np.ascontiguousarray(mu[:, :, s0:s1])
+    h0 = np.full(nF, -3.5)
+    ref = orc.half_ls_rotsym(nlI, nlF, md, degI, degF, lvI, lvF, vibI, vibF,
+                             s0, s1, piece_mu, nc, W, x, out=h0)
+    T = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)
+    bufs = [T(a) for a in (degI, degF, lvI, lvF, vibI, vibF, piece_mu, W, x,
+                           h0)]
+    torch.cuda.synchronize()
+    _lib.check(L.gain_copy_dipole_constant(nc))
+    d = [b.data_ptr() for b in bufs]
+    _lib.check(L.gain_compute_half_linestrength_rotsym(
+        nF, nI, nlI, nlF, md, d[0], d[1], d[2], d[3], d[4], d[5], s0, s1,
+        s1 - s0, d[6], d[7], d[8], d[9], None))
+    torch.cuda.synchronize()
+    got = bufs[9].cpu().numpy()
+    np.testing.assert_allclose(got, ref, rtol=0,
+                               atol=1e-14 * np.abs(ref).max())
+    inside = (vibF >= s0) & (vibF < s1)
+    assert np.all(got[~inside] == -3.5)
+    rc = L.gain_compute_half_linestrength_rotsym(
+        nF, nI, nlI, nlF, md, d[0], d[1], d[2], d[3], d[4], d[5], s0, s1,
+        s1 - s0 + 1, d[6], d[7], d[8], d[9], None)
+    assert rc == _lib.ERR_ARG
+
+
 def test_error_paths_return_codes():
     """Missing uploads and bad arguments come back as status codes."""
     import ctypes as C

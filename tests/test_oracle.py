@@ -181,6 +181,87 @@ def test_half_ls_dipole_piece_assigns_only_its_rows():
     assert np.all(part[~inside] == 0.0)
 
 
+# ---------------------------------------------------------------- ROTSYM
+def _rotsym_levels(b):
+    """Level index 2k + tau of every primitive of a K-block basis."""
+    k = np.repeat(np.arange(b.J + 1), b.kblock)
+    return 2 * k + (np.asarray(b.tau) & 1), 2 * (b.J + 1)
+
+
+@pytest.mark.parametrize("jI,jF,seed,piece", [(2, 3, 1, (0, 7)),
+                                              (3, 2, 2, (2, 6)),
+                                              (3, 3, 3, (1, 7)),
+                                              (1, 0, 4, (0, 7))])
+def test_half_ls_rotsym_reduces_to_kblock_operator(jI, jF, seed, piece):
+    """A symmetric-top basis written as ROTSYM levels (k, tau) with the
+    Wigner table filled by the K-block rotational factor, scaled by an
+    asymmetric degeneracy matrix c[dI, dF], is the K-block half line
+    strength of x restricted to each initial degeneracy, mixed by c."""
+    rng = np.random.default_rng(seed)
+    nc = 7
+    prob = synth.make_problem([jI, jF] if jI != jF else [jI], nc, "c1")
+    bI = irregular_basis(jI, nc, rng, max_per_block=9)
+    bF = irregular_basis(jF, nc, rng, max_per_block=9)
+    prob.bases = [bI, bF]
+    jmax = max(jI, jF)
+    tab = orc.threej_table(jmax)
+    lvI, nlI = _rotsym_levels(bI)
+    lvF, nlF = _rotsym_levels(bF)
+    c = np.array([[1.0, 2.0], [3.0, 5.0]])
+    W = np.zeros((2, 2, nlF, nlI, 3))
+    for kI in range(jI + 1):
+        for tI in (0, 1):
+            for kF in range(jF + 1):
+                for tF in (0, 1):
+                    v, a = orc.rot_coeff(jmax, tab, jI, jF, kI, tI, kF, tF)
+                    if a >= 0:
+                        for dI in (0, 1):
+                            for dF in (0, 1):
+                                W[dF, dI, 2 * kF + tF, 2 * kI + tI, a] = (
+                                    v * c[dI, dF])
+    degI = rng.integers(0, 2, bI.nprim)
+    degF = rng.integers(0, 2, bF.nprim)
+    x = rng.standard_normal(bI.nprim)
+    s0, s1 = piece
+    mu = np.ascontiguousarray(prob.mu[:, :, s0:s1])
+    h = orc.half_ls_rotsym(nlI, nlF, 2, degI, degF, lvI, lvF, bI.vib,
+                           bF.vib, s0, s1, mu, nc, W, x)
+    parts = [orc.half_ls(prob, bI, bF, x * (degI == d), tab, s0, s1)
+             for d in (0, 1)]
+    ref = sum(c[d, degF] * parts[d] for d in (0, 1))
+    scale = max(np.abs(ref).max(), 1e-300)
+    np.testing.assert_allclose(h, ref, rtol=0, atol=1e-14 * scale)
+    inside = (bF.vib >= s0) & (bF.vib < s1)
+    assert np.all(h[~inside] == 0.0)
+
+
+def test_half_ls_rotsym_delta_wigner_is_a_gathered_gemv():
+    """W[n] = delta(n, z) leaves h = x . mu_z[vibI, vibF] (numpy GEMV);
+    entries outside the dipole piece keep the caller's values."""
+    rng = np.random.default_rng(11)
+    nc, nI, nF, nlI, nlF, md = 9, 300, 200, 4, 6, 3
+    mu = rng.standard_normal((3, nc, nc))
+    vibI = rng.integers(0, nc, nI)
+    vibF = rng.integers(0, nc, nF)
+    degI, degF = rng.integers(0, md, nI), rng.integers(0, md, nF)
+    lvI, lvF = rng.integers(0, nlI, nI), rng.integers(0, nlF, nF)
+    x = rng.standard_normal(nI)
+    s0, s1 = 2, 7
+    sentinel = np.full(nF, 7.25)
+    for z in range(3):
+        W = np.zeros((md, md, nlF, nlI, 3))
+        W[..., z] = 1.0
+        h = orc.half_ls_rotsym(nlI, nlF, md, degI, degF, lvI, lvF, vibI,
+                               vibF, s0, s1,
+                               np.ascontiguousarray(mu[:, :, s0:s1]), nc, W,
+                               x, out=sentinel)
+        inside = (vibF >= s0) & (vibF < s1)
+        ref = x @ mu[z][vibI][:, vibF]
+        np.testing.assert_allclose(h[inside], ref[inside], rtol=0,
+                                   atol=1e-13 * np.abs(ref).max())
+        assert np.all(h[~inside] == 7.25)
+
+
 # ---------------------------------------------------------------- dot
 def test_dot_is_correctly_rounded_on_ill_conditioned_input():
     # condition number ~1e9: double-double accumulation still rounds to within
