@@ -153,7 +153,7 @@ int crt_beta(int nm, int64_t K);
 void slice_rows_crt(int nm, int beta, const double* src, int64_t ld,
                     int rows, int K, int8_t* planes, int64_t plane,
                     int* expo, hipStream_t s);
-constexpr int kCrtConfigs = 8;
+constexpr int kCrtConfigs = 12;
 const char* crt_config_name(int cfg);
 // C[z][n*ldc + m] = 2^{eA + eB - 2(beta-1)} * (exact integer product of the
 // operands), g.S = number of moduli, g.beta as sliced, no split-K: one

@@ -263,6 +263,21 @@ __device__ inline void wait_vmcnt() {
                              (15 << 8));
 }
 
+// Wait until at most `later` * DPW of this wave's loads are in flight
+// (`later` stages issued after the one needed), for later <= MAXL.
+template <int DPW, int MAXL>
+__device__ inline void wait_stages(int later) {
+  if constexpr (MAXL > 0) {
+    if (later >= MAXL) {
+      wait_vmcnt<MAXL * DPW>();
+      return;
+    }
+    wait_stages<DPW, MAXL - 1>(later);
+  } else {
+    wait_vmcnt<0>();
+  }
+}
+
 // Operands staged through LDS: workgroup tile (32 TM WM) x (32 TN WN), 4
 // waves of (32 TM) x (32 TN) (the S level accumulators of a wave stay
 // within the 256 accumulation registers), NST k-block stages filled by
@@ -655,13 +670,21 @@ constexpr I8Cfg kCrtCfg[kCrtConfigs] = {
     {4, 4, 2, 2, 1},  // 256 x 256 through LDS (4 stages)
     {2, 4, 2, 2, 1},  // 128 x 256 through LDS
     {2, 2, 2, 4, 1},  // 128 x 256, 8 waves, through LDS
+    {4, 4, 2, 2, 4},  // 256 x 256 through LDS, 6 stages
+    {4, 4, 2, 2, 2},  // 256 x 256, LDS + fragment prefetch, 4 stages
+    {4, 4, 2, 2, 3},  // 256 x 256, LDS + fragment prefetch, 6 stages
+    {2, 4, 2, 2, 3},  // 128 x 256, LDS + fragment prefetch, 6 stages
 };
+// LDS pipeline of a configuration: stage count and fragment prefetch
+constexpr int crt_nst(int lds) { return lds == 1 || lds == 2 ? 4 : 6; }
+constexpr bool crt_pf(int lds) { return lds == 2 || lds == 3; }
 
 const char* crt_config_name(int cfg) {
   static const char* names[kCrtConfigs] = {
       "256x256/w128x128", "128x256/w64x128", "256x128/w128x64",
       "128x128/w64x64", "128x256/8w/w64x64", "256x256/w128x128/lds",
-      "128x256/w64x128/lds", "128x256/8w/lds"};
+      "128x256/w64x128/lds", "128x256/8w/lds", "256x256/lds6",
+      "256x256/lds4/pf", "256x256/lds6/pf", "128x256/lds6/pf"};
   return cfg >= 0 && cfg < kCrtConfigs ? names[cfg] : "?";
 }
 
@@ -746,11 +769,16 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res(
 // Same product through LDS: NST k-block stages of the A and B blocks of the
 // workgroup tile, filled by direct global -> LDS loads (each wave-wide
 // global_load_lds_dwordx4 copies one 1 KB block; lane l fetches chunk
-// sigma(l) so the fragment reads are bank-conflict-free), stage kb + NST - 1
-// requested before the products of stage kb, one barrier per k block.  The
-// deep prefetch costs no registers, which matters here: a k block carries
-// only TM x TN products per wave.
-template <int TM, int TN, int WM, int WN, int NST>
+// sigma(l) so the fragment reads are bank-conflict-free), one barrier per
+// k block.  The deep prefetch costs no registers, which matters here: a k
+// block carries only TM x TN products per wave.
+//   PF = false: stage kb + NST - 1 is requested, then the fragments of
+//     stage kb are read and multiplied.
+//   PF = true: the fragments of stage kb + 1 are read from LDS (into a
+//     second register set) before the products of stage kb are issued, so
+//     the LDS latency hides behind the matrix pipe; stage kb + 1 must have
+//     landed by then, so NST - 2 stages stay in flight.
+template <int TM, int TN, int WM, int WN, int NST, bool PF>
 __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res_lds(
     I8GemmArgs g, CrtConsts cc, int ntm, int ntn, int rb0, int nkb,
     int64_t ldr, int nr) {
@@ -759,7 +787,9 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res_lds(
   constexpr int NBLK = BMB + BNB;
   constexpr int NBLKP = (NBLK + NW - 1) / NW * NW;
   constexpr int DPW = NBLKP / NW;
-  static_assert(NST >= 2 && NST <= 4, "stage count");
+  static_assert(NST >= 2 && NST <= 6 && (NST - 2) * DPW < 64,
+                "stage count");
+  static_assert(!PF || NST >= 3, "fragment prefetch needs 3 stages");
   __shared__ __attribute__((aligned(1024))) int8_t lds[NST * NBLKP * 1024];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -795,40 +825,92 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res_lds(
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[a][b][e] = 0;
   const int rd_off = 16 * i8_sigma(2 * (lane & 31) + (lane >> 5));
-#pragma unroll
-  for (int st = 0; st < NST - 1; ++st)
-    if (st < nkb) issue(st, st);
-  int stage = 0;
-  for (int kb = 0; kb < nkb; ++kb) {
-    const int later = min(NST - 2, nkb - 1 - kb);
-    if (later >= 2)
-      wait_vmcnt<(NST >= 4 ? 2 * DPW : 0)>();
-    else if (later == 1)
-      wait_vmcnt<(NST >= 3 ? DPW : 0)>();
-    else
-      wait_vmcnt<0>();
-    __builtin_amdgcn_s_barrier();
-    if (kb + NST - 1 < nkb)
-      issue(stage == 0 ? NST - 1 : stage - 1, kb + NST - 1);
+  auto read = [&](i4v(&fa)[TM], i4v(&fb)[TN], int stage) {
     const int8_t* base = lds + stage * (NBLKP * 1024) + rd_off;
-    i4v fa[TM], fb[TN];
 #pragma unroll
     for (int a = 0; a < TM; ++a)
       fa[a] = *(const i4v*)(base + (TM * wm + a) * 1024);
 #pragma unroll
     for (int b = 0; b < TN; ++b)
       fb[b] = *(const i4v*)(base + (BMB + TN * wn + b) * 1024);
+  };
+  // products of A fragments [a0, a1)
+  auto mma = [&](const i4v(&fa)[TM], const i4v(&fb)[TN], int a0 = 0,
+                 int a1 = TM) {
 #pragma unroll
     for (int a = 0; a < TM; ++a)
 #pragma unroll
       for (int b = 0; b < TN; ++b)
-        acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[a], fb[b],
-                                                          acc[a][b], 0, 0, 0);
-    stage = stage == NST - 1 ? 0 : stage + 1;
+        if (a >= a0 && a < a1)
+          acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+              fa[a], fb[b], acc[a][b], 0, 0, 0);
+  };
+  // the epilogue's scalars, loaded (and waited for) before the loop, so
+  // that no scalar load is in flight inside it (scalar and LDS returns
+  // share one counter)
+  int8_t* R = g.R + (int64_t)z * nr * ldr;
+  double m = (double)cc.m[mi], minv = cc.minv_d[mi];
+  asm volatile("" : "+v"(R), "+v"(m), "+v"(minv));
+#pragma unroll
+  for (int st = 0; st < NST - 1; ++st)
+    if (st < nkb) issue(st, st);
+  if constexpr (!PF) {
+    int stage = 0;
+    for (int kb = 0; kb < nkb; ++kb) {
+      wait_stages<DPW, NST - 2>(min(NST - 2, nkb - 1 - kb));
+      __builtin_amdgcn_s_barrier();
+      if (kb + NST - 1 < nkb)
+        issue(stage == 0 ? NST - 1 : stage - 1, kb + NST - 1);
+      i4v fa[TM], fb[TN];
+      read(fa, fb, stage);
+      mma(fa, fb);
+      stage = stage == NST - 1 ? 0 : stage + 1;
+    }
+  } else {
+    i4v fa0[TM], fb0[TN], fa1[TM], fb1[TN];
+    wait_stages<DPW, NST - 2>(min(NST - 2, nkb - 1));
+    __builtin_amdgcn_s_barrier();
+    read(fa0, fb0, 0);
+    // iteration kb: stage kb + 1 landed -> barrier -> request stage
+    // kb + NST - 1 into the buffer of stage kb - 1 (read before the
+    // previous barrier and consumed by products that precede this one) ->
+    // read stage kb + 1 -> products of stage kb
+    // With LDS DMA in flight the LDS counter is not in order, so the wait
+    // before a product whose fragments came from LDS drains every LDS read:
+    // the reads of stage kb + 1 are therefore issued after the first half
+    // of the products of stage kb (whose wait then finds the reads made
+    // half a k block earlier complete) and before the second half.  The
+    // steady-state loop reads unconditionally (a read on one path only
+    // would also force a full drain).
+    int nxt = 1 % NST;  // buffer of stage kb + 1
+    auto step = [&](const i4v(&ca)[TM], const i4v(&cb)[TN], i4v(&na)[TM],
+                    i4v(&nb)[TN], int kb) {
+      wait_stages<DPW, NST - 3>(min(NST - 3, nkb - 2 - kb));
+      __builtin_amdgcn_s_barrier();
+      if (kb + NST - 1 < nkb)
+        issue(nxt == 0 ? NST - 2 : (nxt == 1 ? NST - 1 : nxt - 2),
+              kb + NST - 1);
+      mma(ca, cb, 0, (TM + 1) / 2);
+      __builtin_amdgcn_sched_barrier(0);
+      read(na, nb, nxt);
+      nxt = nxt == NST - 1 ? 0 : nxt + 1;
+      __builtin_amdgcn_sched_barrier(0);
+      mma(ca, cb, (TM + 1) / 2, TM);
+    };
+    int kb = 0;
+    for (; kb + 2 < nkb; kb += 2) {
+      step(fa0, fb0, fa1, fb1, kb);
+      step(fa1, fb1, fa0, fb0, kb + 1);
+    }
+    // one or two k blocks left
+    if (kb + 1 < nkb) {
+      step(fa0, fb0, fa1, fb1, kb);
+      mma(fa1, fb1);
+    } else {
+      mma(fa0, fb0);
+    }
   }
 
-  const double m = (double)cc.m[mi], minv = cc.minv_d[mi];
-  int8_t* R = g.R + (int64_t)z * nr * ldr;
   const int arb = arb0 + TM * wm, brb = brb0 + TN * wn;
 #pragma unroll
   for (int b = 0; b < TN; ++b) {
@@ -972,15 +1054,20 @@ void gemm_crt_i8(const I8GemmArgs& g, hipStream_t s) {
 #undef GAIN_CRT_CASE
 #define GAIN_CRT_LDS(CC)                                                     \
   case CC:                                                                   \
-    hipLaunchKernelGGL((k_gemm_i8_res_lds<kCrtCfg[CC].tm, kCrtCfg[CC].tn,    \
-                                          kCrtCfg[CC].wm, kCrtCfg[CC].wn,    \
-                                          4>),                               \
+    hipLaunchKernelGGL((k_gemm_i8_res_lds<                                  \
+                           kCrtCfg[CC].tm, kCrtCfg[CC].tn, kCrtCfg[CC].wm,   \
+                           kCrtCfg[CC].wn, crt_nst(kCrtCfg[CC].lds),         \
+                           crt_pf(kCrtCfg[CC].lds)>),                        \
                        grid, block, 0, s, g, cc, L.ntm, L.ntn, L.rb0, L.nkb, \
                        L.ldr, L.nr);                                         \
     break;
     GAIN_CRT_LDS(5)
     GAIN_CRT_LDS(6)
     GAIN_CRT_LDS(7)
+    GAIN_CRT_LDS(8)
+    GAIN_CRT_LDS(9)
+    GAIN_CRT_LDS(10)
+    GAIN_CRT_LDS(11)
 #undef GAIN_CRT_LDS
   }
   check_launch("gemm_i8_res");
