@@ -79,7 +79,7 @@ void gemm_sliced_bf16(const SlicedGemmArgs& g, hipStream_t s);
 // of 32 rows x 32 bytes (1 KB), so one MFMA operand fragment is one
 // contiguous 1 KB block.  Rows are padded to i8_rows_pad(R) and K to
 // i8_kpad(K) with zeros, so the GEMM never bounds-checks its loads.
-inline int64_t i8_kpad(int64_t K) { return (K + 31) / 32 * 32; }
+inline int64_t i8_kpad(int64_t K) { return (K + 63) / 64 * 64; }
 inline int64_t i8_rows_pad(int64_t R) { return (R + 31) / 32 * 32 + 256; }
 inline int64_t i8_max_krange(int S) {
   return ((1ll << 31) - 1) / (4096ll * S) / 32 * 32;
@@ -153,7 +153,7 @@ int crt_beta(int nm, int64_t K);
 void slice_rows_crt(int nm, int beta, const double* src, int64_t ld,
                     int rows, int K, int8_t* planes, int64_t plane,
                     int* expo, hipStream_t s);
-constexpr int kCrtConfigs = 12;
+constexpr int kCrtConfigs = 14;
 const char* crt_config_name(int cfg);
 // C[z][n*ldc + m] = 2^{eA + eB - 2(beta-1)} * (exact integer product of the
 // operands), g.S = number of moduli, g.beta as sliced, no split-K: one

@@ -674,17 +674,22 @@ constexpr I8Cfg kCrtCfg[kCrtConfigs] = {
     {4, 4, 2, 2, 2},  // 256 x 256, LDS + fragment prefetch, 4 stages
     {4, 4, 2, 2, 3},  // 256 x 256, LDS + fragment prefetch, 6 stages
     {2, 4, 2, 2, 3},  // 128 x 256, LDS + fragment prefetch, 6 stages
+    {4, 4, 2, 2, 5},  // 256 x 256, prefetch, 4 stages of 2 k blocks
+    {4, 2, 2, 4, 5},  // 256 x 256, 8 waves of 128 x 64, as above
 };
-// LDS pipeline of a configuration: stage count and fragment prefetch
-constexpr int crt_nst(int lds) { return lds == 1 || lds == 2 ? 4 : 6; }
-constexpr bool crt_pf(int lds) { return lds == 2 || lds == 3; }
+// LDS pipeline of a configuration: stage count, fragment prefetch, k
+// blocks per stage
+constexpr int crt_nst(int lds) { return lds == 1 || lds == 2 || lds == 5 ? 4 : 6; }
+constexpr bool crt_pf(int lds) { return lds == 2 || lds == 3 || lds == 5; }
+constexpr int crt_kpb(int lds) { return lds == 5 ? 2 : 1; }
 
 const char* crt_config_name(int cfg) {
   static const char* names[kCrtConfigs] = {
       "256x256/w128x128", "128x256/w64x128", "256x128/w128x64",
       "128x128/w64x64", "128x256/8w/w64x64", "256x256/w128x128/lds",
       "128x256/w64x128/lds", "128x256/8w/lds", "256x256/lds6",
-      "256x256/lds4/pf", "256x256/lds6/pf", "128x256/lds6/pf"};
+      "256x256/lds4/pf", "256x256/lds6/pf", "128x256/lds6/pf",
+      "256x256/lds4x2/pf", "256x256/8w/lds4x2/pf"};
   return cfg >= 0 && cfg < kCrtConfigs ? names[cfg] : "?";
 }
 
@@ -766,19 +771,20 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res(
   }
 }
 
-// Same product through LDS: NST k-block stages of the A and B blocks of the
-// workgroup tile, filled by direct global -> LDS loads (each wave-wide
-// global_load_lds_dwordx4 copies one 1 KB block; lane l fetches chunk
-// sigma(l) so the fragment reads are bank-conflict-free), one barrier per
-// k block.  The deep prefetch costs no registers, which matters here: a k
-// block carries only TM x TN products per wave.
-//   PF = false: stage kb + NST - 1 is requested, then the fragments of
-//     stage kb are read and multiplied.
-//   PF = true: the fragments of stage kb + 1 are read from LDS (into a
-//     second register set) before the products of stage kb are issued, so
-//     the LDS latency hides behind the matrix pipe; stage kb + 1 must have
-//     landed by then, so NST - 2 stages stay in flight.
-template <int TM, int TN, int WM, int WN, int NST, bool PF>
+// Same product through LDS: NST stages of KPB k blocks of the A and B
+// blocks of the workgroup tile, filled by direct global -> LDS loads (each
+// wave-wide global_load_lds_dwordx4 copies one 1 KB block; lane l fetches
+// chunk sigma(l) so the fragment reads are bank-conflict-free), one
+// barrier per stage.  The deep prefetch costs no registers, which matters
+// here: a k block carries only TM x TN products per wave.
+//   PF = false (KPB = 1): stage kb + NST - 1 is requested, then the
+//     fragments of stage kb are read and multiplied.
+//   PF = true: the fragments of stage s + 1 are read from LDS (into a
+//     second register set) while the products of stage s are issued, so
+//     the LDS latency hides behind the matrix pipe; stage s + 1 must have
+//     landed by then, so NST - 2 stages stay in flight.  KPB = 2 halves the
+//     barriers per k block (the k block count must then be even).
+template <int TM, int TN, int WM, int WN, int NST, bool PF, int KPB>
 __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res_lds(
     I8GemmArgs g, CrtConsts cc, int ntm, int ntn, int rb0, int nkb,
     int64_t ldr, int nr) {
@@ -787,10 +793,12 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res_lds(
   constexpr int NBLK = BMB + BNB;
   constexpr int NBLKP = (NBLK + NW - 1) / NW * NW;
   constexpr int DPW = NBLKP / NW;
-  static_assert(NST >= 2 && NST <= 6 && (NST - 2) * DPW < 64,
+  constexpr int SB = NBLKP * KPB;  // blocks per stage
+  static_assert(NST >= 2 && NST <= 6 && (NST - 2) * DPW * KPB < 64,
                 "stage count");
   static_assert(!PF || NST >= 3, "fragment prefetch needs 3 stages");
-  __shared__ __attribute__((aligned(1024))) int8_t lds[NST * NBLKP * 1024];
+  static_assert(KPB == 1 || (PF && KPB == 2), "k blocks per stage");
+  __shared__ __attribute__((aligned(1024))) int8_t lds[NST * SB * 1024];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int wm = wave % WM, wn = wave / WM;
@@ -801,21 +809,27 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res_lds(
   const int src_off = 16 * i8_sigma(lane);
   const int8_t* Az = g.A + za * g.a_z + mi * g.a_plane;
   const int8_t* Bz = g.B + mi * g.b_plane;
-  // slot q of a stage: A blocks, then B blocks
-  auto issue = [&](int stage, int kb) {
-    int8_t* dst = lds + stage * (NBLKP * 1024);
+  const int nst = nkb / KPB;  // stages
+  // slot j * NBLKP + q of a stage: k block j of the stage, A blocks then B
+  // blocks
+  auto issue = [&](int stage, int st) {
+    int8_t* dst = lds + stage * (SB * 1024);
 #pragma unroll
-    for (int t = 0; t < DPW; ++t) {
-      const int q = wave + NW * t;
-      const int qq = q < NBLK ? q : 0;  // padding slots repeat block 0
-      const int8_t* src =
-          qq < BMB ? Az + ((int64_t)(arb0 + qq) * nkb + kb) * 1024
-                   : Bz + ((int64_t)(brb0 + qq - BMB) * nkb + kb) * 1024;
-      __builtin_amdgcn_global_load_lds(
-          (const void*)(src + src_off),
-          (__attribute__((address_space(3))) void*)(dst + q * 1024), 16, 0,
-          0);
-    }
+    for (int j = 0; j < KPB; ++j)
+#pragma unroll
+      for (int t = 0; t < DPW; ++t) {
+        const int kb = st * KPB + j;
+        const int q = wave + NW * t;
+        const int qq = q < NBLK ? q : 0;  // padding slots repeat block 0
+        const int8_t* src =
+            qq < BMB ? Az + ((int64_t)(arb0 + qq) * nkb + kb) * 1024
+                     : Bz + ((int64_t)(brb0 + qq - BMB) * nkb + kb) * 1024;
+        __builtin_amdgcn_global_load_lds(
+            (const void*)(src + src_off),
+            (__attribute__((address_space(3))) void*)(dst +
+                                                      (j * NBLKP + q) * 1024),
+            16, 0, 0);
+      }
   };
   i16v acc[TM][TN];
 #pragma unroll
@@ -825,8 +839,8 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res_lds(
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[a][b][e] = 0;
   const int rd_off = 16 * i8_sigma(2 * (lane & 31) + (lane >> 5));
-  auto read = [&](i4v(&fa)[TM], i4v(&fb)[TN], int stage) {
-    const int8_t* base = lds + stage * (NBLKP * 1024) + rd_off;
+  auto read = [&](i4v(&fa)[TM], i4v(&fb)[TN], int stage, int j) {
+    const int8_t* base = lds + (stage * SB + j * NBLKP) * 1024 + rd_off;
 #pragma unroll
     for (int a = 0; a < TM; ++a)
       fa[a] = *(const i4v*)(base + (TM * wm + a) * 1024);
@@ -853,7 +867,7 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res_lds(
   asm volatile("" : "+v"(R), "+v"(m), "+v"(minv));
 #pragma unroll
   for (int st = 0; st < NST - 1; ++st)
-    if (st < nkb) issue(st, st);
+    if (st < nst) issue(st, st);
   if constexpr (!PF) {
     int stage = 0;
     for (int kb = 0; kb < nkb; ++kb) {
@@ -862,52 +876,58 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res_lds(
       if (kb + NST - 1 < nkb)
         issue(stage == 0 ? NST - 1 : stage - 1, kb + NST - 1);
       i4v fa[TM], fb[TN];
-      read(fa, fb, stage);
+      read(fa, fb, stage, 0);
       mma(fa, fb);
       stage = stage == NST - 1 ? 0 : stage + 1;
     }
   } else {
-    i4v fa0[TM], fb0[TN], fa1[TM], fb1[TN];
-    wait_stages<DPW, NST - 2>(min(NST - 2, nkb - 1));
+    i4v fa0[KPB][TM], fb0[KPB][TN], fa1[KPB][TM], fb1[KPB][TN];
+    wait_stages<DPW * KPB, NST - 2>(min(NST - 2, nst - 1));
     __builtin_amdgcn_s_barrier();
-    read(fa0, fb0, 0);
-    // iteration kb: stage kb + 1 landed -> barrier -> request stage
-    // kb + NST - 1 into the buffer of stage kb - 1 (read before the
-    // previous barrier and consumed by products that precede this one) ->
-    // read stage kb + 1 -> products of stage kb
+#pragma unroll
+    for (int j = 0; j < KPB; ++j) read(fa0[j], fb0[j], 0, j);
+    // iteration s: stage s + 1 landed -> barrier -> request stage
+    // s + NST - 1 into the buffer of stage s - 1 (read before the previous
+    // barrier and consumed by products that precede this one) -> products
+    // of stage s, with the reads of stage s + 1 among them.
     // With LDS DMA in flight the LDS counter is not in order, so the wait
     // before a product whose fragments came from LDS drains every LDS read:
-    // the reads of stage kb + 1 are therefore issued after the first half
-    // of the products of stage kb (whose wait then finds the reads made
-    // half a k block earlier complete) and before the second half.  The
-    // steady-state loop reads unconditionally (a read on one path only
-    // would also force a full drain).
-    int nxt = 1 % NST;  // buffer of stage kb + 1
-    auto step = [&](const i4v(&ca)[TM], const i4v(&cb)[TN], i4v(&na)[TM],
-                    i4v(&nb)[TN], int kb) {
-      wait_stages<DPW, NST - 3>(min(NST - 3, nkb - 2 - kb));
+    // the reads of k block j of stage s + 1 are therefore issued after the
+    // first half of the products of k block j of stage s (whose wait then
+    // finds the reads made half a k block earlier complete) and before the
+    // second half.  The steady-state loop reads unconditionally (a read on
+    // one path only would also force a full drain).
+    int nxt = 1 % NST;  // buffer of stage s + 1
+    auto step = [&](const i4v(&ca)[KPB][TM], const i4v(&cb)[KPB][TN],
+                    i4v(&na)[KPB][TM], i4v(&nb)[KPB][TN], int st) {
+      wait_stages<DPW * KPB, NST - 3>(min(NST - 3, nst - 2 - st));
       __builtin_amdgcn_s_barrier();
-      if (kb + NST - 1 < nkb)
+      if (st + NST - 1 < nst)
         issue(nxt == 0 ? NST - 2 : (nxt == 1 ? NST - 1 : nxt - 2),
-              kb + NST - 1);
-      mma(ca, cb, 0, (TM + 1) / 2);
-      __builtin_amdgcn_sched_barrier(0);
-      read(na, nb, nxt);
+              st + NST - 1);
+#pragma unroll
+      for (int j = 0; j < KPB; ++j) {
+        mma(ca[j], cb[j], 0, (TM + 1) / 2);
+        __builtin_amdgcn_sched_barrier(0);
+        read(na[j], nb[j], nxt, j);
+        __builtin_amdgcn_sched_barrier(0);
+        mma(ca[j], cb[j], (TM + 1) / 2, TM);
+      }
       nxt = nxt == NST - 1 ? 0 : nxt + 1;
-      __builtin_amdgcn_sched_barrier(0);
-      mma(ca, cb, (TM + 1) / 2, TM);
     };
-    int kb = 0;
-    for (; kb + 2 < nkb; kb += 2) {
-      step(fa0, fb0, fa1, fb1, kb);
-      step(fa1, fb1, fa0, fb0, kb + 1);
+    int st = 0;
+    for (; st + 2 < nst; st += 2) {
+      step(fa0, fb0, fa1, fb1, st);
+      step(fa1, fb1, fa0, fb0, st + 1);
     }
-    // one or two k blocks left
-    if (kb + 1 < nkb) {
-      step(fa0, fb0, fa1, fb1, kb);
-      mma(fa1, fb1);
+    // one or two stages left
+    if (st + 1 < nst) {
+      step(fa0, fb0, fa1, fb1, st);
+#pragma unroll
+      for (int j = 0; j < KPB; ++j) mma(fa1[j], fb1[j]);
     } else {
-      mma(fa0, fb0);
+#pragma unroll
+      for (int j = 0; j < KPB; ++j) mma(fa0[j], fb0[j]);
     }
   }
 
@@ -1022,7 +1042,7 @@ void gemm_crt_i8(const I8GemmArgs& g, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0) return;
   GAIN_CHECK_ARG(g.cfg >= 0 && g.cfg < kCrtConfigs, "CRT gemm: config %d",
                  g.cfg);
-  GAIN_CHECK_ARG(g.Kp > 0 && g.Kp % 32 == 0 && g.ksplit == 1 && g.nz >= 1 &&
+  GAIN_CHECK_ARG(g.Kp > 0 && g.Kp % 64 == 0 && g.ksplit == 1 && g.nz >= 1 &&
                      g.row0 >= 0 && g.R,
                  "CRT gemm: Kp %lld, split %d, residue buffer %p",
                  (long long)g.Kp, g.ksplit, (const void*)g.R);
@@ -1057,7 +1077,8 @@ void gemm_crt_i8(const I8GemmArgs& g, hipStream_t s) {
     hipLaunchKernelGGL((k_gemm_i8_res_lds<                                  \
                            kCrtCfg[CC].tm, kCrtCfg[CC].tn, kCrtCfg[CC].wm,   \
                            kCrtCfg[CC].wn, crt_nst(kCrtCfg[CC].lds),         \
-                           crt_pf(kCrtCfg[CC].lds)>),                        \
+                           crt_pf(kCrtCfg[CC].lds),                          \
+                           crt_kpb(kCrtCfg[CC].lds)>),                       \
                        grid, block, 0, s, g, cc, L.ntm, L.ntn, L.rb0, L.nkb, \
                        L.ldr, L.nr);                                         \
     break;
@@ -1068,6 +1089,8 @@ void gemm_crt_i8(const I8GemmArgs& g, hipStream_t s) {
     GAIN_CRT_LDS(9)
     GAIN_CRT_LDS(10)
     GAIN_CRT_LDS(11)
+    GAIN_CRT_LDS(12)
+    GAIN_CRT_LDS(13)
 #undef GAIN_CRT_LDS
   }
   check_launch("gemm_i8_res");

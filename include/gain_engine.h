@@ -236,7 +236,7 @@ int gain_get_timing(gain_ctx* ctx, const char* name, double* total_ms,
  * "proj_cfg" / "half_ls_cfg" (force a configuration, -1 = automatic),
  * "i8_cfg" / "i8_split" (split-INT8 GEMMs: force a tile configuration
  * 0..11, -1 = automatic; projection split count, 0 = automatic),
- * "crt_cfg" (CRT-form int8 GEMMs: force a tile configuration 0..11),
+ * "crt_cfg" (CRT-form int8 GEMMs: force a tile configuration 0..13),
  * "gemv_rows" (2/4/8), "gemv_ksplit", "mu_ksplit" (0 = automatic),
  * "mfma" (0: vector-ALU GEMMs, for cross-checks). */
 int gain_set_tuning(gain_ctx* ctx, const char* key, int value);

@@ -390,7 +390,7 @@ def test_batched_crt_error_vs_fp64_oracle(precision, bound):
     assert es <= bound
 
 
-@pytest.mark.parametrize("cfg", list(range(12)))
+@pytest.mark.parametrize("cfg", list(range(14)))
 def test_every_crt_tile_configuration(cfg):
     """Each CRT-form tile shape for both GEMM kinds on sizes with partial
     tiles; a frequency window starts the projected final rows off a 32-row
