@@ -142,8 +142,12 @@ struct CrtConsts {
   int m[kCrtMaxModuli];
   float minv[kCrtMaxModuli];   // 1 / m_i
   double minv_d[kCrtMaxModuli];
-  float inv[kCrtMaxModuli];    // (m_0 ... m_{i-1})^{-1} mod m_i
-  float w[kCrtMaxModuli][kCrtMaxModuli];  // (m_0 ... m_{j-1}) mod m_i, j < i
+  // Garner weights as symmetric residues (|.| <= 127): digit i of the
+  // mixed-radix form is v_i = (r_i inv_i + sum_{j<i} v_j nw_ij) mod m_i with
+  // inv_i = (m_0 ... m_{i-1})^{-1} mod m_i and
+  // nw_ij = -(m_0 ... m_{j-1}) inv_i mod m_i.
+  float inv[kCrtMaxModuli];
+  float nw[kCrtMaxModuli][kCrtMaxModuli];
 };
 const CrtConsts& crt_consts(int nm);
 // Largest beta with 2 K 4^{beta-1} < prod_{i<nm} m_i.

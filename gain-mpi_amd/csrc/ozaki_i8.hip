@@ -527,6 +527,12 @@ int64_t inv_mod(int64_t a, int64_t m) {  // a^{-1} mod m, gcd(a, m) = 1
   return ((t % m) + m) % m;
 }
 
+// a mod m in [-(m - 1) / 2, (m - 1) / 2] (m odd)
+int64_t sym_res(int64_t a, int64_t m) {
+  int64_t r = ((a % m) + m) % m;
+  return r > (m - 1) / 2 ? r - m : r;
+}
+
 CrtConsts make_crt(int nm) {
   CrtConsts c{};
   c.nm = nm;
@@ -535,12 +541,15 @@ CrtConsts make_crt(int nm) {
     c.m[i] = (int)m;
     c.minv[i] = 1.0f / (float)m;
     c.minv_d[i] = 1.0 / (double)m;
-    int64_t prod = 1;  // m_0 ... m_{j-1} mod m_i
+    int64_t pw[kCrtMaxModuli];  // m_0 ... m_{j-1} mod m_i
+    int64_t prod = 1;
     for (int j = 0; j < i; ++j) {
-      c.w[i][j] = (float)prod;
+      pw[j] = prod;
       prod = prod * kCrtModuli[j] % m;
     }
-    c.inv[i] = i == 0 ? 1.0f : (float)inv_mod(prod, m);
+    const int64_t inv = i == 0 ? 1 : inv_mod(prod, m);
+    c.inv[i] = (float)sym_res(inv, m);
+    for (int j = 0; j < i; ++j) c.nw[i][j] = (float)sym_res(-pw[j] * inv, m);
   }
   return c;
 }
@@ -563,28 +572,24 @@ int crt_beta(int nm, int64_t K) {
   const long double lim = (lp - 1.0L - log2l((long double)K)) / 2.0L;
   int b1 = (int)floorl(lim);
   if ((long double)b1 >= lim) --b1;  // strict inequality
-  GAIN_CHECK_ARG(b1 >= 8 && b1 <= 52, "CRT: no integer width for K = %lld",
+  GAIN_CHECK_ARG(b1 >= 8, "CRT: no integer width for K = %lld",
                  (long long)K);
-  return b1 + 1;
+  // at most 50-bit integers: their residues are taken exactly in FP64
+  // (sym_mod_d), and 50 bits already exceed the FP64 product accuracy
+  return std::min(b1, 49) + 1;
 }
 
-// Residue of an integer-valued double |x| < 2^52 modulo m, in the symmetric
-// range [-(m-1)/2, (m-1)/2] (m odd).
+// x mod m in [-(m - 1) / 2, (m - 1) / 2] for an odd modulus m and an
+// integer-valued x with |x| < 2^51, minv = fl(1/m).  x * minv is within
+// |x/m| 2^-52 of x/m, and x/m (m odd) is at least 1/(2m) away from every
+// half-integer, so rint(x * minv) = rint(x/m) and the result is exact.
 __device__ inline double sym_mod_d(double x, double m, double minv) {
-  double r = x - rint(x * minv) * m;  // exact; rint may be off by one
-  const double h = 0.5 * (m - 1.0);
-  if (r > h) r -= m;
-  if (r < -h) r += m;
-  return r;
+  return fma(-rint(x * minv), m, x);
 }
 
-// Same for an integer-valued float |x| < 2^24.
+// Same in FP32 for an integer-valued |x| < 2^22 (error |x/m| 2^-23 < 1/(2m)).
 __device__ inline float sym_mod_f(float x, float m, float minv) {
-  float r = fmaf(-rintf(x * minv), m, x);
-  const float h = 0.5f * (m - 1.0f);
-  if (r > h) r -= m;
-  if (r < -h) r += m;
-  return r;
+  return fmaf(-rintf(x * minv), m, x);
 }
 
 template <int NM>
@@ -953,10 +958,11 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res_lds(
 }
 
 // Garner's mixed-radix conversion of the nm residues of every output: digit
-// i from the digits below it by a dot product with the (m_0...m_{j-1}) mod
-// m_i weights (exact small integers in FP32), then the mixed-radix number in
-// FP64, scaled by the operand exponents.  Four consecutive outputs per
-// thread (one 32-bit load per residue plane; residue rows are 4-aligned).
+// i = (r_i inv_i + sum_{j<i} v_j nw_ij) mod m_i, one symmetric reduction of
+// an exact FP32 integer (|.| <= 14 * 127^2 < 2^22), then the mixed-radix
+// number in FP64, scaled by the operand exponents.  Four consecutive outputs
+// per thread (one 32-bit load per residue plane; residue rows are
+// 4-aligned).
 template <int NM>
 __global__ __launch_bounds__(256) void k_crt_combine(I8GemmArgs g,
                                                      CrtConsts cc, int rb0,
@@ -983,11 +989,10 @@ __global__ __launch_bounds__(256) void k_crt_combine(I8GemmArgs g,
       if (i == 0) {
         v[0][q] = r;
       } else {
-        float sum = 0.0f;  // |sum| < 14 * 127 * 255 < 2^24: exact
+        float t = r * cc.inv[i];
 #pragma unroll
-        for (int j = 0; j < i; ++j) sum = fmaf(v[j][q], cc.w[i][j], sum);
-        sum = sym_mod_f(sum, mf, cc.minv[i]);
-        v[i][q] = sym_mod_f((r - sum) * cc.inv[i], mf, cc.minv[i]);
+        for (int j = 0; j < i; ++j) t = fmaf(v[j][q], cc.nw[i][j], t);
+        v[i][q] = sym_mod_f(t, mf, cc.minv[i]);
       }
     }
   }
