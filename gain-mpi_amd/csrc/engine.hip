@@ -31,7 +31,7 @@ static long double lfact(int n) {
   return f;
 }
 
-static double three_j(int j1, int j2, int j3, int m1, int m2, int m3) {
+double three_j(int j1, int j2, int j3, int m1, int m2, int m3) {
   if (j3 > j1 + j2 || j3 < std::abs(j1 - j2)) return 0.0;
   if (j1 < 0 || j2 < 0 || j3 < 0) return 0.0;
   if (j1 < std::abs(m1) || j2 < std::abs(m2) || j3 < std::abs(m3)) return 0.0;

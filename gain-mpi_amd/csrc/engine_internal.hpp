@@ -19,6 +19,10 @@
 
 namespace gain {
 
+// Wigner 3j symbol (j1 j2 j3; m1 m2 m3), Racah formula in long double.
+double three_j(int j1, int j2, int j3, int m1, int m2, int m3);
+
+
 // ---------------------------------------------------------------------------
 struct DeviceArena {
   std::vector<void*> ptrs;

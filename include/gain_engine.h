@@ -360,6 +360,28 @@ int gain_compute_half_linestrength_rotsym(
     const int* icorrI, const int* icorrF, int startFblock, int endFblock,
     int ncontrF, const double* dipole_me, const double* wigner,
     const double* vector, double* half_ls, void* stream);
+/* ---- symmetric-top (ROTSYM) rotational factors ---------------------------
+ * Groups: "C(M)", "CS(M)", "C2V(M)", "C3V(M)", "D3H(M)" (symmetry.f90:58-538).
+ *
+ * Symmetrised rotational functions of one J (MLrotsymmetry_generate,
+ * trove_wrappers.f90:416-589, with MLsymmetrization_rot :591-914): the
+ * 2J+1 Wang functions (k = 0, then k = 1..J with tau = 0, 1) combined into
+ * nlevels levels, each of one irrep (level_irrep[nlevels]) and with
+ * maxdeg components; root_of[level * maxdeg + deg] is the column of the
+ * function in vectors[(2J+1) x (2J+1)] (row = Wang function, row major),
+ * -1 past the irrep's degeneracy.  operations (nullable) receives the real
+ * part of the rotation matrix of every group operation in the Wang basis,
+ * noper x (2J+1) x (2J+1).  Output arrays may be NULL (sizes only). */
+int gain_rotsym_functions(const char* group, int J, int* nlevels,
+                          int* maxdeg, int* level_irrep, int* root_of,
+                          double* vectors, double* operations);
+/* c_wignerGen (contraced_rotational_dipole, trove_wrappers.f90:170-414):
+ * the rotational dipole factors between the symmetrised functions of J_I
+ * and J_F (|J_F - J_I| <= 1), rot[3 x nlevelsI x nlevelsF x maxdeg x maxdeg]
+ * (first index fastest), the table gain_compute_half_linestrength_rotsym
+ * takes.  rot may be NULL (sizes only). */
+int gain_rotsym_wigner(const char* group, int jI, int jF, int* nlevelsI,
+                       int* nlevelsF, int* maxdeg, double* rot);
 /* cublasDdot(handle, n, x, 1, y, 1, result) in DEVICE pointer mode,
  * GpuManager.cpp:741 */
 int gain_ddot(int n, const double* x, const double* y, double* result,

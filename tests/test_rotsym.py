@@ -1,0 +1,221 @@
+"""Symmetric-top (ROTSYM) rotational factors: the symmetrised rotational
+functions and the contracted dipole table of the library
+(gain_rotsym_functions / gain_rotsym_wigner, restating c_wignerGen,
+trove_wrappers.f90:170-914, with the group tables of symmetry.f90), checked
+on the CPU against what the mathematics fixes:
+
+* the rotational levels of a symmetric top fall into the irreps the
+  textbook K rules give (C3v(M), D3h(M)), level by level in count;
+* the functions are orthonormal and every group operation acts on the
+  components of a level by the irrep's character (trace of each block);
+* with no symmetry the table is the K-block rotational factor of the
+  oracle (pinned against the Fortran textbook form in test_oracle.py);
+* in any group, the ROTSYM half line strength (oracle of
+  dipole_GPU.cu:641-723) with this table equals the K-block half line
+  strength (oracle of dipole_GPU.cu:392-515) carried into the symmetrised
+  rotational basis on both sides.
+No GPU is used."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from gain_engine import _lib, synth
+from oracle import oracle as orc
+
+GROUPS = ["C(M)", "CS(M)", "C2V(M)", "C3V(M)", "D3H(M)"]
+NOPER = {"C(M)": 1, "CS(M)": 2, "C2V(M)": 4, "C3V(M)": 6, "D3H(M)": 12}
+
+
+def functions(group, J):
+    L = _lib.load()
+    nl, md = C.c_int(), C.c_int()
+    _lib.check(L.gain_rotsym_functions(group.encode(), J, C.byref(nl),
+                                       C.byref(md), None, None, None, None))
+    n = 2 * J + 1
+    irr = np.zeros(nl.value, np.int32)
+    root = np.zeros(nl.value * md.value, np.int32)
+    vec = np.zeros(n * n)
+    ops = np.zeros(NOPER[group] * n * n)
+    _lib.check(L.gain_rotsym_functions(
+        group.encode(), J, C.byref(nl), C.byref(md), irr.ctypes.data,
+        root.ctypes.data, vec.ctypes.data, ops.ctypes.data))
+    return (nl.value, md.value, irr, root.reshape(nl.value, md.value),
+            vec.reshape(n, n), ops.reshape(NOPER[group], n, n))
+
+
+def wigner(group, jI, jF):
+    L = _lib.load()
+    a, b, m = C.c_int(), C.c_int(), C.c_int()
+    _lib.check(L.gain_rotsym_wigner(group.encode(), jI, jF, C.byref(a),
+                                    C.byref(b), C.byref(m), None))
+    rot = np.zeros(3 * a.value * b.value * m.value * m.value)
+    _lib.check(L.gain_rotsym_wigner(group.encode(), jI, jF, C.byref(a),
+                                    C.byref(b), C.byref(m), rot.ctypes.data))
+    # first index fastest: [degF][degI][levelF][levelI][n]
+    return a.value, b.value, m.value, rot.reshape(m.value, m.value, b.value,
+                                                  a.value, 3)
+
+
+def textbook_levels(group, J):
+    """Irrep counts of the 2J+1 rotational functions of a symmetric top by
+    the K rules of the molecular symmetry group (Bunker & Jensen)."""
+    if group == "C3V(M)":  # A1, A2, E
+        c = [0, 0, 0]
+        c[0 if J % 2 == 0 else 1] += 1
+        for k in range(1, J + 1):
+            if k % 3 == 0:
+                c[0] += 1
+                c[1] += 1
+            else:
+                c[2] += 1
+        return c
+    if group == "D3H(M)":  # A1', A2', E', A1", A2", E"
+        c = [0] * 6
+        c[0 if J % 2 == 0 else 1] += 1
+        for k in range(1, J + 1):
+            r = k % 6
+            if r == 0:
+                c[0] += 1
+                c[1] += 1
+            elif r in (1, 5):
+                c[5] += 1
+            elif r in (2, 4):
+                c[2] += 1
+            else:
+                c[3] += 1
+                c[4] += 1
+        return c
+    raise ValueError(group)
+
+
+@pytest.mark.parametrize("group", ["C3V(M)", "D3H(M)"])
+@pytest.mark.parametrize("J", [0, 1, 2, 3, 4, 5, 6, 7, 9, 12])
+def test_levels_follow_the_symmetric_top_k_rules(group, J):
+    nl, md, irr, root, vec, ops = functions(group, J)
+    want = textbook_levels(group, J)
+    got = np.bincount(irr, minlength=len(want)).tolist()
+    assert got == want
+    degen = [1, 1, 2] if group == "C3V(M)" else [1, 1, 2, 1, 1, 2]
+    assert sum(degen[s] for s in irr) == 2 * J + 1
+
+
+# character tables [irrep][operation] with operations class by class
+CHARS = {
+    "C(M)": [[1]],
+    "CS(M)": [[1, 1], [1, -1]],
+    "C2V(M)": [[1, 1, 1, 1], [1, 1, -1, -1], [1, -1, -1, 1], [1, -1, 1, -1]],
+    "C3V(M)": [[1, 1, 1, 1, 1, 1], [1, 1, 1, -1, -1, -1],
+               [2, -1, -1, 0, 0, 0]],
+    "D3H(M)": [[1] * 12, [1, 1, 1, -1, -1, -1, 1, 1, 1, -1, -1, -1],
+               [2, -1, -1, 0, 0, 0, 2, -1, -1, 0, 0, 0],
+               [1, 1, 1, 1, 1, 1, -1, -1, -1, -1, -1, -1],
+               [1, 1, 1, -1, -1, -1, -1, -1, -1, 1, 1, 1],
+               [2, -1, -1, 0, 0, 0, -2, 1, 1, 0, 0, 0]],
+}
+
+
+@pytest.mark.parametrize("group", GROUPS)
+@pytest.mark.parametrize("J", [0, 1, 2, 3, 5, 8])
+def test_functions_orthonormal_and_blocks_carry_the_irrep(group, J):
+    nl, md, irr, root, vec, ops = functions(group, J)
+    n = 2 * J + 1
+    np.testing.assert_allclose(vec.T @ vec, np.eye(n), atol=1e-13)
+    for op in range(NOPER[group]):
+        t = ops[op]
+        np.testing.assert_allclose(t.T @ t, np.eye(n), atol=1e-12)
+        q = vec.T @ t @ vec  # operation in the symmetrised basis
+        blocks = [[r for r in root[l] if r >= 0] for l in range(nl)]
+        mask = np.zeros((n, n), bool)
+        for l, cols in enumerate(blocks):
+            sub = q[np.ix_(cols, cols)]
+            assert abs(np.trace(sub) - CHARS[group][irr[l]][op]) < 1e-10
+            mask[np.ix_(cols, cols)] = True
+        assert np.abs(q[~mask]).max(initial=0.0) < 1e-10
+
+
+def wang_index(b):
+    """Wang function (k = 0; then k, tau = 0, 1) of every K-block primitive."""
+    k = np.repeat(np.arange(b.J + 1), b.kblock)
+    return np.where(k == 0, 0, 2 * k - 1 + (np.asarray(b.tau) & 1))
+
+
+@pytest.mark.parametrize("jI,jF", [(0, 1), (1, 1), (2, 3), (3, 2), (4, 4)])
+def test_no_symmetry_table_is_the_kblock_factor(jI, jF):
+    nlI, nlF, md, rot = wigner("C(M)", jI, jF)
+    assert (nlI, nlF, md) == (2 * jI + 1, 2 * jF + 1, 1)
+    jmax = max(jI, jF)
+    tab = orc.threej_table(jmax)
+    kt = lambda J: [(0, J % 2)] + [(k, t) for k in range(1, J + 1)
+                                   for t in (0, 1)]
+    for a, (kI, tI) in enumerate(kt(jI)):
+        for b, (kF, tF) in enumerate(kt(jF)):
+            v, comp = orc.rot_coeff(jmax, tab, jI, jF, kI, tI, kF, tF)
+            want = np.zeros(3)
+            if comp >= 0:
+                # the oracle carries the kernel's truncated 1/sqrt(2)
+                s = 1.0 / 0.70710678118 / math.sqrt(2.0)
+                want[comp] = v * (s if (kI != kF and kI * kF != 0) else 1.0)
+            np.testing.assert_allclose(rot[0, 0, b, a], want, rtol=0,
+                                       atol=1e-14)
+
+
+@pytest.mark.parametrize("group", GROUPS)
+@pytest.mark.parametrize("jI,jF", [(1, 2), (2, 2), (3, 2), (4, 5)])
+def test_rotsym_half_ls_is_the_kblock_one_in_the_symmetrised_basis(
+        group, jI, jF):
+    rng = np.random.default_rng(7 * jI + jF)
+    nc = 5
+    prob = synth.make_problem([jI, jF] if jI != jF else [jI], nc, "c1")
+    bI = prob.bases[0]
+    bF = prob.bases[-1]
+    jmax = max(jI, jF)
+    tab = orc.threej_table(jmax)
+    x_kb = rng.standard_normal(bI.nprim)
+    h_kb = orc.half_ls(prob, bI, bF, x_kb, tab)
+    fI = functions(group, jI)
+    fF = functions(group, jF)
+    nlI, nlF, md, rot = wigner(group, jI, jF)
+
+    def rotsym_basis(f, b):
+        nl, md_, irr, root, vec, _ = f
+        levels, degs, vibs, roots = [], [], [], []
+        for v in range(nc):
+            for l in range(nl):
+                for d in range(md_):
+                    if root[l, d] >= 0:
+                        levels.append(l)
+                        degs.append(d)
+                        vibs.append(v)
+                        roots.append(root[l, d])
+        # primitive (v, Wang function) -> position in the K-block basis
+        pos = {(int(v), int(w)): i for i, (v, w) in
+               enumerate(zip(b.vib, wang_index(b)))}
+        return (np.array(levels), np.array(degs), np.array(vibs),
+                np.array(roots), vec, pos)
+
+    lvI, dgI, vbI, rtI, EI, posI = rotsym_basis(fI, bI)
+    lvF, dgF, vbF, rtF, EF, posF = rotsym_basis(fF, bF)
+    nI, nF = 2 * jI + 1, 2 * jF + 1
+    # x in the symmetrised basis: x_rs[v, p] = sum_w E_I[w, p] x_kb[v, w]
+    x_rs = np.array([sum(EI[w, p] * x_kb[posI[(v, w)]] for w in range(nI))
+                     for v, p in zip(vbI, rtI)])
+    h_rs = orc.half_ls_rotsym(nlI, nlF, md, dgI, dgF, lvI, lvF, vbI, vbF, 0,
+                              nc, prob.mu, nc, rot, x_rs)
+    want = np.array([sum(EF[w, q] * h_kb[posF[(v, w)]] for w in range(nF))
+                     for v, q in zip(vbF, rtF)])
+    scale = np.abs(want).max()
+    # the K-block oracle uses the kernel's truncated 1/sqrt(2)
+    # (relative difference 9.3e-12)
+    np.testing.assert_allclose(h_rs, want, rtol=0, atol=1e-10 * scale)
+
+
+def test_errors():
+    L = _lib.load()
+    a, b, m = C.c_int(), C.c_int(), C.c_int()
+    assert L.gain_rotsym_wigner(b"C2V(M)", 2, 4, C.byref(a), C.byref(b),
+                                C.byref(m), None) == _lib.ERR_ARG
+    assert L.gain_rotsym_wigner(b"TD(X)", 2, 2, C.byref(a), C.byref(b),
+                                C.byref(m), None) == _lib.ERR_ARG
+    assert b"group" in L.gain_last_error()
