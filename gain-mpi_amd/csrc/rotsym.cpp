@@ -3,7 +3,7 @@
 // the Wigner table that compute_gpu_half_linestrength_rotsym_
 // (dipole_GPU.cu:641-723) consumes and that the reference builds in Fortran
 // (c_wignerGen, trove_wrappers.f90:170-414; MLrotsymmetry_generate
-// :416-589; MLsymmetrization_rot :591-914; group tables symmetry.f90;
+// :416-589; MLsymmetrization_rot :591-914; group tables symmetry.f90:58-745;
 // Wigner D function fort_func.f90:150-256).  Host code, long double.
 #include <algorithm>
 #include <array>
@@ -125,6 +125,62 @@ bool make_group(const std::string& name, Group& g) {
         M2{-1, 0, 0, -1}, M2{a, b, -b, a},   M2{a, -b, b, a},
         M2{1, 0, 0, -1},  M2{-a, -b, -b, a}, M2{-a, b, b, a}};
     finish(g, {e1, e2});
+    return true;
+  }
+  if (name == "TD(M)" || name == "TD") {  // symmetry.f90:539-745
+    g.nrep = 5, g.noper = 24, g.nclass = 5;
+    g.degen = {1, 1, 2, 3, 3}, g.nelem = {1, 8, 3, 6, 6};
+    g.chars = {{1, 1, 1, 1, 1}, {1, 1, 1, -1, -1}, {2, -1, 2, 0, 0},
+               {3, 0, -1, 1, -1}, {3, 0, -1, -1, 1}};
+    const ld p2 = 0.5L * pi, p3 = 1.5L * pi;
+    g.euler = {{o, o, o},    {p3, p2, o},  {pi, p2, p3}, {o, p2, p3},
+               {p3, p2, pi}, {p2, p2, o},  {pi, p2, p2}, {o, p2, p2},
+               {p2, p2, pi}, {p2, pi, p3}, {o, pi, o},   {pi, o, o},
+               {o, p3, o},   {o, p2, o},   {p2, o, o},   {p3, o, o},
+               {p2, p3, p3}, {p2, p2, p3}, {p2, p2, p2}, {o, p2, pi},
+               {o, pi, p2},  {pi, pi, p2}, {pi, p2, o},  {p3, p2, p3}};
+    g.class_of.clear();
+    for (int c = 0; c < g.nclass; ++c)
+      for (int e = 0; e < g.nelem[c]; ++e) g.class_of.push_back(c);
+    g.maxdeg = 3;
+    g.rep.assign(g.nrep, std::vector<std::vector<ld>>(g.noper));
+    for (int r = 0; r < 2; ++r)
+      for (int op = 0; op < g.noper; ++op)
+        g.rep[r][op] = {g.chars[r][g.class_of[op]]};
+    // degenerate irreps from the generators (123) (operation 2) and (14)*
+    // (operation 21), rows of the matrices, and the products of
+    // symmetry.f90:714-737
+    typedef std::vector<ld> Mx;
+    auto mul = [](const Mx& x, const Mx& y, int d) {
+      Mx z((size_t)d * d, 0.0L);
+      for (int i = 0; i < d; ++i)
+        for (int k = 0; k < d; ++k)
+          for (int j = 0; j < d; ++j) z[i * d + j] += x[i * d + k] * y[k * d + j];
+      return z;
+    };
+    const Mx gen2[3] = {{-a, b, -b, -a},
+                        {0, 1, 0, 0, 0, -1, -1, 0, 0},
+                        {0, -1, 0, 0, 0, -1, 1, 0, 0}};
+    const Mx gen21[3] = {{-a, b, b, a},
+                         {0, 1, 0, 1, 0, 0, 0, 0, -1},
+                         {0, 0, -1, 0, 1, 0, -1, 0, 0}};
+    for (int r = 2; r < 5; ++r) {
+      const int d = g.degen[r];
+      std::vector<Mx>& m = g.rep[r];
+      m[0].assign((size_t)d * d, 0.0L);
+      for (int i = 0; i < d; ++i) m[0][i * d + i] = 1.0L;
+      m[1] = gen2[r - 2];
+      m[20] = gen21[r - 2];
+      // 1-based operation numbers: (target, left, right)
+      static const int seq[][3] = {
+          {3, 2, 2},    {13, 21, 2},  {8, 13, 21},  {5, 8, 3},
+          {4, 5, 5},    {6, 4, 3},    {7, 6, 6},    {9, 8, 8},
+          {10, 7, 2},   {11, 8, 2},   {12, 4, 2},   {14, 3, 21},
+          {15, 11, 21}, {16, 10, 21}, {17, 9, 21},  {18, 2, 21},
+          {19, 5, 21},  {20, 7, 21},  {22, 12, 21}, {23, 4, 21},
+          {24, 6, 21}};
+      for (const auto& q : seq) m[q[0] - 1] = mul(m[q[1] - 1], m[q[2] - 1], d);
+    }
     return true;
   }
   return false;
@@ -383,7 +439,8 @@ const Group& group_of(const char* name) {
     Group t;
     if (!make_group(s, t))
       fail(GAIN_ERR_ARG,
-           "symmetry group \"%s\": C(M), CS(M), C2V(M), C3V(M), D3H(M)",
+           "symmetry group \"%s\": C(M), CS(M), C2V(M), C3V(M), D3H(M), "
+           "TD(M)",
            name);
     g = t;
     last = s;

@@ -361,7 +361,8 @@ int gain_compute_half_linestrength_rotsym(
     int ncontrF, const double* dipole_me, const double* wigner,
     const double* vector, double* half_ls, void* stream);
 /* ---- symmetric-top (ROTSYM) rotational factors ---------------------------
- * Groups: "C(M)", "CS(M)", "C2V(M)", "C3V(M)", "D3H(M)" (symmetry.f90:58-538).
+ * Groups: "C(M)", "CS(M)", "C2V(M)", "C3V(M)", "D3H(M)", "TD(M)"
+ * (symmetry.f90:58-745).
  *
  * Symmetrised rotational functions of one J (MLrotsymmetry_generate,
  * trove_wrappers.f90:416-589, with MLsymmetrization_rot :591-914): the

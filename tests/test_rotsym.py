@@ -24,8 +24,16 @@ import pytest
 from gain_engine import _lib, synth
 from oracle import oracle as orc
 
-GROUPS = ["C(M)", "CS(M)", "C2V(M)", "C3V(M)", "D3H(M)"]
-NOPER = {"C(M)": 1, "CS(M)": 2, "C2V(M)": 4, "C3V(M)": 6, "D3H(M)": 12}
+GROUPS = ["C(M)", "CS(M)", "C2V(M)", "C3V(M)", "D3H(M)", "TD(M)"]
+NOPER = {"C(M)": 1, "CS(M)": 2, "C2V(M)": 4, "C3V(M)": 6, "D3H(M)": 12,
+         "TD(M)": 24}
+# Td(M): classes E, (123), (12)(34), (1423)*, (12)*; sizes; the rotation
+# angle of the equivalent rotation of each class; characters of
+# A1, A2, E, F1, F2
+TD_SIZES = [1, 8, 3, 6, 6]
+TD_ANGLE = [0.0, 2 * math.pi / 3, math.pi, math.pi / 2, math.pi]
+TD_CHARS = [[1, 1, 1, 1, 1], [1, 1, 1, -1, -1], [2, -1, 2, 0, 0],
+            [3, 0, -1, 1, -1], [3, 0, -1, -1, 1]]
 
 
 def functions(group, J):
@@ -90,6 +98,22 @@ def textbook_levels(group, J):
     raise ValueError(group)
 
 
+@pytest.mark.parametrize("J", [0, 1, 2, 3, 4, 5, 6, 7, 8, 10, 13])
+def test_td_levels_reduce_the_rotation_group_representation(J):
+    """A spherical top's 2J+1 functions span D^(J); its reduction over the
+    rotations equivalent to the Td(M) operations (character of a rotation by
+    a: sin((J + 1/2) a) / sin(a/2)) fixes the irrep counts."""
+    def chi(a):
+        return 2 * J + 1 if a == 0.0 else \
+            math.sin((J + 0.5) * a) / math.sin(0.5 * a)
+    want = [round(sum(n * c[k] * chi(a) for k, (n, a) in
+                      enumerate(zip(TD_SIZES, TD_ANGLE))) / 24)
+            for c in TD_CHARS]
+    nl, md, irr, root, vec, ops = functions("TD(M)", J)
+    assert md == 3
+    assert np.bincount(irr, minlength=5).tolist() == want
+
+
 @pytest.mark.parametrize("group", ["C3V(M)", "D3H(M)"])
 @pytest.mark.parametrize("J", [0, 1, 2, 3, 4, 5, 6, 7, 9, 12])
 def test_levels_follow_the_symmetric_top_k_rules(group, J):
@@ -113,6 +137,8 @@ CHARS = {
                [1, 1, 1, 1, 1, 1, -1, -1, -1, -1, -1, -1],
                [1, 1, 1, -1, -1, -1, -1, -1, -1, 1, 1, 1],
                [2, -1, -1, 0, 0, 0, -2, 1, 1, 0, 0, 0]],
+    "TD(M)": [sum(([c[k]] * n for k, n in enumerate(TD_SIZES)), [])
+              for c in TD_CHARS],
 }
 
 
@@ -216,6 +242,6 @@ def test_errors():
     a, b, m = C.c_int(), C.c_int(), C.c_int()
     assert L.gain_rotsym_wigner(b"C2V(M)", 2, 4, C.byref(a), C.byref(b),
                                 C.byref(m), None) == _lib.ERR_ARG
-    assert L.gain_rotsym_wigner(b"TD(X)", 2, 2, C.byref(a), C.byref(b),
+    assert L.gain_rotsym_wigner(b"OH(M)", 2, 2, C.byref(a), C.byref(b),
                                 C.byref(m), None) == _lib.ERR_ARG
     assert b"group" in L.gain_last_error()
