@@ -110,13 +110,16 @@ __global__ __launch_bounds__(256) void k_half_ls_kblock(
 }
 
 // Half line strength in the symmetric-top (ROTSYM) basis with the
-// reference's argument meaning (dipole_GPU.cu:641-723): one thread per
-// final primitive; initial primitives are walked in 256-wide tiles whose
-// coefficient, dipole row offset and Wigner offset (level and degeneracy
-// part) are staged in LDS.  The 3-component Wigner factor is indexed
-// n + 3 lvI + 3 nlvI lvF + 3 nlvI nlvF dgI + 3 nlvI nlvF maxdeg dgF and the
-// dipole components lie ncontrF * dip_stride apart.  Only entries whose
-// vibrational index lies in [startFblock, endFblock) are written.
+// reference's argument meaning (dipole_GPU.cu:641-723).  A workgroup takes
+// 64 final primitives; its four waves split every 256-wide tile of initial
+// primitives (coefficient, dipole row and Wigner offset staged in LDS) into
+// four 64-entry slices, and the four partial sums of each final primitive
+// are added in LDS in a fixed order, so each entry is still assigned once
+// (and a final block fills four times as many workgroups).  The
+// 3-component Wigner factor is indexed n + 3 lvI + 3 nlvI lvF + 3 nlvI nlvF
+// dgI + 3 nlvI nlvF maxdeg dgF and the dipole components lie
+// ncontrF * dip_stride apart.  Only entries whose vibrational index lies in
+// [startFblock, endFblock) are written.
 __global__ __launch_bounds__(256) void k_half_ls_rotsym(
     int dimenF, int dimenI, int nlevelI, int nlevelF, int maxdeg,
     const int* __restrict__ degI, const int* __restrict__ degF,
@@ -129,8 +132,10 @@ __global__ __launch_bounds__(256) void k_half_ls_rotsym(
   __shared__ double s_coef[256];
   __shared__ int64_t s_row[256];
   __shared__ int64_t s_w[256];
+  __shared__ double s_part[4][64];
   const int t = threadIdx.x;
-  const int rF = blockIdx.x * blockDim.x + t;
+  const int f = t & 63, sl = t >> 6;  // final primitive, slice (= wave)
+  const int rF = blockIdx.x * 64 + f;
   const bool inblk = rF < dimenF;
   int vF = inblk ? vibF[rF] : startFblock - 1;
   const bool valid = inblk && vF >= startFblock && vF < endFblock;
@@ -138,8 +143,7 @@ __global__ __launch_bounds__(256) void k_half_ls_rotsym(
   const int64_t comp = (int64_t)ncontrF * dip_stride;
   const int64_t sF = 3 * (int64_t)nlevelI, sdI = sF * nlevelF,
                 sdF = sdI * maxdeg;
-  const int64_t wF =
-      valid ? sF * levelF[rF] + sdF * degF[rF] : 0;
+  const int64_t wF = valid ? sF * levelF[rF] + sdF * degF[rF] : 0;
   const double* dF = dip + (valid ? vF : 0);
   double acc = 0.0;
   for (int b0 = 0; b0 < dimenI; b0 += 256) {
@@ -152,18 +156,22 @@ __global__ __launch_bounds__(256) void k_half_ls_rotsym(
     }
     __syncthreads();
     if (valid) {
-      const int n = min(256, dimenI - b0);
-      for (int j = 0; j < n; ++j) {
+      const int j1 = min(64 * sl + 64, dimenI - b0);
+      for (int j = 64 * sl; j < j1; ++j) {
         const double* w = wigner + wF + s_w[j];
         const double* d = dF + s_row[j];
-        double f = w[0] * d[0];
-        f = fma(w[1], d[comp], f);
-        f = fma(w[2], d[2 * comp], f);
-        acc = fma(s_coef[j], f, acc);
+        double v = w[0] * d[0];
+        v = fma(w[1], d[comp], v);
+        v = fma(w[2], d[2 * comp], v);
+        acc = fma(s_coef[j], v, acc);
       }
     }
   }
-  if (valid) half_ls[rF] = acc;
+  s_part[sl][f] = acc;
+  __syncthreads();
+  if (sl == 0 && valid)
+    half_ls[rF] = ((s_part[0][f] + s_part[1][f]) + s_part[2][f]) +
+                  s_part[3][f];
 }
 
 // Deterministic single-workgroup dot product (fixed reduction order).
@@ -287,7 +295,7 @@ extern "C" int gain_compute_half_linestrength_rotsym(
                        dipole_me && wigner && vector && half_ls,
                    "compute_half_linestrength_rotsym: null pointer");
     if (dimenF == 0) return;
-    hipLaunchKernelGGL(k_half_ls_rotsym, dim3((dimenF + 255) / 256),
+    hipLaunchKernelGGL(k_half_ls_rotsym, dim3((dimenF + 63) / 64),
                        dim3(256), 0, (hipStream_t)stream, dimenF, dimenI,
                        nlevelI, nlevelF, sym_max_degen, kI, kF, ktauI, ktauF,
                        icorrI, icorrF, startFblock, endFblock, ncontrF,
