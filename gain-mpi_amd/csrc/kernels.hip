@@ -155,14 +155,80 @@ __device__ inline void gather_body(const GatherArgs& a, int bx, int b) {
   a.H[(int64_t)b * a.hstride + r] = neg ? -acc : acc;
 }
 
+// Same gather for NB consecutive vectors per thread: the primitive's K, tau
+// and vibrational index and the rotational factors are formed once, and the
+// up to 6 NB loads of Y (K_I = K-1..K+1, tau_I = 0, 1, NB vectors) are all
+// independent, so a thread keeps many loads in flight.
+template <int NB>
 __global__ __launch_bounds__(256) void k_gather_h(GatherArgs a) {
-  gather_body(a, blockIdx.x, blockIdx.y);
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= a.nprimF) return;
+  const int b0 = blockIdx.y * NB;
+  const int K = a.kvalF[r];
+  const int tF = a.tauF[r] & 1;
+  const int v = a.vibF[r];
+  const int n1 = a.jmax + 1;
+  double coef[3][2];
+  int alpha[3][2], kcol[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int KI = K - 1 + q;
+    const bool in = KI >= 0 && KI <= a.jI;
+    kcol[q] = in ? KI : 0;
+    const int dk = K - KI;
+    const double f3j =
+        in ? a.threej[a.jI + KI * n1 + (a.jF - a.jI + 1) * n1 * n1 +
+                      (dk + 1) * n1 * n1 * 3]
+           : 0.0;
+    const double sq = (dk != 0 && KI * K != 0) ? kSqrt2Ref : 1.0;
+#pragma unroll
+    for (int tI = 0; tI < 2; ++tI) {
+      int al, m;
+      if (dk == 0) {
+        al = 2;
+        m = tF - tI;
+      } else if (tF != tI) {
+        al = 0;
+        m = (tF - tI) * dk;
+      } else {
+        al = 1;
+        m = -1;
+      }
+      alpha[q][tI] = al;
+      coef[q][tI] = f3j * sq * (double)m;
+    }
+  }
+  double acc[NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) acc[i] = 0.0;
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int tI = 0; tI < 2; ++tI) {
+      if (coef[q][tI] == 0.0) continue;
+      const double* yp = a.Y + alpha[q][tI] * a.ys_alpha +
+                         (2 * kcol[q] + tI) * (int64_t)a.nc + v;
+#pragma unroll
+      for (int i = 0; i < NB; ++i) {
+        const int b = min(b0 + i, a.nvec - 1);
+        const double* p = yp + (int64_t)b * a.ncolI * a.nc;
+        double y = 0.0;
+        for (int ks = 0; ks < a.ksplit; ++ks) y += p[ks * a.ys_ks];
+        acc[i] = fma(coef[q][tI], y, acc[i]);
+      }
+    }
+  const bool neg = ((K % 3) * tF) & 1;
+#pragma unroll
+  for (int i = 0; i < NB; ++i)
+    if (b0 + i < a.nvec)
+      a.H[(int64_t)(b0 + i) * a.hstride + r] = neg ? -acc[i] : acc[i];
 }
 
 void launch_gather_h(const GatherArgs& a, hipStream_t s) {
   if (a.nprimF == 0 || a.nvec == 0) return;
-  dim3 grid((a.nprimF + 255) / 256, a.nvec);
-  hipLaunchKernelGGL(k_gather_h, grid, dim3(256), 0, s, a);
+  constexpr int NB = 4;
+  dim3 grid((a.nprimF + 255) / 256, (a.nvec + NB - 1) / NB);
+  hipLaunchKernelGGL(k_gather_h<NB>, grid, dim3(256), 0, s, a);
   check_launch("gather_h");
 }
 
