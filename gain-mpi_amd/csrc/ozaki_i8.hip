@@ -592,44 +592,78 @@ __device__ inline float sym_mod_f(float x, float m, float minv) {
   return fmaf(-rintf(x * minv), m, x);
 }
 
-template <int NM>
-__global__ __launch_bounds__(256) void k_slice_crt(
-    const double* __restrict__ src, int64_t ld, int rows, int K, int64_t Kp,
-    int64_t plane, int8_t* __restrict__ planes, int* __restrict__ expo,
-    int beta, CrtConsts cc) {
+// Exponent of every padded row: |x| < 2^e over the row (0 for a zero or
+// padding row).  One workgroup per row.
+__global__ __launch_bounds__(256) void k_row_expo(const double* __restrict__ src,
+                                                  int64_t ld, int rows, int K,
+                                                  int* __restrict__ expo) {
   __shared__ double red[4];
   const int r = blockIdx.x;
   const int tid = threadIdx.x;
-  const bool live = r < rows;
-  const double* x = src + (live ? (int64_t)r * ld : 0);
   double mx = 0.0;
-  if (live)
-    for (int k = tid; k < K; k += 256) mx = fmax(mx, fabs(x[k]));
+  if (r < rows) {
+    const double* x = src + (int64_t)r * ld;
+    double m2 = 0.0;
+    int k = tid;
+    for (; k + 256 < K; k += 512) {
+      mx = fmax(mx, fabs(x[k]));
+      m2 = fmax(m2, fabs(x[k + 256]));
+    }
+    if (k < K) mx = fmax(mx, fabs(x[k]));
+    mx = fmax(mx, m2);
+  }
   for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
   if ((tid & 63) == 0) red[tid >> 6] = mx;
   __syncthreads();
-  mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-  int e = 0;
-  if (mx > 0.0) frexp(mx, &e);  // |x| < 2^e
-  if (tid == 0) expo[r] = e;
-  int8_t* rowbase = planes + (int64_t)(r >> 5) * (Kp << 5) + (r & 31) * 32;
-  for (int64_t k0 = 4 * tid; k0 < Kp; k0 += 1024) {
+  if (tid == 0) {
+    mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    int e = 0;
+    if (mx > 0.0) frexp(mx, &e);  // |x| < 2^e
+    expo[r] = e;
+  }
+}
+
+// Residues of the beta-bit integers rint(x 2^{beta-1-e}) of a tile row
+// (32 rows) over kKch k blocks: thread t converts row t / 8, columns
+// 4 (t % 8) .. + 3 of each 32-column block, and every wave writes 256
+// contiguous bytes of each residue plane (the workgroup one whole 1 KB
+// block).  Rows past `rows` (padding) and columns past K are zero.
+constexpr int kSliceKch = 16;  // k blocks per workgroup
+template <int NM>
+__global__ __launch_bounds__(256) void k_slice_crt(
+    const double* __restrict__ src, int64_t ld, int rows, int K, int64_t Kp,
+    int64_t plane, int8_t* __restrict__ planes,
+    const int* __restrict__ expo, int beta, CrtConsts cc) {
+  const int t = threadIdx.x;
+  const int rr = t >> 3, kk = 4 * (t & 7);
+  const int r = blockIdx.x * 32 + rr;
+  const bool live = r < rows;
+  const int sh = live ? beta - 1 - expo[r] : 0;
+  const double* x = src + (live ? (int64_t)r * ld : 0);
+  const int64_t nkb = Kp >> 5;
+  const int64_t kb0 = (int64_t)blockIdx.y * kSliceKch;
+  const int64_t kb1 = min(kb0 + kSliceKch, nkb);
+  int8_t* tile = planes + (int64_t)blockIdx.x * (Kp << 5) + rr * 32 + kk;
+  for (int64_t kb = kb0; kb < kb1; ++kb) {
+    const int64_t k0 = kb * 32 + kk;
+    double xv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      xv[q] = (live && k0 + q < K) ? x[k0 + q] : 0.0;
     uint32_t w[NM];
 #pragma unroll
     for (int i = 0; i < NM; ++i) w[i] = 0u;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const int64_t k = k0 + q;
-      // |X| <= 2^{beta-1} <= 2^51: exact in double
-      const double X =
-          (live && k < K) ? rint(ldexp(x[k], beta - 1 - e)) : 0.0;
+      // |X| <= 2^{beta-1} <= 2^49: exact in double
+      const double X = rint(ldexp(xv[q], sh));
 #pragma unroll
       for (int i = 0; i < NM; ++i) {
         const double v = sym_mod_d(X, (double)cc.m[i], cc.minv_d[i]);
         w[i] |= (uint32_t)(uint8_t)(int8_t)(int)v << (8 * q);
       }
     }
-    int8_t* dst = rowbase + (k0 >> 5) * 1024 + (k0 & 31);
+    int8_t* dst = tile + kb * 1024;
 #pragma unroll
     for (int i = 0; i < NM; ++i) *(uint32_t*)(dst + i * plane) = w[i];
   }
@@ -644,13 +678,18 @@ void slice_rows_crt(int nm, int beta, const double* src, int64_t ld,
                  (long long)plane, (long long)Rp, (long long)Kp);
   GAIN_CHECK_ARG(((uintptr_t)planes & 3) == 0 && (plane & 3) == 0,
                  "slice_rows_crt: planes must be 4-byte aligned");
-  GAIN_CHECK_ARG(beta >= 9 && beta <= 53, "slice_rows_crt: beta %d", beta);
+  GAIN_CHECK_ARG(beta >= 9 && beta <= 50, "slice_rows_crt: beta %d", beta);
   const CrtConsts& cc = crt_consts(nm);
+  hipLaunchKernelGGL(k_row_expo, dim3((unsigned)Rp), dim3(256), 0, s, src,
+                     ld, rows, K, expo);
+  check_launch("row_expo");
+  const dim3 grid((unsigned)(Rp / 32),
+                  (unsigned)((Kp / 32 + kSliceKch - 1) / kSliceKch));
   switch (nm) {
 #define GAIN_CRT_SLICE(NN)                                                   \
   case NN:                                                                   \
-    hipLaunchKernelGGL(k_slice_crt<NN>, dim3((unsigned)Rp), dim3(256), 0, s, \
-                       src, ld, rows, K, Kp, plane, planes, expo, beta, cc); \
+    hipLaunchKernelGGL(k_slice_crt<NN>, grid, dim3(256), 0, s, src, ld,      \
+                       rows, K, Kp, plane, planes, expo, beta, cc);          \
     break;
     GAIN_CRT_SLICE(11)
     GAIN_CRT_SLICE(12)
