@@ -737,10 +737,40 @@ const char* crt_config_name(int cfg) {
   return cfg >= 0 && cfg < kCrtConfigs ? names[cfg] : "?";
 }
 
+// Residues of one 32 x 32 accumulator tile into the residue layout
+// [row block][column][32 rows] (int8): lane l holds column l % 32, rows
+// 8 w + 4 (l / 32) + (0..3) in word w; lanes l and l ^ 32 trade words so
+// that each stores two runs of 8 rows (0-7 and 16-23 for l < 32, 8-15 and
+// 24-31 otherwise), and every store instruction writes 32 whole 32-byte
+// column segments.  `col` is the column's segment of the tile's row block.
+__device__ inline void store_residue_tile(const i16v& acc, double m,
+                                          double minv, int8_t* col,
+                                          bool live, int lane) {
+  uint32_t wd[4];
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    uint32_t word = 0u;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double r = sym_mod_d((double)acc[4 * w + q], m, minv);
+      word |= (uint32_t)(uint8_t)(int8_t)(int)r << (8 * q);
+    }
+    wd[w] = word;
+  }
+  const int hi = lane >> 5;
+  const uint32_t o0 = __shfl_xor(hi ? wd[0] : wd[1], 32);
+  const uint32_t o1 = __shfl_xor(hi ? wd[2] : wd[3], 32);
+  if (!live) return;
+  const uint2 s0 = hi ? make_uint2(o0, wd[1]) : make_uint2(wd[0], o0);
+  const uint2 s1 = hi ? make_uint2(o1, wd[3]) : make_uint2(wd[2], o1);
+  *(uint2*)(col + 8 * hi) = s0;
+  *(uint2*)(col + 16 + 8 * hi) = s1;
+}
+
 // Per-modulus GEMM: workgroup z in [0, nz * nm) takes modulus z % nm of
 // operand matrix z / nm; the int32 products of the residue planes are
-// reduced modulo m and stored as int8 residues R[z][n][r] (r = A row
-// relative to the first row block; 4 consecutive rows per 32-bit store).
+// reduced modulo m and stored as int8 residues R[z][row block][n][32]
+// (rows relative to the first row block; store_residue_tile).
 template <int TM, int TN, int WM, int WN>
 __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res(
     I8GemmArgs g, CrtConsts cc, int ntm, int ntn, int rb0, int nkb,
@@ -798,20 +828,11 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res(
 #pragma unroll
   for (int b = 0; b < TN; ++b) {
     const int n = (brb + b) * 32 + (lane & 31);
-    if (n >= g.N) continue;
 #pragma unroll
     for (int a = 0; a < TM; ++a)
-#pragma unroll
-      for (int w = 0; w < 4; ++w) {
-        uint32_t word = 0u;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const double r = sym_mod_d((double)acc[a][b][4 * w + q], m, minv);
-          word |= (uint32_t)(uint8_t)(int8_t)(int)r << (8 * q);
-        }
-        const int rr = (arb + a - rb0) * 32 + 8 * w + 4 * (lane >> 5);
-        *(uint32_t*)(R + (int64_t)n * ldr + rr) = word;
-      }
+      store_residue_tile(acc[a][b], m, minv,
+                         R + ((int64_t)(arb + a - rb0) * nr + n) * 32,
+                         n < g.N, lane);
   }
 }
 
@@ -979,20 +1000,11 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res_lds(
 #pragma unroll
   for (int b = 0; b < TN; ++b) {
     const int n = (brb + b) * 32 + (lane & 31);
-    if (n >= g.N) continue;
 #pragma unroll
     for (int a = 0; a < TM; ++a)
-#pragma unroll
-      for (int w = 0; w < 4; ++w) {
-        uint32_t word = 0u;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const double r = sym_mod_d((double)acc[a][b][4 * w + q], m, minv);
-          word |= (uint32_t)(uint8_t)(int8_t)(int)r << (8 * q);
-        }
-        const int rr = (arb + a - rb0) * 32 + 8 * w + 4 * (lane >> 5);
-        *(uint32_t*)(R + (int64_t)n * ldr + rr) = word;
-      }
+      store_residue_tile(acc[a][b], m, minv,
+                         R + ((int64_t)(arb + a - rb0) * nr + n) * 32,
+                         n < g.N, lane);
   }
 }
 
@@ -1006,14 +1018,18 @@ template <int NM>
 __global__ __launch_bounds__(256) void k_crt_combine(I8GemmArgs g,
                                                      CrtConsts cc, int rb0,
                                                      int64_t ldr, int nr) {
-  const int n = blockIdx.y;
+  // workgroup: one residue row block x 32 columns; thread: column
+  // t / 8, rows 4 (t % 8) .. + 3, so a wave reads 256 contiguous bytes of
+  // each residue plane
+  const int t = threadIdx.x;
+  const int n = blockIdx.y * 32 + (t >> 3);
   const int za = blockIdx.z;
-  // residue row of this thread's first output, rounded down to 4
-  const int r0 = 4 * (blockIdx.x * 256 + threadIdx.x);
+  const int r0 = blockIdx.x * 32 + 4 * (t & 7);  // residue row, 4-aligned
   const int rlo = g.row0 - rb0 * 32;  // residue row of output 0
-  if (r0 >= rlo + g.M || r0 + 4 <= rlo) return;
-  const uint32_t* R = (const uint32_t*)(g.R + ((int64_t)za * NM * nr + n) *
-                                                  ldr + r0);
+  if (n >= g.N || r0 >= rlo + g.M || r0 + 4 <= rlo) return;
+  const uint32_t* R = (const uint32_t*)(g.R + (int64_t)za * NM * nr * ldr +
+                                        ((int64_t)blockIdx.x * nr + n) * 32 +
+                                        4 * (t & 7));
   const int64_t pst = (int64_t)nr * ldr / 4;
   uint32_t w[NM];
 #pragma unroll
@@ -1138,9 +1154,10 @@ void gemm_crt_i8(const I8GemmArgs& g, hipStream_t s) {
 #undef GAIN_CRT_LDS
   }
   check_launch("gemm_i8_res");
-  // residue rows [row0 - 32 rb0, row0 - 32 rb0 + M), four per thread
+  // residue rows [row0 - 32 rb0, row0 - 32 rb0 + M): one workgroup per
+  // row block and 32 columns
   const int rspan = g.row0 - L.rb0 * 32 + g.M;
-  const dim3 cgrid((unsigned)((rspan + 1023) / 1024), (unsigned)g.N,
+  const dim3 cgrid((unsigned)((rspan + 31) / 32), (unsigned)((g.N + 31) / 32),
                    (unsigned)g.nz);
   switch (g.S) {
 #define GAIN_CRT_COMB(NN)                                                    \
