@@ -143,27 +143,36 @@ def run_mixed_sweep(nreplay):
         if not bool(ok.all()):
             worst = max(worst, float((d / bound.clamp_min(1e-300)).max()))
     sw.close()
-    # the same sweep with split-INT8 x6 mu X products (projections FP64)
-    sw8 = Sweep(eng, full, precision=5)
-    sw8.run()
-    eng.synchronize()
-    e0.record(stream)
-    for _ in range(nreplay):
-        sw8.run()
-    e1.record(stream)
-    eng.synchronize()
-    sweep8_ms = e0.elapsed_time(e1) / nreplay
-    same8, err8 = True, 0.0
-    for o, q in zip(sw_out, pl_out):
-        same8 &= bool(torch.equal(o["flags"], q["flags"]))
-        R = q["S"]
-        strong = R > 1e-6 * R.max(dim=1, keepdim=True).values
-        if bool(strong.any()):
-            rel = ((o["S"] - R).abs() / R.clamp_min(1e-300))[strong]
-            err8 = max(err8, float(rel.max()))
-    sw8.close()
-    log(f"mixed-J sweep, split-INT8 x6 mu X: {sweep8_ms:.2f} ms per replay, "
-        f"flags identical {same8}, strong-line S max rel err {err8:.2e}")
+    # the same sweep with mu X products in a reduced-precision form
+    # (projections FP64): split-INT8 x6 and the CRT form with 12 moduli
+    def reduced(precision, name):
+        swr = Sweep(eng, full, precision=precision)
+        swr.run()
+        eng.synchronize()
+        e0.record(stream)
+        for _ in range(nreplay):
+            swr.run()
+        e1.record(stream)
+        eng.synchronize()
+        ms = e0.elapsed_time(e1) / nreplay
+        same, err = True, 0.0
+        for o, q in zip(sw_out, pl_out):
+            same &= bool(torch.equal(o["flags"], q["flags"]))
+            R = q["S"]
+            strong = R > 1e-6 * R.max(dim=1, keepdim=True).values
+            if bool(strong.any()):
+                rel = ((o["S"] - R).abs() / R.clamp_min(1e-300))[strong]
+                err = max(err, float(rel.max()))
+        swr.close()
+        log(f"mixed-J sweep, {name} mu X: {ms:.2f} ms per replay, flags "
+            f"identical {same}, strong-line S max rel err {err:.2e}")
+        return {"ms_per_sweep": ms, "value": trans / (ms / 1e3),
+                "transition_flags_identical": same,
+                "strong_max_rel_err_vs_plain_fp64": err,
+                "note": f"mu X products as {name}, projections FP64"}
+
+    red = {"split_int8_x6": reduced(5, "split-INT8 x6"),
+           "crt12": reduced(8, "int8 CRT (12 moduli)")}
     nI = len(w.initials)
     r = {"workload": "256 initial states drawn from J_I in [5,30] (4 irreps, "
                      "N_J=(2J+1)*750), Nc=3000, 2000 states per block, "
@@ -182,12 +191,7 @@ def run_mixed_sweep(nreplay):
          "s_within_fp64_bound": worst == 0.0,
          "note": "one hipGraph replay per sweep; the bound is |dS| <= "
                  "2e-12 S + 2e-14 sqrt(S S_max) per initial state",
-         "split_int8_x6": {"ms_per_sweep": sweep8_ms,
-                           "value": trans / (sweep8_ms / 1e3),
-                           "transition_flags_identical": same8,
-                           "strong_max_rel_err_vs_plain_fp64": err8,
-                           "note": "mu X products as split-INT8 x6, "
-                                   "projections FP64"}}
+         "split_int8_x6": red["split_int8_x6"], "crt12": red["crt12"]}
     log(f"mixed-J sweep: {len(tasks)} tasks, {trans} transitions, "
         f"{sweep_ms:.2f} ms per replay -> {r['value']:.3e}/s; unbatched "
         f"plain {plain_ms:.1f} ms; flags identical {same_flags}, S within "

@@ -87,9 +87,10 @@ struct gain_sweep {
   GemmArgs hl{};
   bool has_hl = false;
   int64_t ncol_all = 0;
-  // split-INT8 mu X (precision GAIN_PREC_I8X*): the sweep's own int8 copy
-  // of mu, X sliced at every replay
+  // split-INT8 (GAIN_PREC_I8X*) or CRT-form (GAIN_PREC_I8CRT*) mu X: the
+  // sweep's own int8 copy of mu, X cut into planes at every replay
   int slices = 0;
+  int crt = 0;  // moduli of the CRT form, 0 otherwise
   I8GemmArgs hl8{};
   int64_t x8_plane = 0;
   GemmArgs* d_pj_probs = nullptr;
@@ -197,7 +198,52 @@ void build(gain_sweep* s, const gain_task* tasks, int ntasks) {
     a.nz = 3;
     a.ksplit = 1;
     s->has_hl = true;
-    if (s->slices == 0) {
+    if (s->crt > 0) {
+      const int nm = s->crt;
+      double* muT = s->arena.alloc<double>((size_t)3 * nc * nc);
+      transpose_mu(c->d_mu, nc, muT, c->stream);
+      const int64_t Kp = i8_kpad(nc), Rm = i8_rows_pad(nc);
+      const int beta = crt_beta(nm, Kp);
+      int8_t* m8 = s->arena.alloc<int8_t>((size_t)nm * 3 * Rm * Kp);
+      int* me = s->arena.alloc<int>((size_t)3 * Rm);
+      for (int al = 0; al < 3; ++al)
+        slice_rows_crt(nm, beta, muT + (size_t)al * nc * nc, nc, nc, nc,
+                       m8 + al * Rm * Kp, 3 * Rm * Kp, me + al * Rm,
+                       c->stream);
+      GAIN_HIP(hipStreamSynchronize(c->stream));
+      s->arena.release(muT);
+      const int64_t Rx = i8_rows_pad(s->ncol_all);
+      s->x8_plane = Rx * Kp;
+      I8GemmArgs& g = s->hl8;
+      g.S = nm;
+      g.beta = beta;
+      g.M = nc;
+      g.N = (int)s->ncol_all;
+      g.row0 = 0;
+      g.Kp = Kp;
+      g.A = m8;
+      g.a_plane = 3 * Rm * Kp;
+      g.a_z = Rm * Kp;
+      g.eA = me;
+      g.ea_z = Rm;
+      g.B = s->arena.alloc<int8_t>((size_t)nm * s->x8_plane);
+      g.b_plane = s->x8_plane;
+      g.eB = s->arena.alloc<int>((size_t)Rx);
+      g.C = Y;
+      g.ldc = nc;
+      g.c_z = s->ncol_all * nc;
+      g.nz = 3;
+      int64_t rb = 0;  // residue buffer for every tile configuration
+      for (int cfg = 0; cfg < kCrtConfigs; ++cfg) {
+        g.cfg = cfg;
+        rb = std::max(rb, gemm_crt_res_bytes(g));
+      }
+      g.R = s->arena.alloc<int8_t>((size_t)rb);
+      // cut X once so the tuner times real operands
+      slice_rows_crt(nm, beta, X, nc, (int)s->ncol_all, nc, (int8_t*)g.B,
+                     s->x8_plane, (int*)g.eB, c->stream);
+      g.cfg = crt_cfg_of(c, 1, g);
+    } else if (s->slices == 0) {
       a.cfg = gemm_cfg(c, 1, a);  // tuned here: replays must not time
     } else {
       const int S = s->slices;
@@ -554,7 +600,13 @@ void run_stage(const std::vector<gain_sweep::Launch>& v, hipStream_t st) {
 // grouped projection GEMM and GEMVs, the epilogue of every task.
 void enqueue(gain_sweep* s, hipStream_t st) {
   run_stage(s->stage_scatter, st);
-  if (s->has_hl && s->slices == 0) gemm_f64(s->hl, st);
+  if (s->has_hl && s->slices == 0 && s->crt == 0) gemm_f64(s->hl, st);
+  if (s->has_hl && s->crt > 0) {
+    slice_rows_crt(s->crt, s->hl8.beta, s->hl.B, s->c->nc, (int)s->ncol_all,
+                   s->c->nc, (int8_t*)s->hl8.B, s->x8_plane,
+                   (int*)s->hl8.eB, st);
+    gemm_crt_i8(s->hl8, st);
+  }
   if (s->has_hl && s->slices > 0) {
     slice_rows_i8(s->slices, s->hl.B, s->c->nc, (int)s->ncol_all, s->c->nc,
                   (int8_t*)s->hl8.B, s->x8_plane, (int*)s->hl8.eB, st);
@@ -579,8 +631,10 @@ extern "C" int gain_sweep_create(gain_ctx* c, const gain_task* tasks,
     GAIN_CHECK_ARG(out && tasks && ntasks > 0, "bad sweep arguments");
     *out = nullptr;
     GAIN_CHECK_ARG(precision == GAIN_PREC_FP64 ||
-                       i8_slices_of(precision) > 0,
-                   "sweeps run FP64 or split-INT8 products (precision %d)",
+                       i8_slices_of(precision) > 0 ||
+                       crt_moduli_of(precision) > 0,
+                   "sweeps run FP64, split-INT8 or CRT-form products "
+                   "(precision %d)",
                    precision);
     if (!c->params_set) fail(GAIN_ERR_STATE, "gain_set_params not called");
     if (!c->d_mu) fail(GAIN_ERR_STATE, "dipole not uploaded");
@@ -588,6 +642,7 @@ extern "C" int gain_sweep_create(gain_ctx* c, const gain_task* tasks,
     s = new gain_sweep();
     s->c = c;
     s->slices = i8_slices_of(precision);
+    s->crt = crt_moduli_of(precision);
     if (c->force_cfg[0] >= 0) s->pj_cfg = c->force_cfg[0];
     build(s, tasks, ntasks);
     GAIN_HIP(hipStreamSynchronize(c->stream));

@@ -182,8 +182,9 @@ int gain_batched_lines(gain_ctx* ctx, int jindI, int gammaI,
  * all batches (tasks sharing an initial batch share them) run as one GEMM,
  * the projections of all tasks as one persistent grouped FP64 MFMA launch
  * and one grouped GEMV launch, and every per-task kernel as one grouped
- * launch per stage.  `precision` is GAIN_PREC_FP64 or a split-INT8 mode
- * (GAIN_PREC_I8X5..7: the mu X products only; the projections stay FP64).
+ * launch per stage.  `precision` is GAIN_PREC_FP64, a split-INT8 mode
+ * (GAIN_PREC_I8X5..7) or a CRT mode (GAIN_PREC_I8CRT11/12/14); the reduced
+ * modes apply to the mu X products only (the projections stay FP64).
  * gain_sweep_run replays the graph on the context's stream
  * (asynchronously; gain_synchronize waits).  Task outputs must be device
  * memory.  Re-uploading the dipole, a basis or a block used by a sweep

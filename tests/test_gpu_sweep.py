@@ -83,6 +83,37 @@ def test_mixed_j_sweep_graph_replays_match_unbatched(cfgs):
         sw.close()
 
 
+@pytest.mark.parametrize("precision,rel", [(8, 1e-7), (9, 1e-9)])
+def test_mixed_j_sweep_crt(precision, rel):
+    """A sweep whose mu X products run in the CRT form (12 or 14 moduli, the
+    sweep's own residues of mu, X cut into residue planes inside the graph)
+    against the unbatched plain FP64 path: transition lists identical, S
+    within `rel` on strong lines, on two replays with eager CRT and FP64
+    work on the context between them."""
+    import torch
+    prob = synth.make_problem([5, 6, 7, 8], 12, "c2v4")
+    prob.params.erange_lower = (0.0, 1e9)
+    rng = np.random.default_rng(7)
+    with Engine.from_problem(prob) as eng:
+        tasks, outs = _tasks(prob, eng, rng, max_per_block=24)
+        sw = Sweep(eng, tasks, precision=precision)
+        for rep in range(2):
+            for o in outs:
+                for v in o.values():
+                    v.zero_()
+            torch.cuda.synchronize()
+            sw.run()
+            eng.synchronize()
+            assert _check(eng, tasks, outs, rel=rel) > 0
+            blk = next(iter(prob.blocks))
+            jF = blk[0] + 1 if blk[0] + 1 < len(prob.bases) else blk[0]
+            eng.batched_lines(blk[0], blk[1], [0, 1], jF,
+                              eng.gamma_pair(blk[1]), precision=precision)
+            eng.batched_lines(blk[0], blk[1], [0, 1], jF,
+                              eng.gamma_pair(blk[1]))
+        sw.close()
+
+
 @pytest.mark.parametrize("precision", [5, 6])
 def test_mixed_j_sweep_split_int8(precision):
     """A sweep whose mu X products run as split-INT8 (6 or 7 slices, the
