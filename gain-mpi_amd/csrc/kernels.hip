@@ -115,55 +115,14 @@ void launch_mu_gemm_valu(const double* mu, int nc, const double* X,
 //           f3j * s * sum_{tau_I} m * Y_alpha[v(r'), 2 K_I + tau_I]
 // with alpha, m and s as in the reference kernel (s = truncated SQRT2 when
 // K_I != K_F and K_I K_F != 0).
-__device__ inline void gather_body(const GatherArgs& a, int bx, int b) {
+// Each thread forms one primitive of NB consecutive vectors: the
+// primitive's K, tau and vibrational index and the rotational factors are
+// formed once, and the up to 6 NB loads of Y (K_I = K-1..K+1, tau_I = 0, 1,
+// NB vectors) are all independent, so a thread keeps many loads in flight.
+template <int NB>
+__device__ inline void gather_body(const GatherArgs& a, int bx, int b0) {
   const int r = bx * 256 + threadIdx.x;
   if (r >= a.nprimF) return;
-  const int K = a.kvalF[r];
-  const int tF = a.tauF[r] & 1;
-  const int v = a.vibF[r];
-  const int n1 = a.jmax + 1;
-  double acc = 0.0;
-  const int klo = max(K - 1, 0), khi = min(K + 1, a.jI);
-  for (int KI = klo; KI <= khi; ++KI) {
-    const int dk = K - KI;
-    const double f3j =
-        a.threej[a.jI + KI * n1 + (a.jF - a.jI + 1) * n1 * n1 +
-                 (dk + 1) * n1 * n1 * 3];
-    const double sq = (dk != 0 && KI * K != 0) ? kSqrt2Ref : 1.0;
-#pragma unroll
-    for (int tI = 0; tI < 2; ++tI) {
-      int alpha, m;
-      if (dk == 0) {
-        alpha = 2;
-        m = tF - tI;
-      } else if (tF != tI) {
-        alpha = 0;
-        m = (tF - tI) * dk;
-      } else {
-        alpha = 1;
-        m = -1;
-      }
-      if (m == 0) continue;
-      const int64_t col = (int64_t)b * a.ncolI + 2 * KI + tI;
-      const double* yp = a.Y + alpha * a.ys_alpha + col * a.nc + v;
-      double y = 0.0;
-      for (int ks = 0; ks < a.ksplit; ++ks) y += yp[ks * a.ys_ks];
-      acc = fma(f3j * sq * (double)m, y, acc);
-    }
-  }
-  const bool neg = ((K % 3) * tF) & 1;
-  a.H[(int64_t)b * a.hstride + r] = neg ? -acc : acc;
-}
-
-// Same gather for NB consecutive vectors per thread: the primitive's K, tau
-// and vibrational index and the rotational factors are formed once, and the
-// up to 6 NB loads of Y (K_I = K-1..K+1, tau_I = 0, 1, NB vectors) are all
-// independent, so a thread keeps many loads in flight.
-template <int NB>
-__global__ __launch_bounds__(256) void k_gather_h(GatherArgs a) {
-  const int r = blockIdx.x * 256 + threadIdx.x;
-  if (r >= a.nprimF) return;
-  const int b0 = blockIdx.y * NB;
   const int K = a.kvalF[r];
   const int tF = a.tauF[r] & 1;
   const int v = a.vibF[r];
@@ -224,11 +183,16 @@ __global__ __launch_bounds__(256) void k_gather_h(GatherArgs a) {
       a.H[(int64_t)(b0 + i) * a.hstride + r] = neg ? -acc[i] : acc[i];
 }
 
+constexpr int kGatherNB = 4;  // vectors per thread
+
+__global__ __launch_bounds__(256) void k_gather_h(GatherArgs a) {
+  gather_body<kGatherNB>(a, blockIdx.x, blockIdx.y * kGatherNB);
+}
+
 void launch_gather_h(const GatherArgs& a, hipStream_t s) {
   if (a.nprimF == 0 || a.nvec == 0) return;
-  constexpr int NB = 4;
-  dim3 grid((a.nprimF + 255) / 256, (a.nvec + NB - 1) / NB);
-  hipLaunchKernelGGL(k_gather_h<NB>, grid, dim3(256), 0, s, a);
+  dim3 grid((a.nprimF + 255) / 256, (a.nvec + kGatherNB - 1) / kGatherNB);
+  hipLaunchKernelGGL(k_gather_h, grid, dim3(256), 0, s, a);
   check_launch("gather_h");
 }
 
@@ -775,11 +739,13 @@ struct GatherOp {
     return (a.nprimF + 255) / 256;
   }
   static int64_t blocks(const Args& a) {
-    return a.nprimF == 0 ? 0 : (int64_t)nbx(a) * a.nvec;
+    return a.nprimF == 0
+               ? 0
+               : (int64_t)nbx(a) * ((a.nvec + kGatherNB - 1) / kGatherNB);
   }
   static __device__ void run(const Args& a, int local) {
     const int n = nbx(a);
-    gather_body(a, local % n, local / n);
+    gather_body<kGatherNB>(a, local % n, (local / n) * kGatherNB);
   }
 };
 struct ExpandTOp {
