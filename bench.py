@@ -365,6 +365,9 @@ def main():
                          "traffic); the rooflines use the published peaks"}
     log(f"measured MFMA ceiling: int8 {mfma_peak['i8_top_s']:.0f} TOP/s, "
         f"fp64 {mfma_peak['f64_tflop_s']:.1f} TFLOP/s")
+    # measured HBM read ceiling (plain streaming kernel, 8 GiB)
+    read_gbs = eng.hbm_read_peak() / 1e9
+    log(f"measured HBM read rate of a streaming kernel: {read_gbs:.0f} GB/s")
 
     # ---------------- plain FP64 GEMV drop-in
     pl = {k: torch.zeros(nF, dtype=torch.float64, device=dev)
@@ -600,6 +603,8 @@ def main():
                            "achieved": gemv_gbs, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": gemv_gbs / HBM_PEAK_GBS,
                            "device_copy_gbs": copy_gbs,
+                           "streaming_read_gbs": read_gbs,
+                           "frac_of_streaming_read": gemv_gbs / read_gbs,
                            "traffic": gemv_traffic},
         "mfma_peak_measured": mfma_peak,
         "mixed_j_sweep": sweep,

@@ -1,6 +1,7 @@
-// Matrix-core throughput calibration: register-only chains of one MFMA
-// instruction (no memory traffic), to set the measured ceiling beside the
-// published peaks the rooflines use.
+// Calibration: matrix-core throughput from register-only chains of one MFMA
+// instruction (no memory traffic) and the HBM read rate of a plain streaming
+// kernel, the measured ceilings beside the published peaks the rooflines
+// use.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -60,7 +61,67 @@ __global__ __launch_bounds__(256) void k_peak_f64(int trips, int store,
       out[blockIdx.x * 256 + t] += acc[j][0] + acc[j][3];
 }
 
+// Read-only HBM stream: grid-stride 16-byte non-temporal loads, eight per
+// trip, summed so the loads stay live; one double written per thread.
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+__global__ __launch_bounds__(256) void k_read_stream(
+    const dbl2* __restrict__ p, int64_t n, double* __restrict__ out) {
+  const int64_t tid = blockIdx.x * 256ll + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  dbl2 s = {0.0, 0.0};
+  int64_t i = tid;
+  for (; i + 7 * stride < n; i += 8 * stride) {
+    dbl2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      v[u] = __builtin_nontemporal_load(p + i + u * stride);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += v[u];
+  }
+  for (; i < n; i += stride) s += p[i];
+  out[tid] = s.x + s.y;
+}
+
 }  // namespace
+
+extern "C" int gain_hbm_read_peak(gain_ctx* c, double* bytes_per_s) {
+  return guarded([&] {
+    need(c);
+    GAIN_CHECK_ARG(bytes_per_s, "null argument");
+    DeviceGuard g(c->device);
+    const int64_t n = 1ll << 29;  // 8 GiB of 16-byte elements
+    const int blocks = 8192;
+    void* buf = nullptr;
+    GAIN_HIP(hipMalloc(&buf, (size_t)n * 16));
+    double* out = c->w_tmp.get<double>((size_t)blocks * 256);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 0.f;
+    hipError_t err = hipMemsetAsync(buf, 0, (size_t)n * 16, c->stream);
+    if (err == hipSuccess) err = hipEventCreate(&e0);
+    if (err == hipSuccess) err = hipEventCreate(&e1);
+    auto launch = [&] {
+      hipLaunchKernelGGL(k_read_stream, dim3(blocks), dim3(256), 0,
+                         c->stream, (const dbl2*)buf, n, out);
+    };
+    if (err == hipSuccess) {
+      launch();  // warm
+      err = hipEventRecord(e0, c->stream);
+    }
+    const int reps = 4;
+    if (err == hipSuccess) {
+      for (int r = 0; r < reps; ++r) launch();
+      err = hipEventRecord(e1, c->stream);
+    }
+    if (err == hipSuccess) err = hipEventSynchronize(e1);
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(buf);
+    if (err != hipSuccess)
+      fail(GAIN_ERR_HIP, "hbm read peak: %s", hipGetErrorString(err));
+    *bytes_per_s = (double)n * 16.0 * reps / (ms * 1e-3);
+  });
+}
 
 extern "C" int gain_mfma_peak(gain_ctx* c, const char* kind,
                               double* ops_per_s) {

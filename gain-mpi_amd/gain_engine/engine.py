@@ -272,6 +272,12 @@ class Engine:
         check(self._L.gain_mfma_peak(self._h, kind.encode(), C.byref(v)))
         return float(v.value)
 
+    def hbm_read_peak(self) -> float:
+        """Measured read-only HBM streaming rate, bytes/s."""
+        v = C.c_double(0.0)
+        check(self._L.gain_hbm_read_peak(self._h, C.byref(v)))
+        return float(v.value)
+
     def tuning_report(self) -> str:
         buf = C.create_string_buffer(1 << 16)
         check(self._L.gain_tuning_report(self._h, buf, 1 << 16))

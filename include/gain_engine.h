@@ -255,6 +255,10 @@ int gain_tuning_report(gain_ctx* ctx, char* buf, int len);
  * counterpart: it calibrates the rooflines of the split-INT8 and FP64
  * GEMMs. */
 int gain_mfma_peak(gain_ctx* ctx, const char* kind, double* ops_per_s);
+/* Read-only HBM rate of a plain streaming kernel (16-byte non-temporal
+ * loads over an 8 GiB buffer allocated for the call), in bytes/s: the
+ * achievable ceiling for the HBM-bound kernels. */
+int gain_hbm_read_peak(gain_ctx* ctx, double* bytes_per_s);
 
 /* ---- transition list (main.cpp:238-338 + Output.cpp:52-112) ----------- */
 typedef struct {
