@@ -289,9 +289,11 @@ def run_rotsym(nrep):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=250)
+    ap.add_argument("--batch", type=int, default=500,
+                    help="initial states per batch (the 2000 lowest initial "
+                         "states are walked in order, wrapping around)")
     ap.add_argument("--plain-states", type=int, default=100)
     ap.add_argument("--precision", type=int, default=8,
                     help="contraction precision of the headline line (8: "

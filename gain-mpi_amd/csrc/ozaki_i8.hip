@@ -138,7 +138,10 @@ __device__ inline void i8_tile(int ntm, int ntn, int ksplit, int* z, int* ksi,
   *z = p / per_z;
   const int rem = p - *z * per_z;
   *ksi = rem / ntiles;
-  tile_order(rem - *ksi * ntiles, ntm, ntn, 8, tm, tn);
+  // with a few column tiles, the column tiles of one row tile run back to
+  // back (same XCD, same time), so the second one's reads of the shared A
+  // blocks hit L2
+  tile_order(rem - *ksi * ntiles, ntm, ntn, ntn <= 4 ? 1 : 8, tm, tn);
 }
 
 template <int S, int TM, int TN, int WM, int WN>
