@@ -90,6 +90,60 @@ bool make_group(const std::string& name, Group& g) {
     finish(g, {});
     return true;
   }
+  // Abelian groups of order 4 and 8: one operation per class; their
+  // equivalent rotations are the identity and the twofold rotations about
+  // z (pi, 0, 0), y (0, pi, 0) and x (pi/2, pi, 3pi/2).
+  const std::array<ld, 3> E0 = {o, o, o}, Rz = {pi, o, o}, Ry = {o, pi, o},
+                          Rx = {0.5L * pi, pi, 1.5L * pi};
+  auto abelian = [&](std::vector<std::vector<ld>> chars,
+                     std::vector<std::array<ld, 3>> euler) {
+    const int n = (int)chars.size();
+    g.nrep = g.noper = g.nclass = n;
+    g.degen.assign(n, 1), g.nelem.assign(n, 1);
+    g.chars = std::move(chars);
+    g.euler = std::move(euler);
+    finish(g, {});
+    return true;
+  };
+  const std::vector<std::vector<ld>> klein = {
+      {1, 1, 1, 1}, {1, 1, -1, -1}, {1, -1, -1, 1}, {1, -1, 1, -1}};
+  if (name == "C2H(M)" || name == "C2H")  // symmetry.f90:140-170
+    return abelian(klein, {E0, Ry, Rx, Rz});
+  if (name == "CS(EM)")  // symmetry.f90:171-202
+    return abelian({{1, 1, 1, 1}, {1, 1, -1, -1}, {1, -1, 1, -1}, {1, -1, -1, 1}},
+                   {E0, Rz, Rx, Ry});
+  if (name == "G4(M)" || name == "G4")  // symmetry.f90:203-230
+    return abelian(klein, {E0, Rx, Ry, Rz});
+  if (name == "G4(EM)")  // symmetry.f90:231-282
+    return abelian({{1, 1, 1, 1, 1, 1, 1, 1},
+                    {1, 1, -1, -1, 1, 1, -1, -1},
+                    {1, -1, -1, 1, 1, -1, -1, 1},
+                    {1, -1, 1, -1, 1, -1, 1, -1},
+                    {1, 1, -1, -1, -1, -1, 1, 1},
+                    {1, 1, 1, 1, -1, -1, -1, -1},
+                    {1, -1, 1, -1, -1, 1, -1, 1},
+                    {1, -1, -1, 1, -1, 1, 1, -1}},
+                   {E0, Ry, Rx, Rz, Rz, Rx, Ry, E0});
+  if (name == "D2H(M)")  // symmetry.f90:283-323
+    return abelian({{1, 1, 1, 1, 1, 1, 1, 1},
+                    {1, 1, 1, 1, -1, -1, -1, -1},
+                    {1, 1, -1, -1, -1, -1, 1, 1},
+                    {1, 1, -1, -1, 1, 1, -1, -1},
+                    {1, -1, 1, -1, -1, 1, -1, 1},
+                    {1, -1, 1, -1, 1, -1, 1, -1},
+                    {1, -1, -1, 1, 1, -1, -1, 1},
+                    {1, -1, -1, 1, -1, 1, 1, -1}},
+                   {E0, Rz, Ry, Rx, Rx, Ry, Rz, E0});
+  if (name == "D2H")  // symmetry.f90:324-364
+    return abelian({{1, 1, 1, 1, 1, 1, 1, 1},
+                    {1, 1, 1, 1, -1, -1, -1, -1},
+                    {1, 1, -1, -1, 1, 1, -1, -1},
+                    {1, 1, -1, -1, -1, -1, 1, 1},
+                    {1, -1, 1, -1, 1, -1, 1, -1},
+                    {1, -1, 1, -1, -1, 1, -1, 1},
+                    {1, -1, -1, 1, 1, -1, -1, 1},
+                    {1, -1, -1, 1, -1, 1, 1, -1}},
+                   {E0, Rz, Ry, Rx, Rz, E0, Rx, Ry});
   const ld a = 0.5L, b = 0.5L * std::sqrt(3.0L);
   if (name == "C3V(M)" || name == "C3V") {  // symmetry.f90:365-415
     g.nrep = 3, g.noper = 6, g.nclass = 3;
@@ -439,8 +493,8 @@ const Group& group_of(const char* name) {
     Group t;
     if (!make_group(s, t))
       fail(GAIN_ERR_ARG,
-           "symmetry group \"%s\": C(M), CS(M), C2V(M), C3V(M), D3H(M), "
-           "TD(M)",
+           "symmetry group \"%s\": C(M), CS(M), C2V(M), C2H(M), CS(EM), "
+           "G4(M), G4(EM), D2H(M), D2H, C3V(M), D3H(M), TD(M)",
            name);
     g = t;
     last = s;

@@ -366,8 +366,11 @@ int gain_compute_half_linestrength_rotsym(
     int ncontrF, const double* dipole_me, const double* wigner,
     const double* vector, double* half_ls, void* stream);
 /* ---- symmetric-top (ROTSYM) rotational factors ---------------------------
- * Groups: "C(M)", "CS(M)", "C2V(M)", "C3V(M)", "D3H(M)", "TD(M)"
- * (symmetry.f90:58-745).
+ * Groups: "C(M)", "CS(M)", "C2V(M)", "C2H(M)", "CS(EM)", "G4(M)",
+ * "G4(EM)", "D2H(M)", "D2H", "C3V(M)", "D3H(M)", "TD(M)" (symmetry.f90:58-745;
+ * the short names "C", "CS", "C2V", "C2H", "G4", "C3V", "D3H", "TD" too).
+ * DNH(M) and DINFTYH(M), whose order comes from the input file, are not
+ * available.
  *
  * Symmetrised rotational functions of one J (MLrotsymmetry_generate,
  * trove_wrappers.f90:416-589, with MLsymmetrization_rot :591-914): the

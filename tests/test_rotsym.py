@@ -6,6 +6,10 @@ on the CPU against what the mathematics fixes:
 
 * the rotational levels of a symmetric top fall into the irreps the
   textbook K rules give (C3v(M), D3h(M)), level by level in count;
+* in Td(M) and the abelian groups (C2h(M), Cs(EM), G4(M), G4(EM), D2h(M),
+  D2h) the irrep counts are the reduction of D^(J) by the closed-form
+  SO(3) characters, and in the abelian groups each level has one parity of
+  k, matching its character under the rotation by pi about z;
 * the functions are orthonormal and every group operation acts on the
   components of a level by the irrep's character (trace of each block);
 * with no symmetry the table is the K-block rotational factor of the
@@ -24,9 +28,38 @@ import pytest
 from gain_engine import _lib, synth
 from oracle import oracle as orc
 
-GROUPS = ["C(M)", "CS(M)", "C2V(M)", "C3V(M)", "D3H(M)", "TD(M)"]
-NOPER = {"C(M)": 1, "CS(M)": 2, "C2V(M)": 4, "C3V(M)": 6, "D3H(M)": 12,
-         "TD(M)": 24}
+GROUPS = ["C(M)", "CS(M)", "C2V(M)", "C2H(M)", "CS(EM)", "G4(M)", "G4(EM)",
+          "D2H(M)", "D2H", "C3V(M)", "D3H(M)", "TD(M)"]
+NOPER = {"C(M)": 1, "CS(M)": 2, "C2V(M)": 4, "C2H(M)": 4, "CS(EM)": 4,
+         "G4(M)": 4, "G4(EM)": 8, "D2H(M)": 8, "D2H": 8, "C3V(M)": 6,
+         "D3H(M)": 12, "TD(M)": 24}
+# Abelian groups (symmetry.f90:140-364): characters [irrep][operation], the
+# rotation angle of each operation's equivalent rotation (0 for the
+# identity, pi for a twofold axis) and the operations equivalent to the
+# twofold rotation about z.
+KLEIN = [[1, 1, 1, 1], [1, 1, -1, -1], [1, -1, -1, 1], [1, -1, 1, -1]]
+D2H_M = [[1, 1, 1, 1, 1, 1, 1, 1], [1, 1, 1, 1, -1, -1, -1, -1],
+         [1, 1, -1, -1, -1, -1, 1, 1], [1, 1, -1, -1, 1, 1, -1, -1],
+         [1, -1, 1, -1, -1, 1, -1, 1], [1, -1, 1, -1, 1, -1, 1, -1],
+         [1, -1, -1, 1, 1, -1, -1, 1], [1, -1, -1, 1, -1, 1, 1, -1]]
+PI = math.pi
+ABELIAN = {
+    "C2H(M)": (KLEIN, [0, PI, PI, PI], [3]),
+    "CS(EM)": ([[1, 1, 1, 1], [1, 1, -1, -1], [1, -1, 1, -1], [1, -1, -1, 1]],
+               [0, PI, PI, PI], [1]),
+    "G4(M)": (KLEIN, [0, PI, PI, PI], [3]),
+    "G4(EM)": ([[1, 1, 1, 1, 1, 1, 1, 1], [1, 1, -1, -1, 1, 1, -1, -1],
+                [1, -1, -1, 1, 1, -1, -1, 1], [1, -1, 1, -1, 1, -1, 1, -1],
+                [1, 1, -1, -1, -1, -1, 1, 1], [1, 1, 1, 1, -1, -1, -1, -1],
+                [1, -1, 1, -1, -1, 1, -1, 1], [1, -1, -1, 1, -1, 1, 1, -1]],
+               [0, PI, PI, PI, PI, PI, PI, 0], [3, 4]),
+    "D2H(M)": (D2H_M, [0, PI, PI, PI, PI, PI, PI, 0], [1, 6]),
+    "D2H": ([[1, 1, 1, 1, 1, 1, 1, 1], [1, 1, 1, 1, -1, -1, -1, -1],
+             [1, 1, -1, -1, 1, 1, -1, -1], [1, 1, -1, -1, -1, -1, 1, 1],
+             [1, -1, 1, -1, 1, -1, 1, -1], [1, -1, 1, -1, -1, 1, -1, 1],
+             [1, -1, -1, 1, 1, -1, -1, 1], [1, -1, -1, 1, -1, 1, 1, -1]],
+            [0, PI, PI, PI, PI, 0, PI, PI], [1, 4]),
+}
 # Td(M): classes E, (123), (12)(34), (1423)*, (12)*; sizes; the rotation
 # angle of the equivalent rotation of each class; characters of
 # A1, A2, E, F1, F2
@@ -140,6 +173,44 @@ CHARS = {
     "TD(M)": [sum(([c[k]] * n for k, n in enumerate(TD_SIZES)), [])
               for c in TD_CHARS],
 }
+CHARS.update({g: v[0] for g, v in ABELIAN.items()})
+
+
+def so3_character(J, a):
+    """Character of D^(J) for a rotation by a: sin((J + 1/2) a) / sin(a/2)."""
+    return 2 * J + 1 if a == 0.0 else \
+        math.sin((J + 0.5) * a) / math.sin(0.5 * a)
+
+
+@pytest.mark.parametrize("group", sorted(ABELIAN))
+@pytest.mark.parametrize("J", [0, 1, 2, 3, 4, 5, 8, 11])
+def test_abelian_levels_reduce_the_rotation_group_representation(group, J):
+    """Irrep counts from the reduction of D^(J) over the rotations
+    equivalent to the group's operations (closed-form SO(3) characters)."""
+    chars, angles, _ = ABELIAN[group]
+    want = [round(sum(c * so3_character(J, a) for c, a in zip(ch, angles))
+                  / len(angles)) for ch in chars]
+    nl, md, irr, root, vec, ops = functions(group, J)
+    assert md == 1 and nl == 2 * J + 1
+    assert np.bincount(irr, minlength=len(chars)).tolist() == want
+
+
+@pytest.mark.parametrize("group", sorted(ABELIAN))
+@pytest.mark.parametrize("J", [1, 2, 3, 6, 9])
+def test_abelian_levels_carry_the_parity_of_k(group, J):
+    """A rotation by pi about z multiplies |J, k> by (-1)^k: each level is
+    built from Wang functions of one parity of k, and its irrep's character
+    of every operation equivalent to that rotation is (-1)^k."""
+    chars, _, rz = ABELIAN[group]
+    nl, md, irr, root, vec, ops = functions(group, J)
+    k_of_row = np.array([0] + [(i + 1) // 2 for i in range(1, 2 * J + 1)])
+    for l in range(nl):
+        rows = np.nonzero(np.abs(vec[:, root[l, 0]]) > 1e-12)[0]
+        par = set((k_of_row[rows] % 2).tolist())
+        assert len(par) == 1, (group, J, l)
+        p = par.pop()
+        for op in rz:
+            assert chars[irr[l]][op] == (-1) ** p, (group, J, l, op)
 
 
 @pytest.mark.parametrize("group", GROUPS)
