@@ -902,6 +902,22 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res_lds(
             16, 0, 0);
       }
   };
+  // request idx = j * DPW + t of those issue() makes
+  auto issue_one = [&](int stage, int st, int idx) {
+    const int j = idx / DPW, t = idx - j * DPW;
+    int8_t* dst = lds + stage * (SB * 1024);
+    const int kb = st * KPB + j;
+    const int q = wave + NW * t;
+    const int qq = q < NBLK ? q : 0;
+    const int8_t* src =
+        qq < BMB ? Az + ((int64_t)(arb0 + qq) * nkb + kb) * 1024
+                 : Bz + ((int64_t)(brb0 + qq - BMB) * nkb + kb) * 1024;
+    __builtin_amdgcn_global_load_lds(
+        (const void*)(src + src_off),
+        (__attribute__((address_space(3))) void*)(dst +
+                                                  (j * NBLKP + q) * 1024),
+        16, 0, 0);
+  };
   i16v acc[TM][TN];
 #pragma unroll
   for (int a = 0; a < TM; ++a)
@@ -986,7 +1002,53 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res_lds(
       }
       nxt = nxt == NST - 1 ? 0 : nxt + 1;
     };
+    // The steps that request a stage (all but the last NST - 1) run in a
+    // loop of their own in which every step requests unconditionally, the
+    // requests spread over the first products of the step, one after each,
+    // so that a wave held while its requests are accepted still has
+    // products in its matrix pipe (the requests issued back to back after
+    // the barrier leave the pipe idle meanwhile: the projection runs 4%
+    // slower that way).  A request under a branch among the products would
+    // make the two paths keep the accumulators in different registers,
+    // hence the separate loop.
+    auto step_ilv = [&](const i4v(&ca)[KPB][TM], const i4v(&cb)[KPB][TN],
+                        i4v(&na)[KPB][TM], i4v(&nb)[KPB][TN], int st) {
+      constexpr int HALF = (TM + 1) / 2 * TN;  // first-half products
+      static_assert(DPW * KPB <= HALF, "one request per product");
+      wait_stages<DPW * KPB, NST - 3>(NST - 3);
+      __builtin_amdgcn_s_barrier();
+      const int slot = nxt == 0 ? NST - 2 : (nxt == 1 ? NST - 1 : nxt - 2);
+#pragma unroll
+      for (int j = 0; j < KPB; ++j) {
+        if (j == 0) {
+          // request idx after product idx of the first k block
+#pragma unroll
+          for (int a = 0; a < (TM + 1) / 2; ++a)
+#pragma unroll
+            for (int b = 0; b < TN; ++b) {
+              acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                  ca[0][a], cb[0][b], acc[a][b], 0, 0, 0);
+              if (a * TN + b < DPW * KPB) {
+                __builtin_amdgcn_sched_barrier(0);
+                issue_one(slot, st + NST - 1, a * TN + b);
+                __builtin_amdgcn_sched_barrier(0);
+              }
+            }
+        } else {
+          mma(ca[j], cb[j], 0, (TM + 1) / 2);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        read(na[j], nb[j], nxt, j);
+        __builtin_amdgcn_sched_barrier(0);
+        mma(ca[j], cb[j], (TM + 1) / 2, TM);
+      }
+      nxt = nxt == NST - 1 ? 0 : nxt + 1;
+    };
     int st = 0;
+    for (; st + 2 < nst && st + NST < nst; st += 2) {
+      step_ilv(fa0, fb0, fa1, fb1, st);
+      step_ilv(fa1, fb1, fa0, fb0, st + 1);
+    }
     for (; st + 2 < nst; st += 2) {
       step(fa0, fb0, fa1, fb1, st);
       step(fa1, fb1, fa0, fb0, st + 1);
