@@ -341,6 +341,23 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_lds(
           0);
     }
   };
+  // request t of those issue() makes
+  auto issue_one = [&](int stage, int kb, int t) {
+    int8_t* dst = lds + stage * (NBLKP * 1024);
+    const int q = wave + NW * t;
+    const int8_t* src;
+    if (q < S * BMB || q >= NBLK) {
+      const int qa = q < NBLK ? q : 0;
+      const int i = qa / BMB, rb = qa % BMB;
+      src = Az + i * g.a_plane + ((int64_t)(arb0 + rb) * nkb + kb) * 1024;
+    } else {
+      const int i = (q - S * BMB) / BNB, rb = (q - S * BMB) % BNB;
+      src = g.B + i * g.b_plane + ((int64_t)(brb0 + rb) * nkb + kb) * 1024;
+    }
+    __builtin_amdgcn_global_load_lds(
+        (const void*)(src + src_off),
+        (__attribute__((address_space(3))) void*)(dst + q * 1024), 16, 0, 0);
+  };
 
   i16v acc[S][TM][TN];
 #pragma unroll
@@ -357,7 +374,47 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_lds(
   for (int st = 0; st < NST - 1; ++st)
     if (kb0 + st < kb1) issue(st, kb0 + st);
   int stage = 0;
-  for (int kb = kb0; kb < kb1; ++kb) {
+  int kb = kb0;
+  // k blocks that request a stage, in a loop of their own: the fragment
+  // reads first, then the requests spread over the first products, one
+  // after each (as in k_gemm_i8_res_lds; the projection 3-4% faster than
+  // with the requests issued between the barrier and the reads)
+  for (; kb + NST - 1 < kb1; ++kb) {
+    wait_vmcnt<(NST >= 3 ? (NST - 2) * DPW : 0)>();
+    __builtin_amdgcn_s_barrier();
+    const int slot = stage == 0 ? NST - 1 : stage - 1;
+    const int8_t* base = lds + stage * (NBLKP * 1024) + rd_off;
+    i4v fa[S][TM];
+#pragma unroll
+    for (int i = 0; i < S; ++i)
+#pragma unroll
+      for (int a = 0; a < TM; ++a)
+        fa[i][a] = *(const i4v*)(base + (BMB * i + TM * wm + a) * 1024);
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      i4v fb[TN];
+#pragma unroll
+      for (int b = 0; b < TN; ++b)
+        fb[b] = *(const i4v*)(base + (S * BMB + BNB * j + TN * wn + b) * 1024);
+#pragma unroll
+      for (int i = 0; i < S - j; ++i)
+#pragma unroll
+        for (int a = 0; a < TM; ++a)
+#pragma unroll
+          for (int b = 0; b < TN; ++b) {
+            acc[i + j][a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                fa[i][a], fb[b], acc[i + j][a][b], 0, 0, 0);
+            const int m = (i * TM + a) * TN + b;
+            if (j == 0 && m < DPW) {
+              __builtin_amdgcn_sched_barrier(0);
+              issue_one(slot, kb + NST - 1, m);
+              __builtin_amdgcn_sched_barrier(0);
+            }
+          }
+    }
+    stage = stage == NST - 1 ? 0 : stage + 1;
+  }
+  for (; kb < kb1; ++kb) {
     // loads of later stages may stay in flight: min(NST - 2, left) of them
     const int later = min(NST - 2, kb1 - 1 - kb);
     if (later >= 2)
