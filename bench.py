@@ -289,9 +289,9 @@ def run_rotsym(nrep):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=500,
+    ap.add_argument("--batch", type=int, default=1000,
                     help="initial states per batch (the 2000 lowest initial "
                          "states are walked in order, wrapping around)")
     ap.add_argument("--plain-states", type=int, default=100)
