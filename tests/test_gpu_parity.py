@@ -442,6 +442,62 @@ def test_every_int8_tile_configuration(cfg, split):
     assert ns > 0 and es <= 5e-8, es
 
 
+_LONG_K = {}
+
+
+def _long_k_case():
+    """J 4 and 5 with Nc = 400 (records of 900 and 1100 elements, a 400 x 400
+    dipole): the GEMMs have 14 and 36 k blocks of 32 bytes, so every
+    pipelined int8 tile form runs its steady loop (the Nc = 20 cases above
+    have 2); six initial states against the oracle, computed once."""
+    if not _LONG_K:
+        prob = synth.make_problem([4, 5], 400, "c2v4")
+        prob.params.erange_lower = (0.0, 1e9)
+        prob.params.freq_window = (1500.0, 1e9)
+        n = len(prob.state_table())
+        initials = [1 + k * (n // 6) for k in range(6)]
+        tr, _ = orc.run(prob, initials)
+        _LONG_K.update(prob=prob, initials=initials, ref=oracle_map(tr))
+    return _LONG_K["prob"], _LONG_K["initials"], _LONG_K["ref"]
+
+
+@pytest.mark.parametrize("cfg", list(range(14)))
+def test_every_crt_tile_configuration_long_k(cfg):
+    """Each CRT-form tile shape with K long enough for its steady loop
+    (stage requests interleaved with the products): S within 1e-7 of the
+    FP64 oracle on strong lines, transition list identical."""
+    prob, initials, ref = _long_k_case()
+    with Engine.from_problem(prob) as eng:
+        eng.set_tuning("crt_cfg", cfg)
+        got = engine_lines(eng, prob, initials, "batched", 8)
+        assert eng.get_tuning("proj_cfg") == cfg
+    assert set(got) == set(ref)
+    es, _, ns, _ = _strong_weak_err(got, ref)
+    print(f"crt12 configuration {cfg}, long K: {ns} strong lines, max rel "
+          f"err {es:.3e}")
+    assert ns > 0 and es <= 1e-7, es
+
+
+@pytest.mark.parametrize("cfg", list(range(12)))
+@pytest.mark.parametrize("split", [1, 2])
+def test_every_int8_tile_configuration_long_k(cfg, split):
+    """Each split-INT8 tile shape and split-K 1/2 with K long enough for
+    the steady loop of the LDS-staged forms: S within 5e-8 of the FP64
+    oracle on strong lines, transition list identical."""
+    prob, initials, ref = _long_k_case()
+    with Engine.from_problem(prob) as eng:
+        eng.set_tuning("i8_cfg", cfg)
+        eng.set_tuning("i8_split", split)
+        got = engine_lines(eng, prob, initials, "batched", 5)
+        assert eng.get_tuning("proj_cfg") == cfg
+        assert eng.get_tuning("proj_split") == split
+    assert set(got) == set(ref)
+    es, _, ns, _ = _strong_weak_err(got, ref)
+    print(f"int8 x6 configuration {cfg} split {split}, long K: {ns} strong "
+          f"lines, max rel err {es:.3e}")
+    assert ns > 0 and es <= 5e-8, es
+
+
 @pytest.mark.parametrize("cfg", list(range(15)))
 def test_every_gemm_tile_configuration(cfg):
     """Each tile shape the autotuner can pick, for both GEMM kinds, on
