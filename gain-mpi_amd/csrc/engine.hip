@@ -106,11 +106,18 @@ void ensure_slots(gain_ctx* c, Basis& b) {
   for (int s = 0; s < nslot; ++s) cnt[s + 1] += cnt[s];
   std::vector<int> fill(cnt.begin(), cnt.end() - 1), prim(b.nprim);
   for (int r = 0; r < b.nprim; ++r) prim[fill[slot[r]]++] = r;
+  if (b.slot_ptr || b.slot_prim) ++c->gen;
   if (b.slot_ptr) c->arena.release(b.slot_ptr);
   if (b.slot_prim) c->arena.release(b.slot_prim);
   b.slot_ptr = c->arena.upload(cnt.data(), cnt.size());
   b.slot_prim = c->arena.upload(prim.data(), prim.size());
   b.slot_nc = c->nc;
+}
+
+void build_slots_all(gain_ctx* c) {
+  if (!c->d_mu) return;
+  for (Basis& b : c->bases)
+    if (b.present && b.nvib <= c->nc) ensure_slots(c, b);
 }
 
 int slices_of(int precision) {
@@ -528,6 +535,13 @@ Plan make_plan(gain_ctx* c, int jindI, int gI, const int* levels, int nI,
 // Tile configuration for a GEMM of one kind and shape: forced, tuned (the
 // autotuner times every configuration on the real operands once per shape,
 // on the context's stream), or the default.
+int gemm_cfg_lookup(const gain_ctx* c, int kind, const GemmArgs& g) {
+  if (c->force_cfg[kind] >= 0) return c->force_cfg[kind];
+  const auto key = std::make_tuple(kind, g.M >= 8192 ? -1 : g.M, g.N, g.K);
+  auto it = c->tuned.find(key);
+  return it != c->tuned.end() ? it->second % 100 : -1;
+}
+
 int gemm_cfg(gain_ctx* c, int kind, GemmArgs g) {
   int ks = 0;
   return gemm_cfg_split(c, kind, g, &ks, false);
@@ -600,32 +614,25 @@ int gemm_cfg_split(gain_ctx* c, int kind, GemmArgs g, int* ksplit,
   return best;
 }
 
-int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
-                    const Small& sm, int nvecX, double* H, int precision) {
-  Basis& bI = *P.bI;
-  Basis& bF = *P.bF;
-  ensure_slots(c, bI);
-  if (bF.nvib > c->nc)
-    fail(GAIN_ERR_ARG,
-         "vibrational index %d of J=%d exceeds the dipole dimension %d",
-         bF.nvib - 1, bF.J, c->nc);
-  const Irrep& ir = bI.irrep[P.gI];
+// Split-K count of the FP64 mu X GEMM when none is forced: enough
+// workgroups to fill 256 CUs (64-row tiles x column tiles x 3 components).
+int mu_ksplit_auto(const gain_ctx* c, int ncolTot) {
+  if (c->mu_ksplit > 0) return c->mu_ksplit;
   const int nc = c->nc;
-  const int64_t xstride = (int64_t)bI.ncol * nc;
-  double* X = c->w_X.get<double>((size_t)nvecX * xstride);
-  const int ncolTot = nvecX * bI.ncol;
-  int ks = c->mu_ksplit;
-  if (ks <= 0) {
-    // enough workgroups to fill 256 CUs: 64-row tiles x column tiles x 3
-    const int64_t tiles = (int64_t)((nc + 63) / 64) *
-                          ((ncolTot + (ncolTot > 64 ? 127 : 63)) /
-                           (ncolTot > 64 ? 128 : 64)) * 3;
-    ks = (int)std::min<int64_t>(16, std::max<int64_t>(1, 2048 / tiles));
-  }
-  double* Y = c->w_Y.get<double>((size_t)ks * 3 * ncolTot * nc);
-  Timed t(c, "half_ls");
+  const int64_t tiles = (int64_t)((nc + 63) / 64) *
+                        ((ncolTot + (ncolTot > 64 ? 127 : 63)) /
+                         (ncolTot > 64 ? 128 : 64)) * 3;
+  return (int)std::min<int64_t>(16, std::max<int64_t>(1, 2048 / tiles));
+}
+
+// Scatter + expansion of nvecX initial vectors into X[vec][2K+tau][v].
+ScatterArgs make_scatter_args(const gain_ctx* c, const Plan& P,
+                              const double* cbase, const Small& sm,
+                              int nvecX, double* X) {
+  const Basis& bI = *P.bI;
+  const Irrep& ir = bI.irrep[P.gI];
   ScatterArgs sa{};
-  sa.nslot = bI.ncol * nc;
+  sa.nslot = bI.ncol * c->nc;
   sa.nprim = bI.nprim;
   sa.slot_ptr = bI.slot_ptr;
   sa.slot_prim = bI.slot_prim;
@@ -642,8 +649,142 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
   sa.ideg = sm.ideg;
   sa.nvec = nvecX;
   sa.X = X;
-  sa.xstride = xstride;
-  launch_scatter_expand(sa, c->stream);
+  sa.xstride = (int64_t)bI.ncol * c->nc;
+  return sa;
+}
+
+// FP64 mu X GEMM: Y[ks][alpha][col][vF] for ncolTot columns of X.
+GemmArgs make_mu_gemm_args(const gain_ctx* c, const double* X, int ncolTot,
+                           double* Y, int ks) {
+  const int nc = c->nc;
+  GemmArgs g{};
+  g.M = nc;
+  g.N = ncolTot;
+  g.K = nc;
+  g.A = c->d_mu;
+  g.lda = nc;
+  g.sAz = (int64_t)nc * nc;
+  g.a_kcontig = false;
+  g.B = X;
+  g.ldb = nc;
+  g.C = Y;
+  g.ldc = nc;
+  g.sCz = (int64_t)ncolTot * nc;
+  g.sCks = 3 * (int64_t)ncolTot * nc;
+  g.nz = 3;
+  g.ksplit = ks;
+  return g;
+}
+
+// Gather of the half line strengths H[vec][r'] from Y (3j, rotational
+// factor and signs per final primitive).
+GatherArgs make_gather_args(const gain_ctx* c, const Plan& P, int nvecX,
+                            int ks, const double* Y, double* H) {
+  const Basis& bI = *P.bI;
+  const Basis& bF = *P.bF;
+  const int64_t ncolTot = (int64_t)nvecX * bI.ncol;
+  GatherArgs ga{};
+  ga.nprimF = bF.nprim;
+  ga.nc = c->nc;
+  ga.jI = bI.J;
+  ga.jF = bF.J;
+  ga.jmax = c->jmax;
+  ga.ncolI = bI.ncol;
+  ga.nvec = nvecX;
+  ga.ksplit = ks;
+  ga.kvalF = bF.kval;
+  ga.tauF = bF.tau;
+  ga.vibF = bF.vib;
+  ga.threej = c->d_threej;
+  ga.Y = Y;
+  ga.ys_alpha = ncolTot * c->nc;
+  ga.ys_ks = 3 * ncolTot * c->nc;
+  ga.H = H;
+  ga.hstride = bF.nprim;
+  return ga;
+}
+
+// Transposed expansion G[v] = P_{F,dF(v)}^T H[h_of_vec[v]].
+ExpandTArgs make_expand_T_args(const Plan& P, const double* H,
+                               const Small& sm, int nvecG, double* G,
+                               int64_t gstride) {
+  const Basis& bF = *P.bF;
+  const Irrep& irF = bF.irrep[P.gF];
+  ExpandTArgs ea{};
+  ea.ntot = irF.ntot;
+  ea.nprim = bF.nprim;
+  ea.sdeg = irF.sdeg;
+  ea.tcase = irF.tcase;
+  ea.case_ptr = bF.case_ptr;
+  ea.case_prim = bF.case_prim;
+  ea.icontr = bF.icontr;
+  ea.repres = irF.repres;
+  ea.H = H;
+  ea.hstride = bF.nprim;
+  ea.h_of_vec = sm.h_of_vec;
+  ea.ideg_of_vec = sm.ideg_of_vec;
+  ea.nvec = nvecG;
+  ea.G = G;
+  ea.gstride = gstride;
+  return ea;
+}
+
+// Epilogue arguments (filters, S, A, intensity) of one plan; the caller
+// sets the projected-strength source and the outputs.
+LinesArgs make_lines_args(const gain_ctx* c, const Plan& P, const Small& sm,
+                          int nvpi, bool compute) {
+  const gain_params& p = c->params;
+  LinesArgs la{};
+  la.nI = P.nI;
+  la.nF = P.blkF->n;
+  la.Flo = compute ? P.Flo : 0;
+  la.Fhi = compute ? P.Fhi : 0;
+  la.maxdeg = c->maxdeg;
+  la.nvec_per_i = nvpi;
+  la.slot_of_vec = sm.slot_of_vec;
+  la.ksplit = 1;
+  la.eI = sm.eI;
+  la.levelI = sm.levelI;
+  la.okI = sm.okI;
+  la.eF = P.blkF->d_energy;
+  la.same_block = (P.jindI == P.jindF && P.gI == P.gF);
+  la.block_ok = P.block_ok;
+  la.jI = P.bI->J;
+  la.jF = P.bF->J;
+  la.ndegI = p.sym_degen[P.gI];
+  la.ndegF = p.sym_degen[P.gF];
+  la.reduced = p.reduced;
+  la.gnsF = p.gns[P.gF];
+  la.zpe = p.zpe;
+  la.threshold = p.threshold;
+  la.beta = kPlanck * kVellgt / (kBoltz * p.temperature);
+  la.partition = p.partition;
+  la.fw0 = p.freq_window[0];
+  la.fw1 = p.freq_window[1];
+  la.eu0 = p.erange_upper[0];
+  la.eu1 = p.erange_upper[1];
+  la.isym_do_F = p.isym_do[P.gF];
+  return la;
+}
+
+int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
+                    const Small& sm, int nvecX, double* H, int precision) {
+  Basis& bI = *P.bI;
+  Basis& bF = *P.bF;
+  ensure_slots(c, bI);
+  if (bF.nvib > c->nc)
+    fail(GAIN_ERR_ARG,
+         "vibrational index %d of J=%d exceeds the dipole dimension %d",
+         bF.nvib - 1, bF.J, c->nc);
+  const int nc = c->nc;
+  const int64_t xstride = (int64_t)bI.ncol * nc;
+  double* X = c->w_X.get<double>((size_t)nvecX * xstride);
+  const int ncolTot = nvecX * bI.ncol;
+  int ks = mu_ksplit_auto(c, ncolTot);
+  double* Y = c->w_Y.get<double>((size_t)ks * 3 * ncolTot * nc);
+  Timed t(c, "half_ls");
+  launch_scatter_expand(make_scatter_args(c, P, cbase, sm, nvecX, X),
+                        c->stream);
   const int S = slices_of(precision);
   if (S > 0) {
     ks = 1;
@@ -759,22 +900,7 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
     g.ksplit = 1;
     gemm_f32acc(g, c->stream);
   } else if (c->use_mfma) {
-    GemmArgs g{};
-    g.M = nc;
-    g.N = ncolTot;
-    g.K = nc;
-    g.A = c->d_mu;
-    g.lda = nc;
-    g.sAz = (int64_t)nc * nc;
-    g.a_kcontig = false;
-    g.B = X;
-    g.ldb = nc;
-    g.C = Y;
-    g.ldc = nc;
-    g.sCz = (int64_t)ncolTot * nc;
-    g.sCks = 3 * (int64_t)ncolTot * nc;
-    g.nz = 3;
-    g.ksplit = ks;
+    GemmArgs g = make_mu_gemm_args(c, X, ncolTot, Y, ks);
     g.cfg = gemm_cfg(c, 1, g);
     gemm_f64(g, c->stream);
     c->last["half_ls_cfg"] = g.cfg < 0 ? gemm_default_config(g.N) : g.cfg;
@@ -783,25 +909,7 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
   } else {
     launch_mu_gemm_valu(c->d_mu, nc, X, ncolTot, Y, ks, c->stream);
   }
-  GatherArgs ga{};
-  ga.nprimF = bF.nprim;
-  ga.nc = nc;
-  ga.jI = bI.J;
-  ga.jF = bF.J;
-  ga.jmax = c->jmax;
-  ga.ncolI = bI.ncol;
-  ga.nvec = nvecX;
-  ga.ksplit = ks;
-  ga.kvalF = bF.kval;
-  ga.tauF = bF.tau;
-  ga.vibF = bF.vib;
-  ga.threej = c->d_threej;
-  ga.Y = Y;
-  ga.ys_alpha = (int64_t)ncolTot * nc;
-  ga.ys_ks = 3 * (int64_t)ncolTot * nc;
-  ga.H = H;
-  ga.hstride = bF.nprim;
-  launch_gather_h(ga, c->stream);
+  launch_gather_h(make_gather_args(c, P, nvecX, ks, Y, H), c->stream);
   return bF.nprim;
 }
 
@@ -984,23 +1092,8 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
     double* G = c->w_G.get<double>((size_t)nvecG * gstride);
     {
       Timed t(c, "expand_T");
-      ExpandTArgs ea{};
-      ea.ntot = ntotF;
-      ea.nprim = bF.nprim;
-      ea.sdeg = irF.sdeg;
-      ea.tcase = irF.tcase;
-      ea.case_ptr = bF.case_ptr;
-      ea.case_prim = bF.case_prim;
-      ea.icontr = bF.icontr;
-      ea.repres = irF.repres;
-      ea.H = H;
-      ea.hstride = hstride;
-      ea.h_of_vec = sm.h_of_vec;
-      ea.ideg_of_vec = sm.ideg_of_vec;
-      ea.nvec = nvecG;
-      ea.G = G;
-      ea.gstride = gstride;
-      launch_expand_T(ea, c->stream);
+      launch_expand_T(make_expand_T_args(P, H, sm, nvecG, G, gstride),
+                      c->stream);
     }
     const double* Crows = blkF.vec;
     if (plain && nI == 1) {
@@ -1194,40 +1287,12 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
   }
   {
     Timed t(c, "epilogue");
-    LinesArgs la{};
-    la.nI = nI;
-    la.nF = nF;
-    la.Flo = compute ? P.Flo : 0;
-    la.Fhi = compute ? P.Fhi : 0;
-    la.maxdeg = c->maxdeg;
-    la.nvec_per_i = nvpi;
-    la.slot_of_vec = sm.slot_of_vec;
+    LinesArgs la = make_lines_args(c, P, sm, nvpi, compute);
     la.src = src;
     la.ksplit = ksplit;
     la.src_ks = src_ks;
     la.src_f = src_f;
     la.src_v = src_v;
-    la.eI = sm.eI;
-    la.levelI = sm.levelI;
-    la.okI = sm.okI;
-    la.eF = blkF.d_energy;
-    la.same_block = (P.jindI == P.jindF && P.gI == P.gF);
-    la.block_ok = P.block_ok;
-    la.jI = P.bI->J;
-    la.jF = bF.J;
-    la.ndegI = p.sym_degen[P.gI];
-    la.ndegF = p.sym_degen[P.gF];
-    la.reduced = p.reduced;
-    la.gnsF = p.gns[P.gF];
-    la.zpe = p.zpe;
-    la.threshold = p.threshold;
-    la.beta = kPlanck * kVellgt / (kBoltz * p.temperature);
-    la.partition = p.partition;
-    la.fw0 = p.freq_window[0];
-    la.fw1 = p.freq_window[1];
-    la.eu0 = p.erange_upper[0];
-    la.eu1 = p.erange_upper[1];
-    la.isym_do_F = p.isym_do[P.gF];
     la.ls = o_ls;
     la.S = o_S;
     la.A = o_A;
@@ -1282,23 +1347,8 @@ void run_adjoint(gain_ctx* c, const Plan& P, const double* dS_dev,
   run_half_ls(c, P, st.cbase, sm, nvecX, H, GAIN_PREC_FP64);
   const double* X = (const double*)c->w_X.p;
   double* G = c->w_G.get<double>((size_t)nvecG * gstride);
-  ExpandTArgs ea{};
-  ea.ntot = ntotF;
-  ea.nprim = bF.nprim;
-  ea.sdeg = irF.sdeg;
-  ea.tcase = irF.tcase;
-  ea.case_ptr = bF.case_ptr;
-  ea.case_prim = bF.case_prim;
-  ea.icontr = bF.icontr;
-  ea.repres = irF.repres;
-  ea.H = H;
-  ea.hstride = hstride;
-  ea.h_of_vec = sm.h_of_vec;
-  ea.ideg_of_vec = sm.ideg_of_vec;
-  ea.nvec = nvecG;
-  ea.G = G;
-  ea.gstride = gstride;
-  launch_expand_T(ea, c->stream);
+  launch_expand_T(make_expand_T_args(P, H, sm, nvecG, G, gstride),
+                  c->stream);
   const double* Crows = blkF.vec + (int64_t)P.Flo * blkF.ld;
   // adjoint work buffers: LS and dLS share w_part, flags in w_out
   double* LS = c->w_part.get<double>((size_t)2 * nR * nvecG);
@@ -1306,39 +1356,10 @@ void run_adjoint(gain_ctx* c, const Plan& P, const double* dS_dev,
   gemm_projection(GAIN_PREC_FP64, nR, nvecG, ntotF, Crows, blkF.ld, G,
                   gstride, LS, nR, c->stream);
   uint8_t* flags = c->w_out.get<uint8_t>((size_t)nI * nF);
-  LinesArgs la{};
-  la.nI = nI;
-  la.nF = nF;
-  la.Flo = P.Flo;
-  la.Fhi = P.Fhi;
-  la.maxdeg = c->maxdeg;
-  la.nvec_per_i = nvpi;
-  la.slot_of_vec = sm.slot_of_vec;
+  LinesArgs la = make_lines_args(c, P, sm, nvpi, true);
   la.src = LS;
-  la.ksplit = 1;
   la.src_f = 1;
   la.src_v = nR;
-  la.eI = sm.eI;
-  la.levelI = sm.levelI;
-  la.okI = sm.okI;
-  la.eF = blkF.d_energy;
-  la.same_block = (P.jindI == P.jindF && P.gI == P.gF);
-  la.block_ok = P.block_ok;
-  la.jI = bI.J;
-  la.jF = bF.J;
-  la.ndegI = p.sym_degen[P.gI];
-  la.ndegF = p.sym_degen[P.gF];
-  la.reduced = p.reduced;
-  la.gnsF = p.gns[P.gF];
-  la.zpe = p.zpe;
-  la.threshold = p.threshold;
-  la.beta = kPlanck * kVellgt / (kBoltz * p.temperature);
-  la.partition = p.partition;
-  la.fw0 = p.freq_window[0];
-  la.fw1 = p.freq_window[1];
-  la.eu0 = p.erange_upper[0];
-  la.eu1 = p.erange_upper[1];
-  la.isym_do_F = p.isym_do[P.gF];
   la.flags = flags;
   launch_lines(la, c->stream);
   // ---- backward
@@ -1352,6 +1373,8 @@ void run_adjoint(gain_ctx* c, const Plan& P, const double* dS_dev,
   da.nvpi = nvpi;
   da.dS = dS_dev;
   da.flags = flags;
+  da.eI = sm.eI;
+  da.eF = blkF.d_energy;
   da.ls = LS;
   da.scale = (p.reduced && ndegF != 1 && ndegI != 1) ? 2.0
                                                      : 2.0 / (double)ndegI;
@@ -1587,10 +1610,13 @@ extern "C" int gain_upload_dipole(gain_ctx* c, const double* mu, int nc,
            "the engine keeps the whole dipole resident: need startF=0, "
            "endF=nc (got %d,%d of %d)", startF, endF, nc);
     DeviceGuard g(c->device);
+    ++c->gen;
     release_sliced(c, c->mu_sl);
     if (c->d_mu) c->arena.release(c->d_mu);
+    c->d_mu = nullptr;
     c->d_mu = c->arena.upload(mu, (size_t)3 * nc * nc);
     c->nc = nc;
+    build_slots_all(c);
   });
 }
 
@@ -1637,6 +1663,7 @@ extern "C" int gain_upload_basis(gain_ctx* c, int jind, int J, int nprim,
     b.tau = c->arena.upload(b.htau.data(), nprim);
     b.vib = c->arena.upload(b.hvib.data(), nprim);
     b.present = true;
+    if (c->d_mu && b.nvib <= c->nc) ensure_slots(c, b);
   });
 }
 
@@ -1707,6 +1734,7 @@ static void register_block(gain_ctx* c, int jind, int gamma, int nstates,
   DeviceGuard g(c->device);
   auto it = c->blocks.find({jind, gamma});
   if (it != c->blocks.end()) {
+    ++c->gen;
     release_sliced(c, it->second.sl);
     if (it->second.owned && it->second.vec)
       c->arena.release((void*)it->second.vec);
@@ -1951,10 +1979,12 @@ extern "C" int gain_set_dipole_combination(gain_ctx* c, int P,
     DeviceGuard g(c->device);
     if (c->d_mu && c->nc != nc)
       fail(GAIN_ERR_ARG, "dipole dimension %d differs from %d", nc, c->nc);
+    ++c->gen;
     release_sliced(c, c->mu_sl);
     if (!c->d_mu) {
       c->d_mu = c->arena.alloc<double>((size_t)3 * nc * nc);
       c->nc = nc;
+      build_slots_all(c);
     }
     double* dcoef = c->w_c.get<double>(P);
     GAIN_HIP(hipMemcpyAsync(dcoef, coef, sizeof(double) * P,

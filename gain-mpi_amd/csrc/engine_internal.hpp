@@ -215,6 +215,10 @@ struct gain_ctx {
   std::map<std::string, int64_t> last;
   bool timing = false;
   std::map<std::string, TimerSlot> timers;
+  // Bumped whenever device memory a captured sweep may hold is released or
+  // changed in place (dipole, replaced block, rebuilt slot tables); a sweep
+  // records it at creation and refuses to replay once it differs.
+  uint64_t gen = 0;
 };
 
 namespace gain {
@@ -299,6 +303,25 @@ gain_ctx* need(gain_ctx* c);
 Basis& basis_of(gain_ctx* c, int jind, bool need_infl);
 Block& block_of(gain_ctx* c, int jind, int gamma);
 void ensure_slots(gain_ctx* c, Basis& b);
+// Builds the scatter tables of every uploaded basis whose vibrational
+// indices fit the dipole, so later calls need no device allocation.
+void build_slots_all(gain_ctx* c);
+int mu_ksplit_auto(const gain_ctx* c, int ncolTot);
+ScatterArgs make_scatter_args(const gain_ctx* c, const Plan& P,
+                              const double* cbase, const Small& sm,
+                              int nvecX, double* X);
+GemmArgs make_mu_gemm_args(const gain_ctx* c, const double* X, int ncolTot,
+                           double* Y, int ks);
+GatherArgs make_gather_args(const gain_ctx* c, const Plan& P, int nvecX,
+                            int ks, const double* Y, double* H);
+ExpandTArgs make_expand_T_args(const Plan& P, const double* H,
+                               const Small& sm, int nvecG, double* G,
+                               int64_t gstride);
+LinesArgs make_lines_args(const gain_ctx* c, const Plan& P, const Small& sm,
+                          int nvpi, bool compute);
+// Tile configuration already fixed for a GEMM shape (forced or tuned), -1
+// for the default; never times anything.
+int gemm_cfg_lookup(const gain_ctx* c, int kind, const GemmArgs& g);
 int slices_of(int precision);
 int i8_slices_of(int precision);
 int crt_moduli_of(int precision);

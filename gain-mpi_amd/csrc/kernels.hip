@@ -535,7 +535,9 @@ __global__ __launch_bounds__(256) void k_dls(DlsArgs a) {
   const int i = v / a.nvpi;
   const int F = a.Flo + f;
   const int64_t o = (int64_t)i * a.nF + F;
-  const double g = (a.flags[o] & 1) ? a.dS[o] : 0.0;
+  // the forward leaves S = 0 for nu < 1e-40 (Output.cpp:65)
+  const double nu = a.eF[F] - a.eI[i];
+  const double g = ((a.flags[o] & 1) && !(nu < 1e-40)) ? a.dS[o] : 0.0;
   const int64_t p = (int64_t)v * a.nR + f;
   a.dls[p] = g * a.scale * a.ls[p];
 }

@@ -135,11 +135,14 @@ namespace gain {
 
 // ---- adjoint (backward of the batched path) ------------------------------
 // dLS[v][F-Flo] = dS[i][F] * dS/dls for every pair that passes the filters
+// and whose S is formed (nu >= 1e-40; below it S is 0 whatever ls is)
 // (S = sum ls^2 / ndegI, or sum ls^2 in the reduced degenerate case).
 struct DlsArgs {
   int nI, nF, Flo, nR, nvpi;
   const double* dS;       // [nI][nF]
   const uint8_t* flags;   // [nI][nF]
+  const double* eI;       // [nI] initial energies
+  const double* eF;       // [nF] final energies
   const double* ls;       // [nvec][nR] forward projection
   double scale;           // 2 / ndegI (or 2)
   double* dls;            // [nvec][nR]

@@ -97,6 +97,7 @@ struct gain_sweep {
   GroupTile* d_pj_tiles = nullptr;
   int n_pj_tiles = 0, pj_cfg = 5;
   int grid = 1024;
+  uint64_t gen = 0;  // context generation the captured pointers belong to
 };
 
 namespace {
@@ -659,6 +660,7 @@ extern "C" int gain_sweep_create(gain_ctx* c, const gain_task* tasks,
     }
     GAIN_HIP(hipStreamEndCapture(c->stream, &s->graph));
     GAIN_HIP(hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0));
+    s->gen = c->gen;
     *out = s;
   });
   if (rc != GAIN_OK && s) {
@@ -672,6 +674,11 @@ extern "C" int gain_sweep_create(gain_ctx* c, const gain_task* tasks,
 extern "C" int gain_sweep_run(gain_sweep* s) {
   return guarded([&] {
     GAIN_CHECK_ARG(s && s->exec, "null sweep");
+    if (s->gen != s->c->gen)
+      fail(GAIN_ERR_STATE,
+           "sweep invalidated: the dipole, a state block or a basis table "
+           "it captured was replaced or changed after gain_sweep_create; "
+           "create the sweep again");
     DeviceGuard g(s->c->device);
     GAIN_HIP(hipGraphLaunch(s->exec, s->c->stream));
   });

@@ -132,3 +132,87 @@ def test_adjoint_matches_dense_fp64_reference(setup):
     np.testing.assert_allclose(g_engine.numpy(), g_ref.cpu().numpy(),
                                rtol=1e-9, atol=1e-12 * float(
                                    g_ref.abs().max()))
+
+
+def _dense_model(prob, levels, jI=0, gI=0, jF=1, gF=1):
+    """Dense FP64 operators of the forward (reference layout of mu): the
+    textbook rotational factor R[alpha][r'][r] and the expanded records."""
+    bI, bF = prob.bases[jI], prob.bases[jF]
+    blkI, blkF = prob.blocks[(jI, gI)], prob.blocks[(jF, gF)]
+    tab = orc.threej_table(prob.jmax)
+    kI, kF = bI.kval(), bF.kval()
+    R = np.zeros((3, bF.nprim, bI.nprim))
+    for rp in range(bF.nprim):
+        for r in range(bI.nprim):
+            if abs(int(kI[r]) - int(kF[rp])) > 1:
+                continue
+            f3j = tab[orc.threej_index(prob.jmax, bI.J, int(kI[r]), bF.J,
+                                       int(kF[rp]))]
+            a, v = textbook_rot(bI.J, bF.J, int(kI[r]), int(bI.tau[r]),
+                                int(kF[rp]), int(bF.tau[rp]) & 1, f3j,
+                                sqrt2=SQRT2_REF_INV)
+            R[a, rp, r] = v
+    XI = np.stack([orc.expand(bI, gI, 0, blkI.vector(l)) for l in levels])
+    XF = np.stack([orc.expand(bF, gF, 0, blkF.vector(f))
+                   for f in range(blkF.n)])
+    return R, XI, XF
+
+
+def _dense_S(R, XI, XF, mu, vI, vF):
+    """S[i][F] = (x_F^T D x_I)^2 with D[r'][r] = sum_a R[a][r'][r]
+    mu[a][vib(r)][vib(r')] (one-dimensional irreps)."""
+    mug = mu[:, vI][:, :, vF].permute(0, 2, 1)
+    D = (R * mug).sum(0)
+    return (XI @ D.T @ XF.T) ** 2
+
+
+def test_adjoint_dmu_elementwise_vs_dense_autograd():
+    """dL/dmu of every dipole element (all 3 nc^2, not only its projection
+    onto a few basis matrices) against torch autograd through the dense FP64
+    model with mu itself as the leaf, for an unmasked loss linear in S and a
+    frequency window that starts below zero, so pairs with
+    fw0 <= nu < 1e-40 pass the filters but have S = 0 (Output.cpp:65): their
+    dS must not reach dL/dmu."""
+    import torch
+    prob = synth.make_problem([2, 3], NC, "pair2", blocks={(0, 0), (1, 1)})
+    prob.params.erange_lower = (0.0, 1e9)
+    prob.params.freq_window = (-20000.0, 15000.0)
+    # a final state degenerate with an initial one: nu = 0 passes the
+    # window but forms no line
+    blkI, blkF = prob.blocks[(0, 0)], prob.blocks[(1, 1)]
+    blkF.energy[3] = blkI.energy[4]
+    blkF.energy.sort()
+    levels = list(range(0, blkI.n, 2))
+    with Engine.from_problem(prob) as eng:
+        nF = blkF.n
+        dev = "cuda:0"
+        res = eng.batched_lines(0, 0, levels, 1, 1)
+        flags, S_eng = res["flags"], res["S"]
+        nu = blkF.energy[None, :] - blkI.energy[levels][:, None]
+        passed = (flags & 1) != 0
+        formed = passed & (nu >= 1e-40)
+        assert (passed & ~formed).any(), "no filtered pair with nu < 1e-40"
+        assert (S_eng[passed & ~formed] == 0).all()
+        w = torch.rand((len(levels), nF), dtype=torch.float64,
+                       generator=torch.Generator().manual_seed(1))
+        wd = w.to(dev).contiguous()
+        dmu = torch.zeros(3 * NC * NC, dtype=torch.float64, device=dev)
+        eng.adjoint(0, 0, levels, 1, 1, wd.data_ptr(), dmu.data_ptr())
+        eng.synchronize()
+        R, XI, XF = _dense_model(prob, levels)
+        mu = torch.tensor(prob.mu, dtype=torch.float64, device=dev,
+                          requires_grad=True)
+        vI = torch.as_tensor(prob.bases[0].vib, device=dev).long()
+        vF = torch.as_tensor(prob.bases[1].vib, device=dev).long()
+        T = lambda a: torch.as_tensor(a, device=dev)
+        S = _dense_S(T(R), T(XI), T(XF), mu, vI, vF)
+        np.testing.assert_allclose(S_eng[formed],
+                                   S.detach().cpu().numpy()[formed],
+                                   rtol=1e-10, atol=1e-14 * S_eng.max())
+        L = (wd * S * T(formed)).sum()
+        (g,) = torch.autograd.grad(L, mu)
+        g = g.reshape(-1)
+        scale = float(g.abs().max())
+        assert scale > 0
+        np.testing.assert_allclose(dmu.cpu().numpy(), g.cpu().numpy(),
+                                   rtol=1e-9, atol=1e-12 * scale)

@@ -197,3 +197,43 @@ def test_mixed_j_sweep_full_shape_matches_unbatched():
         sw.close()
     finally:
         w.close()
+
+
+@pytest.mark.parametrize("change", ["dipole", "block", "combination"])
+def test_sweep_refuses_replay_after_captured_data_changes(change):
+    """A sweep's graph holds raw device pointers: once the dipole is
+    uploaded again, a block it uses is replaced, or mu is changed in place,
+    gain_sweep_run returns GAIN_ERR_STATE without replaying."""
+    import torch
+    from gain_engine import _lib
+    prob = synth.make_problem([2, 3], 12, "c2v4")
+    rng = np.random.default_rng(3)
+    with Engine.from_problem(prob) as eng:
+        tasks, outs = _tasks(prob, eng, rng, 3)
+        sw = Sweep(eng, tasks)
+        sw.run()
+        eng.synchronize()
+        for o in outs:
+            o["flags"].fill_(7)
+        torch.cuda.synchronize()
+        if change == "dipole":
+            eng.upload_dipole(prob.mu)
+        elif change == "block":
+            (j, g), blk = sorted(prob.blocks.items())[0]
+            eng.upload_block(j, g, blk.energy, blk.vectors)
+        else:
+            M = torch.as_tensor(prob.mu.reshape(1, -1), device="cuda:0")
+            eng.set_dipole_combination(np.ones(1), M.data_ptr(), prob.nc)
+        with pytest.raises(_lib.GainError) as e:
+            sw.run()
+        assert e.value.code == _lib.ERR_STATE
+        eng.synchronize()
+        # nothing was replayed: the outputs keep the marker
+        assert all(bool((o["flags"] == 7).all()) for o in outs)
+        sw.close()
+        # a sweep created afterwards runs
+        sw2 = Sweep(eng, tasks)
+        sw2.run()
+        eng.synchronize()
+        assert _check(eng, tasks, outs) > 0
+        sw2.close()
