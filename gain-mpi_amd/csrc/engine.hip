@@ -1501,6 +1501,7 @@ extern "C" int gain_create(gain_ctx** out, int device, int jmax, int nrepres,
                         (kF - kI + 1) * n1 * n1 * 3] =
                   three_j(jI, 1, jF, kI, kF - kI, -kF);
       c->d_threej = c->arena.upload(c->threej.data(), c->threej.size());
+      noalloc_init_device(device);
     } catch (...) {
       if (c->stream) (void)hipStreamDestroy(c->stream);
       delete c;
@@ -1922,6 +1923,9 @@ extern "C" int gain_set_tuning(gain_ctx* c, const char* key, int value) {
     } else if (k == "i8_split") {
       GAIN_CHECK_ARG(value >= 0 && value <= 4, "split 0..4");
       c->force_i8_split = value;
+    } else if (k == "noalloc_cap") {
+      GAIN_CHECK_ARG(value >= 0, "cap >= 0 bytes (0: the whole array)");
+      c->noalloc_cap = value;
     } else if (k == "proj_cfg" || k == "half_ls_cfg") {
       GAIN_CHECK_ARG(value >= -1 && value < kGemmConfigs, "cfg -1..%d",
                      kGemmConfigs - 1);

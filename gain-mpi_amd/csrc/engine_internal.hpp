@@ -219,6 +219,9 @@ struct gain_ctx {
   // changed in place (dipole, replaced block, rebuilt slot tables); a sweep
   // records it at creation and refuses to replay once it differs.
   uint64_t gen = 0;
+  // cap (bytes) of the fixed scratch array the allocation-free entry point
+  // may use; 0 = the whole array
+  int64_t noalloc_cap = 0;
 };
 
 namespace gain {
@@ -337,5 +340,8 @@ int gemm_cfg(gain_ctx* c, int kind, GemmArgs g);
 int gemm_cfg_split(gain_ctx* c, int kind, GemmArgs g, int* ksplit,
                    bool tune_split);
 void resolve_timers(gain_ctx* c, TimerSlot& t);
+// Resolves this GPU's copy of the allocation-free entry point's scratch
+// array (noalloc.hip); called at context creation with `dev` current.
+void noalloc_init_device(int dev);
 
 }  // namespace gain

@@ -38,7 +38,7 @@ EXPORTS = (
     "gain_trove_basis_free", "gain_trove_basis_sizes", "gain_trove_basis_get",
     "gain_trove_upload_basis", "gain_trove_read_states",
     "gain_trove_read_records", "gain_trove_load",
-    "gain_sweep_run", "gain_sweep_destroy",
+    "gain_sweep_run", "gain_sweep_destroy", "gain_plain_lines_noalloc",
 )
 
 
@@ -123,6 +123,8 @@ def load():
                                    C.POINTER(Lines)]),
         "gain_batched_lines": (_ip, [_vp, _ip, _ip, _vp, _ip, _ip, _ip, _ip,
                                      C.POINTER(Lines)]),
+        "gain_plain_lines_noalloc": (_ip, [_vp, _ip, _ip, _ip, _vp, _ip, _ip,
+                                           C.POINTER(Lines)]),
         "gain_set_timing": (_ip, [_vp, _ip]),
         "gain_set_block_levels": (_ip, [_vp, _ip, _ip, _vp]),
         "gain_trove_read_dipole": (_ip, [C.c_char_p, C.POINTER(_ip), _vp]),

@@ -165,6 +165,26 @@ int gain_plain_lines(gain_ctx* ctx, int jindI, int gammaI, int levelI,
                      const double* cI, int jindF, int gammaF,
                      const gain_lines* out);
 
+/* Allocation-free form of gain_plain_lines.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same launches, arguments and results as gain_plain_lines, but
+ * every work buffer lives in one fixed device array of
+ * GAIN_NOALLOC_SCRATCH_BYTES that each GPU holds from the first context on
+ * it to process end, and the call makes no device or pinned-host
+ * allocation, memset or host-to-device copy.  cI_device: the initial record
+ * in device memory, or NULL for the uploaded block's record of levelI.
+ * `out` must be device memory (on_device = 1).  A call whose buffers exceed
+ * the array returns GAIN_ERR_ARG (the tuning keys "mu_ksplit" and
+ * "gemv_ksplit" shrink them).  Returns once queued on the context's stream
+ * (gain_synchronize waits).  Every call on one GPU shares that GPU's array,
+ * which keeps the last call's contents: calls that overlap on one GPU
+ * (several contexts, threads or streams) race. */
+#define GAIN_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int gain_plain_lines_noalloc(gain_ctx* ctx, int jindI, int gammaI,
+                             int levelI, const double* cI_device, int jindF,
+                             int gammaF, const gain_lines* out);
+
 /* Batched engine: nI initial states of block (jindI, gammaI) (levels
  * levelsI[0..nI), vectors from the uploaded block) against every final
  * state of (jindF, gammaF); half line strengths and projection run as
@@ -239,7 +259,8 @@ int gain_get_timing(gain_ctx* ctx, const char* name, double* total_ms,
  * 0..11, -1 = automatic; projection split count, 0 = automatic),
  * "crt_cfg" (CRT-form int8 GEMMs: force a tile configuration 0..13),
  * "gemv_rows" (2/4/8), "gemv_ksplit", "mu_ksplit" (0 = automatic),
- * "mfma" (0: vector-ALU GEMMs, for cross-checks). */
+ * "mfma" (0: vector-ALU GEMMs, for cross-checks), "noalloc_cap" (bytes of
+ * the allocation-free entry point's array a call may use, 0 = all). */
 int gain_set_tuning(gain_ctx* ctx, const char* key, int value);
 /* Current value of a settable key, or what the most recent launch used:
  * "proj_cfg", "proj_split", "proj_grid" (FP64 MFMA projection GEMM),
