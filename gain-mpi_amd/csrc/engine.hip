@@ -1440,6 +1440,7 @@ void run_adjoint(gain_ctx* c, const Plan& P, const double* dS_dev,
   }
   double* XT = c->w_tmp.get<double>((size_t)ncolTot * nc);
   launch_transpose(X, ncolTot, nc, XT, c->stream);
+  Timed tr(c, "adjoint_rankk");
   GemmArgs g{};
   g.M = nc;
   g.N = nc;

@@ -18,6 +18,7 @@ class LineStrengths(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, coef, eng, M, nc, tasks):
+        # (the engine's calls wait for torch's stream: Engine.wait_for_torch)
         eng.set_dipole_combination(coef.detach().cpu().numpy(),
                                    M.data_ptr(), nc)
         outs = []

@@ -9,6 +9,7 @@ of initial states (GEMM engine).
 from __future__ import annotations
 
 import ctypes as C
+import sys
 from typing import Dict, Optional, Sequence
 
 import numpy as np
@@ -50,9 +51,24 @@ class Engine:
         h = C.c_void_p()
         check(L.gain_create(C.byref(h), device, jmax, nrepres, maxdeg))
         self._L, self._h = L, h
+        self.device = device
         self.jmax, self.nrepres, self.maxdeg = jmax, nrepres, maxdeg
         self._keep = []
+        self._xs = None
         self.blocks: Dict = {}
+
+    def wait_for_torch(self):
+        """Order the engine's stream after the work torch has queued on its
+        current stream, so device tensors torch is still writing are
+        complete before engine kernels read or overwrite them (no host
+        synchronisation: the engine's stream waits on a torch event)."""
+        torch = sys.modules.get("torch")
+        if torch is None or not torch.cuda.is_initialized():
+            return
+        if self._xs is None:
+            self._xs = torch.cuda.ExternalStream(self.stream(),
+                                                 device=self.device)
+        self._xs.wait_stream(torch.cuda.current_stream(self.device))
 
     # ------------------------------------------------------------ lifecycle
     def close(self):
@@ -137,6 +153,7 @@ class Engine:
     def bind_block_device(self, jind, gamma, energies, dev_ptr: int,
                           rec_len: int, ld: int):
         e = np.ascontiguousarray(energies, _f64)
+        self.wait_for_torch()
         check(self._L.gain_bind_block_device(self._h, jind, gamma,
                                              e.shape[0], rec_len, ld,
                                              _ptr(e), dev_ptr))
@@ -209,6 +226,7 @@ class Engine:
     def lines_device(self, jindI, gammaI, levels, jindF, gammaF, ptrs: Dict,
                      precision=_lib.PREC_FP64, plain=False, cI=None):
         """Write results into caller device buffers (dict of name -> ptr)."""
+        self.wait_for_torch()
         s = _lib.Lines()
         for k in ("ls", "S", "A", "I", "flags"):
             setattr(s, k, ptrs.get(k))
@@ -293,6 +311,7 @@ def _attach_adjoint():
         device pointers (ints) when on_device, else numpy arrays."""
         lv = np.ascontiguousarray(levels, _i32)
         if on_device:
+            self.wait_for_torch()
             p_dS, p_dmu = int(dS), int(dmu)
         else:
             dS = np.ascontiguousarray(dS, _f64)
@@ -304,11 +323,13 @@ def _attach_adjoint():
 
     def set_dipole_combination(self, coef, M_ptr: int, nc: int):
         cf = np.ascontiguousarray(coef, _f64)
+        self.wait_for_torch()
         check(self._L.gain_set_dipole_combination(self._h, cf.shape[0],
                                                   _ptr(cf), M_ptr, nc))
 
     def dipole_coeff_grad(self, P: int, M_ptr: int, dmu_ptr: int):
         dc = np.zeros(P)
+        self.wait_for_torch()
         check(self._L.gain_dipole_coeff_grad(self._h, P, M_ptr, dmu_ptr,
                                              _ptr(dc)))
         return dc
@@ -346,6 +367,7 @@ class Sweep:
         self._h = h
 
     def run(self):
+        self._eng.wait_for_torch()
         check(self._eng._L.gain_sweep_run(self._h))
 
     def close(self):

@@ -167,3 +167,89 @@ def build_mixed(nc: int = 3000, jmin: int = 5, jmax: int = 30,
         j, g = pool[int(rng.integers(len(pool)))]
         picks.add((j, g, int(rng.integers(nstates))))
     return MixedWorkload(prob, eng, records, sorted(picks))
+
+
+def orthonormal_rows_device(n: int, dim: int, seed: int, dev):
+    """n orthonormal records of length dim (rows), from the QR factorisation
+    of a dim x n Gaussian matrix computed on the device (columns of Q with
+    the sign of R's diagonal folded in)."""
+    import torch
+    g = torch.Generator(device=dev).manual_seed(seed)
+    G = torch.randn((dim, n), generator=g, dtype=torch.float64, device=dev)
+    Q, R = torch.linalg.qr(G)
+    Q = Q * torch.sign(torch.diagonal(R))[None, :]
+    return Q.T.contiguous()
+
+
+def basis_matrices_device(P: int, nc: int, seed: int, dev):
+    """P dipole basis matrices M_p in the dipole layout [p][alpha][vI][vF],
+    Gaussian N(0, 0.1) damped by exp(-|a-b|/50), z symmetric, x and y
+    antisymmetric (the distribution of synth.make_mu), generated on the
+    device."""
+    import torch
+    g = torch.Generator(device=dev).manual_seed(seed)
+    a = torch.arange(nc, device=dev, dtype=torch.float64)
+    damp = torch.exp(-(a[:, None] - a[None, :]).abs() / 50.0)
+    M = torch.empty((P, 3, nc, nc), dtype=torch.float64, device=dev)
+    for p in range(P):
+        for s in range(3):
+            m = torch.randn((nc, nc), generator=g, dtype=torch.float64,
+                            device=dev) * 0.1 * damp
+            up = torch.triu(m, 1)
+            if s == 2:
+                mat = up + up.T + torch.diag(torch.diagonal(m))
+            else:
+                mat = up - up.T
+            M[p, s] = mat.T
+    return M
+
+
+@dataclasses.dataclass
+class AdjointWorkload:
+    """mu = sum_p coef[p] M_p over a (J_I, irrep 0) -> (J_F, irrep 1) pair
+    of the two-irrep scheme, records resident in HBM."""
+    prob: synth.Problem
+    eng: Engine
+    M: object      # torch [P, 3*nc*nc]
+    coef: np.ndarray
+    CI: object     # torch [nI, N_I]
+    CF: object     # torch [nF, N_F]
+
+    @property
+    def nc(self):
+        return self.prob.nc
+
+    def close(self):
+        self.eng.close()
+
+
+def build_adjoint(nc: int = 2000, jI: int = 10, jF: int = 11, P: int = 64,
+                  ninitial: int = 500, nfinal: int = 2000, seed: int = 5,
+                  device: int = 0) -> AdjointWorkload:
+    """The adjoint workload: mu = sum_{p<P} c_p M_p with M_p 3 x nc x nc,
+    c_p ~ N(0, 1); J_I -> J_F in the symmetric/antisymmetric pair scheme
+    (N_J = (2J+1) nc / 2: 21000 -> 23000 at nc = 2000); `ninitial` initial
+    states (irrep 0) x `nfinal` final states (irrep 1) with orthonormal
+    records from QR of a Gaussian, energies uniform and sorted."""
+    import torch
+    dev = torch.device(f"cuda:{device}")
+    ns = {(0, 0): ninitial, (1, 1): nfinal}
+    prob = synth.make_problem([jI, jF], nc, "pair2", blocks={(0, 0), (1, 1)},
+                              kron=True, nstates=ns)
+    blkI, blkF = prob.blocks[(0, 0)], prob.blocks[(1, 1)]
+    CI = orthonormal_rows_device(blkI.n, blkI.rec_len, seed, dev)
+    CF = orthonormal_rows_device(blkF.n, blkF.rec_len, seed + 1, dev)
+    M = basis_matrices_device(P, nc, seed + 2, dev).reshape(P, -1)
+    coef = np.random.default_rng(seed + 3).standard_normal(P)
+    torch.cuda.synchronize()
+    eng = Engine(device, prob.jmax, prob.params.nrepres, prob.params.maxdeg)
+    eng.set_params(prob.params)
+    eng.set_dipole_combination(coef, M.data_ptr(), nc)
+    for jind, bs in enumerate(prob.bases):
+        eng.upload_basis(jind, bs)
+        eng.upload_inflation(jind, bs)
+    eng.bind_block_device(0, 0, blkI.energy, CI.data_ptr(), blkI.rec_len,
+                          blkI.rec_len)
+    eng.bind_block_device(1, 1, blkF.energy, CF.data_ptr(), blkF.rec_len,
+                          blkF.rec_len)
+    return AdjointWorkload(prob, eng, M, coef, CI, CF)

@@ -246,8 +246,10 @@ int gain_dipole_coeff_grad(gain_ctx* ctx, int P, const double* M_device,
 /* ---- per-kernel timing (HIP events around each launch) ------------------ */
 int gain_set_timing(gain_ctx* ctx, int enable);
 /* Accumulated time (ms) and launch count of a named kernel family:
- * "gemv", "gemm_proj", "half_ls", "epilogue". Resets the counters if
- * reset != 0. */
+ * "gemv", "gemm_proj", "half_ls", "expand_T", "epilogue", "adjoint" (the
+ * backward of gain_adjoint_dipole after its forward recomputation) and
+ * "adjoint_rankk" (its rank-k update dL/dmu = dY X^T). Resets the counters
+ * if reset != 0. */
 int gain_get_timing(gain_ctx* ctx, const char* name, double* total_ms,
                     int64_t* launches, int reset);
 
