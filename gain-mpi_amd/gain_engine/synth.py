@@ -13,6 +13,14 @@ information the reference's host classes hand to its device layer:
 * a state table (energy-sorted, States.h:6-25) and eigenvector records, one
   record of rec_len doubles per state (EigenVector.cpp:152-178).
 
+Symmetry schemes (groups of consecutive primitives mixed by a fixed
+orthogonal matrix whose rows are split among the irreps): "c1"; "c2v4"
+(Hadamard quartets, 4 one-dimensional irreps); "pair2" (symmetric and
+antisymmetric pairs); "c3v3" (triplets -> A1, A2 empty, E; C3v selection and
+reduced degeneracy rules); "td5" (decades -> A1, A2, E, F1, F2 with the
+degeneracies, pairs A1-A2, E-E, F1-F2 and reduced degeneracy rules of Td,
+States.cpp:108-115).
+
 Distributions: mu_alpha entries N(0, 0.1) damped by exp(-|a-b|/50), z
 symmetric, x and y antisymmetric; eigenvectors orthonormal (QR of a
 Gaussian, or a Kronecker product of two such factors for blocks too large to
@@ -194,6 +202,10 @@ def _scheme(name: str):
                       np.sqrt(2.0 / 3.0) * np.cos(th),
                       np.sqrt(2.0 / 3.0) * np.sin(th)])
         return 3, [1, 1, 2], 3, [o[0:1], np.zeros((1, 3)), o[1:3]]
+    if name == "td5":  # decades -> A1, A2, E (2), F1 (3), F2 (3) (Td order)
+        q = orthonormal(10, np.random.default_rng(5050))
+        return 5, [1, 1, 2, 3, 3], 10, [q[0:1], q[1:2], q[2:4], q[4:7],
+                                        q[7:10]]
     raise ValueError(name)
 
 
@@ -281,11 +293,11 @@ def make_problem(jvals, nc: int, scheme: str = "c1", blocks=None,
     nrep, sdeg, _, _ = _scheme(scheme)
     if params is None:
         pairs = {"c1": [1], "c2v4": [1, 1, 2, 2], "pair2": [1, 1],
-                 "c3v3": [1, 1, 2]}[scheme]
+                 "c3v3": [1, 1, 2], "td5": [1, 1, 2, 3, 3]}[scheme]
+        stype = {"c3v3": 2, "td5": 3}.get(scheme, 1)
         params = Params(nrepres=nrep, sym_degen=list(sdeg),
                         gns=[1.0] * nrep, isym_pairs=pairs, maxJ=max(jvals),
-                        symmetry_type=2 if scheme == "c3v3" else 1,
-                        reduced=(scheme == "c3v3"))
+                        symmetry_type=stype, reduced=stype > 1)
     mu = make_mu(nc, SEED_MU + seed_offset)
     rng_e = np.random.default_rng(SEED_E + seed_offset)
     rng_c = np.random.default_rng(SEED_CF + seed_offset)

@@ -237,3 +237,24 @@ def test_sweep_refuses_replay_after_captured_data_changes(change):
         eng.synchronize()
         assert _check(eng, tasks, outs) > 0
         sw2.close()
+
+
+@pytest.mark.parametrize("reduced", [True, False])
+@pytest.mark.parametrize("precision", [0, 8])
+def test_mixed_j_sweep_td(reduced, precision):
+    """Td symmetry (3-dimensional irreps; unreduced: nine projected vectors
+    per F1 <-> F2 initial state, the largest GEMV pack) through a captured
+    sweep, against the unbatched plain FP64 path."""
+    prob = synth.make_problem([2, 3, 4], 20, "td5")
+    prob.params.reduced = reduced
+    prob.params.erange_lower = (0.0, 1e9)
+    rng = np.random.default_rng(5)
+    with Engine.from_problem(prob) as eng:
+        tasks, outs = _tasks(prob, eng, rng, max_per_block=6)
+        sw = Sweep(eng, tasks, precision=precision)
+        for _ in range(2):
+            sw.run()
+            eng.synchronize()
+            assert _check(eng, tasks, outs,
+                          rel=1e-12 if precision == 0 else 1e-7) > 0
+        sw.close()

@@ -128,10 +128,10 @@ def _dense_P(b, g, d):
     return P
 
 
-@pytest.mark.parametrize("scheme", ["c2v4", "c3v3", "pair2"])
+@pytest.mark.parametrize("scheme", ["c2v4", "c3v3", "pair2", "td5"])
 def test_expand_matches_dense_operator(scheme):
     rng = np.random.default_rng(11)
-    nc = {"c2v4": 8, "c3v3": 6, "pair2": 6}[scheme]
+    nc = {"c2v4": 8, "c3v3": 6, "pair2": 6, "td5": 10}[scheme]
     b = synth.make_basis(2, nc, scheme)
     for g in range(len(b.ntot)):
         if b.ntot[g] == 0:
@@ -144,6 +144,14 @@ def test_expand_matches_dense_operator(scheme):
             # every component is an isometry for these generators
             np.testing.assert_allclose(P.T @ P, np.eye(b.ntot[g]),
                                        atol=1e-14)
+    # the components of all irreps together are an orthogonal change of
+    # basis of the primitives
+    cols = [_dense_P(b, g, d) for g in range(len(b.ntot))
+            for d in range(b.sdeg[g]) if b.ntot[g]]
+    Pall = np.hstack(cols)
+    if Pall.shape[1] == b.nprim:
+        np.testing.assert_allclose(Pall.T @ Pall, np.eye(b.nprim),
+                                   atol=1e-14)
 
 
 # ---------------------------------------------------------------- half ls
