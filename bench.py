@@ -49,7 +49,8 @@ PEAKS = {0: 78.6, 1: 157.3, 2: 2500.0, 3: 2500.0, 4: 5000.0, 5: 5000.0,
 PRODUCTS = {0: 1, 1: 1, 2: 6, 3: 10, 4: 15, 5: 21, 6: 28, 7: 11, 8: 12,
             9: 14}
 PRODUCTS_SLICES = {4: 5, 5: 6, 6: 7}  # split-INT8 slice counts
-TIMERS = ("gemv", "half_ls", "expand_T", "epilogue", "gemm_proj")
+TIMERS = ("gemv", "half_ls", "expand_T", "epilogue", "gemm_proj",
+          "proj_gemm", "half_ls_gemm")
 # rocprofv3 FETCH_SIZE / WRITE_SIZE per launch for this workload, scaled by
 # the same counters' reading of a 4 GB device copy (profiles/ holds the
 # commands that produced it)
@@ -428,7 +429,7 @@ def main():
     import numpy as np
     import torch
 
-    from gain_engine import workload
+    from gain_engine import synth, workload
 
     t0 = time.time()
     wl = workload.build(nc=args.nc, ninitial=args.ninitial)
@@ -554,28 +555,42 @@ def main():
         torch.cuda.synchronize()
         ms = ev0.elapsed_time(ev1)
         trans = int((bflags & 1).sum().item())
-        gp_ms, gp_n = eng.timing("gemm_proj")
-        hb_ms, hb_n = eng.timing("half_ls")
-        gp_avg = gp_ms / max(gp_n, 1)
+        def avg(name):
+            t, n = eng.timing(name)
+            return t / max(n, 1)
+        gp_avg, pg_avg = avg("gemm_proj"), avg("proj_gemm")
+        hb_avg, hg_avg = avg("half_ls"), avg("half_ls_gemm")
         mrows = np.mean([workload.rows_in_window(
             prob, blkF.energy, [blkI.energy[lv] for lv in levels_of(st)])
             for st in range(nsteps)])
+        # algorithmic flops: projection 2 rows N_F per vector; half line
+        # strength 2 * 3 Nc^2 (2 J_I + 1) per initial state
         flops = 2.0 * mrows * blkF.rec_len * B
-        tfs = flops / (gp_avg / 1e3) / 1e12
+        hflops = 2.0 * 3 * args.nc ** 2 * (2 * prob.bases[0].J + 1) * B
+        tfs = flops / (pg_avg / 1e3) / 1e12
+        htfs = hflops / (hg_avg / 1e3) / 1e12
         prod = PRODUCTS[prec]
         r = {"precision": prec, "dtype": PRECISION_NAMES[prec],
              "transitions": trans, "ms": ms, "tps": trans / (ms / 1e3),
              "speedup_vs_plain": trans / (ms / 1e3) / plain_tps,
-             "gemm_proj_ms": gp_avg, "gemm_proj_tflops": tfs,
-             "half_ls_ms": hb_ms / max(hb_n, 1),
-             "roofline": {"bound": "mfma", "kernel": "gemm_proj",
+             "gemm_proj_ms": gp_avg, "proj_gemm_ms": pg_avg,
+             "gemm_proj_tflops": tfs,
+             "half_ls_ms": hb_avg, "half_ls_gemm_ms": hg_avg,
+             "half_ls_gemm_tflops": htfs,
+             "roofline": {"bound": "mfma", "kernel": "proj_gemm",
                           "achieved": tfs, "peak": PEAKS[prec] / prod,
                           "unit": "TFLOP/s",
-                          "frac": tfs * prod / PEAKS[prec]}}
+                          "frac": tfs * prod / PEAKS[prec]},
+             "roofline_half_ls": {"bound": "mfma", "kernel": "half_ls_gemm",
+                                  "achieved": htfs,
+                                  "peak": PEAKS[prec] / prod,
+                                  "unit": "TFLOP/s",
+                                  "frac": htfs * prod / PEAKS[prec]}}
         log(f"batched[{PRECISION_NAMES[prec]}]: {nsteps} x {B} initial "
             f"states, {trans} transitions in {ms:.1f} ms -> "
-            f"{r['tps']:.3e}/s; projection avg {gp_avg:.2f} ms = {tfs:.1f} "
-            f"algorithmic TF/s; half_ls avg {r['half_ls_ms']:.2f} ms")
+            f"{r['tps']:.3e}/s; projection {gp_avg:.2f} ms (GEMM "
+            f"{pg_avg:.2f} ms = {tfs:.1f} algorithmic TF/s); half_ls "
+            f"{hb_avg:.2f} ms (GEMM {hg_avg:.2f} ms = {htfs:.1f} TF/s)")
         return r
 
     def s_of_batch0(prec):
@@ -633,6 +648,68 @@ def main():
                    "split_k": eng.get_tuning("proj_split"),
                    "grid_items_batch0": eng.get_tuning("proj_grid")}
     proj_traffic = measured_traffic(label, proj_launch["grid_items_batch0"])
+    prefix_h = "half_ls" if tag is None else f"{tag} half_ls"
+    chosen_h = [ln for ln in tune_log.splitlines()
+                if ln.startswith(prefix_h) and "->" in ln]
+    label_h = chosen_h[-1].split("->")[1].split()[0] if chosen_h else None
+    if label_h is not None and tag is not None:
+        label_h = f"{tag} {label_h}"
+    hl_launch = {"tiles": label_h,
+                 "grid_items_batch0": eng.get_tuning("half_ls_grid")}
+    hl_traffic = measured_traffic(label_h, hl_launch["grid_items_batch0"])
+    head["roofline_half_ls"]["traffic"] = hl_traffic
+    head["roofline_half_ls"]["launch"] = hl_launch
+    # one-time work of the headline context: planes of mu and of the final
+    # block, and the autotuner's timing of every tile configuration
+    prep = {"planes_ms": eng.timing("prep")[0],
+            "autotune_s": eng.get_tuning("autotune_us") / 1e6,
+            "note": "planes_ms = HIP-event time of building the reduced-"
+                    "precision planes of mu and of the final block (once per "
+                    "context and precision, all precisions timed here); "
+                    "autotune_s = host time of timing every tile "
+                    "configuration of every GEMM shape (once per shape)"}
+    log(f"one-time work: planes {prep['planes_ms']:.1f} ms, autotune "
+        f"{prep['autotune_s']:.1f} s")
+
+    # ---------------- cold start: exactly the 2000-state workload on a
+    # fresh context (planes built and tiles tuned inside the timed region)
+    def cold_run(tune):
+        e2 = workload.attach(wl)
+        if tune:
+            e2.set_tuning("autotune", 1)
+        e2.set_timing(True)
+        cfl = torch.zeros((nI_all, nF), dtype=torch.uint8, device=dev)
+        o2 = {k: torch.zeros((B, nF), dtype=torch.float64, device=dev)
+              for k in ("S", "A", "I")}
+        torch.cuda.synchronize()
+        t0c = time.time()
+        for b0 in range(0, nI_all, B):
+            lv = list(range(b0, min(nI_all, b0 + B)))
+            e2.lines_device(0, 0, lv, 1, 1,
+                            {k: v.data_ptr() for k, v in o2.items()} |
+                            {"flags": cfl[b0].data_ptr()},
+                            precision=args.precision)
+        e2.synchronize()
+        wall = time.time() - t0c
+        ntr = int((cfl & 1).sum().item())
+        planes = e2.timing("prep")[0]
+        tune_s = e2.get_tuning("autotune_us") / 1e6 if tune else 0.0
+        e2.close()
+        del cfl, o2
+        torch.cuda.empty_cache()
+        return {"transitions": ntr, "wall_s": wall,
+                "value": ntr / wall, "planes_ms": planes,
+                "autotune_s": tune_s}
+    cold = {"untuned": cold_run(False), "autotuned": cold_run(True),
+            "initial_states": nI_all, "batch": B,
+            "note": "fresh context over the HBM-resident records: every "
+                    "initial state in batches of `batch`, planes of mu and "
+                    "the final block built inside the timed region; "
+                    "'untuned' runs the default tiles, 'autotuned' also "
+                    "times every tile configuration on first use"}
+    log(f"cold start: {cold['untuned']['value']:.3e}/s untuned, "
+        f"{cold['autotuned']['value']:.3e}/s with autotuning "
+        f"({cold['autotuned']['autotune_s']:.1f} s of it)")
     tps = head["tps"]
     batched_ms = head["ms"]
     gemm_tfs = head["gemm_proj_tflops"]
@@ -665,8 +742,28 @@ def main():
                          f"{cpu_s:.1f} s: the reference's per-pair loop "
                          f"(expand + half line strength per initial, expand "
                          f"+ dot per final) in plain FP64 with OpenMP"}
+        # the reference's own CPU-sized case in full: J=0 -> J=1, Nc=500,
+        # C1, the 50 lowest J=0 states against every J=1 state
+        prob0 = synth.make_problem([0, 1], 500, "c1")
+        rows0 = prob0.state_table()
+        init0 = [i for i, r in enumerate(rows0) if r[1] == 0][:50]
+        st0 = {}
+        tr0, _ = orc.run(prob0, init0, nthreads=thr, plain_fp64=True,
+                         stats=st0)
+        cpu["host_cpus"] = os.cpu_count()
+        cpu["host_cpus_usable"] = len(os.sched_getaffinity(0))
+        cpu["config0_full"] = {
+            "value": len(tr0) / st0["seconds"], "unit": "transitions/s",
+            "transitions": len(tr0), "seconds": st0["seconds"],
+            "cores": thr,
+            "workload": "J_I=0 -> J_F=1, Nc=500, C1 (N_0=500, N_1=1500): "
+                        "all 50 lowest J=0 states x every J=1 state, the "
+                        "reference's per-pair loop in plain FP64"}
         log(f"cpu baseline: {cpu['value']:.3e} transitions/s on {thr} "
-            f"threads ({cpu_s:.1f}s)")
+            f"threads ({cpu_s:.1f}s); config 0 in full: {len(tr0)} "
+            f"transitions in {st0['seconds']:.2f} s "
+            f"({cpu['config0_full']['value']:.3e}/s); host cpus "
+            f"{cpu['host_cpus']} ({cpu['host_cpus_usable']} usable)")
 
     prod = PRODUCTS[args.precision]
     res = {
@@ -691,18 +788,27 @@ def main():
                   "initial_states": args.plain_states,
                   "ms_per_initial": plain_ms / args.plain_states},
         "speedup_vs_plain": tps / plain_tps,
-        "roofline": {"bound": "mfma", "kernel": "gemm_proj",
+        "roofline": {"bound": "mfma", "kernel": "proj_gemm",
                      "achieved": gemm_tfs, "peak": PEAKS[args.precision] /
                      prod, "unit": "TFLOP/s",
                      "frac": gemm_tfs * prod / PEAKS[args.precision],
                      "note": "achieved = algorithmic 2*M*N*K flops / avg "
-                             "time of the projection (the slicing of the "
-                             "projected vectors included for split modes); "
+                             "HIP-event time of the projection's product "
+                             "kernel (CRT form: k_gemm_i8_res_lds, all "
+                             "moduli in one launch; the reconstruction "
+                             "k_crt_combine is timed in gemm_proj_ms); "
                              "peak = dense MFMA peak of the datatype (int8: "
-                             "TOP/s) / hardware products per FMA (split-INT8 "
-                             "x S: S(S+1)/2); traffic = counter HBM bytes of "
-                             "the chosen launch on batch 0",
+                             "TOP/s) / hardware products per FMA (CRT: one "
+                             "per modulus; split-INT8 x S: S(S+1)/2); "
+                             "traffic = counter HBM bytes of the chosen "
+                             "launch on batch 0; roofline_half_ls is the "
+                             "same for the mu X product kernel",
                      "traffic": proj_traffic, "launch": proj_launch},
+        "roofline_half_ls": head["roofline_half_ls"],
+        "step_ms": {k: head[k] for k in ("gemm_proj_ms", "proj_gemm_ms",
+                                         "half_ls_ms", "half_ls_gemm_ms")},
+        "prep": prep,
+        "cold_start": cold,
         "precision_check": prec_check,
         "variants": variants,
         "autotune": tune_log.strip().splitlines(),

@@ -157,6 +157,7 @@ void release_sliced(gain_ctx* c, Sliced& sl) {
 // slice count.
 const Sliced& mu_sliced(gain_ctx* c, int S) {
   if (c->mu_sl.S == S && !c->mu_sl.i8 && !c->mu_sl.crt) return c->mu_sl;
+  Timed t(c, "prep");  // one-time build of a static operand's planes
   release_sliced(c, c->mu_sl);
   const int nc = c->nc;
   double* muT = c->w_tmp.get<double>((size_t)3 * nc * nc);
@@ -174,6 +175,7 @@ const Sliced& mu_sliced(gain_ctx* c, int S) {
 // Sliced copy of a block's eigenvector records, built once per slice count.
 const Sliced& block_sliced(gain_ctx* c, Block& b, int S) {
   if (b.sl.S == S && !b.sl.i8 && !b.sl.crt) return b.sl;
+  Timed t(c, "prep");  // one-time build of a static operand's planes
   release_sliced(c, b.sl);
   Sliced sl;
   sl.S = S;
@@ -190,6 +192,7 @@ const Sliced& block_sliced(gain_ctx* c, Block& b, int S) {
 // (rows v_F), built once per slice count.
 const Sliced& mu_sliced_i8(gain_ctx* c, int S) {
   if (c->mu_sl.S == S && c->mu_sl.i8 && !c->mu_sl.crt) return c->mu_sl;
+  Timed t(c, "prep");  // one-time build of a static operand's planes
   release_sliced(c, c->mu_sl);
   const int nc = c->nc;
   double* muT = c->w_tmp.get<double>((size_t)3 * nc * nc);
@@ -214,6 +217,7 @@ const Sliced& mu_sliced_i8(gain_ctx* c, int S) {
 // count.
 const Sliced& block_sliced_i8(gain_ctx* c, Block& b, int S) {
   if (b.sl.S == S && b.sl.i8 && !b.sl.crt) return b.sl;
+  Timed t(c, "prep");  // one-time build of a static operand's planes
   release_sliced(c, b.sl);
   Sliced sl;
   sl.S = S;
@@ -233,6 +237,7 @@ const Sliced& block_sliced_i8(gain_ctx* c, Block& b, int S) {
 // component, built once per moduli count.
 const Sliced& mu_sliced_crt(gain_ctx* c, int nm) {
   if (c->mu_sl.S == nm && c->mu_sl.crt) return c->mu_sl;
+  Timed t(c, "prep");  // one-time build of a static operand's planes
   release_sliced(c, c->mu_sl);
   const int nc = c->nc;
   double* muT = c->w_tmp.get<double>((size_t)3 * nc * nc);
@@ -259,6 +264,7 @@ const Sliced& mu_sliced_crt(gain_ctx* c, int nm) {
 // moduli count.
 const Sliced& block_sliced_crt(gain_ctx* c, Block& b, int nm) {
   if (b.sl.S == nm && b.sl.crt) return b.sl;
+  Timed t(c, "prep");  // one-time build of a static operand's planes
   release_sliced(c, b.sl);
   Sliced sl;
   sl.S = nm;
@@ -288,14 +294,15 @@ int8_t* crt_residues(gain_ctx* c, I8GemmArgs g) {
 
 // Tile configuration of a CRT-form GEMM (kind 0: projection, 1: half line
 // strength), timed on the real operands once per shape when autotuning;
-// without tuning the 64 x 128 tile.
+// without tuning kCrtDefaultConfig.
 int crt_cfg_of(gain_ctx* c, int kind, I8GemmArgs g) {
   if (c->force_crt_cfg >= 0) return c->force_crt_cfg;
   const auto key = std::make_tuple(200 + 10 * kind + g.S,
                                    g.M >= 8192 ? -1 : g.M, g.N, (int)g.Kp);
   auto it = c->tuned.find(key);
   if (it != c->tuned.end()) return it->second;
-  if (!c->autotune) return 0;
+  if (!c->autotune) return kCrtDefaultConfig;
+  TuneClock clk(c);
   hipEvent_t e0, e1;
   GAIN_HIP(hipEventCreate(&e0));
   GAIN_HIP(hipEventCreate(&e1));
@@ -358,6 +365,7 @@ int i8_cfg_split(gain_ctx* c, int kind, I8GemmArgs g, int max_split,
   }
   *ksplit = min_split;
   if (!c->autotune) return 0;
+  TuneClock clk(c);
   hipEvent_t e0, e1;
   GAIN_HIP(hipEventCreate(&e0));
   GAIN_HIP(hipEventCreate(&e1));
@@ -566,6 +574,7 @@ int gemm_cfg_split(gain_ctx* c, int kind, GemmArgs g, int* ksplit,
     if (tune_split) *ksplit = std::max(1, c->force_split);
     return -1;
   }
+  TuneClock clk(c);
   hipEvent_t e0, e1;
   GAIN_HIP(hipEventCreate(&e0));
   GAIN_HIP(hipEventCreate(&e1));
@@ -843,7 +852,11 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
     g.C = Y;
     g.R = crt_residues(c, g);
     g.cfg = crt_cfg_of(c, 1, g);
-    gemm_crt_i8(g, c->stream);
+    {
+      Timed tg(c, "half_ls_gemm");
+      gemm_crt_res(g, c->stream);
+    }
+    gemm_crt_combine(g, c->stream);
     c->last["half_ls_cfg"] = g.cfg;
     c->last["half_ls_split"] = 1;
     c->last["half_ls_grid"] = gemm_crt_grid_items(g);
@@ -877,7 +890,10 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
     g.C = Y;
     g.cfg = i8_cfg_split(c, 1, g, 1, &ks);
     g.ksplit = ks;
-    gemm_sliced_i8(g, c->stream);
+    {
+      Timed tg(c, "half_ls_gemm");
+      gemm_sliced_i8(g, c->stream);
+    }
     c->last["half_ls_cfg"] = g.cfg;
     c->last["half_ls_split"] = ks;
     c->last["half_ls_grid"] = gemm_i8_grid_items(g);
@@ -902,7 +918,10 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
   } else if (c->use_mfma) {
     GemmArgs g = make_mu_gemm_args(c, X, ncolTot, Y, ks);
     g.cfg = gemm_cfg(c, 1, g);
-    gemm_f64(g, c->stream);
+    {
+      Timed tg(c, "half_ls_gemm");
+      gemm_f64(g, c->stream);
+    }
     c->last["half_ls_cfg"] = g.cfg < 0 ? gemm_default_config(g.N) : g.cfg;
     c->last["half_ls_split"] = ks;
     c->last["half_ls_grid"] = gemm_grid_items(g);
@@ -1013,6 +1032,7 @@ void gemv_tune(gain_ctx* c, const double* C, int64_t ld, int ntot, int F0,
                int nR, const double* G, int64_t gstride, int nvec) {
   const auto key = std::make_tuple(2, nR >= 8192 ? -1 : nR, nvec, ntot);
   if (c->tuned.count(key)) return;
+  TuneClock clk(c);
   const int rows_opt[3] = {2, 4, 8};
   const int ks_opt[5] = {2, 4, 8, 16, 32};
   hipEvent_t e0, e1;
@@ -1189,7 +1209,11 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
         g.nz = 1;
         g.R = crt_residues(c, g);
         g.cfg = crt_cfg_of(c, 0, g);
-        gemm_crt_i8(g, c->stream);
+        {
+          Timed tg(c, "proj_gemm");
+          gemm_crt_res(g, c->stream);
+        }
+        gemm_crt_combine(g, c->stream);
         c->last["proj_cfg"] = g.cfg;
         c->last["proj_split"] = 1;
         c->last["proj_grid"] = gemm_crt_grid_items(g);
@@ -1219,7 +1243,10 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
         int ks = 1;
         g.cfg = i8_cfg_split(c, 0, g, 4, &ks);
         g.ksplit = ks;
-        gemm_sliced_i8(g, c->stream);
+        {
+          Timed tg(c, "proj_gemm");
+          gemm_sliced_i8(g, c->stream);
+        }
         c->last["proj_cfg"] = g.cfg;
         c->last["proj_split"] = ks;
         c->last["proj_grid"] = gemm_i8_grid_items(g);
@@ -1245,7 +1272,10 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
         int ks = 1;
         g.cfg = gemm_cfg_split(c, 0, g, &ks, true);
         g.ksplit = ks;
-        gemm_f64(g, c->stream);
+        {
+          Timed tg(c, "proj_gemm");
+          gemm_f64(g, c->stream);
+        }
         c->last["proj_cfg"] = g.cfg < 0 ? gemm_default_config(g.N) : g.cfg;
         c->last["proj_split"] = ks;
         c->last["proj_grid"] = gemm_grid_items(g);

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -258,6 +259,21 @@ struct Timed {
     if (!slot) return;
     (void)hipEventRecord(ev.second, c->stream);
     slot->pending.push_back(ev);
+  }
+};
+
+// Host wall clock of an autotuning pass, added to the "autotune_us"
+// tuning counter.
+struct TuneClock {
+  gain_ctx* c;
+  std::chrono::steady_clock::time_point t0;
+  explicit TuneClock(gain_ctx* ctx)
+      : c(ctx), t0(std::chrono::steady_clock::now()) {}
+  ~TuneClock() {
+    c->last["autotune_us"] +=
+        std::chrono::duration_cast<std::chrono::microseconds>(
+            std::chrono::steady_clock::now() - t0)
+            .count();
   }
 };
 

@@ -158,12 +158,20 @@ void slice_rows_crt(int nm, int beta, const double* src, int64_t ld,
                     int rows, int K, int8_t* planes, int64_t plane,
                     int* expo, hipStream_t s);
 constexpr int kCrtConfigs = 14;
+// Tile configuration used when nothing is tuned or forced: 256 x 256
+// workgroup tiles through the 4-stage LDS pipeline, the autotuner's pick for
+// both CRT shapes of the J=20 -> 21 workload.
+constexpr int kCrtDefaultConfig = 5;
 const char* crt_config_name(int cfg);
 // C[z][n*ldc + m] = 2^{eA + eB - 2(beta-1)} * (exact integer product of the
 // operands), g.S = number of moduli, g.beta as sliced, no split-K: one
 // int8 GEMM per (operand matrix, modulus) writing int8 residues to g.R,
 // then the reconstruction kernel.
 void gemm_crt_i8(const I8GemmArgs& g, hipStream_t s);
+// The two launches of gemm_crt_i8: the per-modulus int8 products with their
+// residue epilogue, then the reconstruction into FP64 (timed apart).
+void gemm_crt_res(const I8GemmArgs& g, hipStream_t s);
+void gemm_crt_combine(const I8GemmArgs& g, hipStream_t s);
 int64_t gemm_crt_res_bytes(const I8GemmArgs& g);
 // Work-items of the per-modulus GEMM launch.
 int64_t gemm_crt_grid_items(const I8GemmArgs& g);

@@ -1249,6 +1249,11 @@ int64_t gemm_crt_grid_items(const I8GemmArgs& g) {
 }
 
 void gemm_crt_i8(const I8GemmArgs& g, hipStream_t s) {
+  gemm_crt_res(g, s);
+  gemm_crt_combine(g, s);
+}
+
+void gemm_crt_res(const I8GemmArgs& g, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0) return;
   GAIN_CHECK_ARG(g.cfg >= 0 && g.cfg < kCrtConfigs, "CRT gemm: config %d",
                  g.cfg);
@@ -1304,6 +1309,12 @@ void gemm_crt_i8(const I8GemmArgs& g, hipStream_t s) {
 #undef GAIN_CRT_LDS
   }
   check_launch("gemm_i8_res");
+}
+
+void gemm_crt_combine(const I8GemmArgs& g, hipStream_t s) {
+  if (g.M <= 0 || g.N <= 0) return;
+  const CrtLaunch L = crt_launch(g);
+  const CrtConsts& cc = crt_consts(g.S);
   // residue rows [row0 - 32 rb0, row0 - 32 rb0 + M): one workgroup per
   // row block and 32 columns
   const int rspan = g.row0 - L.rb0 * 32 + g.M;

@@ -69,6 +69,24 @@ def build(nc: int = 4000, jI: int = 20, jF: int = 21, ninitial: int = 2000,
     return Workload(prob, eng, CF, CI, nI)
 
 
+def attach(w: Workload, device: int = 0) -> Engine:
+    """A second, fresh context over the same HBM-resident records (nothing
+    sliced, cached or tuned yet): for cold-start timings."""
+    prob = w.prob
+    eng = Engine(device, prob.jmax, prob.params.nrepres, prob.params.maxdeg)
+    eng.set_params(prob.params)
+    eng.upload_dipole(prob.mu)
+    for jind, bs in enumerate(prob.bases):
+        eng.upload_basis(jind, bs)
+        eng.upload_inflation(jind, bs)
+    blkI, blkF = w.blkI, w.blkF
+    eng.bind_block_device(0, 0, blkI.energy[:w.ninitial], w.CI.data_ptr(),
+                          blkI.rec_len, blkI.rec_len)
+    eng.bind_block_device(1, 1, blkF.energy, w.CF.data_ptr(), blkF.rec_len,
+                          blkF.rec_len)
+    return eng
+
+
 def rows_in_window(prob, eF, eIs) -> int:
     """Final rows the engine streams for these initial energies: the union
     over initials of the states whose energy passes the frequency and upper

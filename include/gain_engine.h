@@ -246,7 +246,11 @@ int gain_dipole_coeff_grad(gain_ctx* ctx, int P, const double* M_device,
 /* ---- per-kernel timing (HIP events around each launch) ------------------ */
 int gain_set_timing(gain_ctx* ctx, int enable);
 /* Accumulated time (ms) and launch count of a named kernel family:
- * "gemv", "gemm_proj", "half_ls", "expand_T", "epilogue", "adjoint" (the
+ * "gemv", "gemm_proj" (batched projection stage), "proj_gemm" (its matrix
+ * product kernel alone), "half_ls" (half line strength stage),
+ * "half_ls_gemm" (its mu X product kernel alone), "prep" (one-time builds
+ * of the reduced-precision planes of the dipole and of final blocks),
+ * "expand_T", "epilogue", "adjoint" (the
  * backward of gain_adjoint_dipole after its forward recomputation) and
  * "adjoint_rankk" (its rank-k update dL/dmu = dY X^T). Resets the counters
  * if reset != 0. */
@@ -267,7 +271,8 @@ int gain_set_tuning(gain_ctx* ctx, const char* key, int value);
 /* Current value of a settable key, or what the most recent launch used:
  * "proj_cfg", "proj_split", "proj_grid" (FP64 MFMA projection GEMM),
  * "half_ls_cfg", "half_ls_split", "half_ls_grid" (mu GEMM), "gemv_grid"
- * (plain GEMV); *_grid = work-items of that launch. No reference
+ * (plain GEMV); *_grid = work-items of that launch; "autotune_us" = host
+ * time spent timing tile configurations so far. No reference
  * counterpart: the reference has no tunable kernels. */
 int gain_get_tuning(gain_ctx* ctx, const char* key, long long* value);
 /* The autotuner's timings and choices, one line per GEMM shape. */
