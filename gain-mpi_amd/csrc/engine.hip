@@ -89,7 +89,8 @@ Block& block_of(gain_ctx* c, int jind, int gamma) {
   return it->second;
 }
 
-// Scatter CSR for a basis: slot = (2K+tau)*nc + v -> primitives.
+// Scatter CSR for a basis: slot = col*nc + v -> primitives, col = 2K+tau
+// (K-block bases) or level * rmd + component (symmetric-top bases).
 void ensure_slots(gain_ctx* c, Basis& b) {
   if (b.slot_nc == c->nc) return;
   if (b.nvib > c->nc)
@@ -100,7 +101,7 @@ void ensure_slots(gain_ctx* c, Basis& b) {
   std::vector<int> cnt(nslot + 1, 0);
   std::vector<int> slot(b.nprim);
   for (int r = 0; r < b.nprim; ++r) {
-    slot[r] = (2 * b.hk[r] + (b.htau[r] & 1)) * c->nc + b.hvib[r];
+    slot[r] = b.hcol[r] * c->nc + b.hvib[r];
     cnt[slot[r] + 1]++;
   }
   for (int s = 0; s < nslot; ++s) cnt[s + 1] += cnt[s];
@@ -501,6 +502,9 @@ Plan make_plan(gain_ctx* c, int jindI, int gI, const int* levels, int nI,
   P.gF = gF;
   P.bI = &basis_of(c, jindI, true);
   P.bF = &basis_of(c, jindF, true);
+  if (P.bI->rotsym != P.bF->rotsym)
+    fail(GAIN_ERR_ARG, "J indices %d and %d mix a symmetric-top and a "
+         "K-block basis", jindI, jindF);
   P.blkI = &block_of(c, jindI, gI);
   P.blkF = &block_of(c, jindF, gF);
   if (!P.blkF->vec)
@@ -711,6 +715,52 @@ GatherArgs make_gather_args(const gain_ctx* c, const Plan& P, int nvecX,
   ga.H = H;
   ga.hstride = bF.nprim;
   return ga;
+}
+
+GatherRotArgs make_gather_rot_args(gain_ctx* c, const Plan& P, int nvecX,
+                                   int ks, const double* Y, double* H,
+                                   int64_t ys_alpha, int64_t ys_ks) {
+  Basis& bI = *P.bI;
+  Basis& bF = *P.bF;
+  auto it = c->wigner.find({P.jindI, P.jindF});
+  if (it == c->wigner.end())
+    fail(GAIN_ERR_STATE,
+         "no rotational (Wigner) table for the symmetric-top pair of J "
+         "indices %d -> %d (gain_upload_wigner)",
+         P.jindI, P.jindF);
+  const WignerTable& w = it->second;
+  ensure_slots(c, bF);
+  GatherRotArgs ga{};
+  ga.nc = c->nc;
+  ga.ncolI = bI.ncol;
+  ga.ncolF = bF.ncol;
+  ga.nvec = nvecX;
+  ga.ksplit = ks;
+  ga.nlI = w.nlI;
+  ga.nlF = w.nlF;
+  ga.md = w.md;
+  ga.W = w.d;
+  ga.Y = Y;
+  ga.ys_alpha = ys_alpha;
+  ga.ys_ks = ys_ks;
+  ga.slot_ptrF = bF.slot_ptr;
+  ga.slot_primF = bF.slot_prim;
+  ga.H = H;
+  ga.hstride = bF.nprim;
+  return ga;
+}
+
+void launch_gather_for(gain_ctx* c, const Plan& P, int nvecX, int ks,
+                       const double* Y, double* H, hipStream_t s) {
+  if (P.bI->rotsym) {
+    const int64_t ncolTot = (int64_t)nvecX * P.bI->ncol;
+    launch_gather_rotsym(make_gather_rot_args(c, P, nvecX, ks, Y, H,
+                                              ncolTot * c->nc,
+                                              3 * ncolTot * c->nc),
+                         s);
+  } else {
+    launch_gather_h(make_gather_args(c, P, nvecX, ks, Y, H), s);
+  }
 }
 
 // Transposed expansion G[v] = P_{F,dF(v)}^T H[h_of_vec[v]].
@@ -928,7 +978,7 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
   } else {
     launch_mu_gemm_valu(c->d_mu, nc, X, ncolTot, Y, ks, c->stream);
   }
-  launch_gather_h(make_gather_args(c, P, nvecX, ks, Y, H), c->stream);
+  launch_gather_for(c, P, nvecX, ks, Y, H, c->stream);
   return bF.nprim;
 }
 
@@ -1361,6 +1411,9 @@ void run_adjoint(gain_ctx* c, const Plan& P, const double* dS_dev,
   Basis& bF = *P.bF;
   Block& blkF = *P.blkF;
   const int nI = P.nI, nF = blkF.n, nc = c->nc;
+  if (bI.rotsym)
+    fail(GAIN_ERR_ARG, "the adjoint covers K-block bases only (J index %d "
+         "is a symmetric-top basis)", P.jindI);
   const Staged st = stage(c, P, nullptr);
   const Small& sm = st.sm;
   launch_fill_zero(dmu_work, (int64_t)3 * nc * nc, c->stream);
@@ -1685,10 +1738,12 @@ extern "C" int gain_upload_basis(gain_ctx* c, int jind, int J, int nprim,
     int nv = 0;
     for (int K = 0; K <= J; ++K)
       for (int r = kstart[K]; r < kstart[K] + kblock[K]; ++r) b.hk[r] = K;
+    b.hcol.resize(nprim);
     for (int r = 0; r < nprim; ++r) {
       GAIN_CHECK_ARG(vib[r] >= 0, "negative vibrational index");
       nv = std::max(nv, vib[r] + 1);
       b.htau[r] &= 1;
+      b.hcol[r] = 2 * b.hk[r] + b.htau[r];
     }
     b.nvib = nv;
     b.kval = c->arena.upload(b.hk.data(), nprim);
@@ -1696,6 +1751,81 @@ extern "C" int gain_upload_basis(gain_ctx* c, int jind, int J, int nprim,
     b.vib = c->arena.upload(b.hvib.data(), nprim);
     b.present = true;
     if (c->d_mu && b.nvib <= c->nc) ensure_slots(c, b);
+  });
+}
+
+extern "C" int gain_upload_basis_rotsym(gain_ctx* c, int jind, int J,
+                                        int nprim, int nlevels, int maxdeg,
+                                        const int* level, const int* deg,
+                                        const int* vib) {
+  return guarded([&] {
+    need(c);
+    GAIN_CHECK_ARG(jind >= 0 && jind < 1024, "J index out of range");
+    GAIN_CHECK_ARG(J >= 0 && J <= c->jmax, "J=%d above jmax %d", J, c->jmax);
+    GAIN_CHECK_ARG(nprim > 0 && level && deg && vib, "bad basis arrays");
+    GAIN_CHECK_ARG(nlevels >= 1 && maxdeg >= 1 && maxdeg <= 3,
+                   "bad level count %d / degeneracy %d", nlevels, maxdeg);
+    DeviceGuard g(c->device);
+    if ((int)c->bases.size() <= jind) c->bases.resize(jind + 1);
+    Basis& b = c->bases[jind];
+    if (b.present) fail(GAIN_ERR_STATE, "J index %d already uploaded", jind);
+    b.J = J;
+    b.nprim = nprim;
+    b.rotsym = true;
+    b.nlevels = nlevels;
+    b.rmd = maxdeg;
+    b.ncol = nlevels * maxdeg;
+    b.hk.assign(nprim, 0);  // no K-block sign on the initial side
+    b.htau.assign(nprim, 0);
+    b.hvib.assign(vib, vib + nprim);
+    b.hcol.resize(nprim);
+    int nv = 0;
+    for (int r = 0; r < nprim; ++r) {
+      GAIN_CHECK_ARG(level[r] >= 0 && level[r] < nlevels && deg[r] >= 0 &&
+                         deg[r] < maxdeg && vib[r] >= 0,
+                     "primitive %d: level %d / component %d / vib %d out of "
+                     "range", r, level[r], deg[r], vib[r]);
+      b.hcol[r] = level[r] * maxdeg + deg[r];
+      nv = std::max(nv, vib[r] + 1);
+    }
+    b.nvib = nv;
+    b.kval = c->arena.upload(b.hk.data(), nprim);
+    b.tau = c->arena.upload(b.htau.data(), nprim);
+    b.vib = c->arena.upload(b.hvib.data(), nprim);
+    b.present = true;
+    if (c->d_mu && b.nvib <= c->nc) ensure_slots(c, b);
+  });
+}
+
+extern "C" int gain_upload_wigner(gain_ctx* c, int jindI, int jindF,
+                                  int nlevelsI, int nlevelsF, int maxdeg,
+                                  const double* rot) {
+  return guarded([&] {
+    need(c);
+    Basis& bI = basis_of(c, jindI, false);
+    Basis& bF = basis_of(c, jindF, false);
+    GAIN_CHECK_ARG(bI.rotsym && bF.rotsym,
+                   "rotational tables belong to symmetric-top bases");
+    GAIN_CHECK_ARG(std::abs(bF.J - bI.J) <= 1, "|J_F - J_I| > 1");
+    GAIN_CHECK_ARG(rot && nlevelsI == bI.nlevels && nlevelsF == bF.nlevels &&
+                       maxdeg == bI.rmd && maxdeg == bF.rmd,
+                   "table %d x %d levels, degeneracy %d; bases have %d x %d, "
+                   "%d / %d", nlevelsI, nlevelsF, maxdeg, bI.nlevels,
+                   bF.nlevels, bI.rmd, bF.rmd);
+    DeviceGuard g(c->device);
+    auto it = c->wigner.find({jindI, jindF});
+    if (it != c->wigner.end()) {
+      ++c->gen;  // a sweep may hold the old table
+      c->arena.release(it->second.d);
+      c->wigner.erase(it);
+    }
+    WignerTable w;
+    w.nlI = nlevelsI;
+    w.nlF = nlevelsF;
+    w.md = maxdeg;
+    w.d = c->arena.upload(rot, (size_t)3 * nlevelsI * nlevelsF * maxdeg *
+                                   maxdeg);
+    c->wigner[{jindI, jindF}] = w;
   });
 }
 

@@ -131,7 +131,12 @@ struct Irrep {
 struct Basis {
   bool present = false, has_infl = false;
   int J = 0, nprim = 0, ncol = 0, nvib = 0;
+  // symmetric-top (ROTSYM) basis: primitives carry a rotational level and a
+  // degenerate component instead of (K, tau); X column = level * rmd + deg
+  bool rotsym = false;
+  int nlevels = 0, rmd = 0;
   std::vector<int> kstart, kblock, hk, htau, hvib;
+  std::vector<int> hcol;  // X column of each primitive
   int *kval = nullptr, *tau = nullptr, *vib = nullptr;
   // scatter CSR over (col, v) slots, built for a given nc
   int slot_nc = -1;
@@ -171,6 +176,13 @@ struct Block {
   std::vector<int> label;  // level number printed per state (default: row)
 };
 
+// Rotational factors of one (J_I, J_F) pair of symmetric-top bases
+// (GpuManager::TransferWigner, GpuManager.cpp:440-470).
+struct WignerTable {
+  int nlI = 0, nlF = 0, md = 0;
+  double* d = nullptr;  // [3][nlI][nlF][md][md], first index fastest
+};
+
 struct TimerSlot {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending, pool;
   double total_ms = 0.0;
@@ -197,6 +209,7 @@ struct gain_ctx {
   Sliced mu_sl;
   std::vector<Basis> bases;
   std::map<std::pair<int, int>, Block> blocks;
+  std::map<std::pair<int, int>, WignerTable> wigner;  // (jindI, jindF)
   DeviceArena arena;
   Work w_c, w_X, w_Y, w_H, w_G, w_part, w_small, w_out, w_tmp;
   Work w_xs, w_xe, w_gs, w_ge, w_res;
@@ -333,6 +346,15 @@ GemmArgs make_mu_gemm_args(const gain_ctx* c, const double* X, int ncolTot,
                            double* Y, int ks);
 GatherArgs make_gather_args(const gain_ctx* c, const Plan& P, int nvecX,
                             int ks, const double* Y, double* H);
+// Symmetric-top gather of a plan whose bases are ROTSYM bases (final slot
+// tables built, Wigner table uploaded).
+GatherRotArgs make_gather_rot_args(gain_ctx* c, const Plan& P, int nvecX,
+                                   int ks, const double* Y, double* H,
+                                   int64_t ys_alpha, int64_t ys_ks);
+// The gather of either kind for a plan (Y in the layout of one mu X GEMM
+// over nvecX vectors).
+void launch_gather_for(gain_ctx* c, const Plan& P, int nvecX, int ks,
+                       const double* Y, double* H, hipStream_t s);
 ExpandTArgs make_expand_T_args(const Plan& P, const double* H,
                                const Small& sm, int nvecG, double* G,
                                int64_t gstride);

@@ -197,6 +197,60 @@ void launch_gather_h(const GatherArgs& a, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
+// Symmetric-top gather: one thread per (v_F, vector) and kRotCF final
+// columns.  The Wigner factors are uniform across the wave (scalar loads);
+// the Y loads of consecutive v_F are coalesced.
+constexpr int kRotCF = 4;
+
+__global__ __launch_bounds__(64) void k_gather_rotsym(GatherRotArgs a) {
+  const int v = blockIdx.x * 64 + threadIdx.x;
+  const int c0 = blockIdx.y * kRotCF;
+  const int b = blockIdx.z;
+  if (v >= a.nc) return;
+  const int64_t sF = 3ll * a.nlI, sdI = sF * a.nlF, sdF = sdI * a.md;
+  int lF[kRotCF], dF[kRotCF];
+#pragma unroll
+  for (int q = 0; q < kRotCF; ++q) {
+    const int cF = min(c0 + q, a.ncolF - 1);
+    lF[q] = cF / a.md;
+    dF[q] = cF - lF[q] * a.md;
+  }
+  double acc[kRotCF];
+#pragma unroll
+  for (int q = 0; q < kRotCF; ++q) acc[q] = 0.0;
+  const double* yb = a.Y + (int64_t)b * a.ncolI * a.nc + v;
+  for (int cI = 0; cI < a.ncolI; ++cI) {
+    const int lI = cI / a.md, dI = cI - lI * a.md;
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+      const double* p = yb + n * a.ys_alpha + (int64_t)cI * a.nc;
+      double y = 0.0;
+      for (int ks = 0; ks < a.ksplit; ++ks) y += p[ks * a.ys_ks];
+      const double* w = a.W + n + 3 * lI + sdI * dI;
+#pragma unroll
+      for (int q = 0; q < kRotCF; ++q)
+        acc[q] = fma(w[sF * lF[q] + sdF * dF[q]], y, acc[q]);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kRotCF; ++q) {
+    const int cF = c0 + q;
+    if (cF >= a.ncolF) break;
+    const int slot = cF * a.nc + v;
+    for (int t = a.slot_ptrF[slot]; t < a.slot_ptrF[slot + 1]; ++t)
+      a.H[(int64_t)b * a.hstride + a.slot_primF[t]] = acc[q];
+  }
+}
+
+void launch_gather_rotsym(const GatherRotArgs& a, hipStream_t s) {
+  if (a.nvec == 0 || a.ncolF == 0 || a.nc == 0) return;
+  GAIN_CHECK_ARG(a.nvec < 65536, "rotsym gather: %d vectors", a.nvec);
+  dim3 grid((a.nc + 63) / 64, (a.ncolF + kRotCF - 1) / kRotCF, a.nvec);
+  hipLaunchKernelGGL(k_gather_rotsym, grid, dim3(64), 0, s, a);
+  check_launch("gather_rotsym");
+}
+
+// ---------------------------------------------------------------------------
 // Transpose of the expansion: g[t] = sum_{r in case(t)}
 //   repres[t + d*Ntot + lambda(r)*sdeg*Ntot] h[r]
 // so that <P c_F, h> = <c_F, g> for every final record c_F

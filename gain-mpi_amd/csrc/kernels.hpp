@@ -64,6 +64,26 @@ struct GatherArgs {
 };
 void launch_gather_h(const GatherArgs& a, hipStream_t s);
 
+// Gather of the symmetric-top (ROTSYM) half line strength from Y_n = mu_n X
+// (dipole_GPU.cu:641-723 regrouped):
+//   h[r'] = sum_n sum_{(l_I, d_I)} W[n, l_I, l_F(r'), d_I, d_F(r')]
+//           Y_n[(l_I, d_I)][v(r')]
+// formed once per final slot (l_F, d_F, v_F) and written to every primitive
+// of the slot (final-basis slot CSR).  X columns are l * md + d.
+struct GatherRotArgs {
+  int nc, ncolI, ncolF, nvec, ksplit;
+  int nlI, nlF, md;
+  const double* W;       // [3][nlI][nlF][md][md], first index fastest
+  const double* Y;
+  int64_t ys_alpha;      // ncolTot * nc
+  int64_t ys_ks;         // 3 * ncolTot * nc
+  const int* slot_ptrF;  // [ncolF * nc + 1]
+  const int* slot_primF;
+  double* H;
+  int64_t hstride;
+};
+void launch_gather_rotsym(const GatherRotArgs& a, hipStream_t s);
+
 struct ExpandTArgs {
   int ntot, nprim, sdeg;
   const int* tcase;      // [ntot] case of each symmetrised function

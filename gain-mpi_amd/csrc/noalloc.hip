@@ -187,7 +187,10 @@ static void plain_noalloc(gain_ctx* c, const Plan& P, const double* cI_dev,
     } else {
       launch_mu_gemm_valu(c->d_mu, nc, X, ncolTot, Y, ks, c->stream);
     }
-    launch_gather_h(make_gather_args(c, P, nvecX, ks, Y, H), c->stream);
+    if (bI.rotsym && bF.slot_nc != nc)
+      fail(GAIN_ERR_STATE, "scatter tables of J index %d are not built",
+           P.jindF);
+    launch_gather_for(c, P, nvecX, ks, Y, H, c->stream);
     launch_expand_T(make_expand_T_args(P, H, sm, nvecG, G, gstride),
                     c->stream);
     GemvArgs a{};

@@ -80,7 +80,10 @@ struct gain_sweep {
     const int* start = nullptr;
     int n = 0, blocks = 0;
   };
-  std::vector<Launch> stage_scatter, stage_gather, stage_gemv, stage_lines;
+  std::vector<Launch> stage_scatter, stage_gather, stage_expand, stage_gemv,
+      stage_lines;
+  std::vector<GatherRotArgs> gather_rot;  // symmetric-top tasks, one launch
+                                          // each
   std::vector<GemvArgs> gemv_single;  // units whose layout the grouped
                                       // GEMV cannot take
   // mu X of every batch: one GEMM over [X of batch 0 | X of batch 1 | ...]
@@ -453,6 +456,10 @@ void build_launches(gain_sweep* s) {
     const gain_sweep::Group& G = s->groups[T.group];
     Basis& bF = *T.P.bF;
     const Irrep& irF = bF.irrep[T.P.gF];
+    if (T.P.bI->rotsym)
+      s->gather_rot.push_back(make_gather_rot_args(
+          c, T.P, G.nvecX, 1, G.Y, T.H, s->ncol_all * nc,
+          3 * s->ncol_all * nc));
     GatherArgs ga{};
     ga.nprimF = bF.nprim;
     ga.nc = nc;
@@ -471,7 +478,7 @@ void build_launches(gain_sweep* s) {
     ga.ys_ks = 3 * s->ncol_all * nc;
     ga.H = T.H;
     ga.hstride = T.hstride;
-    gv.push_back(ga);
+    if (!T.P.bI->rotsym) gv.push_back(ga);
     ExpandTArgs ea{};
     ea.ntot = irF.ntot;
     ea.nprim = bF.nprim;
@@ -493,7 +500,7 @@ void build_launches(gain_sweep* s) {
   // every gather before any projection input: expand_T reads H of its task
   // only, so the two stages are ordered launches
   add_stage(s, s->stage_gather, kGroupedGather, gv, grouped_gather_table(gv));
-  add_stage(s, s->stage_gather, kGroupedExpandT, ev,
+  add_stage(s, s->stage_expand, kGroupedExpandT, ev,
             grouped_expand_T_table(ev));
 
   std::vector<GemvArgs> g4, g8;
@@ -614,6 +621,8 @@ void enqueue(gain_sweep* s, hipStream_t st) {
     gemm_sliced_i8(s->hl8, st);
   }
   run_stage(s->stage_gather, st);
+  for (const GatherRotArgs& a : s->gather_rot) launch_gather_rotsym(a, st);
+  run_stage(s->stage_expand, st);
   gemm_f64_grouped(s->d_pj_probs, s->d_pj_tiles, s->n_pj_tiles, true,
                    s->pj_cfg, s->grid, st);
   run_stage(s->stage_gemv, st);

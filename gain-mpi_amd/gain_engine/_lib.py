@@ -39,6 +39,7 @@ EXPORTS = (
     "gain_trove_upload_basis", "gain_trove_read_states",
     "gain_trove_read_records", "gain_trove_load",
     "gain_sweep_run", "gain_sweep_destroy", "gain_plain_lines_noalloc",
+    "gain_upload_basis_rotsym", "gain_upload_wigner",
 )
 
 
@@ -115,6 +116,9 @@ def load():
         "gain_upload_basis": (_ip, [_vp, _ip, _ip, _ip, _vp, _vp, _vp, _vp]),
         "gain_upload_inflation": (_ip, [_vp, _ip, _ip, _vp, _vp, _ip, _vp,
                                         _vp, _vp, _vp]),
+        "gain_upload_basis_rotsym": (_ip, [_vp, _ip, _ip, _ip, _ip, _ip, _vp,
+                                           _vp, _vp]),
+        "gain_upload_wigner": (_ip, [_vp, _ip, _ip, _ip, _ip, _ip, _vp]),
         "gain_upload_block": (_ip, [_vp, _ip, _ip, _ip, _ip, _vp, _vp]),
         "gain_bind_block_device": (_ip, [_vp, _ip, _ip, _ip, _ip,
                                          C.c_int64, _vp, _vp]),

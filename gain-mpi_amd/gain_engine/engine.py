@@ -117,10 +117,23 @@ class Engine:
         check(self._L.gain_upload_dipole(self._h, _ptr(mu), nc, 0, nc))
 
     def upload_basis(self, jind: int, b):
+        if getattr(b, "rotsym", False):
+            arrs = [np.ascontiguousarray(getattr(b, n), _i32)
+                    for n in ("level", "deg", "vib")]
+            check(self._L.gain_upload_basis_rotsym(
+                self._h, jind, b.J, b.nprim, b.nlevels, b.rmd,
+                *[_ptr(a) for a in arrs]))
+            return
         arrs = [np.ascontiguousarray(getattr(b, n), _i32)
                 for n in ("kstart", "kblock", "tau", "vib")]
         check(self._L.gain_upload_basis(self._h, jind, b.J, b.nprim,
                                         *[_ptr(a) for a in arrs]))
+
+    def upload_wigner(self, jindI: int, jindF: int, nlevelsI: int,
+                      nlevelsF: int, maxdeg: int, rot: np.ndarray):
+        r = np.ascontiguousarray(rot, _f64)
+        check(self._L.gain_upload_wigner(self._h, jindI, jindF, nlevelsI,
+                                         nlevelsF, maxdeg, _ptr(r)))
 
     def upload_inflation(self, jind: int, b):
         nrep = self.nrepres
@@ -170,6 +183,12 @@ class Engine:
         for jind, b in enumerate(prob.bases):
             eng.upload_basis(jind, b)
             eng.upload_inflation(jind, b)
+        for jI, bI in enumerate(prob.bases):
+            for jF, bF in enumerate(prob.bases):
+                if getattr(bI, "rotsym", False) and bI.wig and \
+                        (bF.J - bI.J + 1) in bI.wig:
+                    eng.upload_wigner(jI, jF, bI.nlevels, bF.nlevels, bI.rmd,
+                                      bI.wig[bF.J - bI.J + 1])
         for key, blk in prob.blocks.items():
             if device_blocks and key in device_blocks:
                 ptr, ld = device_blocks[key]

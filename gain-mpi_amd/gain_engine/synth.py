@@ -54,8 +54,19 @@ class Basis:
     repres: List[np.ndarray]  # per irrep float64[ntot*sdeg*matsize]
     ntot: List[int]
     sdeg: List[int]
+    # symmetric-top (ROTSYM) basis: rotational level and degenerate
+    # component per primitive, and the rotational tables to J_F = J + dj - 1
+    # (dj = 0, 1, 2) present in the J list
+    rotsym: bool = False
+    nlevels: int = 0
+    rmd: int = 0
+    level: Optional[np.ndarray] = None
+    deg: Optional[np.ndarray] = None
+    wig: Optional[Dict[int, np.ndarray]] = None
 
     def kval(self) -> np.ndarray:
+        if self.rotsym:
+            return np.zeros(self.nprim, np.int32)
         k = np.empty(self.nprim, np.int32)
         for K in range(self.J + 1):
             k[self.kstart[K]:self.kstart[K] + self.kblock[K]] = K
@@ -224,7 +235,75 @@ def make_basis(J: int, nc: int, scheme: str = "c1") -> Basis:
             tau.extend([t] * nc)
             vib.extend(range(nc))
         pos += len(taus) * nc
-    nprim = pos
+    return _with_inflation(J, pos, np.array(kstart, np.int32),
+                           np.array(kblock, np.int32),
+                           np.array(tau, np.int32), np.array(vib, np.int32),
+                           scheme)
+
+
+def rotsym_levels(group: str, J: int):
+    """(nlevels, maxdeg, root_of[nlevels, maxdeg]) of the symmetrised
+    rotational functions of one J (the library's generator,
+    gain_rotsym_functions); root_of is -1 past a level's degeneracy."""
+    import ctypes as C
+
+    from . import _lib
+    L = _lib.load()
+    nl, md = C.c_int(), C.c_int()
+    _lib.check(L.gain_rotsym_functions(group.encode(), J, C.byref(nl),
+                                       C.byref(md), None, None, None, None))
+    root = np.zeros(nl.value * md.value, np.int32)
+    _lib.check(L.gain_rotsym_functions(group.encode(), J, C.byref(nl),
+                                       C.byref(md), None, root.ctypes.data,
+                                       None, None))
+    return nl.value, md.value, root.reshape(nl.value, md.value)
+
+
+def rotsym_wigner(group: str, jI: int, jF: int) -> np.ndarray:
+    """The rotational table of a J_I -> J_F pair (gain_rotsym_wigner):
+    3 x nlevelsI x nlevelsF x maxdeg x maxdeg, first index fastest."""
+    import ctypes as C
+
+    from . import _lib
+    L = _lib.load()
+    n = [C.c_int(), C.c_int(), C.c_int()]
+    _lib.check(L.gain_rotsym_wigner(group.encode(), jI, jF,
+                                    *[C.byref(x) for x in n], None))
+    rot = np.zeros(3 * n[0].value * n[1].value * n[2].value ** 2)
+    _lib.check(L.gain_rotsym_wigner(group.encode(), jI, jF,
+                                    *[C.byref(x) for x in n],
+                                    rot.ctypes.data))
+    return rot
+
+
+def make_basis_rotsym(J: int, nc: int, group: str,
+                      scheme: str = "c1") -> Basis:
+    """Symmetric-top basis of one J (the do_rotsym basis of
+    TroveBasisSet.cpp:480-483): primitives ordered vibrational index
+    slowest, then rotational level, then degenerate component; the
+    inflation scheme mixes consecutive primitives as in make_basis."""
+    nl, md, root = rotsym_levels(group, J)
+    lv, dg = np.nonzero(root >= 0)
+    nprim = nc * len(lv)
+    _, _, gsize, _ = _scheme(scheme)
+    if nprim % gsize:
+        raise ValueError("primitive count must be a multiple of the "
+                         "symmetry group size")
+    b = _with_inflation(J, nprim, np.zeros(J + 1, np.int32),
+                        np.zeros(J + 1, np.int32),
+                        np.zeros(nprim, np.int32),
+                        np.repeat(np.arange(nc, dtype=np.int32), len(lv)),
+                        scheme)
+    b.rotsym, b.nlevels, b.rmd = True, nl, md
+    b.level = np.tile(lv.astype(np.int32), nc)
+    b.deg = np.tile(dg.astype(np.int32), nc)
+    return b
+
+
+def _with_inflation(J, nprim, kstart, kblock, tau, vib, scheme) -> Basis:
+    """Basis with the inflation data of `scheme` over groups of consecutive
+    primitives."""
+    nrep, sdeg, gsize, rows = _scheme(scheme)
     ncase = nprim // gsize
     case = np.arange(nprim) // gsize
     lam = np.arange(nprim) % gsize
@@ -246,10 +325,8 @@ def make_basis(J: int, nc: int, scheme: str = "c1") -> Basis:
         repres.append(np.ascontiguousarray(rep[:, :, :nt]).reshape(-1)
                       if nt else np.zeros(0))
         ntot.append(nt)
-    return Basis(J=J, nprim=nprim, kstart=np.array(kstart, np.int32),
-                 kblock=np.array(kblock, np.int32),
-                 tau=np.array(tau, np.int32), vib=np.array(vib, np.int32),
-                 maxsymcoeffs=ncase, ijterms=ijterms, icontr=icontr,
+    return Basis(J=J, nprim=nprim, kstart=kstart, kblock=kblock, tau=tau,
+                 vib=vib, maxsymcoeffs=ncase, ijterms=ijterms, icontr=icontr,
                  matsize=gsize, N=N, repres=repres, ntot=ntot, sdeg=sdeg)
 
 
@@ -285,11 +362,19 @@ def make_block(jind: int, gamma: int, ntot: int, rng_e, rng_c,
 def make_problem(jvals, nc: int, scheme: str = "c1", blocks=None,
                  kron: bool = False, emax: float = 10000.0,
                  params: Optional[Params] = None, seed_offset: int = 0,
-                 nstates: Optional[Dict[Tuple[int, int], int]] = None
-                 ) -> Problem:
+                 nstates: Optional[Dict[Tuple[int, int], int]] = None,
+                 rotsym: Optional[str] = None) -> Problem:
     """Problem over a J list.  `blocks` selects which (J index, irrep) blocks
-    carry states (default: every non-empty one)."""
-    bases = [make_basis(J, nc, scheme) for J in jvals]
+    carry states (default: every non-empty one).  `rotsym` names a
+    molecular symmetry group: symmetric-top bases with that group's
+    rotational levels and tables instead of K-block bases."""
+    if rotsym is None:
+        bases = [make_basis(J, nc, scheme) for J in jvals]
+    else:
+        bases = [make_basis_rotsym(J, nc, rotsym, scheme) for J in jvals]
+        for b in bases:
+            b.wig = {jF - b.J + 1: rotsym_wigner(rotsym, b.J, jF)
+                     for jF in jvals if abs(jF - b.J) <= 1}
     nrep, sdeg, _, _ = _scheme(scheme)
     if params is None:
         pairs = {"c1": [1], "c2v4": [1, 1, 2, 2], "pair2": [1, 1],

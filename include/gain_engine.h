@@ -124,6 +124,25 @@ int gain_upload_dipole(gain_ctx* ctx, const double* mu, int nc, int startF,
 int gain_upload_basis(gain_ctx* ctx, int jind, int J, int nprim,
                       const int* kstart, const int* kblock, const int* ktau,
                       const int* vib);
+/* Symmetric-top (ROTSYM) basis of one J, the basis TroveBasisSet builds
+ * with do_rotsym (TroveBasisSet.cpp:480-483: per primitive the rotational
+ * level, stored where the K-block basis keeps ktau, and the degenerate
+ * component, stored where it keeps K; no K blocks), for the half line
+ * strength of dipole_GPU.cu:641-723 (GpuManager::ExecuteRotSymHalfLs,
+ * GpuManager.cpp:547-600, chosen by rotsym_do at :612-618).  `nlevels`
+ * symmetrised rotational levels of up to `maxdeg` components (the layout
+ * dimensions of the rotational table).  Both bases of a pair must be of
+ * the same kind; the adjoint covers K-block bases only. */
+int gain_upload_basis_rotsym(gain_ctx* ctx, int jind, int J, int nprim,
+                             int nlevels, int maxdeg, const int* level,
+                             const int* deg, const int* vib);
+/* Rotational (Wigner) factors of the symmetric-top pair J index jindI ->
+ * jindF, the table GpuManager::TransferWigner uploads
+ * (GpuManager.cpp:440-470) and c_wignerGen fills (TroveBasisSet.cpp:
+ * 611-633): rot[3 x nlevelsI x nlevelsF x maxdeg x maxdeg], first index
+ * fastest (gain_rotsym_wigner produces it for a group). */
+int gain_upload_wigner(gain_ctx* ctx, int jindI, int jindF, int nlevelsI,
+                       int nlevelsF, int maxdeg, const double* rot);
 /* Inflation data of one J (GpuManager::TransferInflation,
  * GpuManager.cpp:381-422): ijterms[maxsymcoeffs*nrepres], icontr[2*nprim],
  * and per irrep N[maxsymcoeffs], repres[ntot*sdeg*matsize]. */

@@ -436,10 +436,24 @@ int64_t orc_run(const orc_params* p, const orc_basis* bases, int nJ,
         orc_expand(bI->nprim, gI, bI->maxsymcoeffs, dI, bI->sdeg[gI],
                    bI->ntot[gI], bI->ijterms, bI->icontr, bI->N[gI],
                    bI->repres[gI], cI, xprim);
-        orc_half_ls(jI, bI->nprim, bI->kstart, bI->kblock, bI->tau, bI->vib,
-                    bF->J, bF->nprim, bF->kstart, bF->kblock, bF->tau,
-                    bF->vib, dipole, nc, 0, nc, tab, jmax, xprim,
-                    h + ((size_t)indF * 3 + dI) * dimmax);
+        if (bI->rotsym) {
+          /* GpuManager.cpp:612-618 -> ExecuteRotSymHalfLs (:547-600) */
+          const int dJ = bF->J - bI->J + 1;
+          const double* w = (dJ >= 0 && dJ <= 2) ? bI->wig[dJ] : NULL;
+          if (!w || !bF->rotsym || bF->rmd != bI->rmd) {
+            count = -1;
+            goto done;
+          }
+          orc_half_ls_rotsym(bF->nprim, bI->nprim, bI->nlevels, bF->nlevels,
+                             bI->rmd, bI->deg, bF->deg, bI->level, bF->level,
+                             bI->vib, bF->vib, 0, nc, nc, dipole, nc, w,
+                             xprim, h + ((size_t)indF * 3 + dI) * dimmax);
+        } else {
+          orc_half_ls(jI, bI->nprim, bI->kstart, bI->kblock, bI->tau,
+                      bI->vib, bF->J, bF->nprim, bF->kstart, bF->kblock,
+                      bF->tau, bF->vib, dipole, nc, 0, nc, tab, jmax, xprim,
+                      h + ((size_t)indF * 3 + dI) * dimmax);
+        }
       }
     }
     int nf = 0;
@@ -489,6 +503,7 @@ int64_t orc_run(const orc_params* p, const orc_basis* bases, int nJ,
     }
     count += nf;
   }
+done:
   free(flist);
   free(h);
   free(xprim);

@@ -126,6 +126,16 @@ typedef struct {
   const double* repres[16];
   int ntot[16];
   int sdeg[16];
+  /* symmetric-top (ROTSYM) basis: level and degenerate component per
+   * primitive instead of K blocks, and the rotational tables to J - 1, J,
+   * J + 1 (index J_F - J + 1; NULL when absent), each
+   * 3 x nlevels x nlevels(J_F) x rmd x rmd, first index fastest */
+  int rotsym;
+  int nlevels;
+  int rmd;
+  const int* level;
+  const int* deg;
+  const double* wig[3];
 } orc_basis;
 
 typedef struct {

@@ -41,7 +41,10 @@ class OrcBasis(C.Structure):
                 ("icontr", C.POINTER(C.c_int)), ("matsize", C.c_int),
                 ("N", C.POINTER(C.c_int) * 16),
                 ("repres", C.POINTER(C.c_double) * 16),
-                ("ntot", C.c_int * 16), ("sdeg", C.c_int * 16)]
+                ("ntot", C.c_int * 16), ("sdeg", C.c_int * 16),
+                ("rotsym", C.c_int), ("nlevels", C.c_int), ("rmd", C.c_int),
+                ("level", C.POINTER(C.c_int)), ("deg", C.POINTER(C.c_int)),
+                ("wig", C.POINTER(C.c_double) * 3)]
 
 
 class OrcStates(C.Structure):
@@ -260,6 +263,16 @@ def basis_struct(b, keep) -> OrcBasis:
         s.repres[g] = _dp(r)
         s.ntot[g] = b.ntot[g]
         s.sdeg[g] = b.sdeg[g]
+    if getattr(b, "rotsym", False):
+        s.rotsym, s.nlevels, s.rmd = 1, b.nlevels, b.rmd
+        for name in ("level", "deg"):
+            arr = np.ascontiguousarray(getattr(b, name), np.int32)
+            keep.append(arr)
+            setattr(s, name, _ip(arr))
+        for dj, rot in (b.wig or {}).items():
+            arr = np.ascontiguousarray(rot, np.float64)
+            keep.append(arr)
+            s.wig[dj] = _dp(arr)
     return s
 
 

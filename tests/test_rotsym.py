@@ -316,3 +316,39 @@ def test_errors():
     assert L.gain_rotsym_wigner(b"OH(M)", 2, 2, C.byref(a), C.byref(b),
                                 C.byref(m), None) == _lib.ERR_ARG
     assert b"group" in L.gain_last_error()
+
+
+@pytest.mark.parametrize("group", ["C3V(M)", "TD(M)", "D2H(M)", "C2V(M)"])
+def test_rotsym_lines_equal_kblock_lines_in_the_symmetrised_basis(group):
+    """The oracle's whole main.cpp:238-338 loop with symmetric-top bases
+    (the ROTSYM half line strength, GpuManager.cpp:612-618) on records
+    carried from a K-block problem into the symmetrised rotational basis
+    gives the K-block problem's transition list and line strengths (up to
+    the K-block kernel's truncated 1/sqrt(2))."""
+    nc = 4
+    kb = synth.make_problem([2, 3], nc, "c1")
+    kb.params.erange_lower = (0.0, 1e9)
+    rs = synth.make_problem([2, 3], nc, "c1", rotsym=group)
+    rs.params = kb.params
+    for jind, (bK, bR) in enumerate(zip(kb.bases, rs.bases)):
+        nl, md_, irr, root, E, _ = functions(group, bK.J)
+        pos = {(int(v), int(w)): i for i, (v, w) in
+               enumerate(zip(bK.vib, wang_index(bK)))}
+        roots = root[bR.level, bR.deg]
+        T = np.zeros((bR.nprim, bK.nprim))
+        for p, (v, q) in enumerate(zip(bR.vib, roots)):
+            for w in range(2 * bK.J + 1):
+                T[p, pos[(int(v), w)]] = E[w, q]
+        blk = kb.blocks[(jind, 0)]
+        rs.blocks[(jind, 0)] = synth.Block(jind, 0, blk.energy.copy(),
+                                           blk.vectors @ T.T, blk.rec_len)
+    rows = kb.state_table()
+    initials = list(range(0, len(rows), 3))
+    trK, _ = orc.run(kb, initials)
+    trR, _ = orc.run(rs, initials)
+    assert len(trK) > 0
+    assert np.array_equal(trK[["iI", "iF", "emitted"]],
+                          trR[["iI", "iF", "emitted"]])
+    smax = trK["S"].max()
+    np.testing.assert_allclose(trR["S"], trK["S"], rtol=1e-9,
+                               atol=1e-13 * smax)
