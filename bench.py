@@ -307,61 +307,42 @@ def run_adjoint(nrep):
 
 
 def run_rotsym(nrep):
-    """Symmetric-top (ROTSYM) half line strength drop-in at J = 10 -> 11,
-    C3V(M), Nc = 2000: one launch of gain_compute_half_linestrength_rotsym
-    per final block, timed with torch events on the default stream.  Each
-    (final, initial) primitive pair reads three dipole elements (24 bytes,
-    gathered by vibrational index) and three Wigner factors."""
+    """Symmetric-top (ROTSYM) half line strength at J = 10 -> 11, C3V(M),
+    Nc = 2000 (N_I = 42000, N_F = 46000 primitives, one irrep): the literal
+    drop-in (gain_compute_half_linestrength_rotsym, one launch per initial
+    vector; each (final, initial) primitive pair reads three dipole
+    elements and three rotational factors), and the engine form for 500
+    initial x 2000 final states (mu X on the matrix cores, contraction with
+    the rotational table), FP64 and CRT (12 moduli); the engine's half line
+    strength of one vector is checked against the literal kernel's."""
     import ctypes as C
 
     import numpy as np
     import torch
 
-    from gain_engine import _lib
+    from gain_engine import _lib, workload
     L = _lib.load()
-    group, jI, jF, nc = b"C3V(M)", 10, 11, 2000
-    nl = [C.c_int(), C.c_int(), C.c_int()]
-    _lib.check(L.gain_rotsym_wigner(group, jI, jF, *[C.byref(x) for x in nl],
-                                    None))
-    nlI, nlF, md = (x.value for x in nl)
-    wig = np.zeros(3 * nlI * nlF * md * md)
-    _lib.check(L.gain_rotsym_wigner(group, jI, jF, *[C.byref(x) for x in nl],
-                                    wig.ctypes.data))
-
-    def basis(J):
-        n_l, m_d = C.c_int(), C.c_int()
-        _lib.check(L.gain_rotsym_functions(group, J, C.byref(n_l),
-                                           C.byref(m_d), None, None, None,
-                                           None))
-        root = np.zeros(n_l.value * m_d.value, np.int32)
-        _lib.check(L.gain_rotsym_functions(group, J, C.byref(n_l),
-                                           C.byref(m_d), None,
-                                           root.ctypes.data, None, None))
-        root = root.reshape(n_l.value, m_d.value)
-        lv, dg = np.nonzero(root >= 0)
-        # vibrational index slowest, then level, then degenerate component
-        vib = np.repeat(np.arange(nc, dtype=np.int32), len(lv))
-        return (np.tile(dg.astype(np.int32), nc),
-                np.tile(lv.astype(np.int32), nc), vib)
-
-    dgI, lvI, vbI = basis(jI)
-    dgF, lvF, vbF = basis(jF)
-    rng = np.random.default_rng(11)
+    group, jI, jF, nc = "C3V(M)", 10, 11, 2000
+    w = workload.build_rotsym(group, nc, jI, jF)
+    eng, prob = w.eng, w.prob
+    bI, bF = prob.bases
+    nI, nF = w.ninitial, prob.blocks[(1, 0)].n
     dev = torch.device("cuda:0")
     T = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)
-    bufs = [T(a) for a in (dgI, dgF, lvI, lvF, vbI, vbF)]
-    d_mu = torch.randn(3 * nc * nc, dtype=torch.float64, device=dev)
-    d_w = T(wig)
-    d_x = torch.randn(len(vbI), dtype=torch.float64, device=dev)
-    d_h = torch.zeros(len(vbF), dtype=torch.float64, device=dev)
+    bufs = [T(a) for a in (bI.deg, bF.deg, bI.level, bF.level, bI.vib,
+                           bF.vib)]
+    d_mu = T(prob.mu.reshape(-1))
+    d_w = T(bI.wig[jF - jI + 1])
+    d_x = w.CI[0].contiguous()      # C1: the record is the primitive vector
+    d_h = torch.zeros(bF.nprim, dtype=torch.float64, device=dev)
     _lib.check(L.gain_copy_dipole_constant(nc))
     p = [b.data_ptr() for b in bufs]
 
     def launch():
         _lib.check(L.gain_compute_half_linestrength_rotsym(
-            len(vbF), len(vbI), nlI, nlF, md, p[0], p[1], p[2], p[3], p[4],
-            p[5], 0, nc, nc, d_mu.data_ptr(), d_w.data_ptr(),
-            d_x.data_ptr(), d_h.data_ptr(), None))
+            bF.nprim, bI.nprim, bI.nlevels, bF.nlevels, bI.rmd, p[0], p[1],
+            p[2], p[3], p[4], p[5], 0, nc, nc, d_mu.data_ptr(),
+            d_w.data_ptr(), d_x.data_ptr(), d_h.data_ptr(), None))
 
     launch()
     torch.cuda.synchronize()
@@ -373,19 +354,77 @@ def run_rotsym(nrep):
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / nrep
-    pairs = float(len(vbI)) * len(vbF)
-    out = {"kernel": "k_half_ls_rotsym", "group": "C3V(M)", "jI": jI,
-           "jF": jF, "nc": nc, "primitives_I": len(vbI),
-           "primitives_F": len(vbF), "ms": ms,
-           "pairs_per_s": pairs / (ms * 1e-3),
+    pairs = float(bI.nprim) * bF.nprim
+    h_lit = d_h.cpu().numpy()
+    h_eng = eng.half_ls(0, 0, 0, w.CI[0].cpu().numpy(), 1, bF.nprim)
+    h_err = float(np.abs(h_eng - h_lit).max() / np.abs(h_lit).max())
+    out = {"kernel": "k_half_ls_rotsym", "group": group, "jI": jI, "jF": jF,
+           "nc": nc, "primitives_I": bI.nprim, "primitives_F": bF.nprim,
+           "ms": ms, "pairs_per_s": pairs / (ms * 1e-3),
            "dipole_gather_gbs": 24.0 * pairs / (ms * 1e-3) / 1e9,
            "note": "literal drop-in (one thread per final primitive, every "
-                   "initial primitive); the dipole piece (96 MB) stays in "
-                   "the on-chip caches, so the gather rate, not HBM, bounds "
-                   "it"}
-    log(f"rotsym drop-in: {len(vbF)} x {len(vbI)} primitive pairs in "
+                   "initial primitive, one initial vector per launch); the "
+                   "dipole (96 MB) stays in the on-chip caches, so the "
+                   "gather rate, not HBM, bounds it"}
+    log(f"rotsym drop-in: {bF.nprim} x {bI.nprim} primitive pairs in "
         f"{ms:.2f} ms -> {out['pairs_per_s']:.3e} pairs/s "
-        f"({out['dipole_gather_gbs']:.0f} GB/s of dipole gathers)")
+        f"({out['dipole_gather_gbs']:.0f} GB/s of dipole gathers); engine "
+        f"half ls vs literal: max rel err {h_err:.2e}")
+    # ---- engine form: every initial state against the final block
+    levels = list(range(nI))
+    o = {k: torch.zeros((nI, nF), dtype=torch.float64, device=dev)
+         for k in ("S", "A", "I")}
+    fl = torch.zeros((nI, nF), dtype=torch.uint8, device=dev)
+    ptrs = {k: v.data_ptr() for k, v in o.items()} | {"flags": fl.data_ptr()}
+    stream = torch.cuda.ExternalStream(eng.stream())
+    eng_out = {}
+    S64 = None
+    for prec in (0, 8):
+        eng.lines_device(0, 0, levels, 1, 0, ptrs, precision=prec)
+        eng.synchronize()
+        eng.set_timing(True)
+        for name in ("half_ls", "half_ls_gemm", "gemm_proj"):
+            eng.timing(name, reset=True)
+        e0.record(stream)
+        for _ in range(nrep):
+            eng.lines_device(0, 0, levels, 1, 0, ptrs, precision=prec)
+        e1.record(stream)
+        eng.synchronize()
+        eng.set_timing(False)
+        tot = e0.elapsed_time(e1) / nrep
+        hl = eng.timing("half_ls")
+        hg = eng.timing("half_ls_gemm")
+        ntr = int((fl & 1).sum().item())
+        S = o["S"].cpu().numpy()
+        if prec == 0:
+            S64 = S
+            err = 0.0
+        else:
+            strong = S64 > 1e-6 * S64.max()
+            err = float((np.abs(S - S64)[strong] / S64[strong]).max())
+        hl_ms = hl[0] / max(hl[1], 1)
+        hg_ms = hg[0] / max(hg[1], 1)
+        # mu X over (levels x components) columns per initial state
+        flops = 2.0 * 3 * nc * nc * (bI.nlevels * bI.rmd) * nI
+        eng_out[PRECISION_NAMES[prec]] = {
+            "ms_per_batch": tot, "transitions": ntr,
+            "value": ntr / (tot / 1e3), "half_ls_ms": hl_ms,
+            "half_ls_gemm_ms": hg_ms,
+            "half_ls_gemm_tflops": flops / (hg_ms / 1e3) / 1e12,
+            "half_ls_ms_per_vector": hl_ms / nI,
+            "speedup_half_ls_vs_literal": ms / (hl_ms / nI),
+            "strong_max_rel_err_vs_fp64": err}
+        log(f"rotsym engine [{PRECISION_NAMES[prec]}]: {nI} x {nF} states "
+            f"in {tot:.2f} ms ({ntr / (tot / 1e3):.3e} transitions/s); half "
+            f"ls {hl_ms:.2f} ms = {hl_ms / nI * 1e3:.2f} us per vector "
+            f"({ms / (hl_ms / nI):.0f}x the literal kernel); strong-line "
+            f"err vs FP64 {err:.2e}")
+    out["engine"] = {"initial_states": nI, "final_states": nF,
+                     "half_ls_vs_literal_max_rel_err": h_err,
+                     **eng_out}
+    w.close()
+    del w, o, fl, bufs, d_mu, d_w, d_h
+    torch.cuda.empty_cache()
     return out
 
 

@@ -271,3 +271,37 @@ def build_adjoint(nc: int = 2000, jI: int = 10, jF: int = 11, P: int = 64,
     eng.bind_block_device(1, 1, blkF.energy, CF.data_ptr(), blkF.rec_len,
                           blkF.rec_len)
     return AdjointWorkload(prob, eng, M, coef, CI, CF)
+
+
+def build_rotsym(group: str = "C3V(M)", nc: int = 2000, jI: int = 10,
+                 jF: int = 11, ninitial: int = 500, nfinal: int = 2000,
+                 seed: int = 21, device: int = 0) -> Workload:
+    """Symmetric-top workload: J_I -> J_F bases of `group`'s rotational
+    levels (one irrep, N_J = (2J+1) nc), `ninitial` x `nfinal` states with
+    orthonormal records from QR of a Gaussian on the device, the
+    rotational tables of the pair uploaded."""
+    import torch
+    dev = torch.device(f"cuda:{device}")
+    ns = {(0, 0): ninitial, (1, 0): nfinal}
+    prob = synth.make_problem([jI, jF], nc, "c1", kron=True, nstates=ns,
+                              rotsym=group)
+    blkI, blkF = prob.blocks[(0, 0)], prob.blocks[(1, 0)]
+    CI = orthonormal_rows_device(blkI.n, blkI.rec_len, seed, dev)
+    CF = orthonormal_rows_device(blkF.n, blkF.rec_len, seed + 1, dev)
+    torch.cuda.synchronize()
+    eng = Engine(device, prob.jmax, prob.params.nrepres, prob.params.maxdeg)
+    eng.set_params(prob.params)
+    eng.upload_dipole(prob.mu)
+    for jind, bs in enumerate(prob.bases):
+        eng.upload_basis(jind, bs)
+        eng.upload_inflation(jind, bs)
+    for a, ba in enumerate(prob.bases):
+        for b, bb in enumerate(prob.bases):
+            if abs(bb.J - ba.J) <= 1:
+                eng.upload_wigner(a, b, ba.nlevels, bb.nlevels, ba.rmd,
+                                  ba.wig[bb.J - ba.J + 1])
+    eng.bind_block_device(0, 0, blkI.energy, CI.data_ptr(), blkI.rec_len,
+                          blkI.rec_len)
+    eng.bind_block_device(1, 0, blkF.energy, CF.data_ptr(), blkF.rec_len,
+                          blkF.rec_len)
+    return Workload(prob, eng, CF, CI, blkI.n)
