@@ -107,6 +107,15 @@ bool make_group(const std::string& name, Group& g) {
   };
   const std::vector<std::vector<ld>> klein = {
       {1, 1, 1, 1}, {1, 1, -1, -1}, {1, -1, -1, 1}, {1, -1, 1, -1}};
+  // D_nh(M) (symmetry.f90:746-1006 for odd n) at the order the reference
+  // runs it: nothing assigns the order sym%N, which keeps its default 1
+  // (symmetry.f90:33), so the group is E, sigma_v, i, C'_2 with the
+  // characters of A1', A2', A1", A2" and no Euler angles assigned (the
+  // C_n / S_n loops that set them are empty for n = 1).
+  if (name == "DNH(M)" || name == "DNH")
+    return abelian({{1, 1, 1, 1}, {1, -1, 1, -1}, {1, 1, -1, -1},
+                    {1, -1, -1, 1}},
+                   {E0, E0, E0, E0});
   if (name == "C2H(M)" || name == "C2H")  // symmetry.f90:140-170
     return abelian(klein, {E0, Ry, Rx, Rz});
   if (name == "CS(EM)")  // symmetry.f90:171-202
@@ -491,10 +500,20 @@ const Group& group_of(const char* name) {
   static thread_local std::string last;
   if (s != last) {
     Group t;
+    // D_infinity_h(M) (symmetry.f90:1444-1543) leaves the matrices of its
+    // 2-dimensional irreps unset and gives sigma_v and C'_inf the identity
+    // as equivalent rotation, so its operations do not form a
+    // representation on the rotational functions
+    if (s == "DINFTYH(M)")
+      fail(GAIN_ERR_ARG,
+           "symmetry group DINFTYH(M): the reference's table does not define "
+           "the matrices of its degenerate irreps or the equivalent "
+           "rotations of sigma_v and C'_inf, so no symmetrised rotational "
+           "functions exist for it");
     if (!make_group(s, t))
       fail(GAIN_ERR_ARG,
            "symmetry group \"%s\": C(M), CS(M), C2V(M), C2H(M), CS(EM), "
-           "G4(M), G4(EM), D2H(M), D2H, C3V(M), D3H(M), TD(M)",
+           "G4(M), G4(EM), D2H(M), D2H, C3V(M), D3H(M), TD(M), DNH(M)",
            name);
     g = t;
     last = s;
@@ -536,10 +555,11 @@ extern "C" int gain_rotsym_wigner(const char* group, int jI, int jF,
                                   int* nlevelsI, int* nlevelsF, int* maxdeg,
                                   double* rot) {
   return guarded([&] {
+    GAIN_CHECK_ARG(nlevelsI && nlevelsF && maxdeg,
+                   "rotsym_wigner: null size output");
     GAIN_CHECK_ARG(jI >= 0 && jF >= 0 && std::abs(jF - jI) <= 1 &&
-                       jI <= 200 && jF <= 200 && nlevelsI && nlevelsF &&
-                       maxdeg,
-                   "rotsym_wigner: need |jF - jI| <= 1");
+                       jI <= 200 && jF <= 200,
+                   "rotsym_wigner: need 0 <= J <= 200 and |jF - jI| <= 1");
     const Group& g = group_of(group);
     const RotSym fi = rot_functions(g, jI), ff = rot_functions(g, jF);
     *nlevelsI = fi.nlevels;

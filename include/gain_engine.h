@@ -415,9 +415,13 @@ int gain_compute_half_linestrength_rotsym(
 /* ---- symmetric-top (ROTSYM) rotational factors ---------------------------
  * Groups: "C(M)", "CS(M)", "C2V(M)", "C2H(M)", "CS(EM)", "G4(M)",
  * "G4(EM)", "D2H(M)", "D2H", "C3V(M)", "D3H(M)", "TD(M)" (symmetry.f90:58-745;
- * the short names "C", "CS", "C2V", "C2H", "G4", "C3V", "D3H", "TD" too).
- * DNH(M) and DINFTYH(M), whose order comes from the input file, are not
- * available.
+ * the short names "C", "CS", "C2V", "C2H", "G4", "C3V", "D3H", "TD" too)
+ * and "DNH(M)" / "DNH" at the order the reference runs it (its order
+ * sym%N is never assigned and stays 1: E, sigma_v, i, C'2, all with the
+ * identity as equivalent rotation; symmetry.f90:746-1006).  "DINFTYH(M)"
+ * is refused: its table (symmetry.f90:1444-1543) leaves the degenerate
+ * irreps' matrices and the equivalent rotations of sigma_v and C'_inf
+ * unset.
  *
  * Symmetrised rotational functions of one J (MLrotsymmetry_generate,
  * trove_wrappers.f90:416-589, with MLsymmetrization_rot :591-914): the

@@ -29,10 +29,10 @@ from gain_engine import _lib, synth
 from oracle import oracle as orc
 
 GROUPS = ["C(M)", "CS(M)", "C2V(M)", "C2H(M)", "CS(EM)", "G4(M)", "G4(EM)",
-          "D2H(M)", "D2H", "C3V(M)", "D3H(M)", "TD(M)"]
+          "D2H(M)", "D2H", "C3V(M)", "D3H(M)", "TD(M)", "DNH(M)"]
 NOPER = {"C(M)": 1, "CS(M)": 2, "C2V(M)": 4, "C2H(M)": 4, "CS(EM)": 4,
          "G4(M)": 4, "G4(EM)": 8, "D2H(M)": 8, "D2H": 8, "C3V(M)": 6,
-         "D3H(M)": 12, "TD(M)": 24}
+         "D3H(M)": 12, "TD(M)": 24, "DNH(M)": 4}
 # Abelian groups (symmetry.f90:140-364): characters [irrep][operation], the
 # rotation angle of each operation's equivalent rotation (0 for the
 # identity, pi for a twofold axis) and the operations equivalent to the
@@ -174,6 +174,9 @@ CHARS = {
               for c in TD_CHARS],
 }
 CHARS.update({g: v[0] for g, v in ABELIAN.items()})
+# DNH(M) at order 1 (E, sigma_v, i, C'2): A1', A2', A1", A2"
+CHARS["DNH(M)"] = [[1, 1, 1, 1], [1, -1, 1, -1], [1, 1, -1, -1],
+                   [1, -1, -1, 1]]
 
 
 def so3_character(J, a):
@@ -352,3 +355,32 @@ def test_rotsym_lines_equal_kblock_lines_in_the_symmetrised_basis(group):
     smax = trK["S"].max()
     np.testing.assert_allclose(trR["S"], trK["S"], rtol=1e-9,
                                atol=1e-13 * smax)
+
+
+@pytest.mark.parametrize("J", [0, 1, 2, 3, 6])
+def test_dnh_at_the_reference_order_is_the_trivial_representation(J):
+    """DNH(M) at the order GAIN-MPI runs it (sym%N keeps its default 1,
+    symmetry.f90:33,746-1006): four operations whose equivalent rotations
+    are all the identity, so every rotational function is A1' and the
+    functions and tables equal the no-symmetry ones."""
+    nl, md, irr, root, vec, ops = functions("DNH(M)", J)
+    c = functions("C(M)", J)
+    assert (nl, md) == (2 * J + 1, 1) and (irr == 0).all()
+    np.testing.assert_array_equal(vec, c[4])
+    np.testing.assert_array_equal(ops, np.stack([np.eye(2 * J + 1)] * 4))
+    for jF in (J - 1, J, J + 1):
+        if jF < 0:
+            continue
+        np.testing.assert_array_equal(wigner("DNH(M)", J, jF)[3],
+                                      wigner("C(M)", J, jF)[3])
+
+
+def test_dinftyh_is_refused_with_its_reason():
+    L = _lib.load()
+    a, b, m = C.c_int(), C.c_int(), C.c_int()
+    assert L.gain_rotsym_wigner(b"DINFTYH(M)", 2, 2, C.byref(a), C.byref(b),
+                                C.byref(m), None) == _lib.ERR_ARG
+    assert b"degenerate irreps" in L.gain_last_error()
+    assert L.gain_rotsym_wigner(b"C3V(M)", 2, 2, None, None, None,
+                                None) == _lib.ERR_ARG
+    assert b"null" in L.gain_last_error()
