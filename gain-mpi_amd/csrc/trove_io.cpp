@@ -458,8 +458,10 @@ extern "C" int gain_trove_basis_get(const gain_trove_basis* b,
   return guarded([&] {
     GAIN_CHECK_ARG(b && name && dst, "null argument");
     const std::string k(name);
+    // (an empty array, e.g. the repres of an irrep with no functions, has
+    // no data pointer: copy nothing)
     auto ints = [&](const std::vector<int>& v) {
-      memcpy(dst, v.data(), v.size() * sizeof(int));
+      if (!v.empty()) memcpy(dst, v.data(), v.size() * sizeof(int));
     };
     if (k == "kstart") {
       ints(b->kstart);
@@ -477,7 +479,7 @@ extern "C" int gain_trove_basis_get(const gain_trove_basis* b,
       GAIN_CHECK_ARG(gamma >= 0 && gamma < b->nrepres, "irrep out of range");
       if (k == "N")
         ints(b->N[gamma]);
-      else
+      else if (!b->repres[gamma].empty())
         memcpy(dst, b->repres[gamma].data(),
                b->repres[gamma].size() * sizeof(double));
     } else {
@@ -526,6 +528,7 @@ extern "C" int gain_trove_read_states(const char* path, int gamma,
     StatesOut s = parse_descr(path, gamma, *p);
     *nstates = (int)s.irec.size();
     if (rec_len) *rec_len = s.dim_basis;
+    if (s.irec.empty()) return;  // no state passes: nothing to copy
     if (irec) memcpy(irec, s.irec.data(), s.irec.size() * sizeof(int));
     if (ilevel) memcpy(ilevel, s.ilevel.data(), s.ilevel.size() * sizeof(int));
     if (energy)

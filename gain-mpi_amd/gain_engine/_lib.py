@@ -9,7 +9,10 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libgain_engine.so")
+# GAIN_ENGINE_LIB selects another build of the same sources (the host
+# sanitizer build, libgain_engine_asan.so); it is loaded as is, without torch
+LIB_OVERRIDE = os.environ.get("GAIN_ENGINE_LIB")
+LIB_PATH = LIB_OVERRIDE or os.path.join(_HERE, "libgain_engine.so")
 MAX_IRREPS = 16
 
 OK, ERR_ARG, ERR_HIP, ERR_STATE, ERR_NOMEM, ERR_CONFIG = range(6)
@@ -93,10 +96,11 @@ def load():
     # One HIP runtime per process: torch ships its own libamdhip64.so.7; when
     # torch is importable it is loaded first so the engine binds to the same
     # runtime (device pointers are then shared with torch tensors).
-    try:
-        import torch  # noqa: F401
-    except ImportError:
-        pass
+    if not LIB_OVERRIDE:
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `make -C gain-mpi_amd` "
