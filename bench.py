@@ -4,7 +4,8 @@
 Workload (synthetic, seeded; gain_engine/workload.py): C2v-like 4-irrep
 basis, initial block (J=20, A1) with N_I = 41000 functions, final block
 (J=21, A2) with N_F = n_F = 43000 states whose 43000 x 43000 FP64 records
-(14.8 GB, orthonormal) are generated in HBM; mu is 3 x 4000 x 4000 FP64.
+(14.8 GB) are the Q factor of the QR factorisation of a Gaussian matrix,
+computed in HBM before anything is timed; mu is 3 x 4000 x 4000 FP64.
 The 2000 lowest initial states go against every final state with a
 0-15000 cm^-1 frequency window.
 
@@ -818,7 +819,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": PRECISION_NAMES[args.precision],
-        "data": "synthetic (seeded generator, orthonormal eigenvectors)",
+        "data": "synthetic (seeded; records = Q of the QR factorisation of "
+                "a Gaussian, computed in HBM; mu Gaussian, damped)",
         "config": {"workload": "J_I=20 -> J_F=21 line strengths, Nc=4000, "
                                "N_I=41000, N_F=n_F=43000, 2000 lowest "
                                "initial states, 0-15000 cm-1 window",

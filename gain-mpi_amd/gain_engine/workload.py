@@ -3,8 +3,11 @@
 C2v-like 4-irrep basis (Hadamard quartets), initial block (J=20, A1) with
 N_I = (2J+1) Nc / 4 functions, final block (J=21, A2) with
 N_F = n_F = (2J+1) Nc / 4 states.  The final records (43000 x 43000 FP64 at
-Nc = 4000, 14.8 GB) are the Kronecker product of two QR factors, generated
-directly in HBM; the `ninitial` lowest initial records likewise.
+Nc = 4000, 14.8 GB) are the Q factor of the QR factorisation of a Gaussian
+matrix, computed in HBM (about 18 s); the `ninitial` lowest initial records
+likewise (the Q factor of a 41000 x ninitial Gaussian).  `records="kron"`
+uses Kronecker products of two small QR factors instead (seconds; the
+mixed-J workload, with 112 blocks, always does).
 """
 from __future__ import annotations
 
@@ -38,7 +41,8 @@ class Workload:
 
 
 def build(nc: int = 4000, jI: int = 20, jF: int = 21, ninitial: int = 2000,
-          device: int = 0, params: Optional[synth.Params] = None) -> Workload:
+          device: int = 0, params: Optional[synth.Params] = None,
+          records: str = "qr", seed: int = 31) -> Workload:
     import torch
 
     prob = synth.make_problem([jI, jF], nc, "c2v4", blocks={(0, 0), (1, 1)},
@@ -46,15 +50,22 @@ def build(nc: int = 4000, jI: int = 20, jF: int = 21, ninitial: int = 2000,
     blkI, blkF = prob.blocks[(0, 0)], prob.blocks[(1, 1)]
     nI = min(ninitial, blkI.n)
     dev = torch.device(f"cuda:{device}")
-    qa, qb = (torch.as_tensor(q, device=dev) for q in blkF.kron)
-    CF = torch.kron(qa.T.contiguous(), qb.T.contiguous())  # rows = records
-    del qa, qb
-    qa, qb = blkI.kron
-    b = qb.shape[0]
-    ka = (nI + b - 1) // b
-    CI = torch.kron(torch.as_tensor(qa.T[:ka].copy(), device=dev),
-                    torch.as_tensor(qb.T.copy(), device=dev))[:nI]
-    CI = CI.contiguous()
+    if records == "qr":
+        CF = orthonormal_rows_device(blkF.n, blkF.rec_len, seed, dev)
+        CI = orthonormal_rows_device(nI, blkI.rec_len, seed + 1, dev)
+        torch.cuda.empty_cache()
+    elif records == "kron":
+        qa, qb = (torch.as_tensor(q, device=dev) for q in blkF.kron)
+        CF = torch.kron(qa.T.contiguous(), qb.T.contiguous())  # rows
+        del qa, qb
+        qa, qb = blkI.kron
+        b = qb.shape[0]
+        ka = (nI + b - 1) // b
+        CI = torch.kron(torch.as_tensor(qa.T[:ka].copy(), device=dev),
+                        torch.as_tensor(qb.T.copy(), device=dev))[:nI]
+        CI = CI.contiguous()
+    else:
+        raise ValueError(records)
     torch.cuda.synchronize()
     eng = Engine(device, prob.jmax, prob.params.nrepres, prob.params.maxdeg)
     eng.set_params(prob.params)
@@ -195,7 +206,10 @@ def orthonormal_rows_device(n: int, dim: int, seed: int, dev):
     g = torch.Generator(device=dev).manual_seed(seed)
     G = torch.randn((dim, n), generator=g, dtype=torch.float64, device=dev)
     Q, R = torch.linalg.qr(G)
-    Q = Q * torch.sign(torch.diagonal(R))[None, :]
+    del G
+    s = torch.sign(torch.diagonal(R))
+    del R
+    Q.mul_(s[None, :])
     return Q.T.contiguous()
 
 
