@@ -77,11 +77,14 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_mixed_sweep(nreplay):
+def run_mixed_sweep(nreplay, phase_scan=(1,)):
     """Mixed-J sweep (256 initial states from J_I in [5, 30], Nc = 3000,
     2000 states per block, every dJ in {-1, 0, +1}): one captured sweep
-    replayed `nreplay` times, and the same tasks through the unbatched plain
-    FP64 path; transition flags and S compared between the two."""
+    (pipelined over the default number of phases) replayed `nreplay` times,
+    and the same tasks through the unbatched plain FP64 path; transition
+    flags and S compared between the two.  The FP64 sweep is also timed
+    with each phase count of `phase_scan` (1 = everything on one
+    stream)."""
     import numpy as np
     import torch
 
@@ -107,11 +110,16 @@ def run_mixed_sweep(nreplay):
                      {k: v.data_ptr() for k, v in o.items()}))
     torch.cuda.synchronize()
     build_s = time.time() - t0
+    # tile shapes of each phase's mu X timed once at creation (and only
+    # there: the unbatched plain calls below run untuned, as in use)
+    eng.set_tuning("autotune", 1)
     t1 = time.time()
     sw = Sweep(eng, full)
     create_s = time.time() - t1
+    eng.set_tuning("autotune", 0)
     gemv_bytes = eng.get_tuning("sweep_gemv_bytes")
     gemv_packs = eng.get_tuning("sweep_gemv_packs")
+    phases = eng.get_tuning("sweep_phases")
     stream = torch.cuda.ExternalStream(eng.stream())
     e0, e1 = torch.cuda.Event(enable_timing=True), \
         torch.cuda.Event(enable_timing=True)
@@ -135,6 +143,7 @@ def run_mixed_sweep(nreplay):
     eng.synchronize()
     plain_ms = e0.elapsed_time(e1)
     same_flags, worst = True, 0.0
+    rel_by_th = {1e-2: 0.0, 1e-4: 0.0, 1e-6: 0.0}
     for o, q in zip(sw_out, pl_out):
         same_flags &= bool(torch.equal(o["flags"], q["flags"]))
         S, R = o["S"], q["S"]
@@ -144,11 +153,42 @@ def run_mixed_sweep(nreplay):
         ok = d <= bound
         if not bool(ok.all()):
             worst = max(worst, float((d / bound.clamp_min(1e-300)).max()))
+        for th in rel_by_th:
+            strong = R > th * rmax
+            if bool(strong.any()):
+                rel_by_th[th] = max(rel_by_th[th], float(
+                    (d / R.clamp_min(1e-300))[strong].max()))
     sw.close()
+    # the same FP64 sweep with other phase counts (same outputs rewritten)
+    scan = {}
+    for ph in phase_scan:
+        eng.set_tuning("sweep_phases", int(ph))
+        eng.set_tuning("autotune", 1)
+        swp = Sweep(eng, full)
+        eng.set_tuning("autotune", 0)
+        for eager in (0, 1):
+            eng.set_tuning("sweep_eager", eager)
+            swp.run()
+            eng.synchronize()
+            e0.record(stream)
+            for _ in range(nreplay):
+                swp.run()
+            e1.record(stream)
+            eng.synchronize()
+            key = str(ph) + ("_eager" if eager else "")
+            scan[key] = e0.elapsed_time(e1) / nreplay
+            log(f"mixed-J sweep, FP64, {ph} phase(s), "
+                f"{'eager launches' if eager else 'graph replay'}: "
+                f"{scan[key]:.2f} ms per replay")
+        eng.set_tuning("sweep_eager", 0)
+        swp.close()
+    eng.set_tuning("sweep_phases", int(phases))
     # the same sweep with mu X products in a reduced-precision form
     # (projections FP64): split-INT8 x6 and the CRT form with 12 moduli
     def reduced(precision, name):
+        eng.set_tuning("autotune", 1)
         swr = Sweep(eng, full, precision=precision)
+        eng.set_tuning("autotune", 0)
         swr.run()
         eng.synchronize()
         e0.record(stream)
@@ -187,12 +227,21 @@ def run_mixed_sweep(nreplay):
          "plain_unbatched_ms": plain_ms,
          "plain_unbatched_tps": trans / (plain_ms / 1e3),
          "speedup_vs_plain": plain_ms / sweep_ms,
-         "graph_capture_s": create_s, "build_s": build_s,
+         "create_s": create_s, "build_s": build_s,
          "gemv_packs": gemv_packs, "gemv_bytes": gemv_bytes,
          "transition_flags_identical": same_flags,
          "s_within_fp64_bound": worst == 0.0,
-         "note": "one hipGraph replay per sweep; the bound is |dS| <= "
-                 "2e-12 S + 2e-14 sqrt(S S_max) per initial state",
+         "s_max_rel_err_above": {f"{th:g} S_max": v
+                                  for th, v in rel_by_th.items()},
+         "phases": phases, "ms_per_sweep_by_phases": scan,
+         "autotune": [ln for ln in eng.tuning_report().splitlines()
+                      if ln.startswith("half_ls")],
+         "note": "one hipGraph replay per sweep (mu X in J-ordered phases on "
+                 "the context's stream, each phase's projections on a "
+                 "second stream); create_s = tile tuning of every phase's "
+                 "mu X + eager validation run + graph capture; the bound "
+                 "is |dS| <= 2e-12 S + 2e-14 sqrt(S S_max) per initial "
+                 "state",
          "split_int8_x6": red["split_int8_x6"], "crt12": red["crt12"]}
     log(f"mixed-J sweep: {len(tasks)} tasks, {trans} transitions, "
         f"{sweep_ms:.2f} ms per replay -> {r['value']:.3e}/s; unbatched "
@@ -450,6 +499,12 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the mixed-J sweep measurement")
+    ap.add_argument("--sweep-phases", type=str, default="1",
+                    help="comma-separated phase counts at which the FP64 "
+                         "sweep is also timed (1: one stream)")
+    ap.add_argument("--sweep-only", action="store_true",
+                    help="run only the mixed-J sweep measurement and print "
+                         "its JSON object")
     ap.add_argument("--no-rotsym", action="store_true",
                     help="skip the symmetric-top half line strength timing")
     ap.add_argument("--no-adjoint", action="store_true",
@@ -471,6 +526,11 @@ def main():
 
     from gain_engine import synth, workload
 
+    if args.sweep_only:
+        scan = tuple(int(x) for x in args.sweep_phases.split(",") if x)
+        print(json.dumps(run_mixed_sweep(max(args.steps, 1), scan)),
+              flush=True)
+        return 0
     t0 = time.time()
     wl = workload.build(nc=args.nc, ninitial=args.ninitial)
     prob, eng = wl.prob, wl.eng
@@ -759,7 +819,9 @@ def main():
     wl.close()
     del wl, outs, bflags, pl, pflags
     torch.cuda.empty_cache()
-    sweep = None if args.no_sweep else run_mixed_sweep(max(args.steps, 1))
+    scan = tuple(int(x) for x in args.sweep_phases.split(",") if x)
+    sweep = None if args.no_sweep else run_mixed_sweep(max(args.steps, 1),
+                                                       scan)
     rotsym = None if args.no_rotsym else run_rotsym(3)
     adjoint = None if args.no_adjoint else run_adjoint(3)
 

@@ -1136,7 +1136,6 @@ void gemv_tune(gain_ctx* c, const double* C, int64_t ld, int ntot, int F0,
 // Lines for nI initial states of one block against one final block.
 void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
                int precision, bool plain, const gain_lines* out) {
-  const gain_params& p = c->params;
   Basis& bF = *P.bF;
   Block& blkF = *P.blkF;
   const int nI = P.nI, nF = blkF.n;
@@ -2084,6 +2083,11 @@ extern "C" int gain_set_tuning(gain_ctx* c, const char* key, int value) {
     } else if (k == "i8_split") {
       GAIN_CHECK_ARG(value >= 0 && value <= 4, "split 0..4");
       c->force_i8_split = value;
+    } else if (k == "sweep_phases") {
+      GAIN_CHECK_ARG(value >= 1 && value <= 16, "phases 1..16");
+      c->sweep_phases = value;
+    } else if (k == "sweep_eager") {
+      c->sweep_eager = value != 0;
     } else if (k == "noalloc_cap") {
       GAIN_CHECK_ARG(value >= 0, "cap >= 0 bytes (0: the whole array)");
       c->noalloc_cap = value;

@@ -216,6 +216,8 @@ struct gain_ctx {
   PinnedRing ring;
   // tuning
   int gemv_rows = 4, gemv_ksplit = 8, mu_ksplit = 0;  // 0: automatic
+  int sweep_phases = 2;  // pipeline phases of a mixed-J sweep
+  bool sweep_eager = false;  // replay sweeps by enqueueing (no graph)
   bool use_mfma = true;
   // GEMM tile autotuner: chosen configuration per (kind, M, N, K)
   bool autotune = false;

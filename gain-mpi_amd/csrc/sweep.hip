@@ -3,25 +3,36 @@
 //
 // Tasks with the same initial batch (J index, irrep, levels) share X and
 // Y = mu X.  The X of every batch sits in one column range of a single
-// matrix, so mu X for the whole sweep is one GEMM (tile shape from the
-// autotuner; FP64, or split-INT8 on the sweep's own int8 copy of mu); the
-// projections C_F G of the tasks with many vectors run as one persistent
-// grouped launch over the tiles of all of them.
+// matrix; the batches are ordered by J_I and cut into phases of about equal
+// column counts, and mu X of a phase is one GEMM over its columns (tile
+// shape from the autotuner; FP64, or split-INT8 / CRT form on the sweep's
+// own int8 copy of mu).  The products run phase after phase on the
+// context's stream; everything downstream of a phase (gathers, transposed
+// expansions, projections, epilogues) runs on a second stream once that
+// phase's product is done, so the HBM-bound projections of finished phases
+// run beside the matrix-core work of the next phase.  A final block of J_F
+// receives tasks from J_I = J_F - 1, J_F, J_F + 1 only, so with J-ordered
+// phases its projection is ready soon after its initial batches.
+// The projections C_F G of the tasks with many vectors run as one
+// persistent grouped launch per phase over the tiles of all of them.
 // Tasks that project at most 9 vectors go through the FP64 GEMV instead (a
 // 128-wide GEMM tile would be mostly padding); such tasks that target the
 // same final block are packed into one GEMV of up to 9 vectors over the
 // union of their energy windows, so the block's records are read once per
-// pack rather than once per task, and every pack of the sweep runs in one
-// grouped GEMV launch.  The per-batch and per-task kernels (scatter, gather,
-// transposed expansion, epilogue) also run as one grouped launch per stage
-// over argument tables built at creation.
+// pack rather than once per task; a pack runs in the phase of its last
+// task, and the packs of one phase run in one grouped GEMV launch.  The
+// per-batch and per-task kernels (scatter, gather, transposed expansion,
+// epilogue) also run as one grouped launch per stage and phase over
+// argument tables built at creation.
 // Everything a replay touches is owned by the sweep (sized at creation), so
 // replays stay valid whatever the context does in between.  The launch
-// sequence contains kernels only (no memset or memcpy nodes).
+// sequence contains kernels and event record/wait pairs only (no memset or
+// memcpy nodes).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <map>
+#include <numeric>
 #include <tuple>
 #include <vector>
 
@@ -34,9 +45,12 @@ struct gain_sweep {
   DeviceArena arena;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  hipStream_t side = nullptr;  // downstream stages of every phase
+  hipEvent_t join = nullptr;   // side stream done (recorded once per run)
   struct Group {
     Plan P;
     int nvecX = 0, ncolTot = 0;
+    int phase = 0;
     double* X = nullptr;
     double* Y = nullptr;
     int64_t* c_off = nullptr;
@@ -45,6 +59,8 @@ struct gain_sweep {
   struct Task {
     Plan P;
     int group = 0;
+    int phase = 0;  // phase of its initial batch
+    int ready = 0;  // phase whose side stage projects it
     int nvpi = 0, nvecG = 0, nR = 0;
     int64_t gstride = 0, hstride = 0;
     bool compute = false;
@@ -67,12 +83,10 @@ struct gain_sweep {
     int ntot = 0;
     int64_t gstride = 0;
     int Flo = 0, Fhi = 0, nvec = 0, ksplit = 1;
+    int ready = 0;  // latest phase among its tasks
     double* G = nullptr;
     double* part = nullptr;  // [ksplit][Fhi-Flo][nvec]
   };
-  std::vector<Group> groups;
-  std::vector<Task> tasks;
-  std::vector<Unit> units;
   // one grouped launch per stage: argument table + workgroup offsets
   struct Launch {
     int kind = 0;
@@ -80,25 +94,36 @@ struct gain_sweep {
     const int* start = nullptr;
     int n = 0, blocks = 0;
   };
-  std::vector<Launch> stage_scatter, stage_gather, stage_expand, stage_gemv,
-      stage_lines;
-  std::vector<GatherRotArgs> gather_rot;  // symmetric-top tasks, one launch
-                                          // each
-  std::vector<GemvArgs> gemv_single;  // units whose layout the grouped
-                                      // GEMV cannot take
-  // mu X of every batch: one GEMM over [X of batch 0 | X of batch 1 | ...]
-  GemmArgs hl{};
-  bool has_hl = false;
+  struct Phase {
+    int64_t col0 = 0, ncol = 0;  // column range of X and Y (col0 % 256 == 0)
+    bool has_hl = false;
+    GemmArgs hl{};    // FP64 mu X of the phase's columns
+    I8GemmArgs hl8{};  // split-INT8 / CRT form of the same
+    std::vector<Launch> scatter, gather, expand, gemv, lines;
+    std::vector<GatherRotArgs> gather_rot;  // symmetric-top tasks, one
+                                            // launch each
+    std::vector<GemvArgs> gemv_single;  // units whose layout the grouped
+                                        // GEMV cannot take
+    GemmArgs* d_pj_probs = nullptr;
+    GroupTile* d_pj_tiles = nullptr;
+    int n_pj_tiles = 0;
+    hipEvent_t done = nullptr;  // the phase's mu X is complete
+  };
+  std::vector<Group> groups;
+  std::vector<Task> tasks;
+  std::vector<Unit> units;
+  std::vector<Phase> phases;
   int64_t ncol_all = 0;
+  double* X = nullptr;
+  double* Y = nullptr;
   // split-INT8 (GAIN_PREC_I8X*) or CRT-form (GAIN_PREC_I8CRT*) mu X: the
   // sweep's own int8 copy of mu, X cut into planes at every replay
   int slices = 0;
   int crt = 0;  // moduli of the CRT form, 0 otherwise
-  I8GemmArgs hl8{};
+  int8_t* x8 = nullptr;
+  int* x8e = nullptr;
   int64_t x8_plane = 0;
-  GemmArgs* d_pj_probs = nullptr;
-  GroupTile* d_pj_tiles = nullptr;
-  int n_pj_tiles = 0, pj_cfg = 5;
+  int pj_cfg = 5;
   int grid = 1024;
   uint64_t gen = 0;  // context generation the captured pointers belong to
 };
@@ -117,6 +142,130 @@ void add_tiles(std::vector<GroupTile>& tiles, int prob, int M, int N, int nz,
   for (int z = 0; z < nz; ++z)
     for (int tm = 0; tm < ntm; ++tm)
       for (int tn = 0; tn < ntn; ++tn) tiles.push_back({prob, tm, tn, z});
+}
+
+template <class A>
+void add_stage(gain_sweep* s, std::vector<gain_sweep::Launch>& out, int kind,
+               const std::vector<A>& v, const GroupedTable& tab) {
+  if (tab.keep.empty()) return;
+  std::vector<A> kept;
+  for (int t : tab.keep) kept.push_back(v[t]);
+  gain_sweep::Launch L;
+  L.kind = kind;
+  L.args = up(s, kept);
+  L.start = up(s, tab.start);
+  L.n = (int)tab.keep.size();
+  L.blocks = tab.start.back();
+  out.push_back(L);
+}
+
+void run_stage(const std::vector<gain_sweep::Launch>& v, hipStream_t st) {
+  for (const gain_sweep::Launch& L : v)
+    launch_grouped(L.kind, L.args, L.start, L.n, L.blocks, st);
+}
+
+// Phases: the batches ordered by J_I (then by creation order) and cut where
+// the running column count passes p/P of the total (P from the tuning key
+// "sweep_phases", at most the number of batches); each phase's columns
+// start on a multiple of 256, so the int8 GEMM of a phase reads no plane
+// row of the next one.  Returns the batches of each phase.
+std::vector<std::vector<int>> assign_phases(gain_sweep* s) {
+  std::vector<int> order;
+  int64_t total = 0;
+  for (size_t g = 0; g < s->groups.size(); ++g)
+    if (s->groups[g].nvecX > 0) {
+      order.push_back((int)g);
+      total += s->groups[g].ncolTot;
+    }
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    return s->groups[x].P.bI->J < s->groups[y].P.bI->J;
+  });
+  const int P =
+      std::max(1, std::min<int>(s->c->sweep_phases, (int)order.size()));
+  std::vector<std::vector<int>> cut(P);
+  int64_t run = 0;
+  int p = 0;
+  for (int g : order) {
+    cut[p].push_back(g);
+    run += s->groups[g].ncolTot;
+    if (p < P - 1 && run * P >= total * (p + 1)) ++p;
+  }
+  std::vector<std::vector<int>> members;
+  for (auto& v : cut)
+    if (!v.empty()) members.push_back(v);
+  if (members.empty()) members.emplace_back();  // a sweep without products
+  s->phases.assign(members.size(), gain_sweep::Phase{});
+  int64_t col = 0;
+  for (size_t pi = 0; pi < members.size(); ++pi) {
+    col = (col + 255) / 256 * 256;
+    s->phases[pi].col0 = col;
+    for (int g : members[pi]) {
+      s->groups[g].phase = (int)pi;
+      col += s->groups[g].ncolTot;
+    }
+    s->phases[pi].ncol = col - s->phases[pi].col0;
+  }
+  s->ncol_all = s->phases.back().col0 + s->phases.back().ncol;
+  return members;
+}
+
+// Scatter launches of every phase (built before the products are tuned, so
+// that the tuner times the real X).
+void build_scatter(gain_sweep* s, const std::vector<std::vector<int>>& mem) {
+  gain_ctx* c = s->c;
+  const int nc = c->nc;
+  for (size_t pi = 0; pi < mem.size(); ++pi) {
+    std::vector<ScatterArgs> sc;
+    for (int g : mem[pi]) {
+      gain_sweep::Group& G = s->groups[g];
+      Basis& bI = *G.P.bI;
+      const Irrep& ir = bI.irrep[G.P.gI];
+      ScatterArgs sa{};
+      sa.nslot = bI.ncol * nc;
+      sa.nprim = bI.nprim;
+      sa.slot_ptr = bI.slot_ptr;
+      sa.slot_prim = bI.slot_prim;
+      sa.kval = bI.kval;
+      sa.tau = bI.tau;
+      sa.icontr = bI.icontr;
+      sa.ijt = bI.ijterms + (int64_t)G.P.gI * bI.maxsymcoeffs;
+      sa.N = ir.N;
+      sa.repres = ir.repres;
+      sa.sdeg = ir.sdeg;
+      sa.ntot = ir.ntot;
+      sa.cbase = G.P.blkI->vec;
+      sa.c_off = G.c_off;
+      sa.ideg = G.ideg;
+      sa.nvec = G.nvecX;
+      sa.X = G.X;
+      sa.xstride = (int64_t)bI.ncol * nc;
+      sc.push_back(sa);
+    }
+    add_stage(s, s->phases[pi].scatter, kGroupedScatter, sc,
+              grouped_scatter_table(sc));
+  }
+}
+
+// mu X of one phase on stream st: the FP64 GEMM, or X cut into int8 planes
+// and the split-INT8 / CRT-form products.
+void products(gain_sweep* s, const gain_sweep::Phase& ph, hipStream_t st) {
+  if (!ph.has_hl) return;
+  const int nc = s->c->nc;
+  if (s->slices == 0 && s->crt == 0) {
+    gemm_f64(ph.hl, st);
+    return;
+  }
+  const double* X = s->X + ph.col0 * nc;
+  int8_t* planes = (int8_t*)ph.hl8.B;
+  if (s->crt > 0) {
+    slice_rows_crt(s->crt, ph.hl8.beta, X, nc, (int)ph.ncol, nc, planes,
+                   s->x8_plane, (int*)ph.hl8.eB, st);
+    gemm_crt_i8(ph.hl8, st);
+  } else {
+    slice_rows_i8(s->slices, X, nc, (int)ph.ncol, nc, planes, s->x8_plane,
+                  (int*)ph.hl8.eB, st);
+    gemm_sliced_i8(ph.hl8, st);
+  }
 }
 
 void build_launches(gain_sweep* s);
@@ -154,8 +303,8 @@ void build(gain_sweep* s, const gain_task* tasks, int ntasks) {
            t);
     s->tasks.push_back(T);
   }
-  // ---- groups: X, Y (column ranges of one matrix), scatter arrays
-  std::vector<int64_t> col0(s->groups.size(), 0);
+  // ---- groups: scatter arrays, phases, X and Y (column ranges of one
+  // matrix)
   for (size_t g = 0; g < s->groups.size(); ++g) {
     gain_sweep::Group& G = s->groups[g];
     Basis& bI = *G.P.bI;
@@ -173,56 +322,86 @@ void build(gain_sweep* s, const gain_task* tasks, int ntasks) {
     if (G.nvecX == 0) continue;
     G.c_off = up(s, off);
     G.ideg = up(s, deg);
-    col0[g] = s->ncol_all;
-    s->ncol_all += G.ncolTot;
   }
+  const std::vector<std::vector<int>> mem = assign_phases(s);
+  for (gain_sweep::Phase& ph : s->phases)
+    GAIN_HIP(hipEventCreateWithFlags(&ph.done, hipEventDisableTiming));
   if (s->ncol_all > 0) {
     GAIN_CHECK_ARG(s->ncol_all < (1ll << 31), "sweep: too many columns");
-    double* X = s->arena.alloc<double>((size_t)s->ncol_all * nc);
-    double* Y = s->arena.alloc<double>((size_t)3 * s->ncol_all * nc);
-    for (size_t g = 0; g < s->groups.size(); ++g) {
-      gain_sweep::Group& G = s->groups[g];
-      if (G.nvecX == 0) continue;
-      G.X = X + col0[g] * nc;
-      G.Y = Y + col0[g] * nc;  // alpha planes are ncol_all * nc apart
+    s->X = s->arena.alloc<double>((size_t)s->ncol_all * nc);
+    s->Y = s->arena.alloc<double>((size_t)3 * s->ncol_all * nc);
+    // the columns between phases stay zero (cleared once, outside the graph)
+    GAIN_HIP(hipMemsetAsync(s->X, 0, sizeof(double) * s->ncol_all * nc,
+                            c->stream));
+    for (size_t pi = 0; pi < mem.size(); ++pi) {
+      int64_t col = s->phases[pi].col0;
+      for (int g : mem[pi]) {
+        gain_sweep::Group& G = s->groups[g];
+        G.X = s->X + col * nc;
+        G.Y = s->Y + col * nc;  // alpha planes are ncol_all * nc apart
+        col += G.ncolTot;
+      }
     }
-    GemmArgs& a = s->hl;
-    a.M = nc;
-    a.N = (int)s->ncol_all;
-    a.K = nc;
-    a.A = c->d_mu;
-    a.lda = nc;
-    a.sAz = (int64_t)nc * nc;
-    a.a_kcontig = false;
-    a.B = X;
-    a.ldb = nc;
-    a.C = Y;
-    a.ldc = nc;
-    a.sCz = s->ncol_all * nc;
-    a.nz = 3;
-    a.ksplit = 1;
-    s->has_hl = true;
-    if (s->crt > 0) {
-      const int nm = s->crt;
+    build_scatter(s, mem);
+    for (const gain_sweep::Phase& ph : s->phases)
+      run_stage(ph.scatter, c->stream);
+    // int8 copy of mu (split-INT8 or CRT form)
+    int8_t* m8 = nullptr;
+    int* me = nullptr;
+    int beta = 0;
+    const int64_t Kp = i8_kpad(nc), Rm = i8_rows_pad(nc);
+    if (s->crt > 0 || s->slices > 0) {
+      const int S = s->crt > 0 ? s->crt : s->slices;
       double* muT = s->arena.alloc<double>((size_t)3 * nc * nc);
       transpose_mu(c->d_mu, nc, muT, c->stream);
-      const int64_t Kp = i8_kpad(nc), Rm = i8_rows_pad(nc);
-      const int beta = crt_beta(nm, Kp);
-      int8_t* m8 = s->arena.alloc<int8_t>((size_t)nm * 3 * Rm * Kp);
-      int* me = s->arena.alloc<int>((size_t)3 * Rm);
-      for (int al = 0; al < 3; ++al)
-        slice_rows_crt(nm, beta, muT + (size_t)al * nc * nc, nc, nc, nc,
-                       m8 + al * Rm * Kp, 3 * Rm * Kp, me + al * Rm,
-                       c->stream);
+      m8 = s->arena.alloc<int8_t>((size_t)S * 3 * Rm * Kp);
+      me = s->arena.alloc<int>((size_t)3 * Rm);
+      if (s->crt > 0) beta = crt_beta(s->crt, Kp);
+      for (int al = 0; al < 3; ++al) {
+        if (s->crt > 0)
+          slice_rows_crt(s->crt, beta, muT + (size_t)al * nc * nc, nc, nc, nc,
+                         m8 + al * Rm * Kp, 3 * Rm * Kp, me + al * Rm,
+                         c->stream);
+        else
+          slice_rows_i8(S, muT + (size_t)al * nc * nc, nc, nc, nc,
+                        m8 + al * Rm * Kp, 3 * Rm * Kp, me + al * Rm,
+                        c->stream);
+      }
       GAIN_HIP(hipStreamSynchronize(c->stream));
       s->arena.release(muT);
       const int64_t Rx = i8_rows_pad(s->ncol_all);
       s->x8_plane = Rx * Kp;
-      I8GemmArgs& g = s->hl8;
-      g.S = nm;
+      s->x8 = s->arena.alloc<int8_t>((size_t)S * s->x8_plane);
+      s->x8e = s->arena.alloc<int>((size_t)Rx);
+    }
+    int64_t rb = 0;  // CRT residue buffer for every phase and configuration
+    for (gain_sweep::Phase& ph : s->phases) {
+      if (ph.ncol == 0) continue;
+      ph.has_hl = true;
+      GemmArgs& a = ph.hl;
+      a.M = nc;
+      a.N = (int)ph.ncol;
+      a.K = nc;
+      a.A = c->d_mu;
+      a.lda = nc;
+      a.sAz = (int64_t)nc * nc;
+      a.a_kcontig = false;
+      a.B = s->X + ph.col0 * nc;
+      a.ldb = nc;
+      a.C = s->Y + ph.col0 * nc;
+      a.ldc = nc;
+      a.sCz = s->ncol_all * nc;
+      a.nz = 3;
+      a.ksplit = 1;
+      if (s->crt == 0 && s->slices == 0) {
+        a.cfg = gemm_cfg(c, 1, a);  // tuned here: replays must not time
+        continue;
+      }
+      I8GemmArgs& g = ph.hl8;
+      g.S = s->crt > 0 ? s->crt : s->slices;
       g.beta = beta;
       g.M = nc;
-      g.N = (int)s->ncol_all;
+      g.N = (int)ph.ncol;
       g.row0 = 0;
       g.Kp = Kp;
       g.A = m8;
@@ -230,72 +409,46 @@ void build(gain_sweep* s, const gain_task* tasks, int ntasks) {
       g.a_z = Rm * Kp;
       g.eA = me;
       g.ea_z = Rm;
-      g.B = s->arena.alloc<int8_t>((size_t)nm * s->x8_plane);
+      g.B = s->x8 + ph.col0 * Kp;  // row blocks of 32 rows x Kp bytes
       g.b_plane = s->x8_plane;
-      g.eB = s->arena.alloc<int>((size_t)Rx);
-      g.C = Y;
-      g.ldc = nc;
-      g.c_z = s->ncol_all * nc;
-      g.nz = 3;
-      int64_t rb = 0;  // residue buffer for every tile configuration
-      for (int cfg = 0; cfg < kCrtConfigs; ++cfg) {
-        g.cfg = cfg;
-        rb = std::max(rb, gemm_crt_res_bytes(g));
-      }
-      g.R = s->arena.alloc<int8_t>((size_t)rb);
-      // cut X once so the tuner times real operands
-      slice_rows_crt(nm, beta, X, nc, (int)s->ncol_all, nc, (int8_t*)g.B,
-                     s->x8_plane, (int*)g.eB, c->stream);
-      g.cfg = crt_cfg_of(c, 1, g);
-    } else if (s->slices == 0) {
-      a.cfg = gemm_cfg(c, 1, a);  // tuned here: replays must not time
-    } else {
-      const int S = s->slices;
-      double* muT = s->arena.alloc<double>((size_t)3 * nc * nc);
-      transpose_mu(c->d_mu, nc, muT, c->stream);
-      const int64_t Kp = i8_kpad(nc), Rm = i8_rows_pad(nc);
-      int8_t* m8 = s->arena.alloc<int8_t>((size_t)S * 3 * Rm * Kp);
-      int* me = s->arena.alloc<int>((size_t)3 * Rm);
-      for (int al = 0; al < 3; ++al)
-        slice_rows_i8(S, muT + (size_t)al * nc * nc, nc, nc, nc,
-                      m8 + al * Rm * Kp, 3 * Rm * Kp, me + al * Rm,
-                      c->stream);
-      GAIN_HIP(hipStreamSynchronize(c->stream));
-      s->arena.release(muT);
-      const int64_t Rx = i8_rows_pad(s->ncol_all);
-      s->x8_plane = Rx * Kp;
-      I8GemmArgs& g = s->hl8;
-      g.S = S;
-      g.M = nc;
-      g.N = (int)s->ncol_all;
-      g.row0 = 0;
-      g.Kp = Kp;
-      g.A = m8;
-      g.a_plane = 3 * Rm * Kp;
-      g.a_z = Rm * Kp;
-      g.eA = me;
-      g.ea_z = Rm;
-      g.B = s->arena.alloc<int8_t>((size_t)S * s->x8_plane);
-      g.b_plane = s->x8_plane;
-      g.eB = s->arena.alloc<int>((size_t)Rx);
-      g.C = Y;
+      g.eB = s->x8e + ph.col0;
+      g.C = s->Y + ph.col0 * nc;
       g.ldc = nc;
       g.c_z = s->ncol_all * nc;
       g.c_ks = 3 * s->ncol_all * nc;
       g.nz = 3;
-      int ks = 1;
-      // slice once so the tuner times real operands
-      slice_rows_i8(S, X, nc, (int)s->ncol_all, nc, (int8_t*)g.B,
-                    s->x8_plane, (int*)g.eB, c->stream);
-      g.cfg = i8_cfg_split(c, 1, g, 1, &ks);
-      g.ksplit = ks;
+      if (s->crt > 0)
+        for (int cfg = 0; cfg < kCrtConfigs; ++cfg) {
+          g.cfg = cfg;
+          rb = std::max(rb, gemm_crt_res_bytes(g));
+        }
+    }
+    int8_t* R = rb > 0 ? s->arena.alloc<int8_t>((size_t)rb) : nullptr;
+    for (gain_sweep::Phase& ph : s->phases) {
+      if (!ph.has_hl || (s->crt == 0 && s->slices == 0)) continue;
+      I8GemmArgs& g = ph.hl8;
+      const double* X = s->X + ph.col0 * nc;
+      // cut the phase's X so the tuner times real operands
+      if (s->crt > 0) {
+        g.R = R;
+        slice_rows_crt(s->crt, beta, X, nc, (int)ph.ncol, nc, (int8_t*)g.B,
+                       s->x8_plane, (int*)g.eB, c->stream);
+        g.cfg = crt_cfg_of(c, 1, g);
+      } else {
+        slice_rows_i8(s->slices, X, nc, (int)ph.ncol, nc, (int8_t*)g.B,
+                      s->x8_plane, (int*)g.eB, c->stream);
+        int ks = 1;
+        g.cfg = i8_cfg_split(c, 1, g, 1, &ks);
+        g.ksplit = ks;
+      }
     }
   }
-  // ---- tasks: H, G, LS, per-pair arrays, projection problems
-  std::vector<GemmArgs> pj;
-  std::vector<GroupTile> pjt;
+  // ---- tasks: H, G, LS, per-pair arrays, projection problems per phase
+  std::vector<std::vector<GemmArgs>> pj(s->phases.size());
+  std::vector<std::vector<GroupTile>> pjt(s->phases.size());
   for (gain_sweep::Task& T : s->tasks) {
     const gain_sweep::Group& G = s->groups[T.group];
+    T.phase = T.ready = G.phase;
     Basis& bF = *T.P.bF;
     const Irrep& irF = bF.irrep[T.P.gF];
     const int nDI = (int)T.P.DI.size();
@@ -348,13 +501,20 @@ void build(gain_sweep* s, const gain_task* tasks, int ntasks) {
     a.ldc = T.nR;
     a.nz = 1;
     a.ksplit = 1;
-    add_tiles(pjt, (int)pj.size(), a.M, a.N, 1, s->pj_cfg);
-    pj.push_back(a);
+    std::vector<GroupTile>& tl = pjt[T.phase];
+    add_tiles(tl, (int)pj[T.phase].size(), a.M, a.N, 1, s->pj_cfg);
+    pj[T.phase].push_back(a);
   }
-  // ---- GEMV packs: tasks of one final block, in task order, up to 9
-  // vectors each
+  // ---- GEMV packs: tasks of one final block, taken phase by phase (then in
+  // task order), up to 9 vectors each; a pack runs in its last task's phase
+  std::vector<int> torder(s->tasks.size());
+  std::iota(torder.begin(), torder.end(), 0);
+  std::stable_sort(torder.begin(), torder.end(), [&](int x, int y) {
+    return s->tasks[x].phase < s->tasks[y].phase;
+  });
   std::map<std::pair<int, int>, int> open_unit;
-  for (gain_sweep::Task& T : s->tasks) {
+  for (int ti : torder) {
+    gain_sweep::Task& T = s->tasks[ti];
     if (!T.compute || !T.gemv) continue;
     const auto key = std::make_pair(T.P.jindF, T.P.gF);
     auto it = open_unit.find(key);
@@ -379,6 +539,7 @@ void build(gain_sweep* s, const gain_task* tasks, int ntasks) {
     U.nvec += T.nvecG;
     U.Flo = std::min(U.Flo, T.P.Flo);
     U.Fhi = std::max(U.Fhi, T.P.Fhi);
+    U.ready = std::max(U.ready, T.phase);
   }
   for (gain_sweep::Unit& U : s->units) {
     U.G = s->arena.alloc<double>((size_t)U.nvec * U.gstride);
@@ -390,74 +551,35 @@ void build(gain_sweep* s, const gain_task* tasks, int ntasks) {
     const gain_sweep::Unit& U = s->units[T.unit];
     T.G = U.G + (int64_t)T.vecoff * U.gstride;
     T.LS = U.part + T.vecoff;
+    T.ready = U.ready;
   }
-  if (!pj.empty()) {
-    s->d_pj_probs = up(s, pj);
-    s->d_pj_tiles = up(s, pjt);
-    s->n_pj_tiles = (int)pjt.size();
+  for (size_t pi = 0; pi < s->phases.size(); ++pi) {
+    if (pj[pi].empty()) continue;
+    gain_sweep::Phase& ph = s->phases[pi];
+    ph.d_pj_probs = up(s, pj[pi]);
+    ph.d_pj_tiles = up(s, pjt[pi]);
+    ph.n_pj_tiles = (int)pjt[pi].size();
   }
   build_launches(s);
 }
 
-template <class A>
-void add_stage(gain_sweep* s, std::vector<gain_sweep::Launch>& out, int kind,
-               const std::vector<A>& v, const GroupedTable& tab) {
-  if (tab.keep.empty()) return;
-  std::vector<A> kept;
-  for (int t : tab.keep) kept.push_back(v[t]);
-  gain_sweep::Launch L;
-  L.kind = kind;
-  L.args = up(s, kept);
-  L.start = up(s, tab.start);
-  L.n = (int)tab.keep.size();
-  L.blocks = tab.start.back();
-  out.push_back(L);
-}
-
-// Argument tables of every stage of the launch sequence (all buffers are
+// Argument tables of the downstream stages of every phase (all buffers are
 // owned by the sweep, so the pointers stay valid for every replay).
 void build_launches(gain_sweep* s) {
   gain_ctx* c = s->c;
   const gain_params& p = c->params;
   const int nc = c->nc;
-  std::vector<ScatterArgs> sc;
-  for (gain_sweep::Group& G : s->groups) {
-    if (G.nvecX == 0) continue;
-    Basis& bI = *G.P.bI;
-    const Irrep& ir = bI.irrep[G.P.gI];
-    ScatterArgs sa{};
-    sa.nslot = bI.ncol * nc;
-    sa.nprim = bI.nprim;
-    sa.slot_ptr = bI.slot_ptr;
-    sa.slot_prim = bI.slot_prim;
-    sa.kval = bI.kval;
-    sa.tau = bI.tau;
-    sa.icontr = bI.icontr;
-    sa.ijt = bI.ijterms + (int64_t)G.P.gI * bI.maxsymcoeffs;
-    sa.N = ir.N;
-    sa.repres = ir.repres;
-    sa.sdeg = ir.sdeg;
-    sa.ntot = ir.ntot;
-    sa.cbase = G.P.blkI->vec;
-    sa.c_off = G.c_off;
-    sa.ideg = G.ideg;
-    sa.nvec = G.nvecX;
-    sa.X = G.X;
-    sa.xstride = (int64_t)bI.ncol * nc;
-    sc.push_back(sa);
-  }
-  add_stage(s, s->stage_scatter, kGroupedScatter, sc,
-            grouped_scatter_table(sc));
-
-  std::vector<GatherArgs> gv;
-  std::vector<ExpandTArgs> ev;
+  const int nph = (int)s->phases.size();
+  std::vector<std::vector<GatherArgs>> gv(nph);
+  std::vector<std::vector<ExpandTArgs>> ev(nph);
   for (gain_sweep::Task& T : s->tasks) {
     if (!T.compute) continue;
     const gain_sweep::Group& G = s->groups[T.group];
+    gain_sweep::Phase& ph = s->phases[T.phase];
     Basis& bF = *T.P.bF;
     const Irrep& irF = bF.irrep[T.P.gF];
     if (T.P.bI->rotsym)
-      s->gather_rot.push_back(make_gather_rot_args(
+      ph.gather_rot.push_back(make_gather_rot_args(
           c, T.P, G.nvecX, 1, G.Y, T.H, s->ncol_all * nc,
           3 * s->ncol_all * nc));
     GatherArgs ga{};
@@ -478,7 +600,7 @@ void build_launches(gain_sweep* s) {
     ga.ys_ks = 3 * s->ncol_all * nc;
     ga.H = T.H;
     ga.hstride = T.hstride;
-    if (!T.P.bI->rotsym) gv.push_back(ga);
+    if (!T.P.bI->rotsym) gv[T.phase].push_back(ga);
     ExpandTArgs ea{};
     ea.ntot = irF.ntot;
     ea.nprim = bF.nprim;
@@ -495,15 +617,19 @@ void build_launches(gain_sweep* s) {
     ea.nvec = T.nvecG;
     ea.G = T.G;
     ea.gstride = T.gstride;
-    ev.push_back(ea);
+    ev[T.phase].push_back(ea);
   }
   // every gather before any projection input: expand_T reads H of its task
   // only, so the two stages are ordered launches
-  add_stage(s, s->stage_gather, kGroupedGather, gv, grouped_gather_table(gv));
-  add_stage(s, s->stage_expand, kGroupedExpandT, ev,
-            grouped_expand_T_table(ev));
+  for (int pi = 0; pi < nph; ++pi) {
+    gain_sweep::Phase& ph = s->phases[pi];
+    add_stage(s, ph.gather, kGroupedGather, gv[pi],
+              grouped_gather_table(gv[pi]));
+    add_stage(s, ph.expand, kGroupedExpandT, ev[pi],
+              grouped_expand_T_table(ev[pi]));
+  }
 
-  std::vector<GemvArgs> g4, g8;
+  std::vector<std::vector<GemvArgs>> g4(nph), g8(nph);
   for (const gain_sweep::Unit& U : s->units) {
     GemvArgs a{};
     a.C = U.blkF->vec;
@@ -523,14 +649,19 @@ void build_launches(gain_sweep* s) {
     // of the projected vectors
     a.rows = U.nvec > 4 ? 8 : 4;
     if (!aligned)
-      s->gemv_single.push_back(a);
+      s->phases[U.ready].gemv_single.push_back(a);
     else if (U.nvec > 4)
-      g8.push_back(a);
+      g8[U.ready].push_back(a);
     else
-      g4.push_back(a);
+      g4[U.ready].push_back(a);
   }
-  add_stage(s, s->stage_gemv, kGroupedGemv4, g4, grouped_gemv_table(g4, 4));
-  add_stage(s, s->stage_gemv, kGroupedGemv8, g8, grouped_gemv_table(g8, 8));
+  for (int pi = 0; pi < nph; ++pi) {
+    gain_sweep::Phase& ph = s->phases[pi];
+    add_stage(s, ph.gemv, kGroupedGemv4, g4[pi],
+              grouped_gemv_table(g4[pi], 4));
+    add_stage(s, ph.gemv, kGroupedGemv8, g8[pi],
+              grouped_gemv_table(g8[pi], 8));
+  }
   // algorithmic bytes of the GEMV stage: record rows inside each pack's
   // window, its projected vectors and partials (tuning getter
   // "sweep_gemv_bytes", for bandwidth figures)
@@ -540,8 +671,9 @@ void build_launches(gain_sweep* s) {
                (int64_t)U.ksplit * (U.Fhi - U.Flo) * U.nvec);
   c->last["sweep_gemv_bytes"] = gb;
   c->last["sweep_gemv_packs"] = (int64_t)s->units.size();
+  c->last["sweep_phases"] = nph;
 
-  std::vector<LinesArgs> lv;
+  std::vector<std::vector<LinesArgs>> lv(nph);
   for (gain_sweep::Task& T : s->tasks) {
     LinesArgs la{};
     la.nI = T.P.nI;
@@ -593,41 +725,58 @@ void build_launches(gain_sweep* s) {
     la.A = T.out.A;
     la.I = T.out.I;
     la.flags = T.out.flags;
-    lv.push_back(la);
+    lv[T.ready].push_back(la);
   }
-  add_stage(s, s->stage_lines, kGroupedLines, lv, grouped_lines_table(lv));
+  for (int pi = 0; pi < nph; ++pi)
+    add_stage(s, s->phases[pi].lines, kGroupedLines, lv[pi],
+              grouped_lines_table(lv[pi]));
 }
 
-void run_stage(const std::vector<gain_sweep::Launch>& v, hipStream_t st) {
-  for (const gain_sweep::Launch& L : v)
-    launch_grouped(L.kind, L.args, L.start, L.n, L.blocks, st);
+// Everything downstream of one phase's mu X, on stream st: gathers,
+// transposed expansions, the phase's grouped projection GEMM, the GEMV packs
+// that became complete, the epilogue of their tasks.
+void downstream(gain_sweep* s, const gain_sweep::Phase& ph, hipStream_t st) {
+  run_stage(ph.gather, st);
+  for (const GatherRotArgs& a : ph.gather_rot) launch_gather_rotsym(a, st);
+  run_stage(ph.expand, st);
+  if (ph.n_pj_tiles > 0)
+    gemm_f64_grouped(ph.d_pj_probs, ph.d_pj_tiles, ph.n_pj_tiles, true,
+                     s->pj_cfg, s->grid, st);
+  run_stage(ph.gemv, st);
+  for (const GemvArgs& a : ph.gemv_single) launch_gemv(a, st);
+  run_stage(ph.lines, st);
 }
 
-// The launch sequence of one sweep (kernels only): scatter of every batch,
-// mu X of every batch as one GEMM, gathers, transposed expansions, the
-// grouped projection GEMM and GEMVs, the epilogue of every task.
+// The launch sequence of one sweep (kernels and event record/wait pairs):
+// per phase, its scatter and mu X on st, then its downstream stages on the
+// side stream after an event; finally st waits for the side stream.  With
+// one phase everything runs on st.
 void enqueue(gain_sweep* s, hipStream_t st) {
-  run_stage(s->stage_scatter, st);
-  if (s->has_hl && s->slices == 0 && s->crt == 0) gemm_f64(s->hl, st);
-  if (s->has_hl && s->crt > 0) {
-    slice_rows_crt(s->crt, s->hl8.beta, s->hl.B, s->c->nc, (int)s->ncol_all,
-                   s->c->nc, (int8_t*)s->hl8.B, s->x8_plane,
-                   (int*)s->hl8.eB, st);
-    gemm_crt_i8(s->hl8, st);
+  const bool two = s->phases.size() > 1;
+  hipStream_t ds = two ? s->side : st;
+  for (const gain_sweep::Phase& ph : s->phases) {
+    run_stage(ph.scatter, st);
+    products(s, ph, st);
+    if (two) {
+      GAIN_HIP(hipEventRecord(ph.done, st));
+      GAIN_HIP(hipStreamWaitEvent(ds, ph.done, 0));
+    }
+    downstream(s, ph, ds);
   }
-  if (s->has_hl && s->slices > 0) {
-    slice_rows_i8(s->slices, s->hl.B, s->c->nc, (int)s->ncol_all, s->c->nc,
-                  (int8_t*)s->hl8.B, s->x8_plane, (int*)s->hl8.eB, st);
-    gemm_sliced_i8(s->hl8, st);
+  if (two) {
+    GAIN_HIP(hipEventRecord(s->join, ds));
+    GAIN_HIP(hipStreamWaitEvent(st, s->join, 0));
   }
-  run_stage(s->stage_gather, st);
-  for (const GatherRotArgs& a : s->gather_rot) launch_gather_rotsym(a, st);
-  run_stage(s->stage_expand, st);
-  gemm_f64_grouped(s->d_pj_probs, s->d_pj_tiles, s->n_pj_tiles, true,
-                   s->pj_cfg, s->grid, st);
-  run_stage(s->stage_gemv, st);
-  for (const GemvArgs& a : s->gemv_single) launch_gemv(a, st);
-  run_stage(s->stage_lines, st);
+}
+
+void release(gain_sweep* s) {
+  if (s->exec) (void)hipGraphExecDestroy(s->exec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  for (gain_sweep::Phase& ph : s->phases)
+    if (ph.done) (void)hipEventDestroy(ph.done);
+  if (s->join) (void)hipEventDestroy(s->join);
+  if (s->side) (void)hipStreamDestroy(s->side);
+  delete s;
 }
 
 }  // namespace
@@ -654,6 +803,8 @@ extern "C" int gain_sweep_create(gain_ctx* c, const gain_task* tasks,
     s->slices = i8_slices_of(precision);
     s->crt = crt_moduli_of(precision);
     if (c->force_cfg[0] >= 0) s->pj_cfg = c->force_cfg[0];
+    GAIN_HIP(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
+    GAIN_HIP(hipEventCreateWithFlags(&s->join, hipEventDisableTiming));
     build(s, tasks, ntasks);
     GAIN_HIP(hipStreamSynchronize(c->stream));
     enqueue(s, c->stream);  // eager run: validates every launch
@@ -673,9 +824,9 @@ extern "C" int gain_sweep_create(gain_ctx* c, const gain_task* tasks,
     *out = s;
   });
   if (rc != GAIN_OK && s) {
-    if (s->exec) (void)hipGraphExecDestroy(s->exec);
-    if (s->graph) (void)hipGraphDestroy(s->graph);
-    delete s;
+    (void)hipStreamSynchronize(c->stream);
+    if (s->side) (void)hipStreamSynchronize(s->side);
+    release(s);
   }
   return rc;
 }
@@ -689,7 +840,10 @@ extern "C" int gain_sweep_run(gain_sweep* s) {
            "it captured was replaced or changed after gain_sweep_create; "
            "create the sweep again");
     DeviceGuard g(s->c->device);
-    GAIN_HIP(hipGraphLaunch(s->exec, s->c->stream));
+    if (s->c->sweep_eager)
+      enqueue(s, s->c->stream);  // the same sequence, launched one by one
+    else
+      GAIN_HIP(hipGraphLaunch(s->exec, s->c->stream));
   });
 }
 
@@ -698,8 +852,7 @@ extern "C" int gain_sweep_destroy(gain_sweep* s) {
     if (!s) return;
     DeviceGuard g(s->c->device);
     (void)hipStreamSynchronize(s->c->stream);
-    if (s->exec) (void)hipGraphExecDestroy(s->exec);
-    if (s->graph) (void)hipGraphDestroy(s->graph);
-    delete s;
+    if (s->side) (void)hipStreamSynchronize(s->side);
+    release(s);
   });
 }

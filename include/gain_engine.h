@@ -217,11 +217,14 @@ int gain_batched_lines(gain_ctx* ctx, int jindI, int gammaI,
  * (J index, irrep) block against one final block (the work of many
  * gain_batched_lines calls, e.g. every J_I -> J_I-1, J_I, J_I+1 pair of a J
  * list).  At creation the engine plans every task, runs it once, and
- * captures the launch sequence in a HIP graph: the half line strengths of
- * all batches (tasks sharing an initial batch share them) run as one GEMM,
- * the projections of all tasks as one persistent grouped FP64 MFMA launch
- * and one grouped GEMV launch, and every per-task kernel as one grouped
- * launch per stage.  `precision` is GAIN_PREC_FP64, a split-INT8 mode
+ * captures the launch sequence in a HIP graph: the initial batches
+ * (tasks sharing an initial batch share their half line strengths) are
+ * ordered by J and cut into phases (tuning key "sweep_phases"); the mu X
+ * products of a phase run as one GEMM on the context's stream, and
+ * everything downstream of it (gathers, projections as one persistent
+ * grouped FP64 MFMA launch and one grouped GEMV launch, every per-task
+ * kernel as one grouped launch per stage) on a second stream of the sweep,
+ * beside the next phase's products.  `precision` is GAIN_PREC_FP64, a split-INT8 mode
  * (GAIN_PREC_I8X5..7) or a CRT mode (GAIN_PREC_I8CRT11/12/14); the reduced
  * modes apply to the mu X products only (the projections stay FP64).
  * gain_sweep_run replays the graph on the context's stream
@@ -285,12 +288,17 @@ int gain_get_timing(gain_ctx* ctx, const char* name, double* total_ms,
  * "crt_cfg" (CRT-form int8 GEMMs: force a tile configuration 0..13),
  * "gemv_rows" (2/4/8), "gemv_ksplit", "mu_ksplit" (0 = automatic),
  * "mfma" (0: vector-ALU GEMMs, for cross-checks), "noalloc_cap" (bytes of
- * the allocation-free entry point's array a call may use, 0 = all). */
+ * the allocation-free entry point's array a call may use, 0 = all),
+ * "sweep_phases" (1..16, default 2: pipeline phases of sweeps created
+ * afterwards; 1 = one stream), "sweep_eager" (1: gain_sweep_run launches
+ * the sweep's kernels one by one instead of replaying its graph). */
 int gain_set_tuning(gain_ctx* ctx, const char* key, int value);
 /* Current value of a settable key, or what the most recent launch used:
  * "proj_cfg", "proj_split", "proj_grid" (FP64 MFMA projection GEMM),
  * "half_ls_cfg", "half_ls_split", "half_ls_grid" (mu GEMM), "gemv_grid"
- * (plain GEMV); *_grid = work-items of that launch; "autotune_us" = host
+ * (plain GEMV); *_grid = work-items of that launch; "sweep_phases",
+ * "sweep_gemv_packs", "sweep_gemv_bytes" (the latest sweep created);
+ * "autotune_us" = host
  * time spent timing tile configurations so far. No reference
  * counterpart: the reference has no tunable kernels. */
 int gain_get_tuning(gain_ctx* ctx, const char* key, long long* value);

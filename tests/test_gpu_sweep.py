@@ -52,8 +52,12 @@ def _check(eng, tasks, outs, rel=1e-12):
     return n
 
 
-@pytest.mark.parametrize("cfgs", [(-1, -1), (0, 7), (9, 3)])
-def test_mixed_j_sweep_graph_replays_match_unbatched(cfgs):
+@pytest.mark.parametrize("cfgs,phases", [((-1, -1), 2), ((0, 7), 1),
+                                         ((9, 3), 3), ((-1, -1), 16)])
+def test_mixed_j_sweep_graph_replays_match_unbatched(cfgs, phases):
+    """Also over the pipeline phases: 1 (one stream), 2 (the default), 3
+    and 16 (one batch per phase: each phase's projections on the second
+    stream beside the next phase's mu X)."""
     import torch
     prob = synth.make_problem([5, 6, 7, 8], 12, "c2v4")
     prob.params.erange_lower = (0.0, 1e9)
@@ -61,6 +65,7 @@ def test_mixed_j_sweep_graph_replays_match_unbatched(cfgs):
     with Engine.from_problem(prob) as eng:
         eng.set_tuning("proj_cfg", cfgs[0])
         eng.set_tuning("half_ls_cfg", cfgs[1])
+        eng.set_tuning("sweep_phases", phases)
         tasks, outs = _tasks(prob, eng, rng, max_per_block=24)
         assert sum(len(t[2]) for t in tasks) > 256
         sw = Sweep(eng, tasks)
@@ -83,18 +88,21 @@ def test_mixed_j_sweep_graph_replays_match_unbatched(cfgs):
         sw.close()
 
 
-@pytest.mark.parametrize("precision,rel", [(8, 1e-7), (9, 1e-9)])
-def test_mixed_j_sweep_crt(precision, rel):
+@pytest.mark.parametrize("precision,rel,phases", [(8, 1e-7, 4),
+                                                   (8, 1e-7, 1),
+                                                   (9, 1e-9, 16)])
+def test_mixed_j_sweep_crt(precision, rel, phases):
     """A sweep whose mu X products run in the CRT form (12 or 14 moduli, the
-    sweep's own residues of mu, X cut into residue planes inside the graph)
-    against the unbatched plain FP64 path: transition lists identical, S
-    within `rel` on strong lines, on two replays with eager CRT and FP64
-    work on the context between them."""
+    sweep's own residues of mu, X cut into residue planes per phase inside
+    the graph) against the unbatched plain FP64 path: transition lists
+    identical, S within `rel` on strong lines, on two replays with eager
+    CRT and FP64 work on the context between them."""
     import torch
     prob = synth.make_problem([5, 6, 7, 8], 12, "c2v4")
     prob.params.erange_lower = (0.0, 1e9)
     rng = np.random.default_rng(7)
     with Engine.from_problem(prob) as eng:
+        eng.set_tuning("sweep_phases", phases)
         tasks, outs = _tasks(prob, eng, rng, max_per_block=24)
         sw = Sweep(eng, tasks, precision=precision)
         for rep in range(2):
@@ -147,31 +155,46 @@ def test_mixed_j_sweep_split_int8(precision):
         sw.close()
 
 
-@pytest.mark.slow
-def test_mixed_j_sweep_full_shape_matches_unbatched():
-    """256 initial states drawn from J_I in [5, 30] (Nc = 3000, uneven
-    blocks N_J = (2J+1) Nc / 4, 2000 states per block, 48 GB of records in
-    HBM), every dJ in {-1, 0, +1} final block, one captured sweep replayed
-    twice, against the unbatched plain FP64 path: transition flags
-    identical; S within the condition-normalised FP64 bound (the two paths
-    sum in different orders)."""
+@pytest.fixture(scope="module")
+def mixed_full():
+    """The full mixed-J shape: 256 initial states drawn from J_I in [5, 30]
+    (Nc = 3000, uneven blocks N_J = (2J+1) Nc / 4, 2000 states per block,
+    48 GB of records in HBM), every dJ in {-1, 0, +1} final block, with the
+    unbatched plain FP64 lines of every task as the reference."""
     import torch
     from gain_engine import workload
     w = workload.build_mixed()
     try:
         eng = w.eng
         tasks = w.tasks()
-        outs, full = [], []
+        refs = []
         for (jI, gI, levels, jF, gF) in tasks:
-            nF = w.prob.blocks[(jF, gF)].n
-            o = {k: torch.zeros((len(levels), nF), dtype=torch.float64,
-                                device="cuda:0") for k in ("S",)}
-            o["flags"] = torch.zeros((len(levels), nF), dtype=torch.uint8,
-                                     device="cuda:0")
-            outs.append(o)
-            full.append((jI, gI, levels, jF, gF,
-                         {k: v.data_ptr() for k, v in o.items()}))
-        sw = Sweep(eng, full)
+            rs = [eng.plain_lines(jI, gI, lv, jF, gF, want_ls=False)
+                  for lv in levels]
+            refs.append((np.stack([r["flags"] for r in rs]),
+                         np.stack([r["S"] for r in rs])))
+        yield w, tasks, refs
+    finally:
+        torch.cuda.synchronize()
+        w.close()
+
+
+def _full_sweep(w, tasks, refs, precision, check):
+    import torch
+    eng = w.eng
+    outs, full = [], []
+    for (jI, gI, levels, jF, gF) in tasks:
+        nF = w.prob.blocks[(jF, gF)].n
+        o = {"S": torch.zeros((len(levels), nF), dtype=torch.float64,
+                              device="cuda:0"),
+             "flags": torch.zeros((len(levels), nF), dtype=torch.uint8,
+                                  device="cuda:0")}
+        outs.append(o)
+        full.append((jI, gI, levels, jF, gF,
+                     {k: v.data_ptr() for k, v in o.items()}))
+    torch.cuda.synchronize()
+    sw = Sweep(eng, full, precision=precision)
+    try:
         for rep in range(2):
             for o in outs:
                 for v in o.values():
@@ -180,23 +203,54 @@ def test_mixed_j_sweep_full_shape_matches_unbatched():
             sw.run()
             eng.synchronize()
             nlines = 0
-            for (jI, gI, levels, jF, gF), o in zip(tasks, outs):
+            for t, o, (rfl, rS) in zip(tasks, outs, refs):
                 fl = o["flags"].cpu().numpy()
                 S = o["S"].cpu().numpy()
-                for i, lv in enumerate(levels):
-                    r = eng.plain_lines(jI, gI, lv, jF, gF, want_ls=False)
-                    assert np.array_equal(r["flags"], fl[i]), (jI, gI, lv, jF)
-                    ref = r["S"]
-                    if ref.max() <= 0:
-                        continue
-                    d = np.abs(S[i] - ref)
-                    assert (d <= 2e-12 * ref + 2e-14 * np.sqrt(
-                        ref * ref.max())).all(), (jI, gI, lv, jF)
-                    nlines += int((fl[i] & 1).sum())
+                assert np.array_equal(fl, rfl), t
+                for i in range(len(t[2])):
+                    if rS[i].max() > 0:
+                        check(S[i], rS[i], t)
+                nlines += int((fl & 1).sum())
             assert nlines > 100000
-        sw.close()
     finally:
-        w.close()
+        sw.close()
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("phases", [2, 1])
+def test_mixed_j_sweep_full_shape_matches_unbatched(mixed_full, phases):
+    """One captured FP64 sweep of the full shape (pipelined over 2 phases,
+    and on one stream) replayed twice, against the unbatched plain FP64
+    path: transition flags identical; S within the condition-normalised FP64
+    bound |dS| <= 2e-12 S + 2e-14 sqrt(S S_max) (the two paths sum in
+    different orders), and within 1e-12 relative on lines with S > 1e-2
+    S_max."""
+    w, tasks, refs = mixed_full
+    w.eng.set_tuning("sweep_phases", phases)
+
+    def check(S, ref, t):
+        d = np.abs(S - ref)
+        assert (d <= 2e-12 * ref + 2e-14 * np.sqrt(ref * ref.max())).all(), t
+        strong = ref > 1e-2 * ref.max()
+        assert (d[strong] <= 1e-12 * ref[strong]).all(), t
+    _full_sweep(w, tasks, refs, 0, check)
+
+
+@pytest.mark.slow
+def test_mixed_j_sweep_full_shape_crt12(mixed_full):
+    """The same full-shape sweep with mu X in the CRT form (12 moduli),
+    pipelined over 4 phases: transition flags identical to the plain FP64
+    path, S within 1e-7 on strong lines (S > 1e-6 S_max of its initial
+    state) and within 1e-7 S_max on every line."""
+    w, tasks, refs = mixed_full
+    w.eng.set_tuning("sweep_phases", 4)
+
+    def check(S, ref, t):
+        d = np.abs(S - ref)
+        strong = ref > 1e-6 * ref.max()
+        assert (d[strong] <= 1e-7 * ref[strong]).all(), t
+        assert (d <= 1e-7 * ref.max()).all(), t
+    _full_sweep(w, tasks, refs, 8, check)
 
 
 @pytest.mark.parametrize("change", ["dipole", "block", "combination"])
