@@ -8,6 +8,7 @@
 #include "common.hpp"
 #include "gemm.hpp"
 #include "kernels.hpp"
+#include "ozaki.hpp"
 
 namespace gain {
 
@@ -15,41 +16,108 @@ typedef double dbl2 __attribute__((ext_vector_type(2)));
 
 // ---------------------------------------------------------------------------
 // Expand + scatter (dipole_GPU.cu:859-889 fused with the K/tau grouping of
-// the half line strength).  One thread per (column, v) slot of X:
+// the half line strength).  One thread per (column, v) slot of X and
+// kScatterNV consecutive vectors:
 //   X[vec][2K+tau][v] = sum_{r in slot} (-1)^{(K mod 3) tau + K} x[r],
 //   x[r] = sum_{i<N[case]} c[t0+i] repres[(t0+i) + d*Ntot + lambda*sdeg*Ntot]
-// The sign is the initial-side factor of dipole_GPU.cu:437-438.
-__device__ inline void scatter_body(const ScatterArgs& a, int bx, int b) {
+// The sign is the initial-side factor of dipole_GPU.cu:437-438.  The slot's
+// primitives and their symmetrisation cases are loaded once for all the
+// thread's vectors, so the dependent index loads are shared.
+__device__ inline int frexp_expo(double x) {
+  int e = kExpoEmpty;
+  if (x != 0.0) frexp(x, &e);  // |x| < 2^e
+  return e;
+}
+
+__device__ inline int wave_max(int v) {
+  for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off));
+  return v;
+}
+
+__device__ inline void scatter_body(const ScatterArgs& a, int bx, int b0) {
   const int s = bx * 256 + threadIdx.x;
-  if (s >= a.nslot) return;
-  const double* c = a.cbase + a.c_off[b];
-  const int ideg = a.ideg[b];
-  double acc = 0.0;
-  for (int q = a.slot_ptr[s]; q < a.slot_ptr[s + 1]; ++q) {
+  const bool live = s < a.nslot;
+  const int nv = min(kScatterNV, a.nvec - b0);
+  double acc[kScatterNV];
+  const double* cv[kScatterNV];
+  int64_t dof[kScatterNV];
+#pragma unroll
+  for (int i = 0; i < kScatterNV; ++i) {
+    acc[i] = 0.0;
+    const int b = b0 + min(i, nv - 1);
+    cv[i] = a.cbase + a.c_off[b];
+    dof[i] = (int64_t)a.ideg[b] * a.ntot;
+  }
+  const int q0 = live ? a.slot_ptr[s] : 0;
+  const int q1 = live ? a.slot_ptr[s + 1] : 0;
+  for (int q = q0; q < q1; ++q) {
     const int r = a.slot_prim[q];
     const int icase = a.icontr[r];
     const int lam = a.icontr[r + a.nprim];
     const int t0 = a.ijt[icase];
     const int ne = a.N[icase];
-    const double* rp = a.repres + (int64_t)ideg * a.ntot +
-                       (int64_t)lam * a.sdeg * a.ntot;
-    double x = 0.0;
-    for (int i = 0; i < ne; ++i) x = fma(c[t0 + i], rp[t0 + i], x);
+    const double* rp = a.repres + (int64_t)lam * a.sdeg * a.ntot + t0;
     const int K = a.kval[r], t = a.tau[r];
-    acc += ((((K % 3) * t + K) & 1) ? -x : x);
+    const bool neg = (((K % 3) * t + K) & 1) != 0;
+#pragma unroll
+    for (int i = 0; i < kScatterNV; ++i) {
+      const double* c = cv[i] + t0;
+      const double* p = rp + dof[i];
+      double x = 0.0;
+      for (int k = 0; k < ne; ++k) x = fma(c[k], p[k], x);
+      acc[i] += neg ? -x : x;
+    }
   }
-  a.X[(int64_t)b * a.xstride + s] = acc;
+#pragma unroll
+  for (int i = 0; i < kScatterNV; ++i) {
+    if (i >= nv) break;
+    const int b = b0 + i;
+    if (live) a.X[(int64_t)b * a.xstride + s] = acc[i];
+    if (!a.expo) continue;
+    // row exponent: the lanes of a wave hold one or two X rows (nc >= 64),
+    // each reduced over the wave and combined by one atomic per row
+    const int row = live ? s / a.nc : -1;
+    const int e = live ? frexp_expo(acc[i]) : kExpoEmpty;
+    const int64_t rbase = (int64_t)b * (a.nslot / a.nc);
+    if (a.nc < 64) {
+      if (live && e != kExpoEmpty) atomicMax(a.expo + rbase + row, e);
+      continue;
+    }
+    const int row0 = __shfl(row, 0);
+    const int rowx = wave_max(row);
+    const int e0 = wave_max(row == row0 ? e : kExpoEmpty);
+    const int e1 = wave_max(row != row0 ? e : kExpoEmpty);
+    if ((threadIdx.x & 63) == 0) {
+      if (row0 >= 0 && e0 != kExpoEmpty) atomicMax(a.expo + rbase + row0, e0);
+      if (rowx != row0 && rowx >= 0 && e1 != kExpoEmpty)
+        atomicMax(a.expo + rbase + rowx, e1);
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void k_scatter_expand(ScatterArgs a) {
-  scatter_body(a, blockIdx.x, blockIdx.y);
+  scatter_body(a, blockIdx.x, blockIdx.y * kScatterNV);
 }
 
 void launch_scatter_expand(const ScatterArgs& a, hipStream_t s) {
   if (a.nslot == 0 || a.nvec == 0) return;
-  dim3 grid((a.nslot + 255) / 256, a.nvec);
+  GAIN_CHECK_ARG(!a.expo || (a.nc > 0 && a.nslot % a.nc == 0),
+                 "scatter: %d slots are not whole rows of %d", a.nslot, a.nc);
+  dim3 grid((a.nslot + 255) / 256, (a.nvec + kScatterNV - 1) / kScatterNV);
   hipLaunchKernelGGL(k_scatter_expand, grid, dim3(256), 0, s, a);
   check_launch("scatter_expand");
+}
+
+__global__ __launch_bounds__(256) void k_fill_int(int* p, int64_t n, int v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+void launch_fill_int(int* p, int64_t n, int v, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((n + 255) / 256)),
+                     dim3(256), 0, s, p, n, v);
+  check_launch("fill_int");
 }
 
 // ---------------------------------------------------------------------------
@@ -782,11 +850,13 @@ struct ScatterOp {
     return (a.nslot + 255) / 256;
   }
   static int64_t blocks(const Args& a) {
-    return a.nslot == 0 ? 0 : (int64_t)nbx(a) * a.nvec;
+    return a.nslot == 0
+               ? 0
+               : (int64_t)nbx(a) * ((a.nvec + kScatterNV - 1) / kScatterNV);
   }
   static __device__ void run(const Args& a, int local) {
     const int n = nbx(a);
-    scatter_body(a, local % n, local / n);
+    scatter_body(a, local % n, (local / n) * kScatterNV);
   }
 };
 struct GatherOp {

@@ -157,6 +157,15 @@ int crt_beta(int nm, int64_t K);
 void slice_rows_crt(int nm, int beta, const double* src, int64_t ld,
                     int rows, int K, int8_t* planes, int64_t plane,
                     int* expo, hipStream_t s);
+// Row exponent of an all-zero row in an exponent array filled by atomic
+// maxima (the scatter's); the slicing stores 0 for such rows, as its own
+// exponent pass does.
+constexpr int kExpoEmpty = -2147483647 - 1;
+// Same as slice_rows_crt with the exponents of the `rows` rows already in
+// `expo` (frexp exponent of each row's largest |x|, or kExpoEmpty).
+void slice_rows_crt_given(int nm, int beta, const double* src, int64_t ld,
+                          int rows, int K, int8_t* planes, int64_t plane,
+                          int* expo, hipStream_t s);
 constexpr int kCrtConfigs = 14;
 // Tile configuration used when nothing is tuned or forced: 256 x 256
 // workgroup tiles through the 4-stage LDS pipeline, the autotuner's pick for

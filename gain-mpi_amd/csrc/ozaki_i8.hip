@@ -695,13 +695,20 @@ constexpr int kSliceKch = 16;  // k blocks per workgroup
 template <int NM>
 __global__ __launch_bounds__(256) void k_slice_crt(
     const double* __restrict__ src, int64_t ld, int rows, int K, int64_t Kp,
-    int64_t plane, int8_t* __restrict__ planes,
-    const int* __restrict__ expo, int beta, CrtConsts cc) {
+    int64_t plane, int8_t* __restrict__ planes, int* __restrict__ expo,
+    int beta, CrtConsts cc) {
   const int t = threadIdx.x;
   const int rr = t >> 3, kk = 4 * (t & 7);
   const int r = blockIdx.x * 32 + rr;
   const bool live = r < rows;
-  const int sh = live ? beta - 1 - expo[r] : 0;
+  int e = live ? expo[r] : 0;
+  if (e == kExpoEmpty) {  // all-zero row of an atomic-max exponent array
+    e = 0;
+    // stored for the reconstruction; workgroups that read the array
+    // meanwhile see either value and take 0 for both
+    if (blockIdx.y == 0 && (t & 7) == 0) expo[r] = 0;
+  }
+  const int sh = live ? beta - 1 - e : 0;
   const double* x = src + (live ? (int64_t)r * ld : 0);
   const int64_t nkb = Kp >> 5;
   const int64_t kb0 = (int64_t)blockIdx.y * kSliceKch;
@@ -732,9 +739,9 @@ __global__ __launch_bounds__(256) void k_slice_crt(
   }
 }
 
-void slice_rows_crt(int nm, int beta, const double* src, int64_t ld,
-                    int rows, int K, int8_t* planes, int64_t plane,
-                    int* expo, hipStream_t s) {
+static void slice_crt_planes(int nm, int beta, const double* src, int64_t ld,
+                             int rows, int K, int8_t* planes, int64_t plane,
+                             int* expo, bool given, hipStream_t s) {
   const int64_t Rp = i8_rows_pad(rows), Kp = i8_kpad(K);
   GAIN_CHECK_ARG(rows >= 0 && K > 0 && plane >= Rp * Kp,
                  "slice_rows_crt: plane of %lld bytes < %lld x %lld",
@@ -743,9 +750,11 @@ void slice_rows_crt(int nm, int beta, const double* src, int64_t ld,
                  "slice_rows_crt: planes must be 4-byte aligned");
   GAIN_CHECK_ARG(beta >= 9 && beta <= 50, "slice_rows_crt: beta %d", beta);
   const CrtConsts& cc = crt_consts(nm);
-  hipLaunchKernelGGL(k_row_expo, dim3((unsigned)Rp), dim3(256), 0, s, src,
-                     ld, rows, K, expo);
-  check_launch("row_expo");
+  if (!given) {
+    hipLaunchKernelGGL(k_row_expo, dim3((unsigned)Rp), dim3(256), 0, s, src,
+                       ld, rows, K, expo);
+    check_launch("row_expo");
+  }
   const dim3 grid((unsigned)(Rp / 32),
                   (unsigned)((Kp / 32 + kSliceKch - 1) / kSliceKch));
   switch (nm) {
@@ -762,6 +771,18 @@ void slice_rows_crt(int nm, int beta, const double* src, int64_t ld,
       fail(GAIN_ERR_ARG, "CRT moduli count %d not supported", nm);
   }
   check_launch("slice_rows_crt");
+}
+
+void slice_rows_crt(int nm, int beta, const double* src, int64_t ld,
+                    int rows, int K, int8_t* planes, int64_t plane,
+                    int* expo, hipStream_t s) {
+  slice_crt_planes(nm, beta, src, ld, rows, K, planes, plane, expo, false, s);
+}
+
+void slice_rows_crt_given(int nm, int beta, const double* src, int64_t ld,
+                          int rows, int K, int8_t* planes, int64_t plane,
+                          int* expo, hipStream_t s) {
+  slice_crt_planes(nm, beta, src, ld, rows, K, planes, plane, expo, true, s);
 }
 
 // Configurations of the per-modulus GEMM: wave tile (32 TM) x (32 TN), WM x
