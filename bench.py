@@ -768,6 +768,14 @@ def main():
     hl_traffic = measured_traffic(label_h, hl_launch["grid_items_batch0"])
     head["roofline_half_ls"]["traffic"] = hl_traffic
     head["roofline_half_ls"]["launch"] = hl_launch
+    if args.precision >= 7:
+        # CRT form: the products skip the dipole's all-zero k blocks (the
+        # entries that round to 0 at the planes' width), so the matrix cores
+        # execute this fraction of the dense products that "achieved" counts
+        kf = eng.get_tuning("half_ls_kblocks_ppm") / 1e6
+        hl_launch["kblocks_multiplied_frac"] = kf
+        head["roofline_half_ls"]["executed_frac"] = \
+            head["roofline_half_ls"]["frac"] * kf
     # one-time work of the headline context: planes of mu and of the final
     # block, and the autotuner's timing of every tile configuration
     prep = {"planes_ms": eng.timing("prep")[0],

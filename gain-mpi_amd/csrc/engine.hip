@@ -151,6 +151,7 @@ void release_sliced(gain_ctx* c, Sliced& sl) {
   if (sl.planes) c->arena.release(sl.planes);
   if (sl.p8) c->arena.release(sl.p8);
   if (sl.expo) c->arena.release(sl.expo);
+  if (sl.kr) c->arena.release(sl.kr);
   sl = Sliced{};
 }
 
@@ -257,6 +258,17 @@ const Sliced& mu_sliced_crt(gain_ctx* c, int nm) {
     slice_rows_crt(nm, sl.beta, muT + (size_t)a * nc * nc, nc, nc, nc,
                    sl.p8 + a * sl.Rp * sl.Kp, sl.plane, sl.expo + a * sl.Rp,
                    c->stream);
+  // the dipole's zero k blocks (entries that round to 0 at beta bits, e.g.
+  // far from the diagonal of a banded dipole) are skipped by the products
+  const int nrb = (int)(sl.Rp / 32);
+  sl.kr = c->arena.alloc<int2>((size_t)3 * nrb);
+  i8_block_kranges(sl.p8, sl.plane, nm, sl.Rp * sl.Kp, 3, nrb,
+                   (int)(sl.Kp / 32), sl.kr, c->stream);
+  sl.kr_host.resize((size_t)3 * nrb);
+  GAIN_HIP(hipMemcpyAsync(sl.kr_host.data(), sl.kr,
+                        sl.kr_host.size() * sizeof(int2),
+                        hipMemcpyDeviceToHost, c->stream));
+  GAIN_HIP(hipStreamSynchronize(c->stream));
   c->mu_sl = sl;
   return c->mu_sl;
 }
@@ -904,11 +916,19 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
     g.ldc = nc;
     g.c_z = (int64_t)ncolTot * nc;
     g.nz = 3;
+    if (c->crt_kskip) {
+      g.a_kr = ms.kr;
+      g.akr_z = ms.Rp / 32;
+      g.akr_n = (int)(ms.Rp / 32);
+    }
     ks = 1;
     Y = c->w_Y.get<double>((size_t)3 * ncolTot * nc);
     g.C = Y;
     g.R = crt_residues(c, g);
     g.cfg = crt_cfg_of(c, 1, g);
+    // k blocks multiplied, parts per million of the dense count
+    c->last["half_ls_kblocks_ppm"] = (int64_t)std::llround(
+        1e6 * crt_kblock_fraction(g, ms.kr_host.data()));
     {
       Timed tg(c, "half_ls_gemm");
       gemm_crt_res(g, c->stream);
@@ -2087,6 +2107,8 @@ extern "C" int gain_set_tuning(gain_ctx* c, const char* key, int value) {
       GAIN_CHECK_ARG(value >= -1 && value < kCrtConfigs, "cfg -1..%d",
                      kCrtConfigs - 1);
       c->force_crt_cfg = value;
+    } else if (k == "crt_kskip") {
+      c->crt_kskip = value != 0;
     } else if (k == "i8_split") {
       GAIN_CHECK_ARG(value >= 0 && value <= 4, "split 0..4");
       c->force_i8_split = value;
@@ -2222,6 +2244,8 @@ extern "C" int gain_get_tuning(gain_ctx* c, const char* key,
       *value = c->use_mfma;
     } else if (k == "autotune") {
       *value = c->autotune;
+    } else if (k == "crt_kskip") {
+      *value = c->crt_kskip;
     } else {
       auto it = c->last.find(k);
       if (it == c->last.end())

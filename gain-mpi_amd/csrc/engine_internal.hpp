@@ -162,6 +162,10 @@ struct Sliced {
   uint16_t* planes = nullptr;
   int8_t* p8 = nullptr;
   int* expo = nullptr;
+  // CRT form of the dipole: nonzero k-block range of every row block of
+  // each component (device and host copies, Rp / 32 entries per component)
+  int2* kr = nullptr;
+  std::vector<int2> kr_host;
 };
 
 struct Block {
@@ -225,6 +229,7 @@ struct gain_ctx {
   int force_split = 1;          // projection split-K when not tuned
   int force_i8_cfg = -1, force_i8_split = 0;  // split-INT8 GEMMs
   int force_crt_cfg = -1;                     // CRT-form int8 GEMMs
+  bool crt_kskip = true;  // CRT dipole products skip zero k blocks
   std::map<std::tuple<int, int, int, int>, int> tuned;
   std::string tune_log;
   // what the most recent launch of each tuned kernel used (tuning getter)

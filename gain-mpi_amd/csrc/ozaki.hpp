@@ -114,6 +114,15 @@ struct I8GemmArgs {
   int cfg = 0;
   int beta = 0;      // CRT form: integer bits per operand element
   int8_t* R = nullptr;  // CRT form: residue buffer, gemm_crt_res_bytes(g)
+  // CRT form, optional: nonzero k-block range of the A operand.
+  // a_kr[z * akr_z + rb] = [x, y) holds every k block (32 columns) of A row
+  // block rb (32 rows) of matrix z that has a nonzero element in some plane
+  // (x >= y: the row block is zero).  A tile multiplies only the k blocks
+  // inside the union of its row blocks' ranges; the products it skips are
+  // exact zeros, so the residues are those of the full product.
+  const int2* a_kr = nullptr;
+  int64_t akr_z = 0;
+  int akr_n = 0;     // row blocks per matrix covered by a_kr
 };
 constexpr int kI8Configs = 12;
 // whether configuration cfg supports S slices
@@ -182,6 +191,18 @@ void gemm_crt_i8(const I8GemmArgs& g, hipStream_t s);
 void gemm_crt_res(const I8GemmArgs& g, hipStream_t s);
 void gemm_crt_combine(const I8GemmArgs& g, hipStream_t s);
 int64_t gemm_crt_res_bytes(const I8GemmArgs& g);
+// Nonzero k-block range of every row block of nmat tiled int8 matrices
+// (nrb row blocks of nkb k blocks each, matrices mat_stride bytes apart
+// inside a plane, nplanes planes `plane` bytes apart): out[m * nrb + rb] =
+// [first, last + 1) over the k blocks with a nonzero byte in any plane,
+// (0, 0) for a zero row block.
+void i8_block_kranges(const int8_t* planes, int64_t plane, int nplanes,
+                      int64_t mat_stride, int nmat, int nrb, int nkb,
+                      int2* out, hipStream_t s);
+// k blocks the product of a CRT-form GEMM with configuration cfg multiplies
+// for the ranges kr (host copy, nz matrices of nrb row blocks), as a
+// fraction of the dense count; 1 without ranges.
+double crt_kblock_fraction(const I8GemmArgs& g, const int2* kr_host);
 // Work-items of the per-modulus GEMM launch.
 int64_t gemm_crt_grid_items(const I8GemmArgs& g);
 

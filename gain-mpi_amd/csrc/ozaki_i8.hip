@@ -851,10 +851,44 @@ __device__ inline void store_residue_tile(const i16v& acc, double m,
   *(uint2*)(col + 16 + 8 * hi) = s1;
 }
 
+// k blocks [*k0, *k0 + *nk) that the workgroup tile with A row blocks
+// [arb0, arb0 + nrb) of matrix za multiplies: the union of the row blocks'
+// nonzero ranges, widened to multiples of `align` (all nkb blocks without
+// ranges; none for a zero tile).  The values are uniform over the
+// workgroup.
+__device__ inline void crt_tile_krange(const I8GemmArgs& g, int za, int arb0,
+                                       int nrb, int nkb, int align, int* k0,
+                                       int* nk) {
+  if (!g.a_kr) {
+    *k0 = 0;
+    *nk = nkb;
+    return;
+  }
+  const int2* kr = g.a_kr + za * g.akr_z + arb0;
+  int lo = nkb, hi = 0;
+  for (int q = 0; q < nrb; ++q) {
+    const int2 r = kr[q];
+    if (r.x < r.y) {
+      lo = min(lo, r.x);
+      hi = max(hi, r.y);
+    }
+  }
+  if (lo >= hi) {
+    *k0 = 0;
+    *nk = 0;
+    return;
+  }
+  lo = lo / align * align;
+  hi = min(nkb, (hi + align - 1) / align * align);
+  *k0 = __builtin_amdgcn_readfirstlane(lo);
+  *nk = __builtin_amdgcn_readfirstlane(hi - lo);
+}
+
 // Per-modulus GEMM: workgroup z in [0, nz * nm) takes modulus z % nm of
 // operand matrix z / nm; the int32 products of the residue planes are
 // reduced modulo m and stored as int8 residues R[z][row block][n][32]
-// (rows relative to the first row block; store_residue_tile).
+// (rows relative to the first row block; store_residue_tile).  With A
+// k-block ranges only the tile's nonzero k blocks are multiplied.
 template <int TM, int TN, int WM, int WN>
 __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res(
     I8GemmArgs g, CrtConsts cc, int ntm, int ntn, int rb0, int nkb,
@@ -865,12 +899,14 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res(
   int z, ksi, tm, tn;
   i8_tile(ntm, ntn, 1, &z, &ksi, &tm, &tn);
   const int za = z / g.S, mi = z - za * g.S;
+  int k0, nk;
+  crt_tile_krange(g, za, rb0 + tm * BMB, BMB, nkb, 2, &k0, &nk);
   const int arb = rb0 + tm * BMB + wm * TM;
   const int brb = tn * BNB + wn * TN;
   const int lane_off = (lane & 31) * 32 + 16 * (lane >> 5);
   const int8_t* Ab = g.A + za * g.a_z + mi * g.a_plane +
-                     (int64_t)arb * nkb * 1024 + lane_off;
-  const int8_t* Bb = g.B + mi * g.b_plane + (int64_t)brb * nkb * 1024 +
+                     ((int64_t)arb * nkb + k0) * 1024 + lane_off;
+  const int8_t* Bb = g.B + mi * g.b_plane + ((int64_t)brb * nkb + k0) * 1024 +
                      lane_off;
   i16v acc[TM][TN];
 #pragma unroll
@@ -896,15 +932,17 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res(
         acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[a], fb[b],
                                                           acc[a][b], 0, 0, 0);
   };
-  load(fa0, fb0, 0);
-  int kb = 0;
-  for (; kb + 1 < nkb; kb += 2) {
-    load(fa1, fb1, kb + 1);
-    mma(fa0, fb0);
-    if (kb + 2 < nkb) load(fa0, fb0, kb + 2);
-    mma(fa1, fb1);
+  if (nk > 0) {
+    load(fa0, fb0, 0);
+    int kb = 0;
+    for (; kb + 1 < nk; kb += 2) {
+      load(fa1, fb1, kb + 1);
+      mma(fa0, fb0);
+      if (kb + 2 < nk) load(fa0, fb0, kb + 2);
+      mma(fa1, fb1);
+    }
+    if (kb < nk) mma(fa0, fb0);
   }
-  if (kb < nkb) mma(fa0, fb0);
 
   // |acc| <= K 127^2 < 2^31: exact in double
   const double m = (double)cc.m[mi], minv = cc.minv_d[mi];
@@ -955,10 +993,14 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res_lds(
   i8_tile(ntm, ntn, 1, &z, &ksi, &tm, &tn);
   const int za = z / g.S, mi = z - za * g.S;
   const int arb0 = rb0 + tm * BMB, brb0 = tn * BNB;
+  // k blocks [k0, k0 + nk) of the tile (all nkb without A ranges); the
+  // operand pointers start at block k0, rows stay nkb blocks long
+  int k0, nk;
+  crt_tile_krange(g, za, arb0, BMB, nkb, 2, &k0, &nk);
   const int src_off = 16 * i8_sigma(lane);
-  const int8_t* Az = g.A + za * g.a_z + mi * g.a_plane;
-  const int8_t* Bz = g.B + mi * g.b_plane;
-  const int nst = nkb / KPB;  // stages
+  const int8_t* Az = g.A + za * g.a_z + mi * g.a_plane + (int64_t)k0 * 1024;
+  const int8_t* Bz = g.B + mi * g.b_plane + (int64_t)k0 * 1024;
+  const int nst = nk / KPB;  // stages
   // slot j * NBLKP + q of a stage: k block j of the stage, A blocks then B
   // blocks
   auto issue = [&](int stage, int st) {
@@ -1040,7 +1082,7 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res_lds(
     // reads first, then the requests spread over the first products, one
     // after each (requests issued between the barrier and the reads kept
     // the matrix pipe idle longer: the half-ls GEMM ran 7% slower)
-    for (; kb + NST - 1 < nkb; ++kb) {
+    for (; kb + NST - 1 < nk; ++kb) {
       wait_stages<DPW, NST - 2>(NST - 2);
       __builtin_amdgcn_s_barrier();
       i4v fa[TM], fb[TN];
@@ -1060,17 +1102,17 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res_lds(
         }
       stage = stage == NST - 1 ? 0 : stage + 1;
     }
-    for (; kb < nkb; ++kb) {
-      wait_stages<DPW, NST - 2>(min(NST - 2, nkb - 1 - kb));
+    for (; kb < nk; ++kb) {
+      wait_stages<DPW, NST - 2>(min(NST - 2, nk - 1 - kb));
       __builtin_amdgcn_s_barrier();
-      if (kb + NST - 1 < nkb)
+      if (kb + NST - 1 < nk)
         issue(stage == 0 ? NST - 1 : stage - 1, kb + NST - 1);
       i4v fa[TM], fb[TN];
       read(fa, fb, stage, 0);
       mma(fa, fb);
       stage = stage == NST - 1 ? 0 : stage + 1;
     }
-  } else {
+  } else if (nst > 0) {
     i4v fa0[KPB][TM], fb0[KPB][TN], fa1[KPB][TM], fb1[KPB][TN];
     wait_stages<DPW * KPB, NST - 2>(min(NST - 2, nst - 1));
     __builtin_amdgcn_s_barrier();
@@ -1260,6 +1302,82 @@ CrtLaunch crt_launch(const I8GemmArgs& g) {
 
 }  // namespace
 
+// One workgroup per (matrix, row block): each thread ORs 16-byte chunks of
+// the row block's k blocks over every plane; the first and last k block
+// with a nonzero chunk are kept by LDS atomics.
+__global__ __launch_bounds__(256) void k_block_kranges(
+    const int8_t* __restrict__ planes, int64_t plane, int nplanes,
+    int64_t mat_stride, int nrb, int nkb, int2* __restrict__ out) {
+  __shared__ int lo_s, hi_s;
+  if (threadIdx.x == 0) {
+    lo_s = nkb;
+    hi_s = 0;
+  }
+  __syncthreads();
+  const int rb = blockIdx.x, mat = blockIdx.y;
+  const int8_t* base =
+      planes + mat * mat_stride + (int64_t)rb * nkb * 1024;
+  int lo = nkb, hi = 0;
+  for (int idx = threadIdx.x; idx < nkb * 64; idx += 256) {
+    const int kb = idx >> 6;
+    const int8_t* p = base + (int64_t)idx * 16;
+    uint4 o = make_uint4(0, 0, 0, 0);
+    for (int q = 0; q < nplanes; ++q) {
+      const uint4 v = *(const uint4*)(p + q * plane);
+      o.x |= v.x;
+      o.y |= v.y;
+      o.z |= v.z;
+      o.w |= v.w;
+    }
+    if (o.x | o.y | o.z | o.w) {
+      lo = min(lo, kb);
+      hi = max(hi, kb + 1);
+    }
+  }
+  if (lo < hi) {
+    atomicMin(&lo_s, lo);
+    atomicMax(&hi_s, hi);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    out[(int64_t)mat * nrb + rb] =
+        lo_s < hi_s ? make_int2(lo_s, hi_s) : make_int2(0, 0);
+}
+
+void i8_block_kranges(const int8_t* planes, int64_t plane, int nplanes,
+                      int64_t mat_stride, int nmat, int nrb, int nkb,
+                      int2* out, hipStream_t s) {
+  if (nmat <= 0 || nrb <= 0) return;
+  GAIN_CHECK_ARG(nplanes >= 1 && nkb >= 1 && nmat <= 65535,
+                 "k-block ranges: %d planes, %d k blocks, %d matrices",
+                 nplanes, nkb, nmat);
+  hipLaunchKernelGGL(k_block_kranges, dim3(nrb, nmat), dim3(256), 0, s,
+                     planes, plane, nplanes, mat_stride, nrb, nkb, out);
+  check_launch("block_kranges");
+}
+
+double crt_kblock_fraction(const I8GemmArgs& g, const int2* kr) {
+  if (!g.a_kr || !kr) return 1.0;
+  const CrtLaunch L = crt_launch(g);
+  const int bmb = kCrtCfg[g.cfg].tm * kCrtCfg[g.cfg].wm;
+  int64_t done = 0, dense = 0;
+  for (int z = 0; z < g.nz; ++z)
+    for (int tm = 0; tm < L.ntm; ++tm) {
+      // as crt_tile_krange
+      int lo = L.nkb, hi = 0;
+      for (int q = 0; q < bmb; ++q) {
+        const int2 r = kr[z * g.akr_z + L.rb0 + tm * bmb + q];
+        if (r.x < r.y) {
+          lo = std::min(lo, r.x);
+          hi = std::max(hi, r.y);
+        }
+      }
+      if (lo < hi) done += std::min(L.nkb, (hi + 1) / 2 * 2) - lo / 2 * 2;
+      dense += L.nkb;
+    }
+  return dense ? (double)done / (double)dense : 1.0;
+}
+
 int64_t gemm_crt_res_bytes(const I8GemmArgs& g) {
   const CrtLaunch L = crt_launch(g);
   return (int64_t)g.nz * g.S * L.nr * L.ldr;
@@ -1291,6 +1409,14 @@ void gemm_crt_res(const I8GemmArgs& g, hipStream_t s) {
                  g.beta, (long long)g.Kp, g.S);
   const CrtLaunch L = crt_launch(g);
   GAIN_CHECK_ARG(L.grid < (1ll << 31), "CRT gemm: grid too large");
+  // every tile's A row blocks must have a range entry
+  GAIN_CHECK_ARG(!g.a_kr ||
+                     (g.akr_z >= g.akr_n &&
+                      L.rb0 + L.ntm * kCrtCfg[g.cfg].tm * kCrtCfg[g.cfg].wm <=
+                          g.akr_n),
+                 "CRT gemm: k-block ranges cover %d row blocks, tiles reach "
+                 "%d", g.akr_n,
+                 L.rb0 + L.ntm * kCrtCfg[g.cfg].tm * kCrtCfg[g.cfg].wm);
   const CrtConsts& cc = crt_consts(g.S);
   const dim3 grid((unsigned)L.grid),
       block(64 * kCrtCfg[g.cfg].wm * kCrtCfg[g.cfg].wn);

@@ -349,6 +349,7 @@ void build(gain_sweep* s, const gain_task* tasks, int ntasks) {
     // int8 copy of mu (split-INT8 or CRT form)
     int8_t* m8 = nullptr;
     int* me = nullptr;
+    int2* mkr = nullptr;  // CRT form: nonzero k blocks of mu's row blocks
     int beta = 0;
     const int64_t Kp = i8_kpad(nc), Rm = i8_rows_pad(nc);
     if (s->crt > 0 || s->slices > 0) {
@@ -367,6 +368,11 @@ void build(gain_sweep* s, const gain_task* tasks, int ntasks) {
           slice_rows_i8(S, muT + (size_t)al * nc * nc, nc, nc, nc,
                         m8 + al * Rm * Kp, 3 * Rm * Kp, me + al * Rm,
                         c->stream);
+      }
+      if (s->crt > 0) {
+        mkr = s->arena.alloc<int2>((size_t)3 * (Rm / 32));
+        i8_block_kranges(m8, 3 * Rm * Kp, s->crt, Rm * Kp, 3, (int)(Rm / 32),
+                         (int)(Kp / 32), mkr, c->stream);
       }
       GAIN_HIP(hipStreamSynchronize(c->stream));
       s->arena.release(muT);
@@ -418,6 +424,11 @@ void build(gain_sweep* s, const gain_task* tasks, int ntasks) {
       g.c_z = s->ncol_all * nc;
       g.c_ks = 3 * s->ncol_all * nc;
       g.nz = 3;
+      if (s->crt > 0 && c->crt_kskip) {
+        g.a_kr = mkr;
+        g.akr_z = Rm / 32;
+        g.akr_n = (int)(Rm / 32);
+      }
       if (s->crt > 0)
         for (int cfg = 0; cfg < kCrtConfigs; ++cfg) {
           g.cfg = cfg;

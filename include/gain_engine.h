@@ -286,6 +286,9 @@ int gain_get_timing(gain_ctx* ctx, const char* name, double* total_ms,
  * "i8_cfg" / "i8_split" (split-INT8 GEMMs: force a tile configuration
  * 0..11, -1 = automatic; projection split count, 0 = automatic),
  * "crt_cfg" (CRT-form int8 GEMMs: force a tile configuration 0..13),
+ * "crt_kskip" (1, the default: the CRT-form dipole products multiply only
+ * the k blocks of the dipole's residue planes that hold a nonzero entry,
+ * with results identical to the dense products; 0: every k block),
  * "gemv_rows" (2/4/8), "gemv_ksplit", "mu_ksplit" (0 = automatic),
  * "mfma" (0: vector-ALU GEMMs, for cross-checks), "noalloc_cap" (bytes of
  * the allocation-free entry point's array a call may use, 0 = all),
@@ -295,7 +298,9 @@ int gain_get_timing(gain_ctx* ctx, const char* name, double* total_ms,
 int gain_set_tuning(gain_ctx* ctx, const char* key, int value);
 /* Current value of a settable key, or what the most recent launch used:
  * "proj_cfg", "proj_split", "proj_grid" (FP64 MFMA projection GEMM),
- * "half_ls_cfg", "half_ls_split", "half_ls_grid" (mu GEMM), "gemv_grid"
+ * "half_ls_cfg", "half_ls_split", "half_ls_grid" (mu GEMM),
+ * "half_ls_kblocks_ppm" (CRT form: k blocks the mu GEMM multiplies, parts
+ * per million of the dense count), "gemv_grid"
  * (plain GEMV); *_grid = work-items of that launch; "sweep_phases",
  * "sweep_gemv_packs", "sweep_gemv_bytes" (the latest sweep created);
  * "autotune_us" = host

@@ -478,6 +478,62 @@ def test_every_crt_tile_configuration_long_k(cfg):
     assert ns > 0 and es <= 1e-7, es
 
 
+_BANDED = {}
+
+
+def _banded_case():
+    """The long-K problem with a banded dipole: entries with |v_I - v_F| > 40
+    set to zero, and v_F in [320, 352) (one 32-row block of the dipole's
+    residue planes) zero for every component, so the dipole's residue planes
+    have zero k blocks at both ends of most row blocks and one zero row
+    block."""
+    if not _BANDED:
+        prob = synth.make_problem([4, 5], 400, "c2v4")
+        prob.params.erange_lower = (0.0, 1e9)
+        prob.params.freq_window = (1500.0, 1e9)
+        a = np.arange(prob.nc)
+        band = np.abs(a[:, None] - a[None, :]) <= 40
+        prob.mu = np.ascontiguousarray(prob.mu * band[None])
+        prob.mu[:, :, 320:352] = 0.0
+        n = len(prob.state_table())
+        initials = [1 + k * (n // 6) for k in range(6)]
+        tr, _ = orc.run(prob, initials)
+        _BANDED.update(prob=prob, initials=initials, ref=oracle_map(tr))
+    return _BANDED["prob"], _BANDED["initials"], _BANDED["ref"]
+
+
+@pytest.mark.parametrize("cfg", list(range(14)))
+def test_crt_zero_kblock_skip_is_exact(cfg):
+    """The CRT-form dipole products multiply only the k blocks of each
+    tile's row blocks that hold a nonzero residue; on a banded dipole with
+    a zero row block every tile configuration gives S, A, I and flags
+    identical bit for bit to the dense products (skipping off), fewer k
+    blocks are multiplied, and S is within 1e-7 of the FP64 oracle on
+    strong lines."""
+    prob, initials, ref = _banded_case()
+    res = {}
+    for skip in (1, 0):
+        with Engine.from_problem(prob) as eng:
+            eng.set_tuning("crt_cfg", cfg)
+            eng.set_tuning("crt_kskip", skip)
+            assert eng.get_tuning("crt_kskip") == skip
+            res[skip] = {"lines": engine_lines(eng, prob, initials,
+                                               "batched", 8),
+                         "ppm": eng.get_tuning("half_ls_kblocks_ppm")}
+    on, off = res[1]["lines"], res[0]["lines"]
+    assert set(on) == set(off) == set(ref)
+    for k in on:
+        for f in ("S", "A", "I", "emitted"):
+            assert on[k][f] == off[k][f], (k, f, on[k][f], off[k][f])
+    assert res[0]["ppm"] == 1_000_000
+    assert res[1]["ppm"] < 900_000, res[1]["ppm"]
+    es, _, ns, _ = _strong_weak_err(on, ref)
+    print(f"crt12 configuration {cfg}, banded dipole: k blocks "
+          f"{res[1]['ppm'] / 1e4:.1f}% of dense, {ns} strong lines, max rel "
+          f"err {es:.3e}")
+    assert ns > 0 and es <= 1e-7, es
+
+
 @pytest.mark.parametrize("cfg", list(range(12)))
 @pytest.mark.parametrize("split", [1, 2])
 def test_every_int8_tile_configuration_long_k(cfg, split):
