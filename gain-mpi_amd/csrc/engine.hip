@@ -325,11 +325,18 @@ int crt_cfg_of(gain_ctx* c, int kind, I8GemmArgs g) {
   snprintf(line, sizeof(line), "crt%d %s M=%d N=%d K=%lld:", g.S,
            kind == 0 ? "projection" : "half_ls", g.M, g.N, (long long)g.Kp);
   std::string rec = line;
+  // the residue products, and their reconstruction into FP64 when the
+  // caller has an output for it (the fused half-line-strength gather reads
+  // the residues itself and passes none)
+  auto run = [&] {
+    gemm_crt_res(g, c->stream);
+    if (g.C) gemm_crt_combine(g, c->stream);
+  };
   for (int cfg = 0; cfg < kCrtConfigs; ++cfg) {
     g.cfg = cfg;
-    gemm_crt_i8(g, c->stream);  // warm
+    run();  // warm
     GAIN_HIP(hipEventRecord(e0, c->stream));
-    for (int r = 0; r < 2; ++r) gemm_crt_i8(g, c->stream);
+    for (int r = 0; r < 2; ++r) run();
     GAIN_HIP(hipEventRecord(e1, c->stream));
     GAIN_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -853,7 +860,10 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
   double* X = c->w_X.get<double>((size_t)nvecX * xstride);
   const int ncolTot = nvecX * bI.ncol;
   int ks = mu_ksplit_auto(c, ncolTot);
-  double* Y = c->w_Y.get<double>((size_t)ks * 3 * ncolTot * nc);
+  // the CRT form sizes its own Y (none when the gather is fused)
+  double* Y = crt_moduli_of(precision) > 0
+                  ? nullptr
+                  : c->w_Y.get<double>((size_t)ks * 3 * ncolTot * nc);
   Timed t(c, "half_ls");
   ScatterArgs sa = make_scatter_args(c, P, cbase, sm, nvecX, X);
   int* xe_crt = nullptr;  // CRT form: row exponents from the scatter
@@ -922,7 +932,9 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
       g.akr_n = (int)(ms.Rp / 32);
     }
     ks = 1;
-    Y = c->w_Y.get<double>((size_t)3 * ncolTot * nc);
+    // K-block bases: reconstruction fused with the gather (no FP64 Y)
+    const bool fuse = c->crt_fuse && !bF.rotsym;
+    Y = fuse ? nullptr : c->w_Y.get<double>((size_t)3 * ncolTot * nc);
     g.C = Y;
     g.R = crt_residues(c, g);
     g.cfg = crt_cfg_of(c, 1, g);
@@ -933,10 +945,28 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
       Timed tg(c, "half_ls_gemm");
       gemm_crt_res(g, c->stream);
     }
-    gemm_crt_combine(g, c->stream);
     c->last["half_ls_cfg"] = g.cfg;
     c->last["half_ls_split"] = 1;
     c->last["half_ls_grid"] = gemm_crt_grid_items(g);
+    if (fuse) {
+      ensure_slots(c, bF);
+      CrtGatherArgs ga{};
+      ga.g = g;
+      ga.nc = nc;
+      ga.nvec = nvecX;
+      ga.ncolI = bI.ncol;
+      ga.jI = bI.J;
+      ga.jF = bF.J;
+      ga.jmax = c->jmax;
+      ga.threej = c->d_threej;
+      ga.slot_ptr = bF.slot_ptr;
+      ga.slot_prim = bF.slot_prim;
+      ga.H = H;
+      ga.hstride = bF.nprim;
+      launch_crt_gather_h(ga, c->stream);
+      return bF.nprim;
+    }
+    gemm_crt_combine(g, c->stream);
   } else if (i8_slices_of(precision) > 0) {
     const int S8 = i8_slices_of(precision);
     const Sliced& ms = mu_sliced_i8(c, S8);
@@ -2109,6 +2139,8 @@ extern "C" int gain_set_tuning(gain_ctx* c, const char* key, int value) {
       c->force_crt_cfg = value;
     } else if (k == "crt_kskip") {
       c->crt_kskip = value != 0;
+    } else if (k == "crt_fuse") {
+      c->crt_fuse = value != 0;
     } else if (k == "i8_split") {
       GAIN_CHECK_ARG(value >= 0 && value <= 4, "split 0..4");
       c->force_i8_split = value;
@@ -2246,6 +2278,8 @@ extern "C" int gain_get_tuning(gain_ctx* c, const char* key,
       *value = c->autotune;
     } else if (k == "crt_kskip") {
       *value = c->crt_kskip;
+    } else if (k == "crt_fuse") {
+      *value = c->crt_fuse;
     } else {
       auto it = c->last.find(k);
       if (it == c->last.end())

@@ -191,6 +191,23 @@ void gemm_crt_i8(const I8GemmArgs& g, hipStream_t s);
 void gemm_crt_res(const I8GemmArgs& g, hipStream_t s);
 void gemm_crt_combine(const I8GemmArgs& g, hipStream_t s);
 int64_t gemm_crt_res_bytes(const I8GemmArgs& g);
+// Half line strength of K-block bases straight from the residues of the
+// CRT-form product Y_alpha = mu_alpha X (g as given to gemm_crt_res, with
+// nz = 3, row0 = 0, residues in g.R): the reconstruction of
+// gemm_crt_combine fused with the gather of launch_gather_h (same
+// arithmetic in the same order, so H is identical), Y never stored.
+// Final primitives are reached through the final basis's (column, v) slot
+// CSR (slot = (2 K + tau) * nc + v).
+struct CrtGatherArgs {
+  I8GemmArgs g;
+  int nc, nvec, ncolI, jI, jF, jmax;
+  const double* threej;
+  const int* slot_ptr;
+  const int* slot_prim;
+  double* H;
+  int64_t hstride;
+};
+void launch_crt_gather_h(const CrtGatherArgs& a, hipStream_t s);
 // Nonzero k-block range of every row block of nmat tiled int8 matrices
 // (nrb row blocks of nkb k blocks each, matrices mat_stride bytes apart
 // inside a plane, nplanes planes `plane` bytes apart): out[m * nrb + rb] =

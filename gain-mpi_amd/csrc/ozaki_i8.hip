@@ -16,6 +16,7 @@
 
 #include "common.hpp"
 #include "gemm.hpp"
+#include "kernels.hpp"
 #include "ozaki.hpp"
 
 namespace gain {
@@ -1221,12 +1222,44 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res_lds(
   }
 }
 
-// Garner's mixed-radix conversion of the nm residues of every output: digit
-// i = (r_i inv_i + sum_{j<i} v_j nw_ij) mod m_i, one symmetric reduction of
-// an exact FP32 integer (|.| <= 14 * 127^2 < 2^22), then the mixed-radix
-// number in FP64, scaled by the operand exponents.  Four consecutive outputs
-// per thread (one 32-bit load per residue plane; residue rows are
-// 4-aligned).
+// Garner's mixed-radix conversion of the nm residues of four outputs (byte q
+// of word w[i] = residue of output q modulo m_i): digit i = (r_i inv_i +
+// sum_{j<i} v_j nw_ij) mod m_i, one symmetric reduction of an exact FP32
+// integer (|.| <= 14 * 127^2 < 2^22), then the mixed-radix number in FP64
+// (the exact integer product of the operands, unscaled).
+template <int NM>
+__device__ inline void crt_recon4(const uint32_t (&w)[NM], const CrtConsts& cc,
+                                  double (&X)[4]) {
+  float v[NM][4];
+#pragma unroll
+  for (int i = 0; i < NM; ++i) {
+    const float mf = (float)cc.m[i];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float r = (float)(int8_t)(w[i] >> (8 * q));
+      if (i == 0) {
+        v[0][q] = r;
+      } else {
+        float t = r * cc.inv[i];
+#pragma unroll
+        for (int j = 0; j < i; ++j) t = fmaf(v[j][q], cc.nw[i][j], t);
+        v[i][q] = sym_mod_f(t, mf, cc.minv[i]);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double x = (double)v[NM - 1][q];
+#pragma unroll
+    for (int i = NM - 2; i >= 0; --i)
+      x = fma(x, (double)cc.m[i], (double)v[i][q]);
+    X[q] = x;
+  }
+}
+
+// Reconstruction of every output of a CRT-form GEMM into FP64, scaled by
+// the operand exponents.  Four consecutive outputs per thread (one 32-bit
+// load per residue plane; residue rows are 4-aligned).
 template <int NM>
 __global__ __launch_bounds__(256) void k_crt_combine(I8GemmArgs g,
                                                      CrtConsts cc, int rb0,
@@ -1247,34 +1280,147 @@ __global__ __launch_bounds__(256) void k_crt_combine(I8GemmArgs g,
   uint32_t w[NM];
 #pragma unroll
   for (int i = 0; i < NM; ++i) w[i] = R[i * pst];
-  float v[NM][4];
-#pragma unroll
-  for (int i = 0; i < NM; ++i) {
-    const float mf = (float)cc.m[i];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const float r = (float)(int8_t)(w[i] >> (8 * q));
-      if (i == 0) {
-        v[0][q] = r;
-      } else {
-        float t = r * cc.inv[i];
-#pragma unroll
-        for (int j = 0; j < i; ++j) t = fmaf(v[j][q], cc.nw[i][j], t);
-        v[i][q] = sym_mod_f(t, mf, cc.minv[i]);
-      }
-    }
-  }
+  double X[4];
+  crt_recon4<NM>(w, cc, X);
   const int eb = g.eB[n] - 2 * (g.beta - 1);
   double* C = g.C + za * g.c_z + (int64_t)n * g.ldc;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int m = r0 + q - rlo;
     if (m < 0 || m >= g.M) continue;
-    double X = (double)v[NM - 1][q];
+    C[m] = ldexp(X[q], g.eA[za * g.ea_z + g.row0 + m] + eb);
+  }
+}
+
+// Fused reconstruction and K-block gather of the half line strength.  One
+// thread per (vector, four consecutive v_F of a 32-row block); it walks
+// K_I = 0..J_I, reconstructs Y_alpha[2 K_I + tau_I][v_F] (as k_crt_combine
+// does) for the three components and both tau_I, and adds each into the
+// finals K_F = K_I + 1, K_I, K_I - 1 that use it with the rotational
+// factors of k_gather_h.  Every final therefore receives its terms in the
+// order k_gather_h adds them (K_I ascending, then tau_I), so H is bit for
+// bit that of the two-kernel path, without the FP64 Y round trip.  A final
+// column is written when its last term (K_I = K_F + 1) has been added:
+// three K_F slots roll along the walk.
+template <int NM>
+__global__ __launch_bounds__(256) void k_crt_gather_h(CrtGatherArgs a,
+                                                      CrtConsts cc,
+                                                      int64_t ldr, int nr) {
+  const I8GemmArgs& g = a.g;
+  const int t = threadIdx.x;
+  const int vec = blockIdx.y * 32 + (t >> 3);
+  const int v0 = blockIdx.x * 32 + 4 * (t & 7);
+  if (vec >= a.nvec || v0 >= a.nc) return;
+  const int64_t pst = (int64_t)nr * ldr / 4;
+  const int64_t zst = (int64_t)NM * nr * ldr;  // between components
+  const int8_t* Rrow =
+      g.R + ((int64_t)blockIdx.x * nr) * 32 + 4 * (t & 7);
+  int ea[3][4];
 #pragma unroll
-    for (int i = NM - 2; i >= 0; --i)
-      X = fma(X, (double)cc.m[i], (double)v[i][q]);
-    C[m] = ldexp(X, g.eA[za * g.ea_z + g.row0 + m] + eb);
+  for (int al = 0; al < 3; ++al)
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      ea[al][q] = g.eA[al * g.ea_z + min(v0 + q, a.nc - 1)];
+  const int n1 = a.jmax + 1;
+  const int64_t colv = (int64_t)vec * a.ncolI;
+  double hp[2][4], hc[2][4], hn[2][4];  // finals K_I - 1, K_I, K_I + 1
+#pragma unroll
+  for (int f = 0; f < 2; ++f)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) hp[f][q] = hc[f][q] = hn[f][q] = 0.0;
+  // final column (K, tau_F) of this thread's four v_F -> every primitive
+  // of its slots
+  auto store = [&](int K, const double (&h)[2][4]) {
+#pragma unroll
+    for (int tF = 0; tF < 2; ++tF) {
+      const bool neg = ((K % 3) * tF) & 1;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (v0 + q >= a.nc) continue;
+        const int sl = (2 * K + tF) * a.nc + v0 + q;
+        const double val = neg ? -h[tF][q] : h[tF][q];
+        for (int p = a.slot_ptr[sl]; p < a.slot_ptr[sl + 1]; ++p)
+          a.H[(int64_t)vec * a.hstride + a.slot_prim[p]] = val;
+      }
+    }
+  };
+  for (int KI = 0; KI <= a.jI; ++KI) {
+    // components this K_I feeds: z into K_F = K_I; x and y into K_I +- 1
+    const bool need_z = KI <= a.jF;
+    const bool need_xy = (KI >= 1 && KI - 1 <= a.jF) || KI + 1 <= a.jF;
+    // tau_I outer: a final takes one component per tau_I, so its terms
+    // arrive as in k_gather_h (tau_I = 0 before 1) whatever the component
+    // order, and only one reconstructed Y column is live at a time
+#pragma unroll
+    for (int tI = 0; tI < 2; ++tI)
+#pragma unroll
+      for (int al = 0; al < 3; ++al) {
+        if (al == 2 ? !need_z : !need_xy) continue;
+        const int64_t n = colv + 2 * KI + tI;
+        const uint32_t* R = (const uint32_t*)(Rrow + al * zst + n * 32);
+        uint32_t w[NM];
+#pragma unroll
+        for (int i = 0; i < NM; ++i) w[i] = R[i * pst];
+        double X[4], y[4];
+        crt_recon4<NM>(w, cc, X);
+        const int eb = g.eB[n] - 2 * (g.beta - 1);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) y[q] = ldexp(X[q], ea[al][q] + eb);
+        // finals K_F = K_I + 1 (q = 0 of k_gather_h), K_I (q = 1), K_I - 1
+        // (q = 2) that take this (component, tau_I)
+#pragma unroll
+        for (int qq = 0; qq < 3; ++qq) {
+          const int K = KI + 1 - qq;
+          if (K < 0 || K > a.jF) continue;
+          double (&h)[2][4] = qq == 0 ? hn : (qq == 1 ? hc : hp);
+          const int dk = K - KI;
+#pragma unroll
+          for (int tF = 0; tF < 2; ++tF) {
+            int alt, m;
+            if (dk == 0) {
+              alt = 2;
+              m = tF - tI;
+            } else if (tF != tI) {
+              alt = 0;
+              m = (tF - tI) * dk;
+            } else {
+              alt = 1;
+              m = -1;
+            }
+            if (alt != al) continue;
+            const double f3j =
+                a.threej[a.jI + KI * n1 + (a.jF - a.jI + 1) * n1 * n1 +
+                         (dk + 1) * n1 * n1 * 3];
+            const double sq = (dk != 0 && KI * K != 0) ? kSqrt2Ref : 1.0;
+            const double coef = f3j * sq * (double)m;
+            if (coef == 0.0) continue;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) h[tF][q] = fma(coef, y[q], h[tF][q]);
+          }
+        }
+      }
+    // K_F = K_I - 1 has all its terms
+    if (KI >= 1 && KI - 1 <= a.jF) store(KI - 1, hp);
+#pragma unroll
+    for (int f = 0; f < 2; ++f)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        hp[f][q] = hc[f][q];
+        hc[f][q] = hn[f][q];
+        hn[f][q] = 0.0;
+      }
+  }
+  // the last two finals reached by the walk, then any final beyond it
+  // (no terms: k_gather_h stores a signed zero)
+  for (int K = a.jI; K <= a.jF; ++K) {
+    if (K == a.jI) {
+      store(K, hp);
+    } else if (K == a.jI + 1) {
+      store(K, hc);
+    } else {
+      const double z[2][4] = {};
+      store(K, z);
+    }
   }
 }
 
@@ -1342,6 +1488,38 @@ __global__ __launch_bounds__(256) void k_block_kranges(
   if (threadIdx.x == 0)
     out[(int64_t)mat * nrb + rb] =
         lo_s < hi_s ? make_int2(lo_s, hi_s) : make_int2(0, 0);
+}
+
+void launch_crt_gather_h(const CrtGatherArgs& a, hipStream_t s) {
+  const I8GemmArgs& g = a.g;
+  if (a.nvec <= 0 || a.nc <= 0) return;
+  GAIN_CHECK_ARG(g.nz == 3 && g.row0 == 0 && g.M == a.nc &&
+                     g.N == a.nvec * a.ncolI && g.R &&
+                     a.ncolI == 2 * (a.jI + 1),
+                 "CRT gather: product shape (nz %d, row0 %d, M %d, N %d) "
+                 "does not match %d vectors of %d columns", g.nz, g.row0,
+                 g.M, g.N, a.nvec, a.ncolI);
+  GAIN_CHECK_ARG(a.H && a.slot_ptr && a.slot_prim && a.threej && g.eA &&
+                     g.eB && a.hstride > 0,
+                 "CRT gather: missing output, final slots, 3j table or "
+                 "exponents");
+  const CrtLaunch L = crt_launch(g);
+  const CrtConsts& cc = crt_consts(g.S);
+  const dim3 grid((unsigned)((a.nc + 31) / 32), (unsigned)((a.nvec + 31) / 32));
+  switch (g.S) {
+#define GAIN_CRT_GATHER(NN)                                                  \
+  case NN:                                                                   \
+    hipLaunchKernelGGL(k_crt_gather_h<NN>, grid, dim3(256), 0, s, a, cc,     \
+                       L.ldr, L.nr);                                         \
+    break;
+    GAIN_CRT_GATHER(11)
+    GAIN_CRT_GATHER(12)
+    GAIN_CRT_GATHER(14)
+#undef GAIN_CRT_GATHER
+    default:
+      fail(GAIN_ERR_ARG, "CRT gather: %d moduli not supported", g.S);
+  }
+  check_launch("crt_gather_h");
 }
 
 void i8_block_kranges(const int8_t* planes, int64_t plane, int nplanes,
@@ -1460,6 +1638,10 @@ void gemm_crt_res(const I8GemmArgs& g, hipStream_t s) {
 
 void gemm_crt_combine(const I8GemmArgs& g, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0) return;
+  GAIN_CHECK_ARG(g.C && g.R && g.eA && g.eB,
+                 "CRT combine: output %p, residues %p, exponents %p / %p",
+                 (const void*)g.C, (const void*)g.R, (const void*)g.eA,
+                 (const void*)g.eB);
   const CrtLaunch L = crt_launch(g);
   const CrtConsts& cc = crt_consts(g.S);
   // residue rows [row0 - 32 rb0, row0 - 32 rb0 + M): one workgroup per

@@ -289,6 +289,9 @@ int gain_get_timing(gain_ctx* ctx, const char* name, double* total_ms,
  * "crt_kskip" (1, the default: the CRT-form dipole products multiply only
  * the k blocks of the dipole's residue planes that hold a nonzero entry,
  * with results identical to the dense products; 0: every k block),
+ * "crt_fuse" (1, the default: the CRT-form half line strength of K-block
+ * bases is reconstructed and gathered by one kernel, identical results
+ * without the FP64 mu X intermediate; 0: the two kernels),
  * "gemv_rows" (2/4/8), "gemv_ksplit", "mu_ksplit" (0 = automatic),
  * "mfma" (0: vector-ALU GEMMs, for cross-checks), "noalloc_cap" (bytes of
  * the allocation-free entry point's array a call may use, 0 = all),

@@ -534,6 +534,85 @@ def test_crt_zero_kblock_skip_is_exact(cfg):
     assert ns > 0 and es <= 1e-7, es
 
 
+def _fused_vs_two_kernels(prob, initials, precision):
+    res = {}
+    for fuse in (1, 0):
+        with Engine.from_problem(prob) as eng:
+            eng.set_tuning("crt_fuse", fuse)
+            assert eng.get_tuning("crt_fuse") == fuse
+            res[fuse] = engine_lines(eng, prob, initials, "batched",
+                                     precision)
+    on, off = res[1], res[0]
+    assert len(on) > 0 and set(on) == set(off)
+    for k in on:
+        for f in ("S", "A", "I", "emitted"):
+            assert on[k][f] == off[k][f], (k, f, on[k][f], off[k][f])
+        assert np.array_equal(on[k]["ls"], off[k]["ls"]), k
+    return on
+
+
+@pytest.mark.parametrize("scheme,jvals,nc", [("c2v4", [2, 3], 16),
+                                             ("c1", [1, 2], 10),
+                                             ("pair2", [3, 4], 40),
+                                             ("c3v3", [1, 2], 12),
+                                             ("c2v4", [5, 6], 72)])
+@pytest.mark.parametrize("precision", [7, 8, 9])
+def test_crt_fused_gather_is_exact(scheme, jvals, nc, precision):
+    """The CRT-form half line strength of K-block bases reconstructed and
+    gathered by one kernel equals the two-kernel path (reconstruction to
+    FP64, then the gather) bit for bit: S, A, I, every degenerate
+    component of ls and the flags, over dJ = -1, 0, +1, degenerate irreps
+    and rows of the last 32-row block beyond Nc, for 11, 12 and 14
+    moduli."""
+    prob = synth.make_problem(jvals, nc, scheme)
+    prob.params.erange_lower = (0.0, 1e9)
+    rows = prob.state_table()
+    initials = list(range(0, len(rows), max(1, len(rows) // 40)))
+    _fused_vs_two_kernels(prob, initials, precision)
+
+
+def test_crt_fused_gather_autotuned():
+    """The autotuner times every CRT tile configuration of the half line
+    strength on a call whose gather is fused (the product has no FP64
+    output, so only the residue products are timed): lines within 1e-7 of
+    the oracle on strong lines, the half-ls shape in the tuning report."""
+    prob = synth.make_problem([4, 5], 20, "c2v4")
+    prob.params.erange_lower = (0.0, 1e9)
+    rows = prob.state_table()
+    initials = list(range(1, len(rows), 2))
+    tr, _ = orc.run(prob, initials)
+    ref = oracle_map(tr)
+    with Engine.from_problem(prob) as eng:
+        eng.set_tuning("autotune", 1)
+        got = engine_lines(eng, prob, initials, "batched", 8)
+        report = eng.tuning_report()
+    assert "crt12 half_ls" in report and "crt12 projection" in report
+    assert set(got) == set(ref)
+    es, _, ns, _ = _strong_weak_err(got, ref)
+    assert ns > 0 and es <= 1e-7, es
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_crt_fused_gather_irregular_basis(seed):
+    """Fused CRT gather on random K-block sizes, tau per primitive and
+    repeated vib indices (final slots holding several primitives, empty
+    slots): identical to the two-kernel path and within 1e-7 of the
+    oracle on strong lines."""
+    rng = np.random.default_rng(seed)
+    nc = 7
+    bases = [irregular_basis(2, nc, rng, 12), irregular_basis(3, nc, rng, 12)]
+    prob = problem_with_bases(bases, nc, seed)
+    prob.params.erange_lower = (0.0, 1e9)
+    rows = prob.state_table()
+    initials = list(range(0, len(rows), 3))
+    tr, _ = orc.run(prob, initials)
+    ref = oracle_map(tr)
+    got = _fused_vs_two_kernels(prob, initials, 8)
+    assert set(got) == set(ref)
+    es, _, ns, _ = _strong_weak_err(got, ref)
+    assert ns > 0 and es <= 1e-7, es
+
+
 @pytest.mark.parametrize("cfg", list(range(12)))
 @pytest.mark.parametrize("split", [1, 2])
 def test_every_int8_tile_configuration_long_k(cfg, split):
