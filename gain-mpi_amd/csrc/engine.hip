@@ -2280,6 +2280,9 @@ extern "C" int gain_get_tuning(gain_ctx* c, const char* key,
       *value = c->crt_kskip;
     } else if (k == "crt_fuse") {
       *value = c->crt_fuse;
+    } else if (k == "autotune_us") {
+      auto it = c->last.find(k);  // nothing tuned yet: 0
+      *value = it == c->last.end() ? 0 : it->second;
     } else {
       auto it = c->last.find(k);
       if (it == c->last.end())

@@ -151,12 +151,20 @@ struct CrtConsts {
   int m[kCrtMaxModuli];
   float minv[kCrtMaxModuli];   // 1 / m_i
   double minv_d[kCrtMaxModuli];
-  // Garner weights as symmetric residues (|.| <= 127): digit i of the
-  // mixed-radix form is v_i = (r_i inv_i + sum_{j<i} v_j nw_ij) mod m_i with
-  // inv_i = (m_0 ... m_{i-1})^{-1} mod m_i and
-  // nw_ij = -(m_0 ... m_{j-1}) inv_i mod m_i.
-  float inv[kCrtMaxModuli];
-  float nw[kCrtMaxModuli][kCrtMaxModuli];
+  // Reconstruction by pairs: the residues modulo m_{2k} and m_{2k+1} are
+  // joined into one modulo P_k = m_{2k} m_{2k+1} (P_k = m_{2k} for the last
+  // of an odd count), r = r_a + m_a ((r_b - r_a) pinv_k mod m_b) with
+  // pinv_k = m_a^{-1} mod m_b, exact in FP32 (|.| < 2^16); then Garner over
+  // the np pair moduli in FP64: digit k is
+  // v_k = (r_k pgi_k + sum_{j<k} v_j pnw_kj) mod P_k with
+  // pgi_k = (P_0 ... P_{k-1})^{-1} mod P_k and
+  // pnw_kj = -(P_0 ... P_{j-1}) pgi_k mod P_k, all symmetric residues, so
+  // every product is below 2^30 and every sum below 2^33 (exact).
+  static constexpr int kMaxPairs = (kCrtMaxModuli + 1) / 2;
+  int np;
+  float pinv[kMaxPairs];
+  double P[kMaxPairs], Pinv[kMaxPairs], pgi[kMaxPairs];
+  double pnw[kMaxPairs][kMaxPairs];
 };
 const CrtConsts& crt_consts(int nm);
 // Largest beta with 2 K 4^{beta-1} < prod_{i<nm} m_i.

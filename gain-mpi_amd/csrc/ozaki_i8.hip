@@ -605,15 +605,34 @@ CrtConsts make_crt(int nm) {
     c.m[i] = (int)m;
     c.minv[i] = 1.0f / (float)m;
     c.minv_d[i] = 1.0 / (double)m;
-    int64_t pw[kCrtMaxModuli];  // m_0 ... m_{j-1} mod m_i
-    int64_t prod = 1;
-    for (int j = 0; j < i; ++j) {
-      pw[j] = prod;
-      prod = prod * kCrtModuli[j] % m;
+  }
+  // pairs of moduli, then the Garner weights over the pair moduli
+  c.np = (nm + 1) / 2;
+  int64_t P[CrtConsts::kMaxPairs];
+  for (int k = 0; k < c.np; ++k) {
+    const int64_t a = kCrtModuli[2 * k];
+    if (2 * k + 1 < nm) {
+      const int64_t b = kCrtModuli[2 * k + 1];
+      c.pinv[k] = (float)sym_res(inv_mod(a, b), b);
+      P[k] = a * b;
+    } else {
+      c.pinv[k] = 0.0f;
+      P[k] = a;
     }
-    const int64_t inv = i == 0 ? 1 : inv_mod(prod, m);
-    c.inv[i] = (float)sym_res(inv, m);
-    for (int j = 0; j < i; ++j) c.nw[i][j] = (float)sym_res(-pw[j] * inv, m);
+    c.P[k] = (double)P[k];
+    c.Pinv[k] = 1.0 / (double)P[k];
+  }
+  for (int k = 0; k < c.np; ++k) {
+    int64_t pw[CrtConsts::kMaxPairs];  // P_0 ... P_{j-1} mod P_k
+    int64_t prod = 1;
+    for (int j = 0; j < k; ++j) {
+      pw[j] = prod;
+      prod = prod * P[j] % P[k];
+    }
+    const int64_t inv = k == 0 ? 1 : inv_mod(prod, P[k]);
+    c.pgi[k] = (double)sym_res(inv, P[k]);
+    for (int j = 0; j < k; ++j)
+      c.pnw[k][j] = (double)sym_res(-pw[j] * inv % P[k], P[k]);
   }
   return c;
 }
@@ -1222,37 +1241,45 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res_lds(
   }
 }
 
-// Garner's mixed-radix conversion of the nm residues of four outputs (byte q
-// of word w[i] = residue of output q modulo m_i): digit i = (r_i inv_i +
-// sum_{j<i} v_j nw_ij) mod m_i, one symmetric reduction of an exact FP32
-// integer (|.| <= 14 * 127^2 < 2^22), then the mixed-radix number in FP64
-// (the exact integer product of the operands, unscaled).
+// Mixed-radix conversion of the nm residues of four outputs (byte q of
+// word w[i] = symmetric residue of output q modulo m_i), by pairs of moduli
+// (CrtConsts): each pair joined exactly in FP32, then Garner's digits over
+// the pair moduli in FP64 (exact integers below 2^33) and the mixed-radix
+// number by Horner in FP64 (the exact integer product of the operands,
+// unscaled, rounded once per step).
 template <int NM>
 __device__ inline void crt_recon4(const uint32_t (&w)[NM], const CrtConsts& cc,
                                   double (&X)[4]) {
-  float v[NM][4];
+  constexpr int NP = (NM + 1) / 2;
+  double v[NP][4];
 #pragma unroll
-  for (int i = 0; i < NM; ++i) {
-    const float mf = (float)cc.m[i];
+  for (int k = 0; k < NP; ++k) {
+    const int ia = 2 * k, ib = 2 * k + 1;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const float r = (float)(int8_t)(w[i] >> (8 * q));
-      if (i == 0) {
-        v[0][q] = r;
+      const float ra = (float)(int8_t)(w[ia] >> (8 * q));
+      float r = ra;
+      if (ib < NM) {
+        const float rb = (float)(int8_t)(w[ib] >> (8 * q));
+        const float d = sym_mod_f((rb - ra) * cc.pinv[k], (float)cc.m[ib],
+                                  cc.minv[ib]);
+        r = fmaf((float)cc.m[ia], d, ra);
+      }
+      if (k == 0) {
+        v[0][q] = (double)r;
       } else {
-        float t = r * cc.inv[i];
+        double t = (double)r * cc.pgi[k];
 #pragma unroll
-        for (int j = 0; j < i; ++j) t = fmaf(v[j][q], cc.nw[i][j], t);
-        v[i][q] = sym_mod_f(t, mf, cc.minv[i]);
+        for (int j = 0; j < k; ++j) t = fma(v[j][q], cc.pnw[k][j], t);
+        v[k][q] = sym_mod_d(t, cc.P[k], cc.Pinv[k]);
       }
     }
   }
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    double x = (double)v[NM - 1][q];
+    double x = v[NP - 1][q];
 #pragma unroll
-    for (int i = NM - 2; i >= 0; --i)
-      x = fma(x, (double)cc.m[i], (double)v[i][q]);
+    for (int k = NP - 2; k >= 0; --k) x = fma(x, cc.P[k], v[k][q]);
     X[q] = x;
   }
 }
@@ -1293,134 +1320,95 @@ __global__ __launch_bounds__(256) void k_crt_combine(I8GemmArgs g,
 }
 
 // Fused reconstruction and K-block gather of the half line strength.  One
-// thread per (vector, four consecutive v_F of a 32-row block); it walks
-// K_I = 0..J_I, reconstructs Y_alpha[2 K_I + tau_I][v_F] (as k_crt_combine
-// does) for the three components and both tau_I, and adds each into the
-// finals K_F = K_I + 1, K_I, K_I - 1 that use it with the rotational
-// factors of k_gather_h.  Every final therefore receives its terms in the
-// order k_gather_h adds them (K_I ascending, then tau_I), so H is bit for
-// bit that of the two-kernel path, without the FP64 Y round trip.  A final
-// column is written when its last term (K_I = K_F + 1) has been added:
-// three K_F slots roll along the walk.
+// workgroup per (32-row block of v_F, vector): phase 1 reconstructs the
+// block's Y_alpha[col][32 rows] for the three components and every X
+// column into LDS exactly as k_crt_combine does (a wave reads 256
+// contiguous bytes of each residue plane); phase 2 forms every final
+// (K_F, tau_F, v_F) of the block from LDS with the rotational factors and
+// the term order of k_gather_h, so H is bit for bit that of the two-kernel
+// path, without the FP64 Y round trip through HBM.
 template <int NM>
 __global__ __launch_bounds__(256) void k_crt_gather_h(CrtGatherArgs a,
                                                       CrtConsts cc,
                                                       int64_t ldr, int nr) {
+  extern __shared__ double ys[];  // [3][ncolI][32]
   const I8GemmArgs& g = a.g;
   const int t = threadIdx.x;
-  const int vec = blockIdx.y * 32 + (t >> 3);
-  const int v0 = blockIdx.x * 32 + 4 * (t & 7);
-  if (vec >= a.nvec || v0 >= a.nc) return;
+  const int rb = blockIdx.x;
+  const int ncolI = a.ncolI;
   const int64_t pst = (int64_t)nr * ldr / 4;
   const int64_t zst = (int64_t)NM * nr * ldr;  // between components
-  const int8_t* Rrow =
-      g.R + ((int64_t)blockIdx.x * nr) * 32 + 4 * (t & 7);
-  int ea[3][4];
-#pragma unroll
-  for (int al = 0; al < 3; ++al)
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-      ea[al][q] = g.eA[al * g.ea_z + min(v0 + q, a.nc - 1)];
   const int n1 = a.jmax + 1;
-  const int64_t colv = (int64_t)vec * a.ncolI;
-  double hp[2][4], hc[2][4], hn[2][4];  // finals K_I - 1, K_I, K_I + 1
+  const int nitem1 = 3 * ncolI * 8;            // (component, column, quad)
+  const int nitem2 = 2 * (a.jF + 1) * 32;      // (final column, row)
+  for (int vec = blockIdx.y; vec < a.nvec; vec += gridDim.y) {
+    const int64_t colv = (int64_t)vec * ncolI;
+#pragma unroll 1
+    for (int i = t; i < nitem1; i += 256) {
+      const int al = i / (ncolI * 8);
+      const int rem = i - al * ncolI * 8;
+      const int col = rem >> 3, quad = rem & 7;
+      const int64_t n = colv + col;
+      const uint32_t* R =
+          (const uint32_t*)(g.R + al * zst + ((int64_t)rb * nr + n) * 32 +
+                            4 * quad);
+      uint32_t w[NM];
 #pragma unroll
-  for (int f = 0; f < 2; ++f)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) hp[f][q] = hc[f][q] = hn[f][q] = 0.0;
-  // final column (K, tau_F) of this thread's four v_F -> every primitive
-  // of its slots
-  auto store = [&](int K, const double (&h)[2][4]) {
-#pragma unroll
-    for (int tF = 0; tF < 2; ++tF) {
-      const bool neg = ((K % 3) * tF) & 1;
+      for (int k = 0; k < NM; ++k) w[k] = R[k * pst];
+      double X[4];
+      crt_recon4<NM>(w, cc, X);
+      const int eb = g.eB[n] - 2 * (g.beta - 1);
+      double* yo = ys + (al * ncolI + col) * 32 + 4 * quad;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        if (v0 + q >= a.nc) continue;
-        const int sl = (2 * K + tF) * a.nc + v0 + q;
-        const double val = neg ? -h[tF][q] : h[tF][q];
-        for (int p = a.slot_ptr[sl]; p < a.slot_ptr[sl + 1]; ++p)
-          a.H[(int64_t)vec * a.hstride + a.slot_prim[p]] = val;
+        const int v = min(rb * 32 + 4 * quad + q, a.nc - 1);
+        yo[q] = ldexp(X[q], g.eA[al * g.ea_z + v] + eb);
       }
     }
-  };
-  for (int KI = 0; KI <= a.jI; ++KI) {
-    // components this K_I feeds: z into K_F = K_I; x and y into K_I +- 1
-    const bool need_z = KI <= a.jF;
-    const bool need_xy = (KI >= 1 && KI - 1 <= a.jF) || KI + 1 <= a.jF;
-    // tau_I outer: a final takes one component per tau_I, so its terms
-    // arrive as in k_gather_h (tau_I = 0 before 1) whatever the component
-    // order, and only one reconstructed Y column is live at a time
+    __syncthreads();
+#pragma unroll 1
+    for (int j = t; j < nitem2; j += 256) {
+      const int colF = j >> 5, r = j & 31;
+      const int v = rb * 32 + r;
+      if (v >= a.nc) continue;
+      const int K = colF >> 1, tF = colF & 1;
+      double acc = 0.0;
 #pragma unroll
-    for (int tI = 0; tI < 2; ++tI)
+      for (int q = 0; q < 3; ++q) {
+        const int KI = K - 1 + q;
+        if (KI < 0 || KI > a.jI) continue;
+        const int dk = K - KI;
+        const double f3j = a.threej[a.jI + KI * n1 +
+                                    (a.jF - a.jI + 1) * n1 * n1 +
+                                    (dk + 1) * n1 * n1 * 3];
+        const double sq = (dk != 0 && KI * K != 0) ? kSqrt2Ref : 1.0;
 #pragma unroll
-      for (int al = 0; al < 3; ++al) {
-        if (al == 2 ? !need_z : !need_xy) continue;
-        const int64_t n = colv + 2 * KI + tI;
-        const uint32_t* R = (const uint32_t*)(Rrow + al * zst + n * 32);
-        uint32_t w[NM];
-#pragma unroll
-        for (int i = 0; i < NM; ++i) w[i] = R[i * pst];
-        double X[4], y[4];
-        crt_recon4<NM>(w, cc, X);
-        const int eb = g.eB[n] - 2 * (g.beta - 1);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) y[q] = ldexp(X[q], ea[al][q] + eb);
-        // finals K_F = K_I + 1 (q = 0 of k_gather_h), K_I (q = 1), K_I - 1
-        // (q = 2) that take this (component, tau_I)
-#pragma unroll
-        for (int qq = 0; qq < 3; ++qq) {
-          const int K = KI + 1 - qq;
-          if (K < 0 || K > a.jF) continue;
-          double (&h)[2][4] = qq == 0 ? hn : (qq == 1 ? hc : hp);
-          const int dk = K - KI;
-#pragma unroll
-          for (int tF = 0; tF < 2; ++tF) {
-            int alt, m;
-            if (dk == 0) {
-              alt = 2;
-              m = tF - tI;
-            } else if (tF != tI) {
-              alt = 0;
-              m = (tF - tI) * dk;
-            } else {
-              alt = 1;
-              m = -1;
-            }
-            if (alt != al) continue;
-            const double f3j =
-                a.threej[a.jI + KI * n1 + (a.jF - a.jI + 1) * n1 * n1 +
-                         (dk + 1) * n1 * n1 * 3];
-            const double sq = (dk != 0 && KI * K != 0) ? kSqrt2Ref : 1.0;
-            const double coef = f3j * sq * (double)m;
-            if (coef == 0.0) continue;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) h[tF][q] = fma(coef, y[q], h[tF][q]);
+        for (int tI = 0; tI < 2; ++tI) {
+          int al, m;
+          if (dk == 0) {
+            al = 2;
+            m = tF - tI;
+          } else if (tF != tI) {
+            al = 0;
+            m = (tF - tI) * dk;
+          } else {
+            al = 1;
+            m = -1;
           }
+          const double coef = f3j * sq * (double)m;
+          if (coef == 0.0) continue;
+          double y = 0.0;  // as k_gather_h sums its (single) split
+          y += ys[(al * ncolI + 2 * KI + tI) * 32 + r];
+          acc = fma(coef, y, acc);
         }
       }
-    // K_F = K_I - 1 has all its terms
-    if (KI >= 1 && KI - 1 <= a.jF) store(KI - 1, hp);
-#pragma unroll
-    for (int f = 0; f < 2; ++f)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        hp[f][q] = hc[f][q];
-        hc[f][q] = hn[f][q];
-        hn[f][q] = 0.0;
-      }
-  }
-  // the last two finals reached by the walk, then any final beyond it
-  // (no terms: k_gather_h stores a signed zero)
-  for (int K = a.jI; K <= a.jF; ++K) {
-    if (K == a.jI) {
-      store(K, hp);
-    } else if (K == a.jI + 1) {
-      store(K, hc);
-    } else {
-      const double z[2][4] = {};
-      store(K, z);
+      const bool neg = ((K % 3) * tF) & 1;
+      const double val = neg ? -acc : acc;
+      const int sl = colF * a.nc + v;
+      for (int p = a.slot_ptr[sl]; p < a.slot_ptr[sl + 1]; ++p)
+        a.H[(int64_t)vec * a.hstride + a.slot_prim[p]] = val;
     }
+    __syncthreads();  // before the next vector overwrites ys
   }
 }
 
@@ -1505,11 +1493,16 @@ void launch_crt_gather_h(const CrtGatherArgs& a, hipStream_t s) {
                  "exponents");
   const CrtLaunch L = crt_launch(g);
   const CrtConsts& cc = crt_consts(g.S);
-  const dim3 grid((unsigned)((a.nc + 31) / 32), (unsigned)((a.nvec + 31) / 32));
+  // LDS of one (row block, vector): Y of 3 components x every X column
+  const size_t lds = (size_t)3 * a.ncolI * 32 * sizeof(double);
+  GAIN_CHECK_ARG(lds <= 64 * 1024, "CRT gather: %d X columns exceed the LDS "
+                 "tile", a.ncolI);
+  const dim3 grid((unsigned)((a.nc + 31) / 32),
+                  (unsigned)std::min(a.nvec, 65535));
   switch (g.S) {
 #define GAIN_CRT_GATHER(NN)                                                  \
   case NN:                                                                   \
-    hipLaunchKernelGGL(k_crt_gather_h<NN>, grid, dim3(256), 0, s, a, cc,     \
+    hipLaunchKernelGGL(k_crt_gather_h<NN>, grid, dim3(256), lds, s, a, cc,   \
                        L.ldr, L.nr);                                         \
     break;
     GAIN_CRT_GATHER(11)
