@@ -73,6 +73,119 @@ def measured_traffic(label, grid):
     return None
 
 
+# Tile configurations of the int8 GEMMs, in the engine's configuration order
+# (ozaki_i8.hip kCrtCfg / kI8Cfg and their names): (tm, tn, wm, wn, lds).
+CRT_CFGS = [((4, 4, 2, 2, 0), "256x256/w128x128"),
+            ((2, 4, 2, 2, 0), "128x256/w64x128"),
+            ((4, 2, 2, 2, 0), "256x128/w128x64"),
+            ((2, 2, 2, 2, 0), "128x128/w64x64"),
+            ((2, 2, 2, 4, 0), "128x256/8w/w64x64"),
+            ((4, 4, 2, 2, 1), "256x256/w128x128/lds"),
+            ((2, 4, 2, 2, 1), "128x256/w64x128/lds"),
+            ((2, 2, 2, 4, 1), "128x256/8w/lds"),
+            ((4, 4, 2, 2, 4), "256x256/lds6"),
+            ((4, 4, 2, 2, 2), "256x256/lds4/pf"),
+            ((4, 4, 2, 2, 3), "256x256/lds6/pf"),
+            ((2, 4, 2, 2, 3), "128x256/lds6/pf"),
+            ((4, 4, 2, 2, 5), "256x256/lds4x2/pf"),
+            ((4, 2, 2, 4, 5), "256x256/8w/lds4x2/pf")]
+# LDS pipeline (stages, fragment prefetch, k blocks per stage) -> lds code
+CRT_PIPES = {(4, "false", 1): 1, (4, "true", 1): 2, (6, "true", 1): 3,
+             (6, "false", 1): 4, (4, "true", 2): 5}
+I8_CFGS = [((1, 2, 2, 2, 0), "64x128/w32x64"), ((2, 1, 2, 2, 0), "128x64/w64x32"),
+           ((1, 2, 4, 1, 0), "128x64/w32x64"), ((1, 2, 1, 4, 0), "32x256/w32x64"),
+           ((2, 1, 4, 1, 0), "256x32/w64x32"), ((2, 1, 1, 4, 0), "64x128/w64x32"),
+           ((1, 2, 2, 2, 1), "64x128/w32x64/lds"),
+           ((2, 1, 2, 2, 1), "128x64/w64x32/lds"),
+           ((1, 1, 4, 2, 0), "128x64/8w"), ((1, 1, 2, 4, 0), "64x128/8w"),
+           ((1, 1, 4, 2, 1), "128x64/8w/lds"), ((1, 1, 2, 4, 1), "64x128/8w/lds")]
+
+
+def kernel_label(name, crt_tag="crt12"):
+    """Label under which bench.py and the autotuner report a launch of the
+    kernel whose demangled name is `name` (None for other kernels)."""
+    import re
+    m = re.search(r"k_gemm_i8_res_lds<(\d+), (\d+), (\d+), (\d+), (\d+), "
+                  r"(true|false), (\d+)>", name)
+    if m:
+        t = tuple(int(x) for x in m.groups()[:4])
+        lds = CRT_PIPES.get((int(m.group(5)), m.group(6), int(m.group(7))))
+        for key, lab in CRT_CFGS:
+            if key == t + (lds,):
+                return f"{crt_tag} {lab}"
+        return None
+    m = re.search(r"k_gemm_i8_res<(\d+), (\d+), (\d+), (\d+)>", name)
+    if m:
+        t = tuple(int(x) for x in m.groups()) + (0,)
+        return next((f"{crt_tag} {lab}" for key, lab in CRT_CFGS
+                     if key == t), None)
+    m = re.search(r"k_gemm_i8(_lds)?<(\d+), (\d+), (\d+), (\d+), (\d+)", name)
+    if m:
+        t = tuple(int(x) for x in m.groups()[2:6]) + (1 if m.group(1) else 0,)
+        return next((f"int8x{m.group(2)} {lab}" for key, lab in I8_CFGS
+                     if key == t), None)
+    m = re.search(r"k_gemm_f64<(\d+), (\d+), (\d+), (\d+), (\d+), "
+                  r"(true|false)>", name)
+    if m:
+        bm, bn, bk, wm, wn = (int(x) for x in m.groups()[:5])
+        return f"{bm}x{bn}x{bk}/{wm * wn}w"
+    m = re.search(r"k_gemv<(\d+),", name)
+    if m:
+        return f"gemv r{m.group(1)}"
+    for k in ("k_crt_gather_h", "k_crt_combine"):
+        if k in name:
+            return k[2:]
+    return None
+
+
+def build_traffic_table(fetch_csv, write_csv, out, copy_gb):
+    """HBM bytes per launch from two rocprofv3 --pmc runs of the same
+    command (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch): averaged per
+    (launch label, grid work-items) and scaled by what the counters read
+    for the device copy of `copy_gb` GB in the same runs (the largest
+    dispatch whose name holds "copy")."""
+    import csv
+    from collections import defaultdict
+
+    def read(path):
+        per, copy = defaultdict(list), 0.0
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                name, v = row["Kernel_Name"], float(row["Counter_Value"])
+                if "copy" in name.lower():
+                    copy = max(copy, v)
+                    continue
+                lab = kernel_label(name)
+                if lab is not None:
+                    per[(lab, int(row["Grid_Size"]))].append(v)
+        return per, copy
+
+    fetch, fcopy = read(fetch_csv)
+    write, wcopy = read(write_csv)
+    moved = copy_gb * 1e9
+    if not (fcopy > 0 and wcopy > 0):
+        raise SystemExit("traffic table: no copy kernel in the counter runs")
+    fs, ws = moved / (fcopy * 1024), moved / (wcopy * 1024)
+    launches = []
+    for key in sorted(set(fetch) & set(write)):
+        f = sum(fetch[key]) / len(fetch[key])
+        w = sum(write[key]) / len(write[key])
+        launches.append({"label": key[0], "grid_items": key[1],
+                         "dispatches": len(fetch[key]), "fetch_kib": f,
+                         "write_kib": w,
+                         "bytes": int((f * fs + w * ws) * 1024)})
+    with open(out, "w") as fo:
+        json.dump({"about": "HBM bytes per launch = (FETCH_SIZE x fetch_scale"
+                            " + WRITE_SIZE x write_scale) x 1024, per (launch "
+                            "label, grid work-items), averaged over dispatches;"
+                            " scales from a device copy of copy_gb GB in the "
+                            "same runs",
+                   "copy_gb": copy_gb, "fetch_scale": fs, "write_scale": ws,
+                   "launches": launches}, fo, indent=1)
+    print(f"{len(launches)} launches -> {out} (fetch x{fs:.3f}, write "
+          f"x{ws:.3f})")
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -524,7 +637,26 @@ def main():
     ap.add_argument("--copy-calib-gb", type=float, default=0.0,
                     help="also run a device-to-device copy of this many GB "
                          "(reference point for HBM byte counters)")
+    ap.add_argument("--records", choices=("qr", "kron"), default="qr",
+                    help="final and initial records: Q of the QR "
+                         "factorisation of a Gaussian (default), or "
+                         "Kronecker products of two smaller orthonormal "
+                         "factors (same shapes and bytes; for hardware-counter "
+                         "runs, whose interception of the QR's library calls "
+                         "crashes)")
+    ap.add_argument("--traffic-table", nargs=3,
+                    metavar=("FETCH_CSV", "WRITE_CSV", "OUT_JSON"),
+                    help="build the per-launch HBM byte table from the "
+                         "counter CSVs of two rocprofv3 --pmc runs (FETCH_SIZE"
+                         ", WRITE_SIZE) made with --copy-calib-gb, then exit "
+                         "(no GPU work)")
     args = ap.parse_args()
+    if args.traffic_table:
+        if args.copy_calib_gb <= 0:
+            raise SystemExit("--traffic-table needs the --copy-calib-gb of "
+                             "the counter runs")
+        build_traffic_table(*args.traffic_table, args.copy_calib_gb)
+        return 0
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
@@ -541,7 +673,8 @@ def main():
               flush=True)
         return 0
     t0 = time.time()
-    wl = workload.build(nc=args.nc, ninitial=args.ninitial)
+    wl = workload.build(nc=args.nc, ninitial=args.ninitial,
+                        records=args.records)
     prob, eng = wl.prob, wl.eng
     blkI, blkF = wl.blkI, wl.blkF
     nI_all, nF = wl.ninitial, blkF.n

@@ -871,12 +871,39 @@ __device__ inline void store_residue_tile(const i16v& acc, double m,
   *(uint2*)(col + 16 + 8 * hi) = s1;
 }
 
+// Workgroup tile of a CRT-form GEMM -> (modulus mi, operand matrix za,
+// tile row tm, tile column tn).  Without A k-block ranges: as i8_tile, z =
+// za * nm + mi outermost.  With ranges and several operand matrices (the
+// three dipole components against one X): the modulus outermost, then
+// groups of the nz matrices of one tile row walked column by column, so
+// the nz tiles of a column run side by side on one XCD with the same k
+// blocks (crt_tile_krange takes the union over the matrices) and share the
+// B blocks in L2.
+__device__ inline void crt_tile(const I8GemmArgs& g, int ntm, int ntn,
+                                int* za, int* mi, int* tm, int* tn) {
+  if (!g.a_kr || g.nz == 1) {
+    int z, ksi;
+    i8_tile(ntm, ntn, 1, &z, &ksi, tm, tn);
+    *za = z / g.S;
+    *mi = z - *za * g.S;
+    return;
+  }
+  const int p = xcd_remap(blockIdx.x, gridDim.x);
+  const int per_mod = ntm * g.nz * ntn;
+  *mi = p / per_mod;
+  int trow;
+  tile_order(p - *mi * per_mod, ntm * g.nz, ntn, g.nz, &trow, tn);
+  *tm = trow / g.nz;
+  *za = trow - *tm * g.nz;
+}
+
 // k blocks [*k0, *k0 + *nk) that the workgroup tile with A row blocks
-// [arb0, arb0 + nrb) of matrix za multiplies: the union of the row blocks'
-// nonzero ranges, widened to multiples of `align` (all nkb blocks without
-// ranges; none for a zero tile).  The values are uniform over the
+// [arb0, arb0 + nrb) multiplies: the union of the row blocks' nonzero
+// ranges over every operand matrix (the nz matrices of one tile row then
+// walk the same k blocks), widened to multiples of `align` (all nkb blocks
+// without ranges; none for a zero tile).  The values are uniform over the
 // workgroup.
-__device__ inline void crt_tile_krange(const I8GemmArgs& g, int za, int arb0,
+__device__ inline void crt_tile_krange(const I8GemmArgs& g, int arb0,
                                        int nrb, int nkb, int align, int* k0,
                                        int* nk) {
   if (!g.a_kr) {
@@ -884,13 +911,15 @@ __device__ inline void crt_tile_krange(const I8GemmArgs& g, int za, int arb0,
     *nk = nkb;
     return;
   }
-  const int2* kr = g.a_kr + za * g.akr_z + arb0;
   int lo = nkb, hi = 0;
-  for (int q = 0; q < nrb; ++q) {
-    const int2 r = kr[q];
-    if (r.x < r.y) {
-      lo = min(lo, r.x);
-      hi = max(hi, r.y);
+  for (int z = 0; z < g.nz; ++z) {
+    const int2* kr = g.a_kr + z * g.akr_z + arb0;
+    for (int q = 0; q < nrb; ++q) {
+      const int2 r = kr[q];
+      if (r.x < r.y) {
+        lo = min(lo, r.x);
+        hi = max(hi, r.y);
+      }
     }
   }
   if (lo >= hi) {
@@ -916,11 +945,11 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res(
   constexpr int BMB = TM * WM, BNB = TN * WN;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave % WM, wn = wave / WM;
-  int z, ksi, tm, tn;
-  i8_tile(ntm, ntn, 1, &z, &ksi, &tm, &tn);
-  const int za = z / g.S, mi = z - za * g.S;
+  int za, mi, tm, tn;
+  crt_tile(g, ntm, ntn, &za, &mi, &tm, &tn);
+  const int z = za * g.S + mi;
   int k0, nk;
-  crt_tile_krange(g, za, rb0 + tm * BMB, BMB, nkb, 2, &k0, &nk);
+  crt_tile_krange(g, rb0 + tm * BMB, BMB, nkb, 2, &k0, &nk);
   const int arb = rb0 + tm * BMB + wm * TM;
   const int brb = tn * BNB + wn * TN;
   const int lane_off = (lane & 31) * 32 + 16 * (lane >> 5);
@@ -1009,14 +1038,14 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res_lds(
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int wm = wave % WM, wn = wave / WM;
-  int z, ksi, tm, tn;
-  i8_tile(ntm, ntn, 1, &z, &ksi, &tm, &tn);
-  const int za = z / g.S, mi = z - za * g.S;
+  int za, mi, tm, tn;
+  crt_tile(g, ntm, ntn, &za, &mi, &tm, &tn);
+  const int z = za * g.S + mi;
   const int arb0 = rb0 + tm * BMB, brb0 = tn * BNB;
   // k blocks [k0, k0 + nk) of the tile (all nkb without A ranges); the
   // operand pointers start at block k0, rows stay nkb blocks long
   int k0, nk;
-  crt_tile_krange(g, za, arb0, BMB, nkb, 2, &k0, &nk);
+  crt_tile_krange(g, arb0, BMB, nkb, 2, &k0, &nk);
   const int src_off = 16 * i8_sigma(lane);
   const int8_t* Az = g.A + za * g.a_z + mi * g.a_plane + (int64_t)k0 * 1024;
   const int8_t* Bz = g.B + mi * g.b_plane + (int64_t)k0 * 1024;
@@ -1532,10 +1561,10 @@ double crt_kblock_fraction(const I8GemmArgs& g, const int2* kr) {
   const CrtLaunch L = crt_launch(g);
   const int bmb = kCrtCfg[g.cfg].tm * kCrtCfg[g.cfg].wm;
   int64_t done = 0, dense = 0;
-  for (int z = 0; z < g.nz; ++z)
-    for (int tm = 0; tm < L.ntm; ++tm) {
-      // as crt_tile_krange
-      int lo = L.nkb, hi = 0;
+  for (int tm = 0; tm < L.ntm; ++tm) {
+    // as crt_tile_krange: one range for the nz matrices of a tile row
+    int lo = L.nkb, hi = 0;
+    for (int z = 0; z < g.nz; ++z)
       for (int q = 0; q < bmb; ++q) {
         const int2 r = kr[z * g.akr_z + L.rb0 + tm * bmb + q];
         if (r.x < r.y) {
@@ -1543,9 +1572,10 @@ double crt_kblock_fraction(const I8GemmArgs& g, const int2* kr) {
           hi = std::max(hi, r.y);
         }
       }
-      if (lo < hi) done += std::min(L.nkb, (hi + 1) / 2 * 2) - lo / 2 * 2;
-      dense += L.nkb;
-    }
+    if (lo < hi)
+      done += g.nz * (std::min(L.nkb, (hi + 1) / 2 * 2) - lo / 2 * 2);
+    dense += g.nz * L.nkb;
+  }
   return dense ? (double)done / (double)dense : 1.0;
 }
 
