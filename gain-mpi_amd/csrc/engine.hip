@@ -107,11 +107,18 @@ void ensure_slots(gain_ctx* c, Basis& b) {
   for (int s = 0; s < nslot; ++s) cnt[s + 1] += cnt[s];
   std::vector<int> fill(cnt.begin(), cnt.end() - 1), prim(b.nprim);
   for (int r = 0; r < b.nprim; ++r) prim[fill[slot[r]]++] = r;
+  std::vector<int> one(nslot);
+  for (int s = 0; s < nslot; ++s) {
+    const int n = cnt[s + 1] - cnt[s];
+    one[s] = n == 1 ? prim[cnt[s]] : (n == 0 ? -1 : -2);
+  }
   if (b.slot_ptr || b.slot_prim) ++c->gen;
   if (b.slot_ptr) c->arena.release(b.slot_ptr);
   if (b.slot_prim) c->arena.release(b.slot_prim);
+  if (b.slot_one) c->arena.release(b.slot_one);
   b.slot_ptr = c->arena.upload(cnt.data(), cnt.size());
   b.slot_prim = c->arena.upload(prim.data(), prim.size());
+  b.slot_one = c->arena.upload(one.data(), one.size());
   b.slot_nc = c->nc;
 }
 
@@ -961,6 +968,7 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
       ga.threej = c->d_threej;
       ga.slot_ptr = bF.slot_ptr;
       ga.slot_prim = bF.slot_prim;
+      ga.slot_one = bF.slot_one;
       ga.H = H;
       ga.hstride = bF.nprim;
       launch_crt_gather_h(ga, c->stream);

@@ -141,6 +141,9 @@ struct Basis {
   // scatter CSR over (col, v) slots, built for a given nc
   int slot_nc = -1;
   int *slot_ptr = nullptr, *slot_prim = nullptr;
+  // per slot: its primitive when it holds exactly one, -1 when none, -2
+  // when several (then the CSR lists them)
+  int* slot_one = nullptr;
   // inflation
   int maxsymcoeffs = 0, matsize = 0;
   int *ijterms = nullptr, *icontr = nullptr;

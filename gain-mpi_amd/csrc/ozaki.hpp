@@ -212,6 +212,8 @@ struct CrtGatherArgs {
   const double* threej;
   const int* slot_ptr;
   const int* slot_prim;
+  const int* slot_one;  // the slot's primitive if it has exactly one, -1
+                        // if none, -2 if several (listed by the CSR)
   double* H;
   int64_t hstride;
 };

@@ -1355,21 +1355,58 @@ __global__ __launch_bounds__(256) void k_crt_combine(I8GemmArgs g,
 // contiguous bytes of each residue plane); phase 2 forms every final
 // (K_F, tau_F, v_F) of the block from LDS with the rotational factors and
 // the term order of k_gather_h, so H is bit for bit that of the two-kernel
-// path, without the FP64 Y round trip through HBM.
+// path, without the FP64 Y round trip through HBM.  The rotational factors
+// of every (final column, K_I, tau_I) are tabulated in LDS once per
+// workgroup (their loads overlap phase 1), and phase 2 takes four outputs
+// per thread at a time, their slot lookups in flight together.
+constexpr int kCrtGatherMaxColF = 64;  // final columns 2 (J_F + 1), J_F < 32
+
 template <int NM>
 __global__ __launch_bounds__(256) void k_crt_gather_h(CrtGatherArgs a,
                                                       CrtConsts cc,
                                                       int64_t ldr, int nr) {
   extern __shared__ double ys[];  // [3][ncolI][32]
+  // factor and component of term (K_I = K_F - 1 + q, tau_I) of final
+  // column colF at [colF * 6 + 2 q + tau_I] (factor 0: no term)
+  __shared__ double cf[kCrtGatherMaxColF * 6];
+  __shared__ int8_t cal[kCrtGatherMaxColF * 6];
   const I8GemmArgs& g = a.g;
   const int t = threadIdx.x;
   const int rb = blockIdx.x;
   const int ncolI = a.ncolI;
+  const int ncolF = 2 * (a.jF + 1);
   const int64_t pst = (int64_t)nr * ldr / 4;
   const int64_t zst = (int64_t)NM * nr * ldr;  // between components
   const int n1 = a.jmax + 1;
+  for (int e = t; e < ncolF * 6; e += 256) {
+    const int colF = e / 6, q = (e % 6) >> 1, tI = e & 1;
+    const int K = colF >> 1, tF = colF & 1;
+    const int KI = K - 1 + q;
+    double coef = 0.0;
+    int al = 0;
+    if (KI >= 0 && KI <= a.jI) {
+      const int dk = K - KI;
+      const double f3j = a.threej[a.jI + KI * n1 + (a.jF - a.jI + 1) * n1 * n1 +
+                                  (dk + 1) * n1 * n1 * 3];
+      const double sq = (dk != 0 && KI * K != 0) ? kSqrt2Ref : 1.0;
+      int m;
+      if (dk == 0) {
+        al = 2;
+        m = tF - tI;
+      } else if (tF != tI) {
+        al = 0;
+        m = (tF - tI) * dk;
+      } else {
+        al = 1;
+        m = -1;
+      }
+      coef = f3j * sq * (double)m;
+    }
+    cf[e] = coef;
+    cal[e] = (int8_t)al;
+  }
   const int nitem1 = 3 * ncolI * 8;            // (component, column, quad)
-  const int nitem2 = 2 * (a.jF + 1) * 32;      // (final column, row)
+  const int nitem2 = ncolF * 32;               // (final column, row)
   for (int vec = blockIdx.y; vec < a.nvec; vec += gridDim.y) {
     const int64_t colv = (int64_t)vec * ncolI;
 #pragma unroll 1
@@ -1395,47 +1432,45 @@ __global__ __launch_bounds__(256) void k_crt_gather_h(CrtGatherArgs a,
       }
     }
     __syncthreads();
+    double* Hv = a.H + (int64_t)vec * a.hstride;
 #pragma unroll 1
-    for (int j = t; j < nitem2; j += 256) {
-      const int colF = j >> 5, r = j & 31;
-      const int v = rb * 32 + r;
-      if (v >= a.nc) continue;
-      const int K = colF >> 1, tF = colF & 1;
-      double acc = 0.0;
+    for (int j0 = t; j0 < nitem2; j0 += 4 * 256) {
+      int pr[4];
+      double val[4];
 #pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const int KI = K - 1 + q;
-        if (KI < 0 || KI > a.jI) continue;
-        const int dk = K - KI;
-        const double f3j = a.threej[a.jI + KI * n1 +
-                                    (a.jF - a.jI + 1) * n1 * n1 +
-                                    (dk + 1) * n1 * n1 * 3];
-        const double sq = (dk != 0 && KI * K != 0) ? kSqrt2Ref : 1.0;
+      for (int u = 0; u < 4; ++u) {
+        const int j = j0 + u * 256;
+        const int colF = j >> 5, r = j & 31;
+        const int v = rb * 32 + r;
+        pr[u] = -1;
+        val[u] = 0.0;
+        if (j >= nitem2 || v >= a.nc) continue;
+        pr[u] = a.slot_one[colF * a.nc + v];
+        const int K = colF >> 1, tF = colF & 1;
+        double acc = 0.0;
 #pragma unroll
-        for (int tI = 0; tI < 2; ++tI) {
-          int al, m;
-          if (dk == 0) {
-            al = 2;
-            m = tF - tI;
-          } else if (tF != tI) {
-            al = 0;
-            m = (tF - tI) * dk;
-          } else {
-            al = 1;
-            m = -1;
-          }
-          const double coef = f3j * sq * (double)m;
+        for (int e = 0; e < 6; ++e) {
+          const double coef = cf[colF * 6 + e];
           if (coef == 0.0) continue;
+          const int KI = K - 1 + (e >> 1);
           double y = 0.0;  // as k_gather_h sums its (single) split
-          y += ys[(al * ncolI + 2 * KI + tI) * 32 + r];
+          y += ys[(cal[colF * 6 + e] * ncolI + 2 * KI + (e & 1)) * 32 + r];
           acc = fma(coef, y, acc);
         }
+        const bool neg = ((K % 3) * tF) & 1;
+        val[u] = neg ? -acc : acc;
       }
-      const bool neg = ((K % 3) * tF) & 1;
-      const double val = neg ? -acc : acc;
-      const int sl = colF * a.nc + v;
-      for (int p = a.slot_ptr[sl]; p < a.slot_ptr[sl + 1]; ++p)
-        a.H[(int64_t)vec * a.hstride + a.slot_prim[p]] = val;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (pr[u] >= 0) {
+          Hv[pr[u]] = val[u];
+        } else if (pr[u] == -2) {  // several primitives share the slot
+          const int j = j0 + u * 256;
+          const int sl = (j >> 5) * a.nc + rb * 32 + (j & 31);
+          for (int p = a.slot_ptr[sl]; p < a.slot_ptr[sl + 1]; ++p)
+            Hv[a.slot_prim[p]] = val[u];
+        }
+      }
     }
     __syncthreads();  // before the next vector overwrites ys
   }
@@ -1516,8 +1551,9 @@ void launch_crt_gather_h(const CrtGatherArgs& a, hipStream_t s) {
                  "CRT gather: product shape (nz %d, row0 %d, M %d, N %d) "
                  "does not match %d vectors of %d columns", g.nz, g.row0,
                  g.M, g.N, a.nvec, a.ncolI);
-  GAIN_CHECK_ARG(a.H && a.slot_ptr && a.slot_prim && a.threej && g.eA &&
-                     g.eB && a.hstride > 0,
+  GAIN_CHECK_ARG(a.H && a.slot_ptr && a.slot_prim && a.slot_one &&
+                     a.threej && g.eA && g.eB && a.hstride > 0 &&
+                     2 * (a.jF + 1) <= kCrtGatherMaxColF,
                  "CRT gather: missing output, final slots, 3j table or "
                  "exponents");
   const CrtLaunch L = crt_launch(g);
