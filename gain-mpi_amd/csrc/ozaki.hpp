@@ -160,6 +160,13 @@ struct CrtConsts {
   // pgi_k = (P_0 ... P_{k-1})^{-1} mod P_k and
   // pnw_kj = -(P_0 ... P_{j-1}) pgi_k mod P_k, all symmetric residues, so
   // every product is below 2^30 and every sum below 2^33 (exact).
+  // Explicit form (at most 12 moduli): X = sum_i r_i C_i - q M with
+  // C_i = (M / m_i) ((M / m_i)^{-1} mod m_i) and q = rint(sum_i r_i ef_i),
+  // ef_i = fl(C_i / M); C_i and M are split at 2^es into a high part (a
+  // multiple of 2^es below 2^(es + 40): every partial sum of the high parts
+  // is exact in FP64) and a low part (below 2^es, summed with rounding).
+  double ef[kCrtMaxModuli], ech[kCrtMaxModuli], ecl[kCrtMaxModuli];
+  double emh, eml;
   static constexpr int kMaxPairs = (kCrtMaxModuli + 1) / 2;
   int np;
   float pinv[kMaxPairs];

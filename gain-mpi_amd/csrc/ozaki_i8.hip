@@ -606,6 +606,29 @@ CrtConsts make_crt(int nm) {
     c.minv[i] = 1.0f / (float)m;
     c.minv_d[i] = 1.0 / (double)m;
   }
+  // explicit form: C_i = M_i (M_i^{-1} mod m_i), split at 2^es with
+  // es = bit length of M - 40
+  typedef unsigned __int128 u128;
+  u128 M = 1;
+  for (int i = 0; i < nm; ++i) M *= (u128)kCrtModuli[i];
+  int bits = 0;
+  for (u128 t = M; t; t >>= 1) ++bits;
+  const int es = std::max(0, bits - 40);
+  auto split = [&](u128 v, double* hi, double* lo) {
+    const u128 h = (v >> es) << es;
+    *hi = std::ldexp((double)(uint64_t)(v >> es), es);  // < 2^40: exact
+    *lo = (double)(v - h);                               // < 2^es, rounded
+  };
+  for (int i = 0; i < nm; ++i) {
+    const int64_t m = kCrtModuli[i];
+    const u128 Mi = M / (u128)m;
+    const int64_t inv = inv_mod((int64_t)(Mi % (u128)m), m);
+    const u128 C = Mi * (u128)inv;  // < M
+    // C / M rounded to FP64 from a 64-bit-mantissa quotient
+    c.ef[i] = (double)((long double)C / (long double)M);
+    split(C, &c.ech[i], &c.ecl[i]);
+  }
+  split(M, &c.emh, &c.eml);
   // pairs of moduli, then the Garner weights over the pair moduli
   c.np = (nm + 1) / 2;
   int64_t P[CrtConsts::kMaxPairs];
@@ -652,7 +675,11 @@ int crt_beta(int nm, int64_t K) {
   long double lp = 0.0L;
   for (int i = 0; i < nm; ++i) lp += log2l((long double)kCrtModuli[i]);
   // 2 K 4^{beta-1} < P  <=>  beta - 1 < (log2 P - 1 - log2 K) / 2
-  const long double lim = (lp - 1.0L - log2l((long double)K)) / 2.0L;
+  // (with a margin of 2^-1e-6, so that |X| / M stays below 1/2 - 3e-7 and
+  // the explicit reconstruction's quotient estimate is never ambiguous; no
+  // K that is a multiple of 64 below 2e5 comes that close to a boundary)
+  const long double lim =
+      (lp - 1.0L - log2l((long double)K) - 1e-6L) / 2.0L;
   int b1 = (int)floorl(lim);
   if ((long double)b1 >= lim) --b1;  // strict inequality
   GAIN_CHECK_ARG(b1 >= 8, "CRT: no integer width for K = %lld",
@@ -1270,15 +1297,41 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res_lds(
   }
 }
 
-// Mixed-radix conversion of the nm residues of four outputs (byte q of
-// word w[i] = symmetric residue of output q modulo m_i), by pairs of moduli
-// (CrtConsts): each pair joined exactly in FP32, then Garner's digits over
-// the pair moduli in FP64 (exact integers below 2^33) and the mixed-radix
-// number by Horner in FP64 (the exact integer product of the operands,
-// unscaled, rounded once per step).
+// The integer product of the operands from the nm residues of four outputs
+// (byte q of word w[i] = symmetric residue of output q modulo m_i), in
+// FP64.
+//   nm <= 12, explicit form (CrtConsts): q = rint(sum r_i ef_i) (its error is
+//   below 2^-39 while |X| / M < 1/2 - 3e-7, crt_beta), then the high parts
+//   sum_i r_i ech_i - q emh exactly and the low parts with rounding; X is
+//   within 2 ulp of the exact product rounded to FP64 for |X| > 2^60 and
+//   within 2^12 below (the operand rounding puts >= 2^44 into every output).
+//   nm = 14, by pairs: each pair of moduli joined exactly in FP32, Garner's
+//   digits over the pair moduli in FP64 (exact integers below 2^33), the
+//   mixed-radix number by Horner in FP64.
 template <int NM>
 __device__ inline void crt_recon4(const uint32_t (&w)[NM], const CrtConsts& cc,
                                   double (&X)[4]) {
+  if constexpr (NM <= 12) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      double r[NM];
+#pragma unroll
+      for (int i = 0; i < NM; ++i)
+        r[i] = (double)(float)(int8_t)(w[i] >> (8 * q));
+      double e = 0.0;
+#pragma unroll
+      for (int i = 0; i < NM; ++i) e = fma(r[i], cc.ef[i], e);
+      const double qt = rint(e);
+      double hi = -qt * cc.emh, lo = -qt * cc.eml;
+#pragma unroll
+      for (int i = 0; i < NM; ++i) {
+        hi = fma(r[i], cc.ech[i], hi);
+        lo = fma(r[i], cc.ecl[i], lo);
+      }
+      X[q] = hi + lo;
+    }
+    return;
+  }
   constexpr int NP = (NM + 1) / 2;
   double v[NP][4];
 #pragma unroll
