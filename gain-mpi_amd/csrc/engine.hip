@@ -2307,3 +2307,26 @@ extern "C" int gain_tuning_report(gain_ctx* c, char* buf, int len) {
     snprintf(buf, len, "%s", c->tune_log.c_str());
   });
 }
+
+extern "C" int gain_crt_moduli(int nm, int* m) {
+  return guarded([&] {
+    GAIN_CHECK_ARG(m, "null argument");
+    for (int i = 0; i < nm; ++i) m[i] = crt_modulus(nm, i);
+  });
+}
+
+extern "C" int gain_crt_beta(int nm, long long K, int* beta) {
+  return guarded([&] {
+    GAIN_CHECK_ARG(beta && K > 0 && (nm == 11 || nm == 12 || nm == 14),
+                   "bad CRT width arguments");
+    *beta = crt_beta(nm, K);
+  });
+}
+
+extern "C" int gain_crt_reconstruct(int nm, const unsigned int* words,
+                                    long long n4, double* out) {
+  return guarded([&] {
+    GAIN_CHECK_ARG(words && out && n4 >= 0, "bad reconstruction arguments");
+    crt_reconstruct_host(nm, words, n4, out);
+  });
+}

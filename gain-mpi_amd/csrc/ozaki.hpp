@@ -174,6 +174,12 @@ struct CrtConsts {
   double pnw[kMaxPairs][kMaxPairs];
 };
 const CrtConsts& crt_consts(int nm);
+// The reconstruction of the CRT-form kernels run on the host: words
+// [n4][nm] (byte q of word i = symmetric residue of output q modulo m_i),
+// out [n4][4] = the integer products in FP64 (unscaled).
+void crt_reconstruct_host(int nm, const uint32_t* words, int64_t n4,
+                          double* out);
+int crt_modulus(int nm, int i);
 // Largest beta with 2 K 4^{beta-1} < prod_{i<nm} m_i.
 int crt_beta(int nm, int64_t K);
 // rows x K FP64 -> nm tiled int8 residue planes (layout and padding as

@@ -693,12 +693,13 @@ int crt_beta(int nm, int64_t K) {
 // integer-valued x with |x| < 2^51, minv = fl(1/m).  x * minv is within
 // |x/m| 2^-52 of x/m, and x/m (m odd) is at least 1/(2m) away from every
 // half-integer, so rint(x * minv) = rint(x/m) and the result is exact.
-__device__ inline double sym_mod_d(double x, double m, double minv) {
+__host__ __device__ inline double sym_mod_d(double x, double m,
+                                            double minv) {
   return fma(-rint(x * minv), m, x);
 }
 
 // Same in FP32 for an integer-valued |x| < 2^22 (error |x/m| 2^-23 < 1/(2m)).
-__device__ inline float sym_mod_f(float x, float m, float minv) {
+__host__ __device__ inline float sym_mod_f(float x, float m, float minv) {
   return fmaf(-rintf(x * minv), m, x);
 }
 
@@ -1303,14 +1304,15 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8_res_lds(
 //   nm <= 12, explicit form (CrtConsts): q = rint(sum r_i ef_i) (its error is
 //   below 2^-39 while |X| / M < 1/2 - 3e-7, crt_beta), then the high parts
 //   sum_i r_i ech_i - q emh exactly and the low parts with rounding; X is
-//   within 2 ulp of the exact product rounded to FP64 for |X| > 2^60 and
-//   within 2^12 below (the operand rounding puts >= 2^44 into every output).
+//   within 2^12 + 2 ulp of the exact product (the operand rounding puts
+//   >= 2^44 into every output).
 //   nm = 14, by pairs: each pair of moduli joined exactly in FP32, Garner's
 //   digits over the pair moduli in FP64 (exact integers below 2^33), the
-//   mixed-radix number by Horner in FP64.
+//   mixed-radix number by Horner in FP64 (within 4 ulp).
 template <int NM>
-__device__ inline void crt_recon4(const uint32_t (&w)[NM], const CrtConsts& cc,
-                                  double (&X)[4]) {
+__host__ __device__ inline void crt_recon4(const uint32_t (&w)[NM],
+                                           const CrtConsts& cc,
+                                           double (&X)[4]) {
   if constexpr (NM <= 12) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -1746,6 +1748,36 @@ void gemm_crt_res(const I8GemmArgs& g, hipStream_t s) {
 #undef GAIN_CRT_LDS
   }
   check_launch("gemm_i8_res");
+}
+
+// The same reconstruction on the host (CPU tests of the constants and the
+// arithmetic): words [n4][nm], out [n4][4].
+template <int NM>
+static void crt_recon_host(const uint32_t* words, int64_t n4, double* out) {
+  const CrtConsts& cc = crt_consts(NM);
+  for (int64_t g = 0; g < n4; ++g) {
+    uint32_t w[NM];
+    for (int i = 0; i < NM; ++i) w[i] = words[g * NM + i];
+    double X[4];
+    crt_recon4<NM>(w, cc, X);
+    for (int q = 0; q < 4; ++q) out[g * 4 + q] = X[q];
+  }
+}
+
+void crt_reconstruct_host(int nm, const uint32_t* words, int64_t n4,
+                          double* out) {
+  switch (nm) {
+    case 11: crt_recon_host<11>(words, n4, out); break;
+    case 12: crt_recon_host<12>(words, n4, out); break;
+    case 14: crt_recon_host<14>(words, n4, out); break;
+    default: fail(GAIN_ERR_ARG, "CRT: %d moduli not supported", nm);
+  }
+}
+
+int crt_modulus(int nm, int i) {
+  GAIN_CHECK_ARG(i >= 0 && i < nm && (nm == 11 || nm == 12 || nm == 14),
+                 "CRT modulus %d of %d", i, nm);
+  return crt_consts(nm).m[i];
 }
 
 void gemm_crt_combine(const I8GemmArgs& g, hipStream_t s) {

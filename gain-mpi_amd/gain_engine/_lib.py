@@ -36,6 +36,7 @@ EXPORTS = (
     "gain_set_dipole_combination", "gain_dipole_coeff_grad",
     "gain_set_tuning", "gain_get_tuning", "gain_tuning_report",
     "gain_mfma_peak", "gain_hbm_read_peak",
+    "gain_crt_moduli", "gain_crt_beta", "gain_crt_reconstruct",
     "gain_sweep_create", "gain_format_line", "gain_write_transitions",
     "gain_set_block_levels", "gain_trove_read_dipole", "gain_trove_read_basis",
     "gain_trove_basis_free", "gain_trove_basis_sizes", "gain_trove_basis_get",
@@ -161,6 +162,10 @@ def load():
         "gain_tuning_report": (_ip, [_vp, C.c_char_p, _ip]),
         "gain_mfma_peak": (_ip, [_vp, C.c_char_p, C.POINTER(_dp)]),
         "gain_hbm_read_peak": (_ip, [_vp, C.POINTER(_dp)]),
+        "gain_crt_moduli": (_ip, [C.c_int, C.POINTER(C.c_int)]),
+        "gain_crt_beta": (_ip, [C.c_int, C.c_longlong, C.POINTER(C.c_int)]),
+        "gain_crt_reconstruct": (_ip, [C.c_int, C.POINTER(C.c_uint),
+                                       C.c_longlong, C.POINTER(_dp)]),
         "gain_sweep_create": (_ip, [_vp, _vp, _ip, _ip, C.POINTER(_vp)]),
         "gain_sweep_run": (_ip, [_vp]),
         "gain_sweep_destroy": (_ip, [_vp]),

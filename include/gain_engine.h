@@ -322,6 +322,18 @@ int gain_mfma_peak(gain_ctx* ctx, const char* kind, double* ops_per_s);
  * loads over an 8 GiB buffer allocated for the call), in bytes/s: the
  * achievable ceiling for the HBM-bound kernels. */
 int gain_hbm_read_peak(gain_ctx* ctx, double* bytes_per_s);
+/* The CRT form's moduli and reconstruction, on the host (no GPU needed).
+ * No reference counterpart: they expose the integer arithmetic of the
+ * int8 CRT kernels for checks against exact integers.
+ * gain_crt_moduli: m[0..nm) for nm = 11, 12 or 14.  gain_crt_beta: the
+ * integer width of the operands of a product of length K.
+ * gain_crt_reconstruct: words[n4][nm] hold the symmetric residues of four
+ * outputs (byte q of word i = residue of output q modulo m_i); out[n4][4]
+ * receives the integer products in FP64, computed as the GPU kernels do. */
+int gain_crt_moduli(int nm, int* m);
+int gain_crt_beta(int nm, long long K, int* beta);
+int gain_crt_reconstruct(int nm, const unsigned int* words, long long n4,
+                         double* out);
 
 /* ---- transition list (main.cpp:238-338 + Output.cpp:52-112) ----------- */
 typedef struct {
