@@ -186,6 +186,31 @@ def build_traffic_table(fetch_csv, write_csv, out, copy_gb):
           f"x{ws:.3f})")
 
 
+def trace_summary(path, pattern=None):
+    """Per (kernel, grid, workgroup) launch count and duration statistics
+    (ms) of a rocprofv3 --kernel-trace CSV, largest total first; only
+    kernels whose name matches `pattern` when given."""
+    import csv
+    import re
+    from collections import defaultdict
+    agg = defaultdict(list)
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            name = row["Kernel_Name"]
+            if pattern and not re.search(pattern, name):
+                continue
+            grid = "x".join(row[k] for k in ("Grid_Size_X", "Grid_Size_Y",
+                                            "Grid_Size_Z"))
+            wg = row["Workgroup_Size_X"]
+            ms = (int(row["End_Timestamp"]) - int(row["Start_Timestamp"])) / 1e6
+            agg[(name[:120], grid, wg)].append(ms)
+    print(f"{'total_ms':>10} {'n':>5} {'avg_ms':>9} {'min_ms':>9} "
+          f"{'max_ms':>9}  grid wg kernel")
+    for (name, grid, wg), v in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
+        print(f"{sum(v):10.3f} {len(v):5d} {sum(v) / len(v):9.4f} "
+              f"{min(v):9.4f} {max(v):9.4f}  {grid} {wg} {name}")
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -644,6 +669,11 @@ def main():
                          "factors (same shapes and bytes; for hardware-counter "
                          "runs, whose interception of the QR's library calls "
                          "crashes)")
+    ap.add_argument("--trace-summary", nargs="+",
+                    metavar=("TRACE_CSV", "NAME_REGEX"),
+                    help="print per-launch duration statistics of a "
+                         "rocprofv3 --kernel-trace CSV grouped by kernel and "
+                         "grid, then exit (no GPU work)")
     ap.add_argument("--traffic-table", nargs=3,
                     metavar=("FETCH_CSV", "WRITE_CSV", "OUT_JSON"),
                     help="build the per-launch HBM byte table from the "
@@ -651,6 +681,9 @@ def main():
                          ", WRITE_SIZE) made with --copy-calib-gb, then exit "
                          "(no GPU work)")
     args = ap.parse_args()
+    if args.trace_summary:
+        trace_summary(*args.trace_summary[:2])
+        return 0
     if args.traffic_table:
         if args.copy_calib_gb <= 0:
             raise SystemExit("--traffic-table needs the --copy-calib-gb of "
