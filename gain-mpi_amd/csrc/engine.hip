@@ -67,6 +67,15 @@ void resolve_timers(gain_ctx* c, TimerSlot& t) {
   (void)c;
 }
 
+void need_mu(const gain_ctx* c) {
+  if (!c->d_mu) fail(GAIN_ERR_STATE, "dipole not uploaded");
+  if (c->mu_missing > 0)
+    fail(GAIN_ERR_STATE,
+         "dipole pieces cover %d of its %d final vibrational columns "
+         "(gain_upload_dipole the rest)",
+         c->nc - c->mu_missing, c->nc);
+}
+
 gain_ctx* need(gain_ctx* c) {
   if (!c) fail(GAIN_ERR_ARG, "null context");
   return c;
@@ -518,7 +527,7 @@ void final_range(const gain_ctx* c, const Block& bF,
 Plan make_plan(gain_ctx* c, int jindI, int gI, const int* levels, int nI,
                int jindF, int gF) {
   if (!c->params_set) fail(GAIN_ERR_STATE, "gain_set_params not called");
-  if (!c->d_mu) fail(GAIN_ERR_STATE, "dipole not uploaded");
+  need_mu(c);
   GAIN_CHECK_ARG(gI >= 0 && gI < c->nrepres && gF >= 0 && gF < c->nrepres,
                  "irrep index out of range");
   Plan P;
@@ -1754,18 +1763,44 @@ extern "C" int gain_upload_dipole(gain_ctx* c, const double* mu, int nc,
   return guarded([&] {
     need(c);
     GAIN_CHECK_ARG(mu && nc > 0, "bad dipole");
-    if (startF != 0 || endF != nc)
-      fail(GAIN_ERR_ARG,
-           "the engine keeps the whole dipole resident: need startF=0, "
-           "endF=nc (got %d,%d of %d)", startF, endF, nc);
+    GAIN_CHECK_ARG(startF >= 0 && startF < endF && endF <= nc,
+                   "dipole piece [%d, %d) outside [0, %d)", startF, endF, nc);
     DeviceGuard g(c->device);
     ++c->gen;
     release_sliced(c, c->mu_sl);
-    if (c->d_mu) c->arena.release(c->d_mu);
-    c->d_mu = nullptr;
-    c->d_mu = c->arena.upload(mu, (size_t)3 * nc * nc);
-    c->nc = nc;
-    build_slots_all(c);
+    if (startF == 0 && endF == nc) {  // the whole matrix
+      if (c->d_mu) c->arena.release(c->d_mu);
+      c->d_mu = nullptr;
+      c->d_mu = c->arena.upload(mu, (size_t)3 * nc * nc);
+      c->nc = nc;
+      c->mu_have.assign(nc, 1);
+      c->mu_missing = 0;
+      build_slots_all(c);
+      return;
+    }
+    // a piece of final columns [startF, endF) (the reference transfers the
+    // dipole in such pieces when it does not fit, TroveDipole.cpp:47-112):
+    // copied into the resident matrix [alpha][vI][vF]
+    if (!c->d_mu || c->nc != nc) {
+      if (c->d_mu) c->arena.release(c->d_mu);
+      c->d_mu = nullptr;
+      c->d_mu = c->arena.alloc<double>((size_t)3 * nc * nc);
+      c->nc = nc;
+      c->mu_have.assign(nc, 0);
+      c->mu_missing = nc;
+      build_slots_all(c);
+    }
+    const int w = endF - startF;
+    for (int a = 0; a < 3; ++a)
+      GAIN_HIP(hipMemcpy2D(c->d_mu + (size_t)a * nc * nc + startF,
+                           sizeof(double) * nc, mu + (size_t)a * w * nc,
+                           sizeof(double) * w, sizeof(double) * w, nc,
+                           hipMemcpyHostToDevice));
+    for (int v = startF; v < endF; ++v)
+      if (!c->mu_have[v]) {
+        c->mu_have[v] = 1;
+        --c->mu_missing;
+      }
   });
 }
 
@@ -2015,7 +2050,7 @@ extern "C" int gain_half_ls(gain_ctx* c, int jindI, int gammaI, int idegI,
                             const double* cI, int jindF, double* h_out) {
   return guarded([&] {
     need(c);
-    if (!c->d_mu) fail(GAIN_ERR_STATE, "dipole not uploaded");
+    need_mu(c);
     GAIN_CHECK_ARG(cI && h_out, "null vector");
     GAIN_CHECK_ARG(gammaI >= 0 && gammaI < c->nrepres, "irrep out of range");
     DeviceGuard g(c->device);
@@ -2224,6 +2259,8 @@ extern "C" int gain_set_dipole_combination(gain_ctx* c, int P,
       c->nc = nc;
       build_slots_all(c);
     }
+    c->mu_have.assign(nc, 1);  // the combination fills every column
+    c->mu_missing = 0;
     double* dcoef = c->w_c.get<double>(P);
     GAIN_HIP(hipMemcpyAsync(dcoef, coef, sizeof(double) * P,
                             hipMemcpyHostToDevice, c->stream));

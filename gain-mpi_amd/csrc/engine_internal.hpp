@@ -213,6 +213,10 @@ struct gain_ctx {
   bool params_set = false;
   double* d_mu = nullptr;
   int nc = 0;
+  // final vibrational columns of mu uploaded so far (pieces of
+  // gain_upload_dipole); products need all nc of them
+  std::vector<char> mu_have;
+  int mu_missing = 0;
   Sliced mu_sl;
   std::vector<Basis> bases;
   std::map<std::pair<int, int>, Block> blocks;
@@ -343,6 +347,8 @@ struct Staged {
 // Half line strengths of every (initial, dI) vector: scatter, mu GEMM and
 // gather.  H rows: i * |DI| + k.  Returns the H stride.
 gain_ctx* need(gain_ctx* c);
+// The whole dipole is resident (every piece uploaded), else GAIN_ERR_STATE.
+void need_mu(const gain_ctx* c);
 Basis& basis_of(gain_ctx* c, int jind, bool need_infl);
 Block& block_of(gain_ctx* c, int jind, int gamma);
 void ensure_slots(gain_ctx* c, Basis& b);

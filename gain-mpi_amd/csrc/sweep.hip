@@ -808,7 +808,7 @@ extern "C" int gain_sweep_create(gain_ctx* c, const gain_task* tasks,
                    "(precision %d)",
                    precision);
     if (!c->params_set) fail(GAIN_ERR_STATE, "gain_set_params not called");
-    if (!c->d_mu) fail(GAIN_ERR_STATE, "dipole not uploaded");
+    need_mu(c);
     DeviceGuard g(c->device);
     s = new gain_sweep();
     s->c = c;

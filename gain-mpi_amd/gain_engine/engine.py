@@ -111,10 +111,16 @@ class Engine:
         check(self._L.gain_get_gamma_pair(self._h, g, C.byref(out)))
         return out.value
 
-    def upload_dipole(self, mu: np.ndarray):
+    def upload_dipole(self, mu: np.ndarray, start: int = 0, end=None):
+        """mu[3][nc(v_I)][w(v_F)]: the whole dipole (w = nc), or the piece
+        of final columns [start, end) in the reference piece layout."""
         mu = np.ascontiguousarray(mu, _f64)
-        nc = mu.shape[-1]
-        check(self._L.gain_upload_dipole(self._h, _ptr(mu), nc, 0, nc))
+        nc = mu.shape[1]
+        end = start + mu.shape[2] if end is None else end
+        if mu.shape != (3, nc, end - start):
+            raise ValueError(f"dipole piece of shape {mu.shape} is not "
+                             f"[3][{nc}][{end - start}]")
+        check(self._L.gain_upload_dipole(self._h, _ptr(mu), nc, start, end))
 
     def upload_basis(self, jind: int, b):
         if getattr(b, "rotsym", False):
@@ -174,12 +180,19 @@ class Engine:
 
     @classmethod
     def from_problem(cls, prob, device: int = 0, upload_vectors=True,
-                     device_blocks: Optional[Dict] = None) -> "Engine":
+                     device_blocks: Optional[Dict] = None,
+                     dipole_pieces=None) -> "Engine":
         """Upload a synth.Problem.  `device_blocks` maps (jind, gamma) to a
-        (device pointer, ld) of records already in HBM (bound, not copied)."""
+        (device pointer, ld) of records already in HBM (bound, not copied);
+        `dipole_pieces` lists (start, end) ranges of final columns to upload
+        the dipole as, in that order, instead of whole."""
         eng = cls(device, prob.jmax, prob.params.nrepres, prob.params.maxdeg)
         eng.set_params(prob.params)
-        eng.upload_dipole(prob.mu)
+        if dipole_pieces is None:
+            eng.upload_dipole(prob.mu)
+        else:
+            for a, b in dipole_pieces:
+                eng.upload_dipole(prob.mu[:, :, a:b], a, b)
         for jind, b in enumerate(prob.bases):
             eng.upload_basis(jind, b)
             eng.upload_inflation(jind, b)

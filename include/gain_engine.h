@@ -113,9 +113,13 @@ int gain_get_gamma_pair(gain_ctx* ctx, int gamma, int* pair);
 
 /* ---- data uploads ------------------------------------------------------ */
 /* Dipole mu in the reference piece layout (TroveDipole.cpp:110):
- * mu[(vF-startF) + vI*(endF-startF) + alpha*(endF-startF)*nc].  The engine
- * needs the whole matrix (startF=0, endF=nc); GpuManager::TransferDipole,
- * GpuManager.cpp:499-527. */
+ * mu[(vF-startF) + vI*(endF-startF) + alpha*(endF-startF)*nc];
+ * GpuManager::TransferDipole, GpuManager.cpp:499-527.  The engine keeps the
+ * whole matrix resident: startF=0, endF=nc replaces it; a piece
+ * [startF, endF) (the reference's split of a dipole that does not fit,
+ * TroveDipole.cpp:47-112) is copied into the resident matrix, and every
+ * product returns GAIN_ERR_STATE until the pieces cover all nc final
+ * columns (a piece of another nc starts a new matrix). */
 int gain_upload_dipole(gain_ctx* ctx, const double* mu, int nc, int startF,
                        int endF);
 /* Primitive basis of one J (GpuManager::TransferBasisSet,

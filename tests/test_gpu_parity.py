@@ -695,3 +695,34 @@ def test_batched_fp32_mfma_error_vs_fp64_oracle():
     print(f"precision 1 (fp32): strong lines {ns} max rel err {es:.3e}; "
           f"weak lines {nw} max rel err {ew:.3e}")
     assert es <= 1e-3
+
+
+def test_dipole_in_pieces():
+    """The dipole uploaded as pieces of final vibrational columns in the
+    reference piece layout (TroveDipole.cpp:47-112), out of order: the lines
+    equal those of the whole upload bit for bit, FP64 and CRT12; while a
+    piece is missing every product returns GAIN_ERR_STATE; a piece outside
+    [0, nc) is refused."""
+    from gain_engine._lib import GainError
+    prob = synth.make_problem([2, 3], 16, "c2v4")
+    prob.params.erange_lower = (0.0, 1e9)
+    rows = prob.state_table()
+    initials = list(range(0, len(rows), 3))
+    with Engine.from_problem(prob) as eng:
+        ref = {p: engine_lines(eng, prob, initials, "batched", p)
+               for p in (0, 8)}
+    with Engine.from_problem(prob, dipole_pieces=[(5, 11), (0, 5)]) as eng:
+        I = initials[0]
+        with pytest.raises(GainError, match="dipole pieces cover 11 of"):
+            eng.batched_lines(rows[I][1], rows[I][2], [rows[I][3]],
+                              rows[I][1], rows[I][2], 0)
+        with pytest.raises(GainError, match="outside"):
+            eng.upload_dipole(np.zeros((3, 16, 7)), 10, 17)
+        eng.upload_dipole(prob.mu[:, :, 11:16], 11, 16)
+        got = {p: engine_lines(eng, prob, initials, "batched", p)
+               for p in (0, 8)}
+    for p in (0, 8):
+        assert set(got[p]) == set(ref[p]) and len(ref[p]) > 0
+        for k in ref[p]:
+            for f in ("S", "A", "I", "emitted"):
+                assert got[p][k][f] == ref[p][k][f], (p, k, f)
