@@ -1269,7 +1269,11 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
       const int S8 = i8_slices_of(precision);
       // split-INT8: final records sliced once per block, projected vectors
       // sliced here, outside the timed projection GEMM
-      const int NMc = crt_moduli_of(precision);
+      // CRT form: the projection may use its own moduli count (tuning key
+      // "proj_crt_moduli"; the half line strength keeps the precision's)
+      const int NMc = crt_moduli_of(precision) > 0 && c->proj_crt > 0
+                          ? c->proj_crt
+                          : crt_moduli_of(precision);
       const Sliced* cs8 = nullptr;
       int8_t* gs8 = nullptr;
       int* ge8 = nullptr;
@@ -2184,6 +2188,10 @@ extern "C" int gain_set_tuning(gain_ctx* c, const char* key, int value) {
       c->crt_kskip = value != 0;
     } else if (k == "crt_fuse") {
       c->crt_fuse = value != 0;
+    } else if (k == "proj_crt_moduli") {
+      GAIN_CHECK_ARG(value == 0 || value == 11 || value == 12 || value == 14,
+                     "projection moduli 0 (as the precision), 11, 12 or 14");
+      c->proj_crt = value;
     } else if (k == "i8_split") {
       GAIN_CHECK_ARG(value >= 0 && value <= 4, "split 0..4");
       c->force_i8_split = value;
@@ -2325,6 +2333,8 @@ extern "C" int gain_get_tuning(gain_ctx* c, const char* key,
       *value = c->crt_kskip;
     } else if (k == "crt_fuse") {
       *value = c->crt_fuse;
+    } else if (k == "proj_crt_moduli") {
+      *value = c->proj_crt;
     } else if (k == "autotune_us") {
       auto it = c->last.find(k);  // nothing tuned yet: 0
       *value = it == c->last.end() ? 0 : it->second;

@@ -293,6 +293,9 @@ int gain_get_timing(gain_ctx* ctx, const char* name, double* total_ms,
  * "crt_kskip" (1, the default: the CRT-form dipole products multiply only
  * the k blocks of the dipole's residue planes that hold a nonzero entry,
  * with results identical to the dense products; 0: every k block),
+ * "proj_crt_moduli" (0, the default: the CRT-form projection uses the
+ * precision's moduli count; 11, 12 or 14: its own, the half line strength
+ * keeping the precision's),
  * "crt_fuse" (1, the default: the CRT-form half line strength of K-block
  * bases is reconstructed and gathered by one kernel, identical results
  * without the FP64 mu X intermediate; 0: the two kernels),

@@ -726,3 +726,18 @@ def test_dipole_in_pieces():
         for k in ref[p]:
             for f in ("S", "A", "I", "emitted"):
                 assert got[p][k][f] == ref[p][k][f], (p, k, f)
+
+
+@pytest.mark.parametrize("pm,bound", [(11, 1e-6), (14, 1e-7)])
+def test_crt_projection_moduli_override(pm, bound):
+    """The CRT projection on its own moduli count (tuning key
+    proj_crt_moduli; the half line strength keeps 12): transition list
+    identical to the oracle's, S within `bound` on strong lines."""
+    prob, initials, ref = _long_k_case()
+    with Engine.from_problem(prob) as eng:
+        eng.set_tuning("proj_crt_moduli", pm)
+        assert eng.get_tuning("proj_crt_moduli") == pm
+        got = engine_lines(eng, prob, initials, "batched", 8)
+    assert set(got) == set(ref)
+    es, _, ns, _ = _strong_weak_err(got, ref)
+    assert ns > 0 and es <= bound, es
