@@ -221,17 +221,19 @@ def _full_sweep(w, tasks, refs, precision, check):
 def test_mixed_j_sweep_full_shape_matches_unbatched(mixed_full, phases):
     """One captured FP64 sweep of the full shape (pipelined over 2 phases,
     and on one stream) replayed twice, against the unbatched plain FP64
-    path: transition flags identical; S within the condition-normalised FP64
-    bound |dS| <= 2e-12 S + 2e-14 sqrt(S S_max) (the two paths sum in
-    different orders), and within 1e-12 relative on lines with S > 1e-2
-    S_max."""
+    path: transition flags identical; S within 1e-12 relative (the
+    config-2 bar) on every line with S > 1e-4 S_max of its initial state
+    (measured: 3.4e-13), and within the condition-normalised FP64 bound
+    |dS| <= 2e-12 S + 2e-14 sqrt(S S_max) on the weaker lines, whose
+    relative error grows with the cancellation in their sums (the two paths
+    add in different orders; 3.3e-12 above 1e-6 S_max)."""
     w, tasks, refs = mixed_full
     w.eng.set_tuning("sweep_phases", phases)
 
     def check(S, ref, t):
         d = np.abs(S - ref)
         assert (d <= 2e-12 * ref + 2e-14 * np.sqrt(ref * ref.max())).all(), t
-        strong = ref > 1e-2 * ref.max()
+        strong = ref > 1e-4 * ref.max()
         assert (d[strong] <= 1e-12 * ref[strong]).all(), t
     _full_sweep(w, tasks, refs, 0, check)
 
