@@ -211,6 +211,47 @@ def trace_summary(path, pattern=None):
               f"{min(v):9.4f} {max(v):9.4f}  {grid} {wg} {name}")
 
 
+def clock_table(counter_csv, trace_csv, pattern=None):
+    """Shader clock and matrix-pipe occupancy per (kernel, grid) from one
+    rocprofv3 run with --pmc GRBM_GUI_ACTIVE SQ_VALU_MFMA_BUSY_CYCLES and
+    --kernel-trace: clock = GRBM_GUI_ACTIVE (summed over the 8 XCDs) / 8 /
+    kernel time; busy = SQ_VALU_MFMA_BUSY_CYCLES / 1024 SIMDs / (GRBM cycles
+    of one XCD)."""
+    import csv
+    import re
+    from collections import defaultdict
+    dur = {}
+    with open(trace_csv) as f:
+        for row in csv.DictReader(f):
+            dur[row["Dispatch_Id"]] = (int(row["End_Timestamp"]) -
+                                       int(row["Start_Timestamp"])) * 1e-9
+    cnt = defaultdict(dict)
+    meta = {}
+    with open(counter_csv) as f:
+        for row in csv.DictReader(f):
+            d = row["Dispatch_Id"]
+            cnt[d][row["Counter_Name"]] = float(row["Counter_Value"])
+            meta[d] = (row["Kernel_Name"], row["Grid_Size"])
+    agg = defaultdict(list)
+    for d, c in cnt.items():
+        name, grid = meta[d]
+        if pattern and not re.search(pattern, name):
+            continue
+        if d not in dur or "GRBM_GUI_ACTIVE" not in c:
+            continue
+        cyc = c["GRBM_GUI_ACTIVE"] / 8
+        busy = c.get("SQ_VALU_MFMA_BUSY_CYCLES", 0.0) / 1024 / max(cyc, 1)
+        agg[(name.split("(")[0][:90], grid)].append(
+            (dur[d] * 1e3, cyc / dur[d] / 1e9, busy))
+    print("| kernel | grid | dispatches | avg (ms) | clock (GHz) | "
+          "matrix pipe busy |\n|---|---:|---:|---:|---:|---:|")
+    for (name, grid), v in sorted(agg.items()):
+        n = len(v)
+        print(f"| `{name}` | {grid} | {n} | {sum(x[0] for x in v) / n:.2f} | "
+              f"{sum(x[1] for x in v) / n:.2f} | "
+              f"{100 * sum(x[2] for x in v) / n:.0f}% |")
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -674,6 +715,13 @@ def main():
                     help="print per-launch duration statistics of a "
                          "rocprofv3 --kernel-trace CSV grouped by kernel and "
                          "grid, then exit (no GPU work)")
+    ap.add_argument("--clock-table", nargs="+",
+                    metavar=("COUNTER_CSV", "TRACE_CSV"),
+                    help="print shader clock and matrix-pipe occupancy per "
+                         "kernel and grid from a rocprofv3 run with --pmc "
+                         "GRBM_GUI_ACTIVE SQ_VALU_MFMA_BUSY_CYCLES "
+                         "--kernel-trace (optional third argument: kernel "
+                         "name regex), then exit (no GPU work)")
     ap.add_argument("--traffic-table", nargs=3,
                     metavar=("FETCH_CSV", "WRITE_CSV", "OUT_JSON"),
                     help="build the per-launch HBM byte table from the "
@@ -681,6 +729,9 @@ def main():
                          ", WRITE_SIZE) made with --copy-calib-gb, then exit "
                          "(no GPU work)")
     args = ap.parse_args()
+    if args.clock_table:
+        clock_table(*args.clock_table[:3])
+        return 0
     if args.trace_summary:
         trace_summary(*args.trace_summary[:2])
         return 0
