@@ -886,6 +886,7 @@ def main():
             return t / max(n, 1)
         gp_avg, pg_avg = avg("gemm_proj"), avg("proj_gemm")
         hb_avg, hg_avg = avg("half_ls"), avg("half_ls_gemm")
+        ex_avg, ep_avg = avg("expand_T"), avg("epilogue")
         mrows = np.mean([workload.rows_in_window(
             prob, blkF.energy, [blkI.energy[lv] for lv in levels_of(st)])
             for st in range(nsteps)])
@@ -903,6 +904,7 @@ def main():
              "gemm_proj_tflops": tfs,
              "half_ls_ms": hb_avg, "half_ls_gemm_ms": hg_avg,
              "half_ls_gemm_tflops": htfs,
+             "expand_T_ms": ex_avg, "epilogue_ms": ep_avg,
              "roofline": {"bound": "mfma", "kernel": "proj_gemm",
                           "achieved": tfs, "peak": PEAKS[prec] / prod,
                           "unit": "TFLOP/s",
@@ -1143,7 +1145,8 @@ def main():
                      "traffic": proj_traffic, "launch": proj_launch},
         "roofline_half_ls": head["roofline_half_ls"],
         "step_ms": {k: head[k] for k in ("gemm_proj_ms", "proj_gemm_ms",
-                                         "half_ls_ms", "half_ls_gemm_ms")},
+                                         "half_ls_ms", "half_ls_gemm_ms",
+                                         "expand_T_ms", "epilogue_ms")},
         "prep": prep,
         "cold_start": cold,
         "precision_check": prec_check,

@@ -339,13 +339,47 @@ __device__ inline void expand_T_body(const ExpandTArgs& a, int bx, int v) {
   a.G[(int64_t)v * a.gstride + t] = acc;
 }
 
+// Same for NV consecutive output vectors per thread: the case's primitive
+// chain (tcase -> case_ptr -> case_prim -> icontr) is loaded once for all of
+// them and their H loads are independent; each vector's sum is formed in
+// the order of expand_T_body, so G is identical.
+template <int NV>
+__device__ inline void expand_T_nv(const ExpandTArgs& a, int bx, int v0) {
+  const int t = bx * 256 + threadIdx.x;
+  if (t >= a.ntot) return;
+  const int nv = min(NV, a.nvec - v0);
+  const int c = a.tcase[t];
+  const double* h[NV];
+  const double* rp[NV];
+  double acc[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    const int v = v0 + min(i, nv - 1);
+    h[i] = a.H + (int64_t)a.h_of_vec[v] * a.hstride;
+    rp[i] = a.repres + t + (int64_t)a.ideg_of_vec[v] * a.ntot;
+    acc[i] = 0.0;
+  }
+  const int64_t lstride = (int64_t)a.sdeg * a.ntot;
+  for (int q = a.case_ptr[c]; q < a.case_ptr[c + 1]; ++q) {
+    const int r = a.case_prim[q];
+    const int64_t off = (int64_t)a.icontr[r + a.nprim] * lstride;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) acc[i] = fma(rp[i][off], h[i][r], acc[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < NV; ++i)
+    if (i < nv) a.G[(int64_t)(v0 + i) * a.gstride + t] = acc[i];
+}
+
+constexpr int kExpandNV = 4;  // output vectors per thread
+
 __global__ __launch_bounds__(256) void k_expand_T(ExpandTArgs a) {
-  expand_T_body(a, blockIdx.x, blockIdx.y);
+  expand_T_nv<kExpandNV>(a, blockIdx.x, blockIdx.y * kExpandNV);
 }
 
 void launch_expand_T(const ExpandTArgs& a, hipStream_t s) {
   if (a.ntot == 0 || a.nvec == 0) return;
-  dim3 grid((a.ntot + 255) / 256, a.nvec);
+  dim3 grid((a.ntot + 255) / 256, (a.nvec + kExpandNV - 1) / kExpandNV);
   hipLaunchKernelGGL(k_expand_T, grid, dim3(256), 0, s, a);
   check_launch("expand_T");
 }
