@@ -363,7 +363,8 @@ def run_mixed_sweep(nreplay, phase_scan=(1,)):
         swp.close()
     eng.set_tuning("sweep_phases", int(phases))
     # the same sweep with mu X products in a reduced-precision form
-    # (projections FP64): split-INT8 x6 and the CRT form with 12 moduli
+    # (projections FP64): split-INT8 x6, the CRT form with 12 and 14 moduli,
+    # FP32 MFMA products and split-BF16 x3
     def reduced(precision, name):
         eng.set_tuning("autotune", 1)
         swr = Sweep(eng, full, precision=precision)
@@ -401,7 +402,9 @@ def run_mixed_sweep(nreplay, phase_scan=(1,)):
 
     red = {"split_int8_x6": reduced(5, "split-INT8 x6"),
            "crt12": reduced(8, "int8 CRT (12 moduli)"),
-           "crt14": reduced(9, "int8 CRT (14 moduli)")}
+           "crt14": reduced(9, "int8 CRT (14 moduli)"),
+           "fp32": reduced(1, "FP32 MFMA"),
+           "split_bf16_x3": reduced(2, "split-BF16 x3")}
     nI = len(w.initials)
     r = {"workload": "256 initial states drawn from J_I in [5,30] (4 irreps, "
                      "N_J=(2J+1)*750), Nc=3000, 2000 states per block, "

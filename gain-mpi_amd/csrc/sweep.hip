@@ -5,9 +5,10 @@
 // Y = mu X.  The X of every batch sits in one column range of a single
 // matrix; the batches are ordered by J_I and cut into phases of about equal
 // column counts, and mu X of a phase is one GEMM over its columns (tile
-// shape from the autotuner; FP64, or split-INT8 / CRT form on the sweep's
-// own int8 copy of mu).  The products run phase after phase on the
-// context's stream; everything downstream of a phase (gathers, transposed
+// shape from the autotuner; FP64, FP32 products, or split-BF16, split-INT8
+// or CRT form on the sweep's own copy of mu).  The products run phase after
+// phase on the context's stream; everything downstream of a phase
+// (gathers, transposed
 // expansions, projections, epilogues) runs on a second stream once that
 // phase's product is done, so the HBM-bound projections of finished phases
 // run beside the matrix-core work of the next phase.  A final block of J_F
@@ -97,8 +98,11 @@ struct gain_sweep {
   struct Phase {
     int64_t col0 = 0, ncol = 0;  // column range of X and Y (col0 % 256 == 0)
     bool has_hl = false;
-    GemmArgs hl{};    // FP64 mu X of the phase's columns
+    GemmArgs hl{};    // FP64 (or FP32-product) mu X of the phase's columns
     I8GemmArgs hl8{};  // split-INT8 / CRT form of the same
+    SlicedGemmArgs hlb{};  // split-BF16 form: X of the phase cut into
+    uint16_t* xs = nullptr;  // its own bf16 planes [S][ncol][Kp]
+    int* xe = nullptr;       // and chunk exponents [ncol][nch]
     std::vector<Launch> scatter, gather, expand, gemv, lines;
     std::vector<GatherRotArgs> gather_rot;  // symmetric-top tasks, one
                                             // launch each
@@ -120,6 +124,10 @@ struct gain_sweep {
   // sweep's own int8 copy of mu, X cut into planes at every replay
   int slices = 0;
   int crt = 0;  // moduli of the CRT form, 0 otherwise
+  // FP32-product (GAIN_PREC_FP32) or split-BF16 (GAIN_PREC_BF16X*, the
+  // sweep's own bf16 slices of mu) mu X
+  bool f32 = false;
+  int bf16 = 0;
   int8_t* x8 = nullptr;
   int* x8e = nullptr;
   int64_t x8_plane = 0;
@@ -252,11 +260,20 @@ void build_scatter(gain_sweep* s, const std::vector<std::vector<int>>& mem) {
 void products(gain_sweep* s, const gain_sweep::Phase& ph, hipStream_t st) {
   if (!ph.has_hl) return;
   const int nc = s->c->nc;
+  const double* X = s->X + ph.col0 * nc;
+  if (s->f32) {
+    gemm_f32acc(ph.hl, st);
+    return;
+  }
+  if (s->bf16 > 0) {
+    slice_rows_bf16(s->bf16, X, nc, (int)ph.ncol, nc, ph.xs, ph.xe, st);
+    gemm_sliced_bf16(ph.hlb, st);
+    return;
+  }
   if (s->slices == 0 && s->crt == 0) {
     gemm_f64(ph.hl, st);
     return;
   }
-  const double* X = s->X + ph.col0 * nc;
   int8_t* planes = (int8_t*)ph.hl8.B;
   if (s->crt > 0) {
     slice_rows_crt(s->crt, ph.hl8.beta, X, nc, (int)ph.ncol, nc, planes,
@@ -381,6 +398,20 @@ void build(gain_sweep* s, const gain_task* tasks, int ntasks) {
       s->x8 = s->arena.alloc<int8_t>((size_t)S * s->x8_plane);
       s->x8e = s->arena.alloc<int>((size_t)Rx);
     }
+    // split-BF16: the sweep's own bf16 slices of mu along v_I
+    uint16_t* mb = nullptr;
+    int* mbe = nullptr;
+    const int64_t Kpb = slice_kpad(nc);
+    const int nchb = s->bf16 > 0 ? (int)slice_nchunk(s->bf16, nc) : 0;
+    if (s->bf16 > 0) {
+      double* muT = s->arena.alloc<double>((size_t)3 * nc * nc);
+      transpose_mu(c->d_mu, nc, muT, c->stream);
+      mb = s->arena.alloc<uint16_t>((size_t)s->bf16 * 3 * nc * Kpb);
+      mbe = s->arena.alloc<int>((size_t)3 * nc * nchb);
+      slice_rows_bf16(s->bf16, muT, nc, 3 * nc, nc, mb, mbe, c->stream);
+      GAIN_HIP(hipStreamSynchronize(c->stream));
+      s->arena.release(muT);
+    }
     int64_t rb = 0;  // CRT residue buffer for every phase and configuration
     for (gain_sweep::Phase& ph : s->phases) {
       if (ph.ncol == 0) continue;
@@ -400,6 +431,30 @@ void build(gain_sweep* s, const gain_task* tasks, int ntasks) {
       a.sCz = s->ncol_all * nc;
       a.nz = 3;
       a.ksplit = 1;
+      if (s->f32) continue;  // one kernel form, nothing to tune
+      if (s->bf16 > 0) {
+        ph.xs = s->arena.alloc<uint16_t>((size_t)s->bf16 * ph.ncol * Kpb);
+        ph.xe = s->arena.alloc<int>((size_t)ph.ncol * nchb);
+        SlicedGemmArgs& g = ph.hlb;
+        g.S = s->bf16;
+        g.M = nc;
+        g.N = (int)ph.ncol;
+        g.Kp = Kpb;
+        g.A = mb;
+        g.a_plane = (int64_t)3 * nc * Kpb;
+        g.a_z = (int64_t)nc * Kpb;
+        g.eA = mbe;
+        g.ea_z = (int64_t)nc * nchb;
+        g.nch = nchb;
+        g.B = ph.xs;
+        g.b_plane = ph.ncol * Kpb;
+        g.eB = ph.xe;
+        g.C = s->Y + ph.col0 * nc;
+        g.ldc = nc;
+        g.c_z = s->ncol_all * nc;
+        g.nz = 3;
+        continue;
+      }
       if (s->crt == 0 && s->slices == 0) {
         a.cfg = gemm_cfg(c, 1, a);  // tuned here: replays must not time
         continue;
@@ -802,10 +857,12 @@ extern "C" int gain_sweep_create(gain_ctx* c, const gain_task* tasks,
     GAIN_CHECK_ARG(out && tasks && ntasks > 0, "bad sweep arguments");
     *out = nullptr;
     GAIN_CHECK_ARG(precision == GAIN_PREC_FP64 ||
+                       precision == GAIN_PREC_FP32 ||
+                       slices_of(precision) > 0 ||
                        i8_slices_of(precision) > 0 ||
                        crt_moduli_of(precision) > 0,
-                   "sweeps run FP64, split-INT8 or CRT-form products "
-                   "(precision %d)",
+                   "sweeps run FP64, FP32, split-BF16, split-INT8 or "
+                   "CRT-form products (precision %d)",
                    precision);
     if (!c->params_set) fail(GAIN_ERR_STATE, "gain_set_params not called");
     need_mu(c);
@@ -814,6 +871,8 @@ extern "C" int gain_sweep_create(gain_ctx* c, const gain_task* tasks,
     s->c = c;
     s->slices = i8_slices_of(precision);
     s->crt = crt_moduli_of(precision);
+    s->f32 = precision == GAIN_PREC_FP32;
+    s->bf16 = slices_of(precision);
     if (c->force_cfg[0] >= 0) s->pj_cfg = c->force_cfg[0];
     GAIN_HIP(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
     GAIN_HIP(hipEventCreateWithFlags(&s->join, hipEventDisableTiming));

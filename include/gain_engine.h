@@ -228,7 +228,8 @@ int gain_batched_lines(gain_ctx* ctx, int jindI, int gammaI,
  * everything downstream of it (gathers, projections as one persistent
  * grouped FP64 MFMA launch and one grouped GEMV launch, every per-task
  * kernel as one grouped launch per stage) on a second stream of the sweep,
- * beside the next phase's products.  `precision` is GAIN_PREC_FP64, a split-INT8 mode
+ * beside the next phase's products.  `precision` is GAIN_PREC_FP64,
+ * GAIN_PREC_FP32, a split-BF16 mode (GAIN_PREC_BF16X3/4), a split-INT8 mode
  * (GAIN_PREC_I8X5..7) or a CRT mode (GAIN_PREC_I8CRT11/12/14); the reduced
  * modes apply to the mu X products only (the projections stay FP64).
  * gain_sweep_run replays the graph on the context's stream

@@ -314,3 +314,33 @@ def test_mixed_j_sweep_td(reduced, precision):
             assert _check(eng, tasks, outs,
                           rel=1e-12 if precision == 0 else 1e-7) > 0
         sw.close()
+
+
+@pytest.mark.parametrize("precision,rel", [(1, 1e-3), (2, 1e-2), (3, 1e-4)])
+def test_mixed_j_sweep_fp32_and_split_bf16(precision, rel):
+    """A sweep whose mu X products run on the FP32 matrix cores (FP32
+    products, FP64 accumulation) or as split-BF16 with 3 or 4 slices (the
+    sweep's own bf16 slices of mu, X sliced inside the graph) against the
+    unbatched plain FP64 path: transition lists identical, S within `rel`
+    on strong lines, on two replays with eager work between them."""
+    import torch
+    prob = synth.make_problem([5, 6, 7, 8], 12, "c2v4")
+    prob.params.erange_lower = (0.0, 1e9)
+    rng = np.random.default_rng(11)
+    with Engine.from_problem(prob) as eng:
+        eng.set_tuning("sweep_phases", 2)
+        tasks, outs = _tasks(prob, eng, rng, max_per_block=24)
+        sw = Sweep(eng, tasks, precision=precision)
+        for rep in range(2):
+            for o in outs:
+                for v in o.values():
+                    v.zero_()
+            torch.cuda.synchronize()
+            sw.run()
+            eng.synchronize()
+            assert _check(eng, tasks, outs, rel=rel) > 0
+            blk = next(iter(prob.blocks))
+            jF = blk[0] + 1 if blk[0] + 1 < len(prob.bases) else blk[0]
+            eng.batched_lines(blk[0], blk[1], [0, 1], jF,
+                              eng.gamma_pair(blk[1]), precision=precision)
+        sw.close()
