@@ -481,13 +481,15 @@ def test_every_crt_tile_configuration_long_k(cfg):
 _BANDED = {}
 
 
-def _banded_case():
+def _banded_case(zero_tiles=False):
     """The long-K problem with a banded dipole: entries with |v_I - v_F| > 40
     set to zero, and v_F in [320, 352) (one 32-row block of the dipole's
     residue planes) zero for every component, so the dipole's residue planes
     have zero k blocks at both ends of most row blocks and one zero row
-    block."""
-    if not _BANDED:
+    block.  With zero_tiles, v_F in [256, 400) is zero as well: whole
+    workgroup tiles (rows 256-511 of the 256-row tiles, 256-383 and 384-511
+    of the 128-row ones) have no k block to multiply."""
+    if zero_tiles not in _BANDED:
         prob = synth.make_problem([4, 5], 400, "c2v4")
         prob.params.erange_lower = (0.0, 1e9)
         prob.params.freq_window = (1500.0, 1e9)
@@ -495,22 +497,26 @@ def _banded_case():
         band = np.abs(a[:, None] - a[None, :]) <= 40
         prob.mu = np.ascontiguousarray(prob.mu * band[None])
         prob.mu[:, :, 320:352] = 0.0
+        if zero_tiles:
+            prob.mu[:, :, 256:400] = 0.0
         n = len(prob.state_table())
         initials = [1 + k * (n // 6) for k in range(6)]
         tr, _ = orc.run(prob, initials)
-        _BANDED.update(prob=prob, initials=initials, ref=oracle_map(tr))
-    return _BANDED["prob"], _BANDED["initials"], _BANDED["ref"]
+        _BANDED[zero_tiles] = (prob, initials, oracle_map(tr))
+    return _BANDED[zero_tiles]
 
 
+@pytest.mark.parametrize("zero_tiles", [False, True])
 @pytest.mark.parametrize("cfg", list(range(14)))
-def test_crt_zero_kblock_skip_is_exact(cfg):
+def test_crt_zero_kblock_skip_is_exact(cfg, zero_tiles):
     """The CRT-form dipole products multiply only the k blocks of each
     tile's row blocks that hold a nonzero residue; on a banded dipole with
-    a zero row block every tile configuration gives S, A, I and flags
-    identical bit for bit to the dense products (skipping off), fewer k
-    blocks are multiplied, and S is within 1e-7 of the FP64 oracle on
-    strong lines."""
-    prob, initials, ref = _banded_case()
+    a zero row block (and, in the second case, whole zero tiles, whose
+    products are skipped entirely) every tile configuration gives S, A, I
+    and flags identical bit for bit to the dense products (skipping off),
+    fewer k blocks are multiplied, and S is within 1e-7 of the FP64 oracle
+    on strong lines."""
+    prob, initials, ref = _banded_case(zero_tiles)
     res = {}
     for skip in (1, 0):
         with Engine.from_problem(prob) as eng:
