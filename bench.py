@@ -14,8 +14,9 @@ line strengths and projection as GEMMs, S/A/I epilogue).  The headline
 contractions run in the CRT form on the int8 matrix cores: operands rounded
 to ~40-bit integers, their exact products formed modulo 12 coprime moduli
 (one int8 GEMM each) and reconstructed in FP64 (S within 1.1e-8 of FP64 on
-strong lines, checked on the first batch).  Batched FP64, split-INT8 x6 and
-the 11-moduli CRT form are timed as variants, and the plain FP64 GEMV
+strong lines, checked on the first batch).  Batched FP64, FP32 matrix-core
+products, split-BF16 x3, split-INT8 x6 and the 11-moduli CRT form are timed
+as variants, and the plain FP64 GEMV
 drop-in (one initial state per call) beside them.  Kernel times
 come from HIP events the engine records on its own stream around every
 launch.  Prints one JSON line.
@@ -682,7 +683,7 @@ def main():
                     help="contraction precision of the headline line (8: "
                          "int8 CRT with 12 moduli, within 1.1e-8 of FP64 on "
                          "strong lines; 5: split-INT8 x6; 0: FP64)")
-    ap.add_argument("--variants", type=str, default="0,5,7",
+    ap.add_argument("--variants", type=str, default="0,1,2,5,7",
                     help="comma-separated extra precisions to time")
     ap.add_argument("--nc", type=int, default=4000)
     ap.add_argument("--ninitial", type=int, default=2000)
@@ -897,8 +898,9 @@ def main():
         # strength 2 * 3 Nc^2 (2 J_I + 1) per initial state
         flops = 2.0 * mrows * blkF.rec_len * B
         hflops = 2.0 * 3 * args.nc ** 2 * (2 * prob.bases[0].J + 1) * B
-        tfs = flops / (pg_avg / 1e3) / 1e12
-        htfs = hflops / (hg_avg / 1e3) / 1e12
+        # (a precision whose product kernels record no timer reports 0)
+        tfs = flops / (pg_avg / 1e3) / 1e12 if pg_avg > 0 else 0.0
+        htfs = hflops / (hg_avg / 1e3) / 1e12 if hg_avg > 0 else 0.0
         prod = PRODUCTS[prec]
         r = {"precision": prec, "dtype": PRECISION_NAMES[prec],
              "transitions": trans, "ms": ms, "tps": trans / (ms / 1e3),

@@ -915,7 +915,10 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
     g.ldc = nc;
     g.c_z = (int64_t)ncolTot * nc;
     g.nz = 3;
-    gemm_sliced_bf16(g, c->stream);
+    {
+      Timed tg(c, "half_ls_gemm");
+      gemm_sliced_bf16(g, c->stream);
+    }
   } else if (crt_moduli_of(precision) > 0) {
     const int nm = crt_moduli_of(precision);
     const Sliced& ms = mu_sliced_crt(c, nm);
@@ -1038,7 +1041,10 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
     g.sCz = (int64_t)ncolTot * nc;
     g.nz = 3;
     g.ksplit = 1;
-    gemm_f32acc(g, c->stream);
+    {
+      Timed tg(c, "half_ls_gemm");
+      gemm_f32acc(g, c->stream, c->f32_cfg);
+    }
   } else if (c->use_mfma) {
     GemmArgs g = make_mu_gemm_args(c, X, ncolTot, Y, ks);
     g.cfg = gemm_cfg(c, 1, g);
@@ -1315,7 +1321,10 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
         g.C = LS;
         g.ldc = nR;
         g.nz = 1;
-        gemm_sliced_bf16(g, c->stream);
+        {
+          Timed tg(c, "proj_gemm");
+          gemm_sliced_bf16(g, c->stream);
+        }
       } else if (NMc > 0) {
         const Sliced& cs = *cs8;
         I8GemmArgs g{};
@@ -1408,11 +1417,12 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
         c->last["proj_grid"] = gemm_grid_items(g);
         ksplit = ks;
         src_ks = (int64_t)nR * nvecG;
-      } else if (precision != GAIN_PREC_FP64)
+      } else if (precision != GAIN_PREC_FP64) {
+        Timed tg(c, "proj_gemm");
         gemm_projection(precision, nR, nvecG, ntotF,
                         Crows + (int64_t)P.Flo * blkF.ld, blkF.ld, G, gstride,
-                        LS, nR, c->stream);
-      else
+                        LS, nR, c->stream, c->f32_cfg);
+      } else
         launch_gemm_tn_valu(nR, nvecG, ntotF,
                             Crows + (int64_t)P.Flo * blkF.ld, blkF.ld, G,
                             gstride, LS, nR, c->stream);
@@ -2188,6 +2198,10 @@ extern "C" int gain_set_tuning(gain_ctx* c, const char* key, int value) {
       c->crt_kskip = value != 0;
     } else if (k == "crt_fuse") {
       c->crt_fuse = value != 0;
+    } else if (k == "f32_cfg") {
+      GAIN_CHECK_ARG(value >= -1 && value < kF32Configs, "cfg -1..%d",
+                     kF32Configs - 1);
+      c->f32_cfg = value;
     } else if (k == "proj_crt_moduli") {
       GAIN_CHECK_ARG(value == 0 || value == 11 || value == 12 || value == 14,
                      "projection moduli 0 (as the precision), 11, 12 or 14");
@@ -2335,6 +2349,8 @@ extern "C" int gain_get_tuning(gain_ctx* c, const char* key,
       *value = c->crt_fuse;
     } else if (k == "proj_crt_moduli") {
       *value = c->proj_crt;
+    } else if (k == "f32_cfg") {
+      *value = c->f32_cfg;
     } else if (k == "autotune_us") {
       auto it = c->last.find(k);  // nothing tuned yet: 0
       *value = it == c->last.end() ? 0 : it->second;

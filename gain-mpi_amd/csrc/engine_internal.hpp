@@ -239,6 +239,7 @@ struct gain_ctx {
   bool crt_kskip = true;  // CRT dipole products skip zero k blocks
   bool crt_fuse = true;   // CRT half ls: reconstruction fused with gather
   int proj_crt = 0;  // CRT projection moduli (0: the precision's)
+  int f32_cfg = -1;  // FP32 GEMM tile configuration (-1: default)
   std::map<std::tuple<int, int, int, int>, int> tuned;
   std::string tune_log;
   // what the most recent launch of each tuned kernel used (tuning getter)

@@ -63,8 +63,12 @@ struct GroupTile {
 void gemm_f64_grouped(const GemmArgs* probs, const GroupTile* tiles,
                       int ntiles, bool a_kcontig, int cfg, int grid,
                       hipStream_t s);
-// FP32 matrix-core products with FP64 accumulation (ksplit, cfg ignored).
-void gemm_f32acc(const GemmArgs& g, hipStream_t s);
+// FP32 matrix-core products with FP64 accumulation over FP64 operands
+// (ksplit and g.cfg ignored): tile configuration cfg in [0, kF32Configs),
+// -1 = default.
+constexpr int kF32Configs = 4;
+void gemm_f32acc(const GemmArgs& g, hipStream_t s, int cfg = -1);
+const char* gemm_f32_config_name(int cfg);
 // Work-items of the launch gemm_f64 makes for g (as profilers report it).
 int64_t gemm_grid_items(const GemmArgs& g);
 int gemm_tile_m(int cfg);
@@ -73,11 +77,12 @@ void gemm_f64(const GemmArgs& g, hipStream_t s);
 int gemm_default_config(int N);
 const char* gemm_config_name(int cfg);
 
-// Projection GEMM of the batched engine at the requested precision:
+// Projection GEMM of the batched engine at the requested precision (FP64
+// or FP32; f32_cfg = the FP32 GEMM's tile configuration):
 //   LS[n*ldc + m] = sum_k A[m*lda + k] * B[n*ldb + k]
 void gemm_projection(int precision, int M, int N, int K, const double* A,
                      int64_t lda, const double* B, int64_t ldb, double* C,
-                     int64_t ldc, hipStream_t s);
+                     int64_t ldc, hipStream_t s, int f32_cfg = -1);
 
 // Zero-fill kernel (used instead of hipMemsetAsync so launch sequences stay
 // capturable in HIP graphs).

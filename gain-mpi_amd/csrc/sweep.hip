@@ -127,6 +127,7 @@ struct gain_sweep {
   // FP32-product (GAIN_PREC_FP32) or split-BF16 (GAIN_PREC_BF16X*, the
   // sweep's own bf16 slices of mu) mu X
   bool f32 = false;
+  int f32_cfg = -1;  // its tile configuration (the context's at creation)
   int bf16 = 0;
   int8_t* x8 = nullptr;
   int* x8e = nullptr;
@@ -262,7 +263,7 @@ void products(gain_sweep* s, const gain_sweep::Phase& ph, hipStream_t st) {
   const int nc = s->c->nc;
   const double* X = s->X + ph.col0 * nc;
   if (s->f32) {
-    gemm_f32acc(ph.hl, st);
+    gemm_f32acc(ph.hl, st, s->f32_cfg);
     return;
   }
   if (s->bf16 > 0) {
@@ -431,7 +432,7 @@ void build(gain_sweep* s, const gain_task* tasks, int ntasks) {
       a.sCz = s->ncol_all * nc;
       a.nz = 3;
       a.ksplit = 1;
-      if (s->f32) continue;  // one kernel form, nothing to tune
+      if (s->f32) continue;  // the context's FP32 tile configuration
       if (s->bf16 > 0) {
         ph.xs = s->arena.alloc<uint16_t>((size_t)s->bf16 * ph.ncol * Kpb);
         ph.xe = s->arena.alloc<int>((size_t)ph.ncol * nchb);
@@ -872,6 +873,7 @@ extern "C" int gain_sweep_create(gain_ctx* c, const gain_task* tasks,
     s->slices = i8_slices_of(precision);
     s->crt = crt_moduli_of(precision);
     s->f32 = precision == GAIN_PREC_FP32;
+    s->f32_cfg = c->f32_cfg;
     s->bf16 = slices_of(precision);
     if (c->force_cfg[0] >= 0) s->pj_cfg = c->force_cfg[0];
     GAIN_HIP(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));

@@ -297,6 +297,8 @@ int gain_get_timing(gain_ctx* ctx, const char* name, double* total_ms,
  * "proj_crt_moduli" (0, the default: the CRT-form projection uses the
  * precision's moduli count; 11, 12 or 14: its own, the half line strength
  * keeping the precision's),
+ * "f32_cfg" (FP32 matrix-core GEMMs: force a tile configuration 0..3,
+ * -1 = the default, 0),
  * "crt_fuse" (1, the default: the CRT-form half line strength of K-block
  * bases is reconstructed and gathered by one kernel, identical results
  * without the FP64 mu X intermediate; 0: the two kernels),

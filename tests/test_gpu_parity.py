@@ -703,6 +703,39 @@ def test_batched_fp32_mfma_error_vs_fp64_oracle():
     assert es <= 1e-3
 
 
+@pytest.mark.parametrize("cfg", list(range(4)))
+def test_every_f32_tile_configuration(cfg):
+    """Each FP32 matrix-core tile shape (operands rounded to FP32, sums of
+    32 consecutive k in FP32 added in FP64), both GEMM kinds, on sizes that
+    leave partial tiles in M, N and K: transition list and emitted bits
+    identical to the FP64 oracle's; the FP32 rounding of the operands
+    perturbs each half line strength by a few 2^-24 of the largest one, so
+    |S - S_oracle| <= 2e-6 sqrt(S_oracle S_max) + 1e-12 S_max on every
+    line."""
+    prob = synth.make_problem([4, 5], 20, "c2v4")
+    prob.params.erange_lower = (0.0, 1e9)
+    rows = prob.state_table()
+    initials = list(range(1, len(rows), 2))
+    tr, _ = orc.run(prob, initials)
+    ref = oracle_map(tr)
+    with Engine.from_problem(prob) as eng:
+        eng.set_tuning("f32_cfg", cfg)
+        assert eng.get_tuning("f32_cfg") == cfg
+        got = engine_lines(eng, prob, initials, "batched", 1)
+    assert set(got) == set(ref)
+    for k in got:
+        assert got[k]["emitted"] == ref[k]["emitted"], k
+    es, ew, ns, nw = _strong_weak_err(got, ref)
+    keys = list(ref)
+    r = np.array([ref[k]["S"] for k in keys])
+    d = np.abs(np.array([got[k]["S"] for k in keys]) - r)
+    bound = 2e-6 * np.sqrt(r * r.max()) + 1e-12 * r.max()
+    print(f"fp32 configuration {cfg}: {ns} strong lines, max rel err "
+          f"{es:.3e}; {nw} weak lines, max rel err {ew:.3e}; max "
+          f"|dS| / bound {(d / bound).max():.3e}")
+    assert ns > 0 and (d <= bound).all(), (d / bound).max()
+
+
 def test_dipole_in_pieces():
     """The dipole uploaded as pieces of final vibrational columns in the
     reference piece layout (TroveDipole.cpp:47-112), out of order: the lines
