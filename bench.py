@@ -11,12 +11,13 @@ The 2000 lowest initial states go against every final state with a
 
 A step is one batch of B initial states through the batched engine (half
 line strengths and projection as GEMMs, S/A/I epilogue).  The headline
-contractions run in the CRT form on the int8 matrix cores: operands rounded
-to ~40-bit integers, their exact products formed modulo 12 coprime moduli
-(one int8 GEMM each) and reconstructed in FP64 (S within 1.1e-8 of FP64 on
-strong lines, checked on the first batch).  Batched FP64, FP32 matrix-core
-products, split-BF16 x3, split-INT8 x6 and the 11-moduli CRT form are timed
-as variants, and the plain FP64 GEMV
+contractions run in the CRT form on the int8 matrix cores: operand rows
+rounded to 43-bit integers relative to their 2-norm, their exact products
+formed modulo 11 coprime moduli (one int8 GEMM each; the row norms bound
+the products by Cauchy-Schwarz) and reconstructed in FP64 (S within 5e-8
+of FP64 on strong lines, checked on the first batch; the target is 1e-6).
+Batched FP64, FP32 matrix-core products, split-BF16 x3, split-INT8 x6 and
+the 12-moduli CRT form are timed as variants, and the plain FP64 GEMV
 drop-in (one initial state per call) beside them.  Kernel times
 come from HIP events the engine records on its own stream around every
 launch.  Prints one JSON line.
@@ -102,7 +103,7 @@ I8_CFGS = [((1, 2, 2, 2, 0), "64x128/w32x64"), ((2, 1, 2, 2, 0), "128x64/w64x32"
            ((1, 1, 4, 2, 1), "128x64/8w/lds"), ((1, 1, 2, 4, 1), "64x128/8w/lds")]
 
 
-def kernel_label(name, crt_tag="crt12"):
+def kernel_label(name, crt_tag="crt11"):
     """Label under which bench.py and the autotuner report a launch of the
     kernel whose demangled name is `name` (None for other kernels)."""
     import re
@@ -139,12 +140,13 @@ def kernel_label(name, crt_tag="crt12"):
     return None
 
 
-def build_traffic_table(fetch_csv, write_csv, out, copy_gb):
+def build_traffic_table(fetch_csv, write_csv, out, copy_gb, crt_tag):
     """HBM bytes per launch from two rocprofv3 --pmc runs of the same
     command (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch): averaged per
     (launch label, grid work-items) and scaled by what the counters read
     for the device copy of `copy_gb` GB in the same runs (the largest
-    dispatch whose name holds "copy")."""
+    dispatch whose name holds "copy").  CRT launches are labelled with
+    `crt_tag` (the moduli count of the runs' headline precision)."""
     import csv
     from collections import defaultdict
 
@@ -156,7 +158,7 @@ def build_traffic_table(fetch_csv, write_csv, out, copy_gb):
                 if "copy" in name.lower():
                     copy = max(copy, v)
                     continue
-                lab = kernel_label(name)
+                lab = kernel_label(name, crt_tag)
                 if lab is not None:
                     per[(lab, int(row["Grid_Size"]))].append(v)
         return per, copy
@@ -402,6 +404,7 @@ def run_mixed_sweep(nreplay, phase_scan=(1,)):
                 "note": f"mu X products as {name}, projections FP64"}
 
     red = {"split_int8_x6": reduced(5, "split-INT8 x6"),
+           "crt11": reduced(7, "int8 CRT (11 moduli)"),
            "crt12": reduced(8, "int8 CRT (12 moduli)"),
            "crt14": reduced(9, "int8 CRT (14 moduli)"),
            "fp32": reduced(1, "FP32 MFMA"),
@@ -433,8 +436,7 @@ def run_mixed_sweep(nreplay, phase_scan=(1,)):
                  "mu X + eager validation run + graph capture; the bound "
                  "is |dS| <= 2e-12 S + 2e-14 sqrt(S S_max) per initial "
                  "state",
-         "split_int8_x6": red["split_int8_x6"], "crt12": red["crt12"],
-         "crt14": red["crt14"]}
+         **red}
     log(f"mixed-J sweep: {len(tasks)} tasks, {trans} transitions, "
         f"{sweep_ms:.2f} ms per replay -> {r['value']:.3e}/s; unbatched "
         f"plain {plain_ms:.1f} ms; flags identical {same_flags}, S within "
@@ -679,11 +681,12 @@ def main():
                     help="initial states per batch (the 2000 lowest initial "
                          "states are walked in order, wrapping around)")
     ap.add_argument("--plain-states", type=int, default=100)
-    ap.add_argument("--precision", type=int, default=8,
-                    help="contraction precision of the headline line (8: "
-                         "int8 CRT with 12 moduli, within 1.1e-8 of FP64 on "
-                         "strong lines; 5: split-INT8 x6; 0: FP64)")
-    ap.add_argument("--variants", type=str, default="0,1,2,5,7",
+    ap.add_argument("--precision", type=int, default=7,
+                    help="contraction precision of the headline line (7: "
+                         "int8 CRT with 11 moduli, within 5e-8 of FP64 on "
+                         "strong lines; 8: 12 moduli, within 3e-9; 5: "
+                         "split-INT8 x6; 0: FP64)")
+    ap.add_argument("--variants", type=str, default="0,1,2,5,8",
                     help="comma-separated extra precisions to time")
     ap.add_argument("--nc", type=int, default=4000)
     ap.add_argument("--ninitial", type=int, default=2000)
@@ -743,7 +746,8 @@ def main():
         if args.copy_calib_gb <= 0:
             raise SystemExit("--traffic-table needs the --copy-calib-gb of "
                              "the counter runs")
-        build_traffic_table(*args.traffic_table, args.copy_calib_gb)
+        build_traffic_table(*args.traffic_table, args.copy_calib_gb,
+                            f"crt{PRODUCTS.get(args.precision, 12)}")
         return 0
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "

@@ -698,7 +698,6 @@ ScatterArgs make_scatter_args(const gain_ctx* c, const Plan& P,
   sa.nvec = nvecX;
   sa.X = X;
   sa.xstride = (int64_t)bI.ncol * c->nc;
-  sa.nc = c->nc;
   return sa;
 }
 
@@ -882,12 +881,6 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
                   : c->w_Y.get<double>((size_t)ks * 3 * ncolTot * nc);
   Timed t(c, "half_ls");
   ScatterArgs sa = make_scatter_args(c, P, cbase, sm, nvecX, X);
-  int* xe_crt = nullptr;  // CRT form: row exponents from the scatter
-  if (crt_moduli_of(precision) > 0) {
-    xe_crt = c->w_xe.get<int>((size_t)i8_rows_pad(ncolTot));
-    launch_fill_int(xe_crt, ncolTot, kExpoEmpty, c->stream);
-    sa.expo = xe_crt;
-  }
   launch_scatter_expand(sa, c->stream);
   const int S = slices_of(precision);
   if (S > 0) {
@@ -924,9 +917,9 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
     const Sliced& ms = mu_sliced_crt(c, nm);
     const int64_t Rx = i8_rows_pad(ncolTot), plane = Rx * ms.Kp;
     int8_t* xs = c->w_xs.get<int8_t>((size_t)nm * plane);
-    int* xe = xe_crt;
-    slice_rows_crt_given(nm, ms.beta, X, nc, ncolTot, nc, xs, plane, xe,
-                         c->stream);
+    int* xe = c->w_xe.get<int>((size_t)Rx);
+    slice_rows_crt(nm, ms.beta, X, nc, ncolTot, nc, xs, plane, xe,
+                   c->stream);
     I8GemmArgs g{};
     g.S = nm;
     g.beta = ms.beta;
@@ -2378,11 +2371,13 @@ extern "C" int gain_crt_moduli(int nm, int* m) {
   });
 }
 
-extern "C" int gain_crt_beta(int nm, long long K, int* beta) {
+extern "C" int gain_crt_beta(int nm, long long K, int* beta,
+                             int* norm_scaled) {
   return guarded([&] {
     GAIN_CHECK_ARG(beta && K > 0 && (nm == 11 || nm == 12 || nm == 14),
                    "bad CRT width arguments");
     *beta = crt_beta(nm, K);
+    if (norm_scaled) *norm_scaled = crt_norm_scaled(nm, K) ? 1 : 0;
   });
 }
 

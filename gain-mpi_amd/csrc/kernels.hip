@@ -23,17 +23,6 @@ typedef double dbl2 __attribute__((ext_vector_type(2)));
 // The sign is the initial-side factor of dipole_GPU.cu:437-438.  The slot's
 // primitives and their symmetrisation cases are loaded once for all the
 // thread's vectors, so the dependent index loads are shared.
-__device__ inline int frexp_expo(double x) {
-  int e = kExpoEmpty;
-  if (x != 0.0) frexp(x, &e);  // |x| < 2^e
-  return e;
-}
-
-__device__ inline int wave_max(int v) {
-  for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off));
-  return v;
-}
-
 __device__ inline void scatter_body(const ScatterArgs& a, int bx, int b0) {
   const int s = bx * 256 + threadIdx.x;
   const bool live = s < a.nslot;
@@ -69,30 +58,8 @@ __device__ inline void scatter_body(const ScatterArgs& a, int bx, int b0) {
     }
   }
 #pragma unroll
-  for (int i = 0; i < kScatterNV; ++i) {
-    if (i >= nv) break;
-    const int b = b0 + i;
-    if (live) a.X[(int64_t)b * a.xstride + s] = acc[i];
-    if (!a.expo) continue;
-    // row exponent: the lanes of a wave hold one or two X rows (nc >= 64),
-    // each reduced over the wave and combined by one atomic per row
-    const int row = live ? s / a.nc : -1;
-    const int e = live ? frexp_expo(acc[i]) : kExpoEmpty;
-    const int64_t rbase = (int64_t)b * (a.nslot / a.nc);
-    if (a.nc < 64) {
-      if (live && e != kExpoEmpty) atomicMax(a.expo + rbase + row, e);
-      continue;
-    }
-    const int row0 = __shfl(row, 0);
-    const int rowx = wave_max(row);
-    const int e0 = wave_max(row == row0 ? e : kExpoEmpty);
-    const int e1 = wave_max(row != row0 ? e : kExpoEmpty);
-    if ((threadIdx.x & 63) == 0) {
-      if (row0 >= 0 && e0 != kExpoEmpty) atomicMax(a.expo + rbase + row0, e0);
-      if (rowx != row0 && rowx >= 0 && e1 != kExpoEmpty)
-        atomicMax(a.expo + rbase + rowx, e1);
-    }
-  }
+  for (int i = 0; i < kScatterNV; ++i)
+    if (i < nv && live) a.X[(int64_t)(b0 + i) * a.xstride + s] = acc[i];
 }
 
 __global__ __launch_bounds__(256) void k_scatter_expand(ScatterArgs a) {
@@ -101,23 +68,9 @@ __global__ __launch_bounds__(256) void k_scatter_expand(ScatterArgs a) {
 
 void launch_scatter_expand(const ScatterArgs& a, hipStream_t s) {
   if (a.nslot == 0 || a.nvec == 0) return;
-  GAIN_CHECK_ARG(!a.expo || (a.nc > 0 && a.nslot % a.nc == 0),
-                 "scatter: %d slots are not whole rows of %d", a.nslot, a.nc);
   dim3 grid((a.nslot + 255) / 256, (a.nvec + kScatterNV - 1) / kScatterNV);
   hipLaunchKernelGGL(k_scatter_expand, grid, dim3(256), 0, s, a);
   check_launch("scatter_expand");
-}
-
-__global__ __launch_bounds__(256) void k_fill_int(int* p, int64_t n, int v) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
-}
-
-void launch_fill_int(int* p, int64_t n, int v, hipStream_t s) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((n + 255) / 256)),
-                     dim3(256), 0, s, p, n, v);
-  check_launch("fill_int");
 }
 
 // ---------------------------------------------------------------------------

@@ -42,16 +42,9 @@ struct ScatterArgs {
   int nvec;
   double* X;
   int64_t xstride;       // ncol * nc
-  // optional: exponent e (|x| < 2^e, frexp) of the largest |X| of each X
-  // row (vec, column), row vec * ncol + column, combined by atomic max into
-  // an array pre-filled with kExpoEmpty (rows without a nonzero element
-  // keep it); the CRT slicing then needs no pass of its own over X
-  int* expo;
-  int nc;                // slots per X row
 };
 constexpr int kScatterNV = 4;  // vectors per scatter thread
 void launch_scatter_expand(const ScatterArgs& a, hipStream_t s);
-void launch_fill_int(int* p, int64_t n, int v, hipStream_t s);
 
 // Ypart[ks][alpha][j][vF] = sum_{vI in chunk ks} mu[alpha][vI][vF] X[j][vI]
 void launch_mu_gemm_valu(const double* mu, int nc, const double* X,

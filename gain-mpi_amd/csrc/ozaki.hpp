@@ -134,17 +134,23 @@ int64_t gemm_i8_grid_items(const I8GemmArgs& g);
 // ---------------------------------------------------------------------------
 // Split-INT8, CRT form (Ozaki scheme with integer moduli).
 //
-// Every FP64 row x is scaled by 2^{beta-1-e} (e the exponent of the row
-// maximum) and rounded to an integer X, |X| <= 2^{beta-1}; X is stored as
-// its residues modulo nm pairwise coprime odd moduli m_i <= 255, each in
-// [-(m_i-1)/2, (m_i-1)/2] (an int8).  For every modulus the int8 matrix
-// cores form the exact int32 product of the residue matrices (exact while
-// K * 127^2 < 2^31); it is reduced modulo m_i, and Garner's mixed-radix
-// conversion of the nm residues gives the exact integer product
-// sum_k X_A X_B, since beta is chosen so that 2 K 4^{beta-1} < prod m_i
-// (crt_beta).  The only error is the rounding of the operands to beta-bit
-// integers, 2^{-beta} of the row maxima per element; the cost is nm int8
-// products per FMA instead of S(S+1)/2 for the sliced form.
+// Every FP64 row x is scaled by 2^{beta-1-e} (e the exponent of a bound of
+// the row's 2-norm, ||x||_2 < 2^e, or, where that width would pass the
+// 50-bit cap, of its maximum) and rounded entry by entry to an integer row
+// X, |X_k| <= 2^{beta-1}; X is stored as its residues modulo
+// nm pairwise coprime odd moduli m_i <= 255, each in [-(m_i-1)/2,
+// (m_i-1)/2] (an int8).  For every modulus the int8 matrix cores form the
+// exact int32 product of the residue matrices (exact while
+// K * 127^2 < 2^31); it is reduced modulo m_i, and the CRT reconstruction
+// of the nm residues gives the exact integer product sum_k X_A X_B, since
+// ||X||_2 < 2^{beta-1} + sqrt(K)/2 and beta is chosen so that
+// (2^{beta-1} + sqrt(K)/2)^2 < prod m_i / 2 (crt_beta; by Cauchy-Schwarz,
+// which bounds the products by the row norms instead of K times the row
+// maxima: about log2(K)/2 more bits per operand at the same moduli; with
+// the maximum scaling 2 K 4^{beta-1} < prod m_i).  The only error is the
+// rounding of the operands to integers, 2^{-beta} of the row norms (or
+// maxima) per element; the cost is nm int8 products per FMA instead of
+// S(S+1)/2 for the sliced form.
 constexpr int kCrtMaxModuli = 14;
 struct CrtConsts {
   int nm;
@@ -180,22 +186,20 @@ const CrtConsts& crt_consts(int nm);
 void crt_reconstruct_host(int nm, const uint32_t* words, int64_t n4,
                           double* out);
 int crt_modulus(int nm, int i);
-// Largest beta with 2 K 4^{beta-1} < prod_{i<nm} m_i.
+// Operand width of a product of length K: the largest beta <= 50 with
+// (2^{beta-1} + sqrt(K)/2)^2 < prod_{i<nm} m_i (1/2 - 3.5e-7), rows
+// scaled by their 2-norm (crt_norm_scaled true), or where that passes 50,
+// the largest with 2 K 4^{beta-1} < prod m_i 2^{-1e-6}, rows scaled by
+// their maximum.
 int crt_beta(int nm, int64_t K);
+bool crt_norm_scaled(int nm, int64_t K);
 // rows x K FP64 -> nm tiled int8 residue planes (layout and padding as
-// slice_rows_i8) and one exponent per padded row.
+// slice_rows_i8) and one exponent per padded row (||row||_2 < 2^e or
+// max|row| < 2^e, as crt_norm_scaled(nm, Kp) says); beta must be
+// crt_beta(nm, Kp), Kp = K padded to 64.
 void slice_rows_crt(int nm, int beta, const double* src, int64_t ld,
                     int rows, int K, int8_t* planes, int64_t plane,
                     int* expo, hipStream_t s);
-// Row exponent of an all-zero row in an exponent array filled by atomic
-// maxima (the scatter's); the slicing stores 0 for such rows, as its own
-// exponent pass does.
-constexpr int kExpoEmpty = -2147483647 - 1;
-// Same as slice_rows_crt with the exponents of the `rows` rows already in
-// `expo` (frexp exponent of each row's largest |x|, or kExpoEmpty).
-void slice_rows_crt_given(int nm, int beta, const double* src, int64_t ld,
-                          int rows, int K, int8_t* planes, int64_t plane,
-                          int* expo, hipStream_t s);
 constexpr int kCrtConfigs = 14;
 // Tile configuration used when nothing is tuned or forced: 256 x 256
 // workgroup tiles through the 4-stage LDS pipeline, the autotuner's pick for

@@ -671,22 +671,53 @@ const CrtConsts& crt_consts(int nm) {
   fail(GAIN_ERR_ARG, "CRT moduli count %d not supported (11, 12, 14)", nm);
 }
 
+// Operand width of a CRT product of length K and the row scaling it goes
+// with.  Rows scaled to a 2-norm below 2^{beta-1} and rounded entry by
+// entry (each entry moves by at most 1/2) have norms below
+// 2^{beta-1} + sqrt(K)/2, so by Cauchy-Schwarz every product
+// sum_k X_A X_B stays below (2^{beta-1} + sqrt(K)/2)^2; rows scaled to a
+// maximum below 2^{beta-1} keep it below K 4^{beta-1}.  Either bound must
+// stay below P (1/2 - 3.5e-7), so that |X| / M < 1/2 - 3e-7 and the
+// explicit reconstruction's quotient estimate is never ambiguous.  The
+// norm scaling gives about log2(K)/2 more bits per operand and is used
+// whenever its width fits the 50-bit cap (at most 50-bit integers: their
+// residues are taken exactly in FP64, sym_mod_d); past the cap the scaling
+// by the maximum keeps more bits below each row's largest entry.
+static int crt_width(int nm, int64_t K, bool* norm) {
+  GAIN_CHECK_ARG(K > 0 && nm >= 1 && nm <= kCrtMaxModuli,
+                 "CRT: %d moduli, product length %lld", nm, (long long)K);
+  long double P = 1.0L;
+  for (int i = 0; i < nm; ++i) P *= (long double)kCrtModuli[i];
+  const long double ln =
+      sqrtl(P * (0.5L - 3.5e-7L)) - sqrtl((long double)K) / 2.0L;
+  if (ln > 256.0L) {
+    int b = (int)floorl(log2l(ln));
+    if (ldexpl(1.0L, b) >= ln) --b;  // strict inequality
+    if (b + 1 <= 50) {
+      *norm = true;
+      return b + 1;
+    }
+  }
+  // 2 K 4^{beta-1} < P 2^{-1e-6} (no K that is a multiple of 64 below 2e5
+  // comes within that margin of a boundary)
+  const long double lm =
+      (log2l(P) - 1.0L - log2l((long double)K) - 1e-6L) / 2.0L;
+  int b = (int)floorl(lm);
+  if ((long double)b >= lm) --b;  // strict inequality
+  GAIN_CHECK_ARG(b >= 8, "CRT: no integer width for K = %lld", (long long)K);
+  *norm = false;
+  return std::min(b, 49) + 1;
+}
+
 int crt_beta(int nm, int64_t K) {
-  long double lp = 0.0L;
-  for (int i = 0; i < nm; ++i) lp += log2l((long double)kCrtModuli[i]);
-  // 2 K 4^{beta-1} < P  <=>  beta - 1 < (log2 P - 1 - log2 K) / 2
-  // (with a margin of 2^-1e-6, so that |X| / M stays below 1/2 - 3e-7 and
-  // the explicit reconstruction's quotient estimate is never ambiguous; no
-  // K that is a multiple of 64 below 2e5 comes that close to a boundary)
-  const long double lim =
-      (lp - 1.0L - log2l((long double)K) - 1e-6L) / 2.0L;
-  int b1 = (int)floorl(lim);
-  if ((long double)b1 >= lim) --b1;  // strict inequality
-  GAIN_CHECK_ARG(b1 >= 8, "CRT: no integer width for K = %lld",
-                 (long long)K);
-  // at most 50-bit integers: their residues are taken exactly in FP64
-  // (sym_mod_d), and 50 bits already exceed the FP64 product accuracy
-  return std::min(b1, 49) + 1;
+  bool norm;
+  return crt_width(nm, K, &norm);
+}
+
+bool crt_norm_scaled(int nm, int64_t K) {
+  bool norm;
+  crt_width(nm, K, &norm);
+  return norm;
 }
 
 // x mod m in [-(m - 1) / 2, (m - 1) / 2] for an odd modulus m and an
@@ -703,33 +734,68 @@ __host__ __device__ inline float sym_mod_f(float x, float m, float minv) {
   return fmaf(-rintf(x * minv), m, x);
 }
 
-// Exponent of every padded row: |x| < 2^e over the row (0 for a zero or
-// padding row).  One workgroup per row.
+// Exponent e of every padded row (0 for a zero or padding row): with
+// `norm`, a bound of the 2-norm, ||x||_2 < 2^e (hence |x_k| < 2^e), else of
+// the maximum, |x_k| < 2^e.  One workgroup per row; the sum of squares runs
+// in a fixed order, so a row gets the same exponent on every run.  The
+// computed sum is within K 2^-53 of the exact one (K < 2^30), so the square
+// root of the sum times 1 + 2^-30, times 1 + 2^-40, bounds the norm from
+// above.  Rows whose squares leave the FP64 range (entries beyond 2^500 or
+// all below 2^-500) take the bound sqrt(K) max|x_k| instead.
 __global__ __launch_bounds__(256) void k_row_expo(const double* __restrict__ src,
                                                   int64_t ld, int rows, int K,
+                                                  bool norm,
                                                   int* __restrict__ expo) {
-  __shared__ double red[4];
+  __shared__ double red[2][4];
   const int r = blockIdx.x;
   const int tid = threadIdx.x;
-  double mx = 0.0;
+  double mx = 0.0, ss = 0.0;
   if (r < rows) {
     const double* x = src + (int64_t)r * ld;
-    double m2 = 0.0;
+    double m2 = 0.0, s2 = 0.0;
     int k = tid;
     for (; k + 256 < K; k += 512) {
-      mx = fmax(mx, fabs(x[k]));
-      m2 = fmax(m2, fabs(x[k + 256]));
+      const double a = x[k], b = x[k + 256];
+      mx = fmax(mx, fabs(a));
+      m2 = fmax(m2, fabs(b));
+      ss = fma(a, a, ss);
+      s2 = fma(b, b, s2);
     }
-    if (k < K) mx = fmax(mx, fabs(x[k]));
+    if (k < K) {
+      const double a = x[k];
+      mx = fmax(mx, fabs(a));
+      ss = fma(a, a, ss);
+    }
     mx = fmax(mx, m2);
+    ss += s2;
   }
-  for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
-  if ((tid & 63) == 0) red[tid >> 6] = mx;
+  // butterfly sums: every lane adds the same pairs (a + b = b + a exactly)
+  for (int off = 32; off >= 1; off >>= 1) {
+    mx = fmax(mx, __shfl_xor(mx, off));
+    ss += __shfl_xor(ss, off);
+  }
+  if ((tid & 63) == 0) {
+    red[0][tid >> 6] = mx;
+    red[1][tid >> 6] = ss;
+  }
   __syncthreads();
   if (tid == 0) {
-    mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    mx = fmax(fmax(red[0][0], red[0][1]), fmax(red[0][2], red[0][3]));
+    ss = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
     int e = 0;
-    if (mx > 0.0) frexp(mx, &e);  // |x| < 2^e
+    if (mx > 0.0) {
+      const double nb = sqrt(ss * (1.0 + 0x1p-30)) * (1.0 + 0x1p-40);
+      if (!norm) {
+        frexp(mx, &e);  // |x_k| < 2^e
+      } else if (ss > 0x1p-1000 && nb < 0x1p500) {
+        frexp(nb, &e);  // ||x|| <= nb < 2^e
+      } else {
+        frexp(mx, &e);  // |x_k| < 2^e, ||x|| < sqrt(K) 2^e
+        int lk = 0;
+        while ((int64_t)1 << lk < K) ++lk;
+        e += (lk + 1) / 2;
+      }
+    }
     expo[r] = e;
   }
 }
@@ -749,13 +815,7 @@ __global__ __launch_bounds__(256) void k_slice_crt(
   const int rr = t >> 3, kk = 4 * (t & 7);
   const int r = blockIdx.x * 32 + rr;
   const bool live = r < rows;
-  int e = live ? expo[r] : 0;
-  if (e == kExpoEmpty) {  // all-zero row of an atomic-max exponent array
-    e = 0;
-    // stored for the reconstruction; workgroups that read the array
-    // meanwhile see either value and take 0 for both
-    if (blockIdx.y == 0 && (t & 7) == 0) expo[r] = 0;
-  }
+  const int e = live ? expo[r] : 0;
   const int sh = live ? beta - 1 - e : 0;
   const double* x = src + (live ? (int64_t)r * ld : 0);
   const int64_t nkb = Kp >> 5;
@@ -787,9 +847,9 @@ __global__ __launch_bounds__(256) void k_slice_crt(
   }
 }
 
-static void slice_crt_planes(int nm, int beta, const double* src, int64_t ld,
-                             int rows, int K, int8_t* planes, int64_t plane,
-                             int* expo, bool given, hipStream_t s) {
+void slice_rows_crt(int nm, int beta, const double* src, int64_t ld,
+                    int rows, int K, int8_t* planes, int64_t plane,
+                    int* expo, hipStream_t s) {
   const int64_t Rp = i8_rows_pad(rows), Kp = i8_kpad(K);
   GAIN_CHECK_ARG(rows >= 0 && K > 0 && plane >= Rp * Kp,
                  "slice_rows_crt: plane of %lld bytes < %lld x %lld",
@@ -797,12 +857,13 @@ static void slice_crt_planes(int nm, int beta, const double* src, int64_t ld,
   GAIN_CHECK_ARG(((uintptr_t)planes & 3) == 0 && (plane & 3) == 0,
                  "slice_rows_crt: planes must be 4-byte aligned");
   GAIN_CHECK_ARG(beta >= 9 && beta <= 50, "slice_rows_crt: beta %d", beta);
+  GAIN_CHECK_ARG(beta == crt_beta(nm, Kp),
+                 "slice_rows_crt: width %d is not that of %d moduli at K %lld",
+                 beta, nm, (long long)Kp);
   const CrtConsts& cc = crt_consts(nm);
-  if (!given) {
-    hipLaunchKernelGGL(k_row_expo, dim3((unsigned)Rp), dim3(256), 0, s, src,
-                       ld, rows, K, expo);
-    check_launch("row_expo");
-  }
+  hipLaunchKernelGGL(k_row_expo, dim3((unsigned)Rp), dim3(256), 0, s, src,
+                     ld, rows, K, crt_norm_scaled(nm, Kp), expo);
+  check_launch("row_expo");
   const dim3 grid((unsigned)(Rp / 32),
                   (unsigned)((Kp / 32 + kSliceKch - 1) / kSliceKch));
   switch (nm) {
@@ -819,18 +880,6 @@ static void slice_crt_planes(int nm, int beta, const double* src, int64_t ld,
       fail(GAIN_ERR_ARG, "CRT moduli count %d not supported", nm);
   }
   check_launch("slice_rows_crt");
-}
-
-void slice_rows_crt(int nm, int beta, const double* src, int64_t ld,
-                    int rows, int K, int8_t* planes, int64_t plane,
-                    int* expo, hipStream_t s) {
-  slice_crt_planes(nm, beta, src, ld, rows, K, planes, plane, expo, false, s);
-}
-
-void slice_rows_crt_given(int nm, int beta, const double* src, int64_t ld,
-                          int rows, int K, int8_t* planes, int64_t plane,
-                          int* expo, hipStream_t s) {
-  slice_crt_planes(nm, beta, src, ld, rows, K, planes, plane, expo, true, s);
 }
 
 // Configurations of the per-modulus GEMM: wave tile (32 TM) x (32 TN), WM x

@@ -248,7 +248,6 @@ void build_scatter(gain_sweep* s, const std::vector<std::vector<int>>& mem) {
       sa.nvec = G.nvecX;
       sa.X = G.X;
       sa.xstride = (int64_t)bI.ncol * nc;
-      sa.nc = nc;
       sc.push_back(sa);
     }
     add_stage(s, s->phases[pi].scatter, kGroupedScatter, sc,

@@ -163,7 +163,8 @@ def load():
         "gain_mfma_peak": (_ip, [_vp, C.c_char_p, C.POINTER(_dp)]),
         "gain_hbm_read_peak": (_ip, [_vp, C.POINTER(_dp)]),
         "gain_crt_moduli": (_ip, [C.c_int, C.POINTER(C.c_int)]),
-        "gain_crt_beta": (_ip, [C.c_int, C.c_longlong, C.POINTER(C.c_int)]),
+        "gain_crt_beta": (_ip, [C.c_int, C.c_longlong, C.POINTER(C.c_int),
+                                C.POINTER(C.c_int)]),
         "gain_crt_reconstruct": (_ip, [C.c_int, C.POINTER(C.c_uint),
                                        C.c_longlong, C.POINTER(_dp)]),
         "gain_sweep_create": (_ip, [_vp, _vp, _ip, _ip, C.POINTER(_vp)]),

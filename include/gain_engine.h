@@ -85,10 +85,11 @@ typedef enum {
   GAIN_PREC_I8X6 = 5,       /* split-INT8, 6 slices (42 bits) */
   GAIN_PREC_I8X7 = 6,       /* split-INT8, 7 slices (49 bits) */
   GAIN_PREC_I8CRT11 = 7,    /* int8 residues modulo 11 coprime moduli
-                               (operands rounded to ~36 bits), exact
-                               integer products by CRT reconstruction */
-  GAIN_PREC_I8CRT12 = 8,    /* 12 moduli (~40 bits) */
-  GAIN_PREC_I8CRT14 = 9     /* 14 moduli (~47 bits) */
+                               (rows rounded to 43-bit integers relative
+                               to their 2-norm), exact integer products by
+                               CRT reconstruction */
+  GAIN_PREC_I8CRT12 = 8,    /* 12 moduli (47 bits of the row norm) */
+  GAIN_PREC_I8CRT14 = 9     /* 14 moduli (47-49 bits of the row maximum) */
 } gain_precision;
 
 const char* gain_last_error(void);
@@ -336,12 +337,14 @@ int gain_hbm_read_peak(gain_ctx* ctx, double* bytes_per_s);
  * No reference counterpart: they expose the integer arithmetic of the
  * int8 CRT kernels for checks against exact integers.
  * gain_crt_moduli: m[0..nm) for nm = 11, 12 or 14.  gain_crt_beta: the
- * integer width of the operands of a product of length K.
+ * integer width of the operands of a product of length K, and (if
+ * norm_scaled is not NULL) 1 when rows are scaled by their 2-norm (the
+ * products then bounded by Cauchy-Schwarz), 0 when by their maximum.
  * gain_crt_reconstruct: words[n4][nm] hold the symmetric residues of four
  * outputs (byte q of word i = residue of output q modulo m_i); out[n4][4]
  * receives the integer products in FP64, computed as the GPU kernels do. */
 int gain_crt_moduli(int nm, int* m);
-int gain_crt_beta(int nm, long long K, int* beta);
+int gain_crt_beta(int nm, long long K, int* beta, int* norm_scaled);
 int gain_crt_reconstruct(int nm, const unsigned int* words, long long n4,
                          double* out);
 

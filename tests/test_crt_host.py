@@ -1,6 +1,9 @@
 """The CRT form's integer arithmetic on the host (no GPU): the moduli are
 pairwise coprime and odd, the operand width leaves the product range inside
-half the moduli product, and the reconstruction the kernels run (explicit
+half the moduli product (rows of 2-norm below 2^(beta-1), rounded entry by
+entry: by Cauchy-Schwarz every product of two of them stays below
+(2^(beta-1) + sqrt(K)/2)^2; past the 50-bit cap rows of maximum below
+2^(beta-1): K 4^(beta-1)), and the reconstruction the kernels run (explicit
 form for 11 and 12 moduli, Garner over pairs for 14) turns the residues of
 exact integers back into those integers: within 2^12 + 2 ulp for 11 and 12
 moduli (the operand rounding of a CRT product puts at least 2^44 into every
@@ -23,11 +26,16 @@ def moduli(nm):
     return list(m)
 
 
-def beta(nm, K):
+def width(nm, K):
+    """(operand width, whether rows are scaled by their 2-norm)"""
     L = _lib.load()
-    b = C.c_int()
-    _lib.check(L.gain_crt_beta(nm, K, C.byref(b)))
-    return b.value
+    b, n = C.c_int(), C.c_int()
+    _lib.check(L.gain_crt_beta(nm, K, C.byref(b), C.byref(n)))
+    return b.value, bool(n.value)
+
+
+def beta(nm, K):
+    return width(nm, K)[0]
 
 
 def reconstruct(nm, values):
@@ -63,16 +71,44 @@ def test_moduli_and_width(nm):
             assert math.gcd(ms[i], ms[j]) == 1
     M = math.prod(ms)
     for K in (64, 4032, 43008):
-        b = beta(nm, K)
-        assert 2 * K * 4 ** (b - 1) < M  # the product range fits
-        assert b == 50 or 2 * K * 4 ** b >= M  # and is the widest that does
+        b, norm = width(nm, K)
+        if norm:
+            # rows scaled by their 2-norm: the Cauchy-Schwarz range fits
+            # with the reconstruction's margin (exact integers:
+            # (2^b + sqrt(K))^2 < 2 M (1 - 7e-7), sqrt(K) bounded above by
+            # isqrt(K) + 1 and below by isqrt(K))
+            lim = 2 * M * (10 ** 7 - 7) // 10 ** 7
+            assert b <= 50 and (2 ** b + math.isqrt(K) + 1) ** 2 < lim
+            # and it is the widest that does
+            assert (2 ** (b + 1) + math.isqrt(K)) ** 2 >= lim
+            # about log2(K)/2 more bits than the bound by K times maxima
+            assert 2 * K * 4 ** (b - 1) >= M
+        else:
+            # past the 50-bit cap: rows scaled by their maximum
+            assert 2 * K * 4 ** (b - 1) < M
+            assert b == 50 or 2 * K * 4 ** b >= M
+
+
+def test_width_grows_with_norm_bound():
+    """The Cauchy-Schwarz bound (row norms) against the bound by K times the
+    row maxima (2 K 4^(beta-1) < M): at K = 43008 the width grows from 40
+    to 47 bits with 12 moduli and from 36 to 43 with 11; 14 moduli would
+    pass the 50-bit cap and keep the scaling by the maxima (47 and 49
+    bits)."""
+    assert width(12, 43008) == (47, True) and width(11, 43008) == (43, True)
+    assert width(12, 4032) == (47, True) and width(11, 4032) == (43, True)
+    assert width(14, 43008) == (47, False) and width(14, 4032) == (49, False)
 
 
 @pytest.mark.parametrize("nm,K", [(11, 4032), (12, 4032), (12, 43008),
                                   (14, 4032), (14, 43008)])
 def test_reconstruction_of_exact_integers(nm, K):
     rng = random.Random(nm * 100003 + K)
-    lim = K * 4 ** (beta(nm, K) - 1)  # largest product of the operands
+    b, norm = width(nm, K)
+    # largest product of two integer rows (Cauchy-Schwarz, or K times the
+    # product of the maxima)
+    lim = ((2 ** (b - 1) + (math.isqrt(K) + 1) // 2) ** 2 if norm
+           else K * 4 ** (b - 1))
     vals = [0, 1, -1, 2 ** 60, -(2 ** 60), lim, -lim, lim - 1]
     vals += [rng.randint(-lim, lim) for _ in range(3000)]
     vals += [rng.randint(-(2 ** 30), 2 ** 30) * rng.choice([1, 2 ** 20, 2 ** 40])
