@@ -680,7 +680,9 @@ def main():
     ap.add_argument("--batch", type=int, default=1000,
                     help="initial states per batch (the 2000 lowest initial "
                          "states are walked in order, wrapping around)")
-    ap.add_argument("--plain-states", type=int, default=100)
+    ap.add_argument("--plain-states", type=int, default=2000,
+                    help="initial states of the plain FP64 GEMV drop-in "
+                         "timing (default: all 2000 of the workload)")
     ap.add_argument("--precision", type=int, default=7,
                     help="contraction precision of the headline line (7: "
                          "int8 CRT with 11 moduli, within 5e-8 of FP64 on "
@@ -811,6 +813,7 @@ def main():
     log(f"measured HBM read rate of a streaming kernel: {read_gbs:.0f} GB/s")
 
     # ---------------- plain FP64 GEMV drop-in
+    args.plain_states = max(1, min(args.plain_states, nI_all))
     pl = {k: torch.zeros(nF, dtype=torch.float64, device=dev)
           for k in ("S", "A", "I")}
     pflags = torch.zeros((args.plain_states, nF), dtype=torch.uint8,
