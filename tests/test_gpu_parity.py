@@ -767,6 +767,34 @@ def test_dipole_in_pieces():
                 assert got[p][k][f] == ref[p][k][f], (p, k, f)
 
 
+@pytest.mark.parametrize("precision,bound", [(7, 1e-6), (8, 1e-7),
+                                             (9, 1e-9)])
+def test_crt_rows_of_unequal_scale(precision, bound):
+    """The CRT form scales every operand row by its own exponent (a bound
+    of its 2-norm, or of its maximum for 14 moduli): a dipole whose final
+    vibrational rows are scaled by powers of two from 2^-40 to 2^40 gives
+    the transition list and emitted bits of the FP64 oracle and S within
+    `bound` on strong lines, as an unscaled one does."""
+    prob = synth.make_problem([3, 4], 24, "c2v4")
+    prob.params.erange_lower = (0.0, 1e9)
+    rng = np.random.default_rng(5)
+    scale = np.ldexp(1.0, rng.integers(-40, 41, size=prob.nc))
+    prob.mu = np.ascontiguousarray(prob.mu * scale[None, None, :])
+    rows = prob.state_table()
+    initials = list(range(0, len(rows), 2))
+    tr, _ = orc.run(prob, initials)
+    ref = oracle_map(tr)
+    with Engine.from_problem(prob) as eng:
+        got = engine_lines(eng, prob, initials, "batched", precision)
+    assert set(got) == set(ref)
+    for k in got:
+        assert got[k]["emitted"] == ref[k]["emitted"], k
+    es, _, ns, _ = _strong_weak_err(got, ref)
+    print(f"precision {precision}, rows scaled by 2^-40..2^40: {ns} strong "
+          f"lines, max rel err {es:.3e}")
+    assert ns > 0 and es <= bound, es
+
+
 @pytest.mark.parametrize("pm,bound", [(11, 1e-6), (14, 1e-7)])
 def test_crt_projection_moduli_override(pm, bound):
     """The CRT projection on its own moduli count (tuning key
