@@ -453,7 +453,8 @@ def run_adjoint(nrep):
     2000), J_I=10 -> J_F=11 (pair scheme, N_I = 21000, N_F = 23000),
     500 initial x 2000 final states.  Times the forward (mu from c, batched
     FP64 lines) and the backward (gain_adjoint_dipole, which recomputes the
-    FP64 forward, + the contraction dL/dc = <M_p, dL/dmu>), with the
+    FP64 forward, + the contraction dL/dc = <M_p, dL/dmu>) both as is and
+    given the forward's pair line strengths (gain_adjoint_dipole_ls), with the
     rank-k update dL/dmu = dY X^T timed on its own; one central difference
     (h = 1e-6) along a random direction of c checks the gradient."""
     import numpy as np
@@ -511,6 +512,27 @@ def run_adjoint(nrep):
     bwd_ms = (time.time() - t1) * 1e3 / nrep
     rk_ms, rk_n = eng.timing("adjoint_rankk", reset=True)
     bw_ms, bw_n = eng.timing("adjoint", reset=True)
+    # the backward given the forward's pair line strengths and flags
+    md2 = eng.maxdeg * eng.maxdeg
+    ls = torch.zeros((len(levels), blkF.n, md2), dtype=torch.float64,
+                     device=M.device)
+    fl = torch.zeros((len(levels), blkF.n), dtype=torch.uint8,
+                     device=M.device)
+    eng.lines_device(0, 0, levels, 1, 1, {"S": S.data_ptr(),
+                                          "ls": ls.data_ptr(),
+                                          "flags": fl.data_ptr()})
+    dmu_ls = torch.zeros_like(dmu)
+    eng.adjoint(0, 0, levels, 1, 1, dS.data_ptr(), dmu_ls.data_ptr(),
+                ls=ls.data_ptr(), flags=fl.data_ptr())
+    eng.synchronize()
+    t1 = time.time()
+    for _ in range(nrep):
+        eng.adjoint(0, 0, levels, 1, 1, dS.data_ptr(), dmu_ls.data_ptr(),
+                    ls=ls.data_ptr(), flags=fl.data_ptr())
+        eng.dipole_coeff_grad(M.shape[0], M.data_ptr(), dmu_ls.data_ptr())
+    eng.synchronize()
+    bwd_ls_ms = (time.time() - t1) * 1e3 / nrep
+    ls_dev = float((dmu_ls - dmu).abs().max() / dmu.abs().max())
     eng.set_timing(False)
     rk_avg = rk_ms / max(rk_n, 1)
     ncol = 2 * (w.prob.bases[0].J + 1) * len(levels)
@@ -526,6 +548,8 @@ def run_adjoint(nrep):
                        "final states; L = sum w (log S - log S*)^2 over "
                        "strong lines",
            "build_s": build_s, "forward_ms": fwd_ms, "backward_ms": bwd_ms,
+           "backward_given_ls_ms": bwd_ls_ms,
+           "given_ls_max_rel_dev": ls_dev,
            "backward_kernels_ms": bw_ms / max(bw_n, 1),
            "rankk_ms": rk_avg,
            "rankk_tflops": rk_flops / (rk_avg / 1e3) / 1e12,
@@ -540,8 +564,12 @@ def run_adjoint(nrep):
                    "dls, C_F^T dls GEMM, expansion, transposed gather, "
                    "rank-k update on the FP64 matrix cores) + "
                    "gain_dipole_coeff_grad (one pass over the 6.1 GB of "
-                   "M_p); host wall clock per call, kernels by HIP events"}
+                   "M_p); backward_given_ls = gain_adjoint_dipole_ls (the "
+                   "forward's pair line strengths and flags given, forward "
+                   "not recomputed) + the same contraction; host wall clock "
+                   "per call, kernels by HIP events"}
     log(f"adjoint: forward {fwd_ms:.1f} ms, backward {bwd_ms:.1f} ms "
+        f"(given ls {bwd_ls_ms:.1f} ms, dev {ls_dev:.1e}) "
         f"(rank-k {rk_avg:.2f} ms = {out['rankk_tflops']:.1f} TF/s); "
         f"dL/dc.d {an:.6e} vs central difference {fd:.6e}")
     w.close()

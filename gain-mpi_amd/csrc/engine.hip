@@ -1485,7 +1485,8 @@ void run_lines(gain_ctx* c, const Plan& P, const double* cI_host,
 //   dls = dS * dS/dls,  dG = C_F^T dls,  dH = P_F dG,  dY = gather^T dH,
 //   dmu_alpha = dY_alpha X^T  (a rank-k update, k = columns of X).
 void run_adjoint(gain_ctx* c, const Plan& P, const double* dS_dev,
-                 double* dmu_work) {
+                 double* dmu_work, const double* ls_fwd,
+                 const uint8_t* flags_fwd) {
   const gain_params& p = c->params;
   Basis& bI = *P.bI;
   Basis& bF = *P.bF;
@@ -1504,27 +1505,44 @@ void run_adjoint(gain_ctx* c, const Plan& P, const double* dS_dev,
   const int ntotF = irF.ntot;
   const int64_t gstride = (ntotF + 1) / 2 * 2;
   const int64_t hstride = bF.nprim;
-  // ---- forward (FP64)
-  double* H = c->w_H.get<double>((size_t)(nvecX + 1) * hstride);
-  launch_fill_zero(H + (size_t)nvecX * hstride, hstride, c->stream);
-  run_half_ls(c, P, st.cbase, sm, nvecX, H, GAIN_PREC_FP64);
-  const double* X = (const double*)c->w_X.p;
-  double* G = c->w_G.get<double>((size_t)nvecG * gstride);
-  launch_expand_T(make_expand_T_args(P, H, sm, nvecG, G, gstride),
-                  c->stream);
   const double* Crows = blkF.vec + (int64_t)P.Flo * blkF.ld;
+  LinesArgs la = make_lines_args(c, P, sm, nvpi, true);
   // adjoint work buffers: LS and dLS share w_part, flags in w_out
   double* LS = c->w_part.get<double>((size_t)2 * nR * nvecG);
   double* dLS = LS + (size_t)nR * nvecG;
-  gemm_projection(GAIN_PREC_FP64, nR, nvecG, ntotF, Crows, blkF.ld, G,
-                  gstride, LS, nR, c->stream);
-  uint8_t* flags = c->w_out.get<uint8_t>((size_t)nI * nF);
-  LinesArgs la = make_lines_args(c, P, sm, nvpi, true);
-  la.src = LS;
-  la.src_f = 1;
-  la.src_v = nR;
-  la.flags = flags;
-  launch_lines(la, c->stream);
+  const double* X = nullptr;
+  const uint8_t* flags = flags_fwd;
+  if (bF.nvib > nc)
+    fail(GAIN_ERR_ARG,
+         "vibrational index %d of J=%d exceeds the dipole dimension %d",
+         bF.nvib - 1, bF.J, nc);
+  if (ls_fwd) {
+    // the forward's line strengths and flags are given: only X (the
+    // scattered initial vectors of the rank-k update) is rebuilt
+    ensure_slots(c, bI);
+    X = c->w_X.get<double>((size_t)nvecX * bI.ncol * nc);
+    launch_scatter_expand(make_scatter_args(c, P, st.cbase, sm, nvecX,
+                                            (double*)X),
+                          c->stream);
+  } else {
+    // ---- forward (FP64)
+    double* H = c->w_H.get<double>((size_t)(nvecX + 1) * hstride);
+    launch_fill_zero(H + (size_t)nvecX * hstride, hstride, c->stream);
+    run_half_ls(c, P, st.cbase, sm, nvecX, H, GAIN_PREC_FP64);
+    X = (const double*)c->w_X.p;
+    double* G = c->w_G.get<double>((size_t)nvecG * gstride);
+    launch_expand_T(make_expand_T_args(P, H, sm, nvecG, G, gstride),
+                    c->stream);
+    gemm_projection(GAIN_PREC_FP64, nR, nvecG, ntotF, Crows, blkF.ld, G,
+                    gstride, LS, nR, c->stream);
+    uint8_t* fl = c->w_out.get<uint8_t>((size_t)nI * nF);
+    la.src = LS;
+    la.src_f = 1;
+    la.src_v = nR;
+    la.flags = fl;
+    launch_lines(la, c->stream);
+    flags = fl;
+  }
   // ---- backward
   Timed t(c, "adjoint");
   const int ndegI = p.sym_degen[P.gI], ndegF = p.sym_degen[P.gF];
@@ -1542,6 +1560,11 @@ void run_adjoint(gain_ctx* c, const Plan& P, const double* dS_dev,
   da.scale = (p.reduced && ndegF != 1 && ndegI != 1) ? 2.0
                                                      : 2.0 / (double)ndegI;
   da.dls = dLS;
+  if (ls_fwd) {
+    da.ls_pair = ls_fwd;
+    da.slot_of_vec = la.slot_of_vec;
+    da.md2 = c->maxdeg * c->maxdeg;
+  }
   launch_dls(da, c->stream);
   // dG[v][t] = sum_f C[Flo+f][t] dLS[v][f]
   double* dG = c->w_G.get<double>((size_t)nvecG * gstride);
@@ -2220,13 +2243,19 @@ extern "C" int gain_set_tuning(gain_ctx* c, const char* key, int value) {
   });
 }
 
-extern "C" int gain_adjoint_dipole(gain_ctx* c, int jindI, int gammaI,
-                                   const int* levelsI, int nI, int jindF,
-                                   int gammaF, const double* dS, double* dmu,
-                                   int on_device, int accumulate) {
+// dL/dmu of one batch (both entry points): the FP64 forward recomputed,
+// or the forward's pair line strengths and flags given (ls_in, flags_in,
+// device or host memory as dS).
+static int adjoint_entry(gain_ctx* c, int jindI, int gammaI,
+                         const int* levelsI, int nI, int jindF, int gammaF,
+                         const double* dS, bool given, const double* ls_in,
+                         const uint8_t* flags_in, double* dmu, int on_device,
+                         int accumulate) {
   return guarded([&] {
     need(c);
     GAIN_CHECK_ARG(levelsI && nI > 0 && dS && dmu, "bad adjoint arguments");
+    GAIN_CHECK_ARG(!given || (ls_in && flags_in),
+                   "the forward's line strengths and flags are required");
     DeviceGuard g(c->device);
     Plan P = make_plan(c, jindI, gammaI, levelsI, nI, jindF, gammaF);
     if (!P.blkI->vec)
@@ -2235,15 +2264,28 @@ extern "C" int gain_adjoint_dipole(gain_ctx* c, int jindI, int gammaI,
     const int nc = c->nc;
     const size_t nF = P.blkF->n;
     const size_t nmu = (size_t)3 * nc * nc;
+    const size_t md2 = (size_t)c->maxdeg * c->maxdeg;
     const double* dS_dev = dS;
+    const double* ls_dev = ls_in;
+    const uint8_t* fl_dev = flags_in;
     if (!on_device) {
       double* d = c->w_c.get<double>((size_t)nI * nF);
       GAIN_HIP(hipMemcpyAsync(d, dS, sizeof(double) * nI * nF,
                               hipMemcpyHostToDevice, c->stream));
       dS_dev = d;
+      if (ls_in) {
+        double* l = c->w_gs.get<double>((size_t)nI * nF * md2);
+        GAIN_HIP(hipMemcpyAsync(l, ls_in, sizeof(double) * nI * nF * md2,
+                                hipMemcpyHostToDevice, c->stream));
+        ls_dev = l;
+        uint8_t* f = c->w_ge.get<uint8_t>((size_t)nI * nF);
+        GAIN_HIP(hipMemcpyAsync(f, flags_in, (size_t)nI * nF,
+                                hipMemcpyHostToDevice, c->stream));
+        fl_dev = f;
+      }
     }
     double* work = c->w_xs.get<double>(nmu);
-    run_adjoint(c, P, dS_dev, work);
+    run_adjoint(c, P, dS_dev, work, ls_dev, fl_dev);
     if (on_device) {
       launch_axpy1(work, dmu, (int64_t)nmu, accumulate, c->stream);
     } else {
@@ -2256,6 +2298,24 @@ extern "C" int gain_adjoint_dipole(gain_ctx* c, int jindI, int gammaI,
     }
     GAIN_HIP(hipStreamSynchronize(c->stream));
   });
+}
+
+extern "C" int gain_adjoint_dipole(gain_ctx* c, int jindI, int gammaI,
+                                   const int* levelsI, int nI, int jindF,
+                                   int gammaF, const double* dS, double* dmu,
+                                   int on_device, int accumulate) {
+  return adjoint_entry(c, jindI, gammaI, levelsI, nI, jindF, gammaF, dS,
+                       false, nullptr, nullptr, dmu, on_device, accumulate);
+}
+
+extern "C" int gain_adjoint_dipole_ls(gain_ctx* c, int jindI, int gammaI,
+                                      const int* levelsI, int nI, int jindF,
+                                      int gammaF, const double* dS,
+                                      const double* ls, const uint8_t* flags,
+                                      double* dmu, int on_device,
+                                      int accumulate) {
+  return adjoint_entry(c, jindI, gammaI, levelsI, nI, jindF, gammaF, dS,
+                       true, ls, flags, dmu, on_device, accumulate);
 }
 
 extern "C" int gain_set_dipole_combination(gain_ctx* c, int P,

@@ -648,7 +648,10 @@ __global__ __launch_bounds__(256) void k_dls(DlsArgs a) {
   const double nu = a.eF[F] - a.eI[i];
   const double g = ((a.flags[o] & 1) && !(nu < 1e-40)) ? a.dS[o] : 0.0;
   const int64_t p = (int64_t)v * a.nR + f;
-  a.dls[p] = g * a.scale * a.ls[p];
+  const double ls =
+      a.ls_pair ? a.ls_pair[o * a.md2 + a.slot_of_vec[v - i * a.nvpi]]
+                : a.ls[p];
+  a.dls[p] = g * a.scale * ls;
 }
 
 void launch_dls(const DlsArgs& a, hipStream_t s) {

@@ -167,6 +167,12 @@ struct DlsArgs {
   const double* ls;       // [nvec][nR] forward projection
   double scale;           // 2 / ndegI (or 2)
   double* dls;            // [nvec][nR]
+  // optional: the forward's per-pair line strengths (gain_lines.ls,
+  // [nI][nF][md2]) read in place of `ls`, vector v of initial i at slot
+  // slot_of_vec[v - i * nvpi]
+  const double* ls_pair = nullptr;
+  const int* slot_of_vec = nullptr;
+  int md2 = 0;
 };
 void launch_dls(const DlsArgs& a, hipStream_t s);
 

@@ -31,7 +31,8 @@ EXPORTS = (
     "gain_copy_symmetry_constants", "gain_copy_jmax_constant",
     "gain_copy_dipole_constant", "gain_transform_vector_primitive",
     "gain_compute_half_linestrength", "gain_compute_half_linestrength_rotsym",
-    "gain_ddot", "gain_adjoint_dipole", "gain_rotsym_functions",
+    "gain_ddot", "gain_adjoint_dipole", "gain_adjoint_dipole_ls",
+    "gain_rotsym_functions",
     "gain_rotsym_wigner",
     "gain_set_dipole_combination", "gain_dipole_coeff_grad",
     "gain_set_tuning", "gain_get_tuning", "gain_tuning_report",
@@ -183,6 +184,8 @@ def load():
         "gain_rotsym_wigner": (_ip, [C.c_char_p, _ip, _ip] + [_vp] * 4),
         "gain_adjoint_dipole": (_ip, [_vp, _ip, _ip, _vp, _ip, _ip, _ip, _vp,
                                       _vp, _ip, _ip]),
+        "gain_adjoint_dipole_ls": (_ip, [_vp, _ip, _ip, _vp, _ip, _ip, _ip,
+                                         _vp, _vp, _vp, _vp, _ip, _ip]),
         "gain_set_dipole_combination": (_ip, [_vp, _ip, _vp, _vp, _ip]),
         "gain_dipole_coeff_grad": (_ip, [_vp, _ip, _vp, _vp, _vp]),
     }

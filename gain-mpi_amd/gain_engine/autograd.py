@@ -14,22 +14,36 @@ import torch
 
 class LineStrengths(torch.autograd.Function):
     """S(c) for tasks [(jindI, gammaI, levels, jindF, gammaF), ...]; output
-    is the concatenation of the [nI, nF] S arrays (float64, device)."""
+    is the concatenation of the [nI, nF] S arrays (float64, device).  With
+    reuse (the default) the forward keeps each task's pair line strengths
+    and flags and the backward hands them to gain_adjoint_dipole_ls, which
+    does not recompute the forward; reuse=False recomputes it
+    (gain_adjoint_dipole)."""
 
     @staticmethod
-    def forward(ctx, coef, eng, M, nc, tasks):
+    def forward(ctx, coef, eng, M, nc, tasks, reuse=True):
         # (the engine's calls wait for torch's stream: Engine.wait_for_torch)
         eng.set_dipole_combination(coef.detach().cpu().numpy(),
                                    M.data_ptr(), nc)
-        outs = []
+        outs, kept = [], []
+        md2 = eng.maxdeg * eng.maxdeg
         for (jI, gI, levels, jF, gF) in tasks:
             nF = eng.blocks[(jF, gF)][0]
             S = torch.zeros((len(levels), nF), dtype=torch.float64,
                             device=M.device)
-            eng.lines_device(jI, gI, levels, jF, gF, {"S": S.data_ptr()})
+            ptrs = {"S": S.data_ptr()}
+            if reuse:
+                ls = torch.zeros((len(levels), nF, md2), dtype=torch.float64,
+                                 device=M.device)
+                fl = torch.zeros((len(levels), nF), dtype=torch.uint8,
+                                 device=M.device)
+                ptrs |= {"ls": ls.data_ptr(), "flags": fl.data_ptr()}
+                kept.append((ls, fl))
+            eng.lines_device(jI, gI, levels, jF, gF, ptrs)
             outs.append(S.reshape(-1))
         eng.synchronize()
         ctx.eng, ctx.M, ctx.nc, ctx.tasks = eng, M, nc, tasks
+        ctx.kept = kept if reuse else None
         ctx.save_for_backward(coef)
         return torch.cat(outs)
 
@@ -37,23 +51,31 @@ class LineStrengths(torch.autograd.Function):
     def backward(ctx, gS):
         (coef,) = ctx.saved_tensors
         eng, M, nc = ctx.eng, ctx.M, ctx.nc
-        eng.set_dipole_combination(coef.detach().cpu().numpy(),
-                                   M.data_ptr(), nc)
+        if ctx.kept is None:
+            # the recomputed forward needs mu = sum_p c_p M_p again
+            eng.set_dipole_combination(coef.detach().cpu().numpy(),
+                                       M.data_ptr(), nc)
         dmu = torch.zeros(3 * nc * nc, dtype=torch.float64, device=M.device)
         gS = gS.contiguous().to(M.device)
         off = 0
-        for (jI, gI, levels, jF, gF) in ctx.tasks:
+        for t, (jI, gI, levels, jF, gF) in enumerate(ctx.tasks):
             nF = eng.blocks[(jF, gF)][0]
             n = len(levels) * nF
             part = gS[off:off + n].contiguous()
-            eng.adjoint(jI, gI, levels, jF, gF, part.data_ptr(),
-                        dmu.data_ptr(), on_device=True, accumulate=True)
+            if ctx.kept is None:
+                eng.adjoint(jI, gI, levels, jF, gF, part.data_ptr(),
+                            dmu.data_ptr(), on_device=True, accumulate=True)
+            else:
+                ls, fl = ctx.kept[t]
+                eng.adjoint(jI, gI, levels, jF, gF, part.data_ptr(),
+                            dmu.data_ptr(), on_device=True, accumulate=True,
+                            ls=ls.data_ptr(), flags=fl.data_ptr())
             off += n
         eng.synchronize()
         dc = eng.dipole_coeff_grad(coef.shape[0], M.data_ptr(),
                                    dmu.data_ptr())
         return (torch.as_tensor(dc, dtype=coef.dtype), None, None, None,
-                None)
+                None, None)
 
 
 def make_basis_matrices(P: int, nc: int, seed: int = 7) -> np.ndarray:

@@ -338,20 +338,37 @@ def _attach_adjoint():
     """Adjoint / dipole-parameterisation methods of Engine."""
 
     def adjoint(self, jindI, gammaI, levels, jindF, gammaF, dS, dmu,
-                on_device=True, accumulate=False):
+                on_device=True, accumulate=False, ls=None, flags=None):
         """dL/dmu for one batch given dS = dL/dS ([nI][nF]); dS and dmu are
-        device pointers (ints) when on_device, else numpy arrays."""
+        device pointers (ints) when on_device, else numpy arrays.  With ls
+        and flags (the FP64 batched forward's gain_lines.ls and .flags for
+        this batch, same kind of memory) the forward is not recomputed."""
         lv = np.ascontiguousarray(levels, _i32)
         if on_device:
             self.wait_for_torch()
             p_dS, p_dmu = int(dS), int(dmu)
+            p_ls = None if ls is None else int(ls)
+            p_fl = None if flags is None else int(flags)
         else:
             dS = np.ascontiguousarray(dS, _f64)
             p_dS, p_dmu = _ptr(dS), _ptr(dmu)
-        check(self._L.gain_adjoint_dipole(self._h, jindI, gammaI, _ptr(lv),
-                                          lv.shape[0], jindF, gammaF, p_dS,
-                                          p_dmu, int(on_device),
-                                          int(accumulate)))
+            if ls is not None:
+                ls = np.ascontiguousarray(ls, _f64)
+                flags = np.ascontiguousarray(flags, np.uint8)
+            p_ls = None if ls is None else _ptr(ls)
+            p_fl = None if flags is None else _ptr(flags)
+        if p_ls is None and p_fl is None:
+            check(self._L.gain_adjoint_dipole(self._h, jindI, gammaI,
+                                              _ptr(lv), lv.shape[0], jindF,
+                                              gammaF, p_dS, p_dmu,
+                                              int(on_device),
+                                              int(accumulate)))
+            return
+        check(self._L.gain_adjoint_dipole_ls(self._h, jindI, gammaI,
+                                             _ptr(lv), lv.shape[0], jindF,
+                                             gammaF, p_dS, p_ls, p_fl, p_dmu,
+                                             int(on_device),
+                                             int(accumulate)))
 
     def set_dipole_combination(self, coef, M_ptr: int, nc: int):
         cf = np.ascontiguousarray(coef, _f64)

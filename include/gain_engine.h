@@ -263,6 +263,18 @@ int gain_adjoint_dipole(gain_ctx* ctx, int jindI, int gammaI,
                         const int* levelsI, int nI, int jindF, int gammaF,
                         const double* dS, double* dmu, int on_device,
                         int accumulate);
+/* The same without recomputing the forward: ls and flags are the
+ * gain_lines.ls and gain_lines.flags that the FP64 batched forward
+ * (gain_batched_lines, GAIN_PREC_FP64) wrote for this batch and block with
+ * the current dipole ([nI][nF][maxdeg*maxdeg] and [nI][nF]; device memory
+ * if on_device, else host).  Only the scattered initial vectors of the
+ * rank-k update are rebuilt.  The caller keeps them matched: results for
+ * another batch, dipole or block give a wrong gradient. */
+int gain_adjoint_dipole_ls(gain_ctx* ctx, int jindI, int gammaI,
+                           const int* levelsI, int nI, int jindF, int gammaF,
+                           const double* dS, const double* ls,
+                           const unsigned char* flags, double* dmu,
+                           int on_device, int accumulate);
 /* mu = sum_{p<P} coef[p] M_p (coef host, M device [P][3][nc][nc]); sets the
  * context's dipole. */
 int gain_set_dipole_combination(gain_ctx* ctx, int P, const double* coef,

@@ -216,3 +216,80 @@ def test_adjoint_dmu_elementwise_vs_dense_autograd():
         assert scale > 0
         np.testing.assert_allclose(dmu.cpu().numpy(), g.cpu().numpy(),
                                    rtol=1e-9, atol=1e-12 * scale)
+
+
+def test_autograd_reuse_equals_recompute(setup):
+    """dL/dc with the forward's line strengths handed to the backward
+    (gain_adjoint_dipole_ls) equals dL/dc with the forward recomputed
+    inside the adjoint (gain_adjoint_dipole)."""
+    import torch
+
+    from gain_engine.autograd import LineStrengths
+    prob, eng, M, Md, coef, tasks = setup
+    c, _, mask, w, target = _prepare(setup)
+    grads = []
+    for reuse in (True, False):
+        S = LineStrengths.apply(c, eng, Md, NC, tasks, reuse)
+        L = (w * (torch.log(S[mask]) - target) ** 2).sum()
+        (g,) = torch.autograd.grad(L, c)
+        grads.append(g.numpy())
+    assert np.abs(grads[1]).max() > 0
+    np.testing.assert_allclose(grads[0], grads[1], rtol=1e-12,
+                               atol=1e-14 * np.abs(grads[1]).max())
+
+
+@pytest.mark.parametrize("gI,gF", [(2, 2), (3, 4)])
+def test_adjoint_given_ls_degenerate_irreps(gI, gF):
+    """Td irreps E -> E and F1 -> F2 (two and three degenerate components,
+    so each pair's line strengths sit at component slots): dL/dmu from the
+    forward's ls and flags equals the recomputed one, for host and device
+    memory, and <dL/dmu, D> equals the central difference of L = sum w S
+    along a dipole direction D, which is exact up to rounding since S is
+    quadratic in mu."""
+    import torch
+    nc = 20
+    prob = synth.make_problem([2, 3], nc, "td5")
+    prob.params.erange_lower = (0.0, 1e9)
+    n0 = prob.blocks[(0, gI)].n
+    levels = list(range(0, n0, 2))
+    dev = "cuda:0"
+    with Engine.from_problem(prob) as eng:
+        nF = eng.blocks[(1, gF)][0]
+        res = eng.batched_lines(0, gI, levels, 1, gF)
+        assert (res["flags"] & 1).sum() > 0, "no pair passes the filters"
+        w = np.random.default_rng(11).random((len(levels), nF))
+        nmu = 3 * nc * nc
+        d_re = np.zeros(nmu)
+        d_ls = np.zeros(nmu)
+        eng.adjoint(0, gI, levels, 1, gF, w, d_re, on_device=False)
+        eng.adjoint(0, gI, levels, 1, gF, w, d_ls, on_device=False,
+                    ls=res["ls"], flags=res["flags"])
+        scale = np.abs(d_re).max()
+        assert scale > 0
+        np.testing.assert_allclose(d_ls, d_re, rtol=1e-12,
+                                   atol=1e-14 * scale)
+        # device memory
+        wd = torch.as_tensor(w, device=dev).contiguous()
+        lsd = torch.as_tensor(np.ascontiguousarray(res["ls"]),
+                              device=dev).contiguous()
+        fld = torch.as_tensor(np.ascontiguousarray(res["flags"]),
+                              device=dev).contiguous()
+        dd = torch.full((nmu,), 7.0, dtype=torch.float64, device=dev)
+        eng.adjoint(0, gI, levels, 1, gF, wd.data_ptr(), dd.data_ptr(),
+                    ls=lsd.data_ptr(), flags=fld.data_ptr())
+        eng.synchronize()
+        np.testing.assert_allclose(dd.cpu().numpy(), d_re, rtol=1e-12,
+                                   atol=1e-14 * scale)
+        # central difference along D through mu(t) = mu + t D
+        D = np.random.default_rng(12).standard_normal(prob.mu.shape)
+        Mt = torch.as_tensor(np.stack([prob.mu.reshape(-1), D.reshape(-1)]),
+                             device=dev).contiguous()
+        h = 0.5
+        Lh = []
+        for t in (h, -h):
+            eng.set_dipole_combination(np.array([1.0, t]), Mt.data_ptr(), nc)
+            Lh.append(float((w * eng.batched_lines(0, gI, levels, 1,
+                                                   gF)["S"]).sum()))
+        fd = (Lh[0] - Lh[1]) / (2 * h)
+        ad = float(d_re @ D.reshape(-1))
+        assert ad == pytest.approx(fd, rel=1e-10, abs=1e-12 * abs(Lh[0]))
