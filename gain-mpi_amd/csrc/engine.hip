@@ -117,9 +117,11 @@ void ensure_slots(gain_ctx* c, Basis& b) {
   std::vector<int> fill(cnt.begin(), cnt.end() - 1), prim(b.nprim);
   for (int r = 0; r < b.nprim; ++r) prim[fill[slot[r]]++] = r;
   std::vector<int> one(nslot);
+  b.slot_multi = false;
   for (int s = 0; s < nslot; ++s) {
     const int n = cnt[s + 1] - cnt[s];
     one[s] = n == 1 ? prim[cnt[s]] : (n == 0 ? -1 : -2);
+    b.slot_multi |= n > 1;
   }
   if (b.slot_ptr || b.slot_prim) ++c->gen;
   if (b.slot_ptr) c->arena.release(b.slot_ptr);
@@ -881,7 +883,13 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
                   : c->w_Y.get<double>((size_t)ks * 3 * ncolTot * nc);
   Timed t(c, "half_ls");
   ScatterArgs sa = make_scatter_args(c, P, cbase, sm, nvecX, X);
-  launch_scatter_expand(sa, c->stream);
+  // CRT form: X built straight into its residue planes when it fits
+  const int64_t repres_len =
+      (int64_t)bI.matsize * bI.irrep[P.gI].sdeg * bI.irrep[P.gI].ntot;
+  const bool xfuse = crt_moduli_of(precision) > 0 && c->crt_xfuse &&
+                     scatter_slice_crt_fits(crt_moduli_of(precision), nc,
+                                            repres_len);
+  if (!xfuse) launch_scatter_expand(sa, c->stream);
   const int S = slices_of(precision);
   if (S > 0) {
     ks = 1;
@@ -918,8 +926,14 @@ int64_t run_half_ls(gain_ctx* c, const Plan& P, const double* cbase,
     const int64_t Rx = i8_rows_pad(ncolTot), plane = Rx * ms.Kp;
     int8_t* xs = c->w_xs.get<int8_t>((size_t)nm * plane);
     int* xe = c->w_xe.get<int>((size_t)Rx);
-    slice_rows_crt(nm, ms.beta, X, nc, ncolTot, nc, xs, plane, xe,
-                   c->stream);
+    if (xfuse)
+      scatter_slice_rows_crt(nm, ms.beta, sa, bI.ncol, nc, ncolTot, xs, plane,
+                             xe, c->w_desc.get<int4>((size_t)sa.nslot),
+                             !bI.slot_multi && bI.irrep[P.gI].maxN <= 1,
+                             c->stream);
+    else
+      slice_rows_crt(nm, ms.beta, X, nc, ncolTot, nc, xs, plane, xe,
+                     c->stream);
     I8GemmArgs g{};
     g.S = nm;
     g.beta = ms.beta;
@@ -2005,6 +2019,9 @@ extern "C" int gain_upload_inflation(gain_ctx* c, int jind, int maxsymcoeffs,
         for (int i = 0; i < N[gm][k]; ++i) tcase[ijt[k] + i] = k;
       }
       ir.N = c->arena.upload(N[gm], maxsymcoeffs);
+      ir.maxN = 0;
+      for (int k = 0; k < maxsymcoeffs; ++k)
+        ir.maxN = std::max(ir.maxN, N[gm][k]);
       const size_t nrep = (size_t)ir.ntot * ir.sdeg * matsize;
       ir.repres = nrep ? c->arena.upload(repres[gm], nrep)
                        : c->arena.alloc<double>(1);
@@ -2214,6 +2231,8 @@ extern "C" int gain_set_tuning(gain_ctx* c, const char* key, int value) {
       c->crt_kskip = value != 0;
     } else if (k == "crt_fuse") {
       c->crt_fuse = value != 0;
+    } else if (k == "crt_xfuse") {
+      c->crt_xfuse = value != 0;
     } else if (k == "f32_cfg") {
       GAIN_CHECK_ARG(value >= -1 && value < kF32Configs, "cfg -1..%d",
                      kF32Configs - 1);
@@ -2400,6 +2419,8 @@ extern "C" int gain_get_tuning(gain_ctx* c, const char* key,
       *value = c->crt_kskip;
     } else if (k == "crt_fuse") {
       *value = c->crt_fuse;
+    } else if (k == "crt_xfuse") {
+      *value = c->crt_xfuse;
     } else if (k == "proj_crt_moduli") {
       *value = c->proj_crt;
     } else if (k == "f32_cfg") {

@@ -126,6 +126,7 @@ struct Irrep {
   int* N = nullptr;
   double* repres = nullptr;
   int* tcase = nullptr;
+  int maxN = 0;  // largest term count of a symmetrisation case
 };
 
 struct Basis {
@@ -144,6 +145,7 @@ struct Basis {
   // per slot: its primitive when it holds exactly one, -1 when none, -2
   // when several (then the CSR lists them)
   int* slot_one = nullptr;
+  bool slot_multi = false;  // some slot holds several primitives
   // inflation
   int maxsymcoeffs = 0, matsize = 0;
   int *ijterms = nullptr, *icontr = nullptr;
@@ -223,7 +225,7 @@ struct gain_ctx {
   std::map<std::pair<int, int>, WignerTable> wigner;  // (jindI, jindF)
   DeviceArena arena;
   Work w_c, w_X, w_Y, w_H, w_G, w_part, w_small, w_out, w_tmp;
-  Work w_xs, w_xe, w_gs, w_ge, w_res;
+  Work w_xs, w_xe, w_gs, w_ge, w_res, w_desc;
   PinnedRing ring;
   // tuning
   int gemv_rows = 4, gemv_ksplit = 8, mu_ksplit = 0;  // 0: automatic
@@ -238,6 +240,7 @@ struct gain_ctx {
   int force_crt_cfg = -1;                     // CRT-form int8 GEMMs
   bool crt_kskip = true;  // CRT dipole products skip zero k blocks
   bool crt_fuse = true;   // CRT half ls: reconstruction fused with gather
+  bool crt_xfuse = true;  // CRT half ls: X scattered straight into residues
   int proj_crt = 0;  // CRT projection moduli (0: the precision's)
   int f32_cfg = -1;  // FP32 GEMM tile configuration (-1: default)
   std::map<std::tuple<int, int, int, int>, int> tuned;

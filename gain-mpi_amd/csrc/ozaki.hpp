@@ -193,6 +193,19 @@ int crt_modulus(int nm, int i);
 // their maximum.
 int crt_beta(int nm, int64_t K);
 bool crt_norm_scaled(int nm, int64_t K);
+struct ScatterArgs;
+// The half line strength's X (the scatter of `sa`: sa.nvec vectors x ncol
+// columns = rows rows of nc) cut straight into nm residue planes and row
+// exponents, as launch_scatter_expand followed by slice_rows_crt, without
+// the FP64 X; desc: sa.nslot entries of scratch.  Fits when nc <= 4096 and
+// the irrep's repres has fewer than 2^31 entries.  `single`: every slot
+// holds at most one primitive and every case one term (the branch-free
+// form).
+bool scatter_slice_crt_fits(int nm, int nc, int64_t repres_len);
+void scatter_slice_rows_crt(int nm, int beta, const ScatterArgs& sa, int ncol,
+                            int nc, int rows, int8_t* planes, int64_t plane,
+                            int* expo, int4* desc, bool single,
+                            hipStream_t s);
 // rows x K FP64 -> nm tiled int8 residue planes (layout and padding as
 // slice_rows_i8) and one exponent per padded row (||row||_2 < 2^e or
 // max|row| < 2^e, as crt_norm_scaled(nm, Kp) says); beta must be

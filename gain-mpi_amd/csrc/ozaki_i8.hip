@@ -742,6 +742,27 @@ __host__ __device__ inline float sym_mod_f(float x, float m, float minv) {
 // root of the sum times 1 + 2^-30, times 1 + 2^-40, bounds the norm from
 // above.  Rows whose squares leave the FP64 range (entries beyond 2^500 or
 // all below 2^-500) take the bound sqrt(K) max|x_k| instead.
+// Exponent e of a row from its largest magnitude mx and its sum of squares
+// ss (any summation order: the bound carries a 2^-30 margin): with norm,
+// ||x|| < 2^e; without, max |x_k| < 2^e; 0 for a zero row.
+__device__ inline int crt_row_expo(double mx, double ss, bool norm, int K) {
+  int e = 0;
+  if (mx > 0.0) {
+    const double nb = sqrt(ss * (1.0 + 0x1p-30)) * (1.0 + 0x1p-40);
+    if (!norm) {
+      frexp(mx, &e);  // |x_k| < 2^e
+    } else if (ss > 0x1p-1000 && nb < 0x1p500) {
+      frexp(nb, &e);  // ||x|| <= nb < 2^e
+    } else {
+      frexp(mx, &e);  // |x_k| < 2^e, ||x|| < sqrt(K) 2^e
+      int lk = 0;
+      while ((int64_t)1 << lk < K) ++lk;
+      e += (lk + 1) / 2;
+    }
+  }
+  return e;
+}
+
 __global__ __launch_bounds__(256) void k_row_expo(const double* __restrict__ src,
                                                   int64_t ld, int rows, int K,
                                                   bool norm,
@@ -782,21 +803,7 @@ __global__ __launch_bounds__(256) void k_row_expo(const double* __restrict__ src
   if (tid == 0) {
     mx = fmax(fmax(red[0][0], red[0][1]), fmax(red[0][2], red[0][3]));
     ss = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
-    int e = 0;
-    if (mx > 0.0) {
-      const double nb = sqrt(ss * (1.0 + 0x1p-30)) * (1.0 + 0x1p-40);
-      if (!norm) {
-        frexp(mx, &e);  // |x_k| < 2^e
-      } else if (ss > 0x1p-1000 && nb < 0x1p500) {
-        frexp(nb, &e);  // ||x|| <= nb < 2^e
-      } else {
-        frexp(mx, &e);  // |x_k| < 2^e, ||x|| < sqrt(K) 2^e
-        int lk = 0;
-        while ((int64_t)1 << lk < K) ++lk;
-        e += (lk + 1) / 2;
-      }
-    }
-    expo[r] = e;
+    expo[r] = crt_row_expo(mx, ss, norm, K);
   }
 }
 
@@ -880,6 +887,233 @@ void slice_rows_crt(int nm, int beta, const double* src, int64_t ld,
       fail(GAIN_ERR_ARG, "CRT moduli count %d not supported", nm);
   }
   check_launch("slice_rows_crt");
+}
+
+// ---------------------------------------------------------------------------
+// The half line strength's X built and cut into residue planes in one pass
+// (no FP64 X in HBM).  k_slot_desc flattens each (column, v) slot of the
+// initial basis once per call: for a slot of one primitive its record
+// offset t0, term count, repres offset (lambda sdeg Ntot + t0) and the
+// initial-side sign (dipole_GPU.cu:437-438) in w; for a slot of several the
+// CSR range with w = 2; an empty slot has no terms.
+__global__ __launch_bounds__(256) void k_slot_desc(ScatterArgs a,
+                                                   int4* __restrict__ desc) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= a.nslot) return;
+  const int q0 = a.slot_ptr[s], q1 = a.slot_ptr[s + 1];
+  int4 d = make_int4(0, 0, 0, 0);
+  if (q1 - q0 == 1) {
+    const int r = a.slot_prim[q0];
+    const int icase = a.icontr[r];
+    const int lam = a.icontr[r + a.nprim];
+    const int t0 = a.ijt[icase];
+    const int K = a.kval[r], t = a.tau[r];
+    d = make_int4(t0, a.N[icase], lam * a.sdeg * a.ntot + t0,
+                  ((K % 3) * t + K) & 1);
+  } else if (q1 > q0) {
+    d = make_int4(q0, q1, 0, 2);
+  }
+  desc[s] = d;
+}
+
+struct XSliceArgs {
+  ScatterArgs sa;  // basis, records and vectors (its X is not used)
+  const int4* desc;
+  int ncol, nc, rows;
+  int64_t Kp, plane;
+  int8_t* planes;
+  int* expo;
+  int beta;
+  bool norm;
+};
+
+// A slot of several primitives: the scatter's loop (scatter_body).
+__device__ __noinline__ double x_value_multi(const ScatterArgs& a,
+                                             const double* cv, int64_t dof,
+                                             int q0, int q1) {
+  double acc = 0.0;
+  for (int q = q0; q < q1; ++q) {
+    const int r = a.slot_prim[q];
+    const int icase = a.icontr[r];
+    const int lam = a.icontr[r + a.nprim];
+    const int t0 = a.ijt[icase];
+    const int ne = a.N[icase];
+    const double* c = cv + t0;
+    const double* p = a.repres + (int64_t)lam * a.sdeg * a.ntot + t0 + dof;
+    const int K = a.kval[r], t = a.tau[r];
+    double x = 0.0;
+    for (int k = 0; k < ne; ++k) x = fma(c[k], p[k], x);
+    acc += (((K % 3) * t + K) & 1) ? -x : x;
+  }
+  return acc;
+}
+
+// X[vec][col][v] with the scatter's arithmetic (identical values).
+__device__ inline double x_value(const XSliceArgs& a, const double* cv,
+                                 int64_t dof, int s) {
+  const int4 d = a.desc[s];
+  if (d.w == 2) return x_value_multi(a.sa, cv, dof, d.x, d.y);
+  const double* c = cv + d.x;
+  const double* p = a.sa.repres + dof + d.z;
+  double x = 0.0;
+  for (int k = 0; k < d.y; ++k) x = fma(c[k], p[k], x);
+  return 0.0 + (d.w ? -x : x);
+}
+
+// One workgroup: 4 consecutive X rows (vector, column) of R * 8 k blocks.
+// Thread t holds row (t / 8) % 4, columns 4 (t % 8) .. + 3 of k blocks
+// 8 j + t / 32 (j < R) in registers; the row's norm comes from butterflies
+// over the lanes of that row and the four waves' partials in a fixed
+// order, then the registers are cut into residues: each 32 lanes write the
+// four rows' 128 contiguous bytes of a 1 KB block of every plane (the
+// layout of k_slice_crt).  ONE: every slot holds at most one primitive of
+// one term, and the values take no branch (an empty or padding slot loads
+// entry 0 and is zeroed), so the loads of all of a thread's values can be
+// in flight together.
+template <int NM, int R, bool ONE>
+__global__ __launch_bounds__(256) void k_scatter_slice_crt(XSliceArgs a,
+                                                           CrtConsts cc) {
+  __shared__ double red[2][4][4];  // [max | sum of squares][wave][row]
+  const int t = threadIdx.x;
+  const int q = t & 7, rr = (t >> 3) & 3, kbo = t >> 5;
+  const int n = blockIdx.x * 4 + rr;
+  const bool live = n < a.rows;
+  const int vec = live ? n / a.ncol : 0;
+  const int col = live ? n - vec * a.ncol : 0;
+  const double* cv = a.sa.cbase + (live ? a.sa.c_off[vec] : 0);
+  const int64_t dof = live ? (int64_t)a.sa.ideg[vec] * a.sa.ntot : 0;
+  const int nkb = (int)(a.Kp >> 5);
+  double x[R][4];
+  double mx = 0.0, ss = 0.0;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = (j * 8 + kbo) * 32 + 4 * q + u;
+      const bool ok = live && k < a.nc;
+      double v;
+      if constexpr (ONE) {
+        const int4 d = a.desc[ok ? col * a.nc + k : 0];
+        const double xv = fma(cv[d.x], a.sa.repres[dof + d.z], 0.0);
+        v = (ok && d.y) ? 0.0 + (d.w ? -xv : xv) : 0.0;
+      } else {
+        v = ok ? x_value(a, cv, dof, col * a.nc + k) : 0.0;
+      }
+      x[j][u] = v;
+      mx = fmax(mx, fabs(v));
+      ss = fma(v, v, ss);
+    }
+  }
+  // lanes of one row in a wave differ in bits 0-2 and 5 of the lane id
+#pragma unroll
+  for (int off = 1; off <= 32; off <<= 1) {
+    if (off == 8 || off == 16) continue;
+    mx = fmax(mx, __shfl_xor(mx, off));
+    ss += __shfl_xor(ss, off);
+  }
+  if ((t & 39) == 0) {  // q = 0 and lane bit 5 clear: one lane per row
+    red[0][t >> 6][rr] = mx;
+    red[1][t >> 6][rr] = ss;
+  }
+  __syncthreads();
+  mx = fmax(fmax(red[0][0][rr], red[0][1][rr]),
+            fmax(red[0][2][rr], red[0][3][rr]));
+  ss = (red[1][0][rr] + red[1][1][rr]) + (red[1][2][rr] + red[1][3][rr]);
+  const int e = crt_row_expo(mx, ss, a.norm, a.nc);
+  if (q == 0 && kbo == 0) a.expo[n] = e;
+  const int sh = a.beta - 1 - e;
+  int8_t* tile = a.planes + (int64_t)(n >> 5) * (a.Kp << 5) + (n & 31) * 32 +
+                 4 * q;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int kb = j * 8 + kbo;
+    if (kb >= nkb) continue;
+    uint32_t w[NM];
+#pragma unroll
+    for (int i = 0; i < NM; ++i) w[i] = 0u;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      // |X| <= 2^{beta-1} <= 2^49: exact in double
+      const double X = rint(ldexp(x[j][u], sh));
+#pragma unroll
+      for (int i = 0; i < NM; ++i) {
+        const double v = sym_mod_d(X, (double)cc.m[i], cc.minv_d[i]);
+        w[i] |= (uint32_t)(uint8_t)(int8_t)(int)v << (8 * u);
+      }
+    }
+    int8_t* dst = tile + (int64_t)kb * 1024;
+#pragma unroll
+    for (int i = 0; i < NM; ++i) *(uint32_t*)(dst + i * a.plane) = w[i];
+  }
+}
+
+bool scatter_slice_crt_fits(int nm, int nc, int64_t repres_len) {
+  return (nm == 11 || nm == 12 || nm == 14) && nc >= 1 &&
+         i8_kpad(nc) <= 32 * 8 * 16 && repres_len < INT32_MAX;
+}
+
+void scatter_slice_rows_crt(int nm, int beta, const ScatterArgs& sa, int ncol,
+                            int nc, int rows, int8_t* planes, int64_t plane,
+                            int* expo, int4* desc, bool single,
+                            hipStream_t s) {
+  const int64_t Rp = i8_rows_pad(rows), Kp = i8_kpad(nc);
+  GAIN_CHECK_ARG(rows >= 0 && ncol >= 1 && sa.nslot == ncol * nc &&
+                     (int64_t)sa.nvec * ncol == rows,
+                 "scatter_slice_rows_crt: %d rows of %d columns, %d vectors,"
+                 " %d slots", rows, ncol, sa.nvec, sa.nslot);
+  GAIN_CHECK_ARG(Kp <= 32 * 8 * 16,
+                 "scatter_slice_rows_crt: K = %d beyond %d", nc, 32 * 8 * 16);
+  GAIN_CHECK_ARG(plane >= Rp * Kp && ((uintptr_t)planes & 3) == 0 &&
+                     (plane & 3) == 0,
+                 "scatter_slice_rows_crt: plane of %lld bytes < %lld x %lld "
+                 "or unaligned", (long long)plane, (long long)Rp,
+                 (long long)Kp);
+  GAIN_CHECK_ARG(beta == crt_beta(nm, Kp),
+                 "scatter_slice_rows_crt: width %d is not that of %d moduli "
+                 "at K %lld", beta, nm, (long long)Kp);
+  hipLaunchKernelGGL(k_slot_desc, dim3((unsigned)((sa.nslot + 255) / 256)),
+                     dim3(256), 0, s, sa, desc);
+  check_launch("slot_desc");
+  XSliceArgs a{};
+  a.sa = sa;
+  a.desc = desc;
+  a.ncol = ncol;
+  a.nc = nc;
+  a.rows = rows;
+  a.Kp = Kp;
+  a.plane = plane;
+  a.planes = planes;
+  a.expo = expo;
+  a.beta = beta;
+  a.norm = crt_norm_scaled(nm, Kp);
+  const CrtConsts& cc = crt_consts(nm);
+  const dim3 grid((unsigned)(Rp / 4));
+  const bool small = Kp <= 32 * 8 * 8;
+  switch (nm) {
+#define GAIN_XSLICE_R(NN, RR)                                                \
+  if (single)                                                                \
+    hipLaunchKernelGGL((k_scatter_slice_crt<NN, RR, true>), grid, dim3(256), \
+                       0, s, a, cc);                                         \
+  else                                                                       \
+    hipLaunchKernelGGL((k_scatter_slice_crt<NN, RR, false>), grid,           \
+                       dim3(256), 0, s, a, cc);
+#define GAIN_XSLICE(NN)                                                      \
+  case NN:                                                                   \
+    if (small) {                                                             \
+      GAIN_XSLICE_R(NN, 8)                                                   \
+    } else {                                                                 \
+      GAIN_XSLICE_R(NN, 16)                                                  \
+    }                                                                        \
+    break;
+    GAIN_XSLICE(11)
+    GAIN_XSLICE(12)
+    GAIN_XSLICE(14)
+#undef GAIN_XSLICE
+#undef GAIN_XSLICE_R
+    default:
+      fail(GAIN_ERR_ARG, "CRT moduli count %d not supported", nm);
+  }
+  check_launch("scatter_slice_crt");
 }
 
 // Configurations of the per-modulus GEMM: wave tile (32 TM) x (32 TN), WM x

@@ -315,6 +315,10 @@ int gain_get_timing(gain_ctx* ctx, const char* name, double* total_ms,
  * "crt_fuse" (1, the default: the CRT-form half line strength of K-block
  * bases is reconstructed and gathered by one kernel, identical results
  * without the FP64 mu X intermediate; 0: the two kernels),
+ * "crt_xfuse" (1, the default: the CRT-form half line strength builds the
+ * initial vectors' X straight into its residue planes, one kernel, when
+ * N_c <= 4096; 0: the FP64 X scattered, then cut by two kernels; the
+ * residues are identical),
  * "gemv_rows" (2/4/8), "gemv_ksplit", "mu_ksplit" (0 = automatic),
  * "mfma" (0: vector-ALU GEMMs, for cross-checks), "noalloc_cap" (bytes of
  * the allocation-free entry point's array a call may use, 0 = all),

@@ -540,12 +540,12 @@ def test_crt_zero_kblock_skip_is_exact(cfg, zero_tiles):
     assert ns > 0 and es <= 1e-7, es
 
 
-def _fused_vs_two_kernels(prob, initials, precision):
+def _fused_vs_two_kernels(prob, initials, precision, key="crt_fuse"):
     res = {}
     for fuse in (1, 0):
         with Engine.from_problem(prob) as eng:
-            eng.set_tuning("crt_fuse", fuse)
-            assert eng.get_tuning("crt_fuse") == fuse
+            eng.set_tuning(key, fuse)
+            assert eng.get_tuning(key) == fuse
             res[fuse] = engine_lines(eng, prob, initials, "batched",
                                      precision)
     on, off = res[1], res[0]
@@ -617,6 +617,65 @@ def test_crt_fused_gather_irregular_basis(seed):
     assert set(got) == set(ref)
     es, _, ns, _ = _strong_weak_err(got, ref)
     assert ns > 0 and es <= 1e-7, es
+
+
+def _xslice_problem(case):
+    if case.startswith("irregular"):
+        seed = int(case[-1])
+        rng = np.random.default_rng(seed)
+        nc = 7
+        bases = [irregular_basis(2, nc, rng, 12),
+                 irregular_basis(3, nc, rng, 12)]
+        prob = problem_with_bases(bases, nc, seed)
+    elif case == "rotsym":
+        prob = synth.make_problem([3, 4], 6, "c3v3", rotsym="C3V(M)")
+    else:
+        scheme, nc = {"c2v4": ("c2v4", 16), "td5": ("td5", 20)}[case]
+        prob = synth.make_problem([2, 3], nc, scheme)
+    prob.params.erange_lower = (0.0, 1e9)
+    return prob
+
+
+@pytest.mark.parametrize("case", ["c2v4", "td5", "rotsym", "irregular1",
+                                  "irregular2"])
+@pytest.mark.parametrize("precision", [7, 8, 9])
+def test_crt_x_scattered_into_residues_is_exact(case, precision):
+    """The CRT-form half line strength's X built straight into its residue
+    planes by one kernel (tuning key crt_xfuse) equals the FP64 scatter
+    followed by the exponent and slicing kernels: identical S, A, I, flags
+    and pair line strengths, on K-block bases, the degenerate Td irreps
+    (records' components 1 and 2), a symmetric-top basis and irregular
+    bases (slots holding several primitives, empty slots); S within 1e-7
+    of the oracle on strong lines."""
+    prob = _xslice_problem(case)
+    rows = prob.state_table()
+    initials = list(range(0, len(rows), 3))
+    tr, _ = orc.run(prob, initials)
+    ref = oracle_map(tr)
+    got = _fused_vs_two_kernels(prob, initials, precision, "crt_xfuse")
+    assert set(got) == set(ref)
+    es, _, ns, _ = _strong_weak_err(got, ref)
+    assert ns > 0 and es <= 1e-7, es
+
+
+def test_crt_x_scattered_into_residues_long_rows():
+    """The same at N_c = 2500 (rows of 2500 dipole columns: the 16-round
+    form of the kernel, 79 k blocks): identical to the three-kernel path
+    for 11 and 12 moduli."""
+    prob = synth.make_problem([1, 2], 2500, "c2v4",
+                              blocks={(0, 0), (1, 1)})
+    prob.params.erange_lower = (0.0, 1e9)
+    n0 = prob.blocks[(0, 0)].n
+    levels = list(range(0, n0, 7))
+    for precision in (7, 8):
+        res = {}
+        for fuse in (1, 0):
+            with Engine.from_problem(prob) as eng:
+                eng.set_tuning("crt_xfuse", fuse)
+                res[fuse] = eng.batched_lines(0, 0, levels, 1, 1, precision)
+        assert (res[1]["flags"] & 1).sum() > 0
+        for f in ("S", "A", "I", "flags", "ls"):
+            assert np.array_equal(res[1][f], res[0][f]), (precision, f)
 
 
 @pytest.mark.parametrize("cfg", list(range(12)))
