@@ -1747,26 +1747,43 @@ __global__ __launch_bounds__(256) void k_crt_gather_h(CrtGatherArgs a,
   const int nitem2 = ncolF * 32;               // (final column, row)
   for (int vec = blockIdx.y; vec < a.nvec; vec += gridDim.y) {
     const int64_t colv = (int64_t)vec * ncolI;
+    // two items per thread at a time: every residue word and exponent of
+    // both is requested before either is reconstructed
 #pragma unroll 1
-    for (int i = t; i < nitem1; i += 256) {
-      const int al = i / (ncolI * 8);
-      const int rem = i - al * ncolI * 8;
-      const int col = rem >> 3, quad = rem & 7;
-      const int64_t n = colv + col;
-      const uint32_t* R =
-          (const uint32_t*)(g.R + al * zst + ((int64_t)rb * nr + n) * 32 +
-                            4 * quad);
-      uint32_t w[NM];
+    for (int i0 = t; i0 < nitem1; i0 += 2 * 256) {
+      uint32_t w[2][NM];
+      int ex[2][4];
+      int off[2];
 #pragma unroll
-      for (int k = 0; k < NM; ++k) w[k] = R[k * pst];
-      double X[4];
-      crt_recon4<NM>(w, cc, X);
-      const int eb = g.eB[n] - 2 * (g.beta - 1);
-      double* yo = ys + (al * ncolI + col) * 32 + 4 * quad;
+      for (int h = 0; h < 2; ++h) {
+        const int i = min(i0 + h * 256, nitem1 - 1);
+        const int al = i / (ncolI * 8);
+        const int rem = i - al * ncolI * 8;
+        const int col = rem >> 3, quad = rem & 7;
+        const int64_t n = colv + col;
+        const uint32_t* R =
+            (const uint32_t*)(g.R + al * zst + ((int64_t)rb * nr + n) * 32 +
+                              4 * quad);
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int v = min(rb * 32 + 4 * quad + q, a.nc - 1);
-        yo[q] = ldexp(X[q], g.eA[al * g.ea_z + v] + eb);
+        for (int k = 0; k < NM; ++k) w[h][k] = R[k * pst];
+        const int eb = g.eB[n] - 2 * (g.beta - 1);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int v = min(rb * 32 + 4 * quad + q, a.nc - 1);
+          ex[h][q] = g.eA[al * g.ea_z + v] + eb;
+        }
+        off[h] = (al * ncolI + col) * 32 + 4 * quad;
+      }
+      // keep both items' requests ahead of the reconstructions
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (i0 + h * 256 >= nitem1) break;
+        double X[4];
+        crt_recon4<NM>(w[h], cc, X);
+        double* yo = ys + off[h];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) yo[q] = ldexp(X[q], ex[h][q]);
       }
     }
     __syncthreads();
