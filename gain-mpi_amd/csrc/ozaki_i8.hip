@@ -1698,6 +1698,7 @@ __global__ __launch_bounds__(256) void k_crt_combine(I8GemmArgs g,
 // workgroup (their loads overlap phase 1), and phase 2 takes four outputs
 // per thread at a time, their slot lookups in flight together.
 constexpr int kCrtGatherMaxColF = 64;  // final columns 2 (J_F + 1), J_F < 32
+constexpr int kCrtGatherVecs = 4;      // vectors per workgroup (grid stride)
 
 template <int NM>
 __global__ __launch_bounds__(256) void k_crt_gather_h(CrtGatherArgs a,
@@ -1917,8 +1918,11 @@ void launch_crt_gather_h(const CrtGatherArgs& a, hipStream_t s) {
   const size_t lds = (size_t)3 * a.ncolI * 32 * sizeof(double);
   GAIN_CHECK_ARG(lds <= 64 * 1024, "CRT gather: %d X columns exceed the LDS "
                  "tile", a.ncolI);
+  // several vectors per workgroup share its coefficient table
   const dim3 grid((unsigned)((a.nc + 31) / 32),
-                  (unsigned)std::min(a.nvec, 65535));
+                  (unsigned)std::min((a.nvec + kCrtGatherVecs - 1) /
+                                         kCrtGatherVecs,
+                                     65535));
   switch (g.S) {
 #define GAIN_CRT_GATHER(NN)                                                  \
   case NN:                                                                   \
