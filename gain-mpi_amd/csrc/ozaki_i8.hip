@@ -971,8 +971,9 @@ __device__ inline double x_value(const XSliceArgs& a, const double* cv,
 // entry 0 and is zeroed), so the loads of all of a thread's values can be
 // in flight together.
 template <int NM, int R, bool ONE>
-__global__ __launch_bounds__(256) void k_scatter_slice_crt(XSliceArgs a,
-                                                           CrtConsts cc) {
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(ONE ? 3 : 1))) void k_scatter_slice_crt(
+    XSliceArgs a, CrtConsts cc) {
   __shared__ double red[2][4][4];  // [max | sum of squares][wave][row]
   const int t = threadIdx.x;
   const int q = t & 7, rr = (t >> 3) & 3, kbo = t >> 5;
